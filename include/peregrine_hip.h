@@ -1,0 +1,167 @@
+/*
+ * libperegrine_hip -- C ABI of the MI355X value-iteration solver.
+ *
+ * This library replaces the in-process GPU boundary of Peregrine's solver:
+ * every kernel launch of `solver/src/Gpu.jl` and the device-array plumbing
+ * that `solver/src/Run.jl` / `solver/src/OutOfCore.jl` do around it.  The
+ * caller owns host memory; the library owns device memory.  Indices are
+ * 0-based.  Every entry point returns an int status (0 = ok, < 0 = error)
+ * and never throws; `pg_last_error()` gives the message of the last failure
+ * on the calling thread.  A solver handle is used by one thread at a time.
+ *
+ * Conventions kept from the reference (file:line into the reference tree):
+ *   - flat index  ((((ix*Ny+iy)*Nh+ih)*Nvx+ivx)*Nvy+ivy)*Nw+iw
+ *     (solver/src/PeregrineSolver.jl:78-81, docs/TABLE_FORMAT.md:225-242)
+ *   - occupancy   one byte per (x, y, h) cell, index (ix*Ny+iy)*Nh+ih
+ *     (solver/src/Pipeline.jl:392-397)
+ *   - controls    structure of arrays, ctl[k], ctl[k+nctl], ctl[k+2*nctl],
+ *     entry 0 is coast (solver/src/Run.jl:283-285,
+ *     solver/src/PeregrineSolver.jl:1263-1275)
+ */
+#ifndef PEREGRINE_HIP_H
+#define PEREGRINE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PG_ABI_VERSION 1
+
+/* status codes */
+#define PG_OK               0
+#define PG_ERR_ARG         -1   /* a bad argument: null, size, range        */
+#define PG_ERR_HIP         -2   /* a HIP runtime call failed                */
+#define PG_ERR_UNSUPPORTED -3   /* a parameter combination not built here   */
+#define PG_ERR_NOMEM       -4   /* device memory budget exceeded            */
+#define PG_ERR_STATE       -5   /* call out of order (e.g. sweep before seed) */
+
+/* Uniform 6-D grid over (x, y, h, vx, vy, w); axis 2 (h) is periodic.
+ * Mirrors `Grid6` (solver/src/PeregrineSolver.jl:47-60): lo/step in float,
+ * heading lo = Float32(-pi), step = Float32(2pi/n). */
+typedef struct pg_grid {
+    int32_t n[6];
+    float   lo[6];
+    float   step[6];
+} pg_grid;
+
+/* Body-frame drivetrain regression, row-major 3x3 blocks.
+ * Mirrors `Model` (solver/src/PeregrineSolver.jl:100-109). knee <= 0 or NaN
+ * disables traction saturation (PeregrineSolver.jl:126-136). */
+typedef struct pg_model {
+    float B[9];
+    float A[9];
+    float q[3];
+    float S[9];
+    float D[9];
+    float c[3];
+    float eps[3];
+    float knee;
+} pg_model;
+
+/* Backup parameters, field for field `Params`
+ * (solver/src/PeregrineSolver.jl:897-916). Booleans are int32 0/1. */
+typedef struct pg_params {
+    float   dt;
+    int32_t nsub;
+    int32_t checks;
+    int32_t adaptive_checks;
+    int32_t nearest;
+    float   cap;
+    int32_t ntau;
+    float   tau_ratio;
+    float   cfl;
+    float   tau_min;
+    float   tau_max;
+    float   hmax;
+    int32_t ncoarse;
+    int32_t rounds;
+    float   delta0;
+    int32_t rk2;
+    int32_t vclamp;
+    int32_t simplex;
+} pg_params;
+
+/* How V is held on the device. */
+#define PG_VSTORE_F32       0   /* plain FP32, Jacobi double buffer          */
+#define PG_VSTORE_F16_TILE  1   /* FP16 with per-tile FP32 scale/offset      */
+
+typedef struct pg_ctx pg_ctx;
+typedef struct pg_solver pg_solver;
+
+/* Library version string and ABI number. */
+const char* pg_version(void);
+int pg_abi_version(void);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* pg_last_error(void);
+
+/* Number of visible HIP devices. */
+int pg_device_count(int32_t* n);
+
+/* Bind a device and create its streams. Replaces `CUDA.functional()` and the
+ * implicit CUDA.jl context (solver/src/Run.jl:1653-1654). */
+int pg_create(int32_t device, pg_ctx** out);
+int pg_destroy(pg_ctx* ctx);
+
+/* Device memory this run may use: free * headroom.
+ * Replaces `gpu_budget` (solver/src/Gpu.jl:181-185). */
+int pg_device_budget(pg_ctx* ctx, double headroom, int64_t* bytes);
+
+/* Would this grid fit whole, before anything is allocated?
+ * Replaces `gpu_fits` (solver/src/Gpu.jl:204-209); the per-cell byte count is
+ * the Jacobi pair of V buffers in the chosen store. */
+int pg_grid_fits(pg_ctx* ctx, const pg_grid* g, int32_t vstore, double headroom,
+                 int32_t* fits, double* need_gib, double* free_gib);
+
+/* Allocate a resident solver for one grid: V (two buffers), occupancy and the
+ * control table go to the device. Replaces the uploads at
+ * solver/src/Run.jl:1776-1784. `occ` holds n[0]*n[1]*n[2] bytes (0 free,
+ * nonzero blocked); `ctl_soa` holds 3*nctl floats with entry 0 = coast. */
+int pg_solver_create(pg_ctx* ctx, const pg_grid* g, const pg_model* m,
+                     const pg_params* p, const float* ctl_soa, int32_t nctl,
+                     const uint8_t* occ, int32_t vstore, pg_solver** out);
+int pg_solver_destroy(pg_solver* s);
+
+/* fill!(V, cap); V[seeds] = 0 (solver/src/Run.jl:1437-1438). The seeds are
+ * kept and reasserted after every sweep (Run.jl:1464). */
+int pg_fill_seed(pg_solver* s, const int64_t* seeds, int64_t nseeds);
+
+/* One Jacobi value-iteration sweep: V' = min(V, T V), seeds reasserted.
+ * Replaces `sweep_gpu!` (solver/src/Gpu.jl:5-37). Returns the sum of the
+ * per-cell decreases (FP64) and their maximum, the sup-norm of the change.
+ * `phase` and `rev` are accepted for interface parity: the plain scheme
+ * scans the whole control table (phase unused) and a Jacobi sweep has no
+ * order (rev unused). */
+int pg_sweep(pg_solver* s, int32_t phase, int32_t rev, double* sum_decrease,
+             float* linf);
+
+/* Sweep until max_iters, or until the change is quiet: sum <= tol_sum (the
+ * reference's rule, Run.jl:1449-1471) when tol_sum >= 0, or sup-norm <
+ * tol_linf when tol_linf > 0 (both may be set; either stops).
+ * Replaces `solve_value!` (solver/src/Run.jl:1434-1474) after the seeding. */
+int pg_solve(pg_solver* s, int32_t max_iters, double tol_sum, float tol_linf,
+             int32_t* iters, double* last_sum, float* last_linf);
+
+/* Argmin control index (0 = coast) of the Bellman backup of the current V,
+ * one byte per cell; blocked cells report 0 (PeregrineSolver.jl:1226-1228).
+ * Ties resolve to the lowest index, as the strict `<` scan does
+ * (PeregrineSolver.jl:1245-1275). nctl must be <= 255. */
+int pg_policy(pg_solver* s, uint8_t* pol_host);
+
+/* Copy n values of the current V (FP32) starting at flat index off. */
+int pg_download_V(pg_solver* s, float* dst, int64_t off, int64_t n);
+int pg_upload_V(pg_solver* s, const float* src, int64_t off, int64_t n);
+
+/* Duration of the last sweep kernel, from HIP events on the solver stream. */
+int pg_last_sweep_ms(pg_solver* s, float* ms);
+
+/* Number of cells of the solver's grid. */
+int pg_solver_ncells(pg_solver* s, int64_t* n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PEREGRINE_HIP_H */

@@ -1,0 +1,555 @@
+// CPU ORACLE -- TEST INFRASTRUCTURE ONLY.
+//
+// A from-source restatement of the value-iteration hot path of Peregrine's
+// solver (Julia, solver/src/PeregrineSolver.jl and solver/src/Pipeline.jl of
+// the reference tree), used as the checker for the HIP library. Only tests/,
+// __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it; the
+// product never links, imports or calls it.
+//
+// Arithmetic contract. Templated on the value type T:
+//   T = float  : follows the reference's Float32 CPU arithmetic operation by
+//                operation -- same order of every sum and product, no fused
+//                multiply-add (build with -ffp-contract=off), IEEE division,
+//                ties-to-even rounding, truncating conversions.
+//   T = double : the same algorithm carried in double (grid, model and
+//                controls promoted from their float values), the FP64 truth
+//                used to measure reduced-precision stores.
+// sincos: Julia's Base.sincos(Float32) is a Float32 front end over double
+// kernels taken from FreeBSD msun (k_sinf.c / k_cosf.c by Bruce D. Evans,
+// e_rem_pio2f.c). This file evaluates those published kernels in double with
+// the medium-range Cody-Waite reduction; the exact bits of Julia's reduction
+// front end are parity-unpinned (no Julia here), the accuracy (<= 1 ulp vs
+// the correctly rounded value) is pinned by tests/test_oracle_math.py.
+#pragma once
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include "../include/peregrine_hip.h"
+
+namespace orc {
+
+// ---------------------------------------------------------------------------
+// sincos
+// ---------------------------------------------------------------------------
+
+// FreeBSD msun __kernel_sindf / __kernel_cosdf coefficients.
+static const double kS1 = -0x15555554cbac77.0p-55;
+static const double kS2 = 0x111110896efbb2.0p-59;
+static const double kS3 = -0x1a00f9e2cae774.0p-65;
+static const double kS4 = 0x16cd878c3b46a7.0p-71;
+static const double kC0 = -0x1ffffffd0c5e81.0p-54;
+static const double kC1 = 0x155553e1053a42.0p-57;
+static const double kC2 = -0x16c087e80f1e27.0p-62;
+static const double kC3 = 0x199342e0ee5069.0p-68;
+// e_rem_pio2f.c medium-range constants: 2/pi, first 33 bits of pi/2, tail.
+static const double kInvPio2 = 6.36619772367581382433e-01;
+static const double kPio2_1 = 1.57079631090164184570e+00;
+static const double kPio2_1t = 1.58932547735281966916e-08;
+
+inline float ksin(double x) {
+    double z = x * x;
+    double w = z * z;
+    double r = kS3 + z * kS4;
+    double s = z * x;
+    return (float)((x + s * (kS1 + z * kS2)) + s * w * r);
+}
+
+inline float kcos(double x) {
+    double z = x * x;
+    double w = z * z;
+    double r = kC2 + z * kC3;
+    return (float)(((1.0 + z * kC0) + w * kC1) + (w * z) * r);
+}
+
+// Julia: sincos(x::Float32) (base/special/trig.jl). Non-finite input gives
+// NaN (the reference never reaches sincos with one: `diverged` guards it,
+// PeregrineSolver.jl:191-192).
+inline void sincos_f32(float x, float& s, float& c) {
+    if (!std::isfinite(x)) {
+        s = c = std::numeric_limits<float>::quiet_NaN();
+        return;
+    }
+    const float pio4f = 3.14159274101257324f / 4.0f;  // Float32(pi)/4
+    if (std::fabs(x) < pio4f) {
+        if (x == 0.0f) { s = x; c = 1.0f; return; }
+        double xd = (double)x;
+        s = ksin(xd);
+        c = kcos(xd);
+        return;
+    }
+    double xd = (double)x;
+    double fn = std::nearbyint(xd * kInvPio2);      // ties-to-even
+    double r = xd - fn * kPio2_1;
+    double w = fn * kPio2_1t;
+    double y = r - w;
+    int64_t n = (int64_t)fn;
+    float si = ksin(y), co = kcos(y);
+    switch (n & 3) {
+        case 0: s = si; c = co; break;
+        case 1: s = co; c = -si; break;
+        case 2: s = -si; c = -co; break;
+        default: s = -co; c = si; break;
+    }
+}
+
+inline void sincos_t(float x, float& s, float& c) { sincos_f32(x, s, c); }
+inline void sincos_t(double x, double& s, double& c) { s = std::sin(x); c = std::cos(x); }
+
+// ---------------------------------------------------------------------------
+// Julia scalar semantics
+// ---------------------------------------------------------------------------
+
+template <class T> inline bool isfin(T v) { return std::isfinite(v); }
+// Julia min/max for non-NaN operands (every operand here is non-NaN).
+template <class T> inline T jmin(T a, T b) { return (a < b) ? a : b; }
+template <class T> inline T jmax(T a, T b) { return (a > b) ? a : b; }
+// clamp(x, lo, hi) = ifelse(x > hi, hi, ifelse(x < lo, lo, x))
+template <class T> inline T jclamp(T x, T lo, T hi) { return x > hi ? hi : (x < lo ? lo : x); }
+// round(x) with RoundNearest: ties to even.
+template <class T> inline T jround(T x) { return std::nearbyint(x); }
+
+// ---------------------------------------------------------------------------
+// Grid (PeregrineSolver.jl:47-81)
+// ---------------------------------------------------------------------------
+
+inline int64_t ncells(const pg_grid& g) {
+    int64_t n = 1;
+    for (int k = 0; k < 6; ++k) n *= (int64_t)g.n[k];
+    return n;
+}
+
+// axisvalue(g, k, i) = g.lo[k] + Float32(i) * g.step[k]   (:65)
+template <class T> inline T axisvalue(const pg_grid& g, int k, int32_t i) {
+    return (T)g.lo[k] + (T)(float)i * (T)g.step[k];
+}
+
+// unflatten (:68-76)
+inline void unflatten(const pg_grid& g, int64_t idx, int32_t i[6]) {
+    int64_t r = idx;
+    for (int k = 5; k >= 1; --k) {
+        i[k] = (int32_t)(r % (int64_t)g.n[k]);
+        r /= (int64_t)g.n[k];
+    }
+    i[0] = (int32_t)r;
+}
+
+// flatten (:78-81)
+inline int64_t flatten(const pg_grid& g, int32_t i1, int32_t i2, int32_t i3, int32_t i4,
+                       int32_t i5, int32_t i6) {
+    return (((((int64_t)i1 * g.n[1] + i2) * g.n[2] + i3) * g.n[3] + i4) * g.n[4] + i5) *
+               g.n[5] + i6;
+}
+
+// ---------------------------------------------------------------------------
+// Drivetrain model (PeregrineSolver.jl:100-171)
+// ---------------------------------------------------------------------------
+
+// traction_gain (:126-136); tanh from the C library.
+template <class T> inline T traction_gain(T u1, T u2, T u3, float knee) {
+    if (std::isnan(knee) || knee <= 0.0f) return (T)1;
+    T m = std::sqrt(u1 * u1 + u2 * u2 + u3 * u3);
+    T r = m / (T)knee;
+    if (r < (T)1.0e-6f) return (T)1;
+    return std::tanh(r) / r;
+}
+
+template <class T> inline void saturate(const pg_model& m, T u1r, T u2r, T u3r, T& u1, T& u2, T& u3) {
+    T gn = traction_gain<T>(u1r, u2r, u3r, m.knee);
+    u1 = u1r * gn; u2 = u2r * gn; u3 = u3r * gn;
+}
+
+// csign(v, e) = clamp(v / e, -1, 1)   (:120)
+template <class T> inline T csign(T v, float e) { return jclamp(v / (T)e, (T)-1, (T)1); }
+
+// body_accel_sat (:150-162): each row is one left-to-right sum of 14 terms.
+template <class T>
+inline void body_accel_sat(const pg_model& m, T u1, T u2, T u3, T vx, T vy, T w, T& ax, T& ay, T& al) {
+    const float* B = m.B; const float* A = m.A; const float* q = m.q;
+    const float* S = m.S; const float* D = m.D; const float* c = m.c;
+    T w2 = w * w;
+    T sx = csign(vx, m.eps[0]), sy = csign(vy, m.eps[1]), sw = csign(w, m.eps[2]);
+    T dx = std::fabs(vx) * vx, dy = std::fabs(vy) * vy, dw = std::fabs(w) * w;
+    for (int r = 0; r < 3; ++r) {
+        T acc = (T)B[3 * r] * u1;
+        acc = acc + (T)B[3 * r + 1] * u2;
+        acc = acc + (T)B[3 * r + 2] * u3;
+        acc = acc + (T)A[3 * r] * vx;
+        acc = acc + (T)A[3 * r + 1] * vy;
+        acc = acc + (T)A[3 * r + 2] * w;
+        acc = acc + (T)q[r] * w2;
+        acc = acc + (T)S[3 * r] * sx;
+        acc = acc + (T)S[3 * r + 1] * sy;
+        acc = acc + (T)S[3 * r + 2] * sw;
+        acc = acc + (T)D[3 * r] * dx;
+        acc = acc + (T)D[3 * r + 1] * dy;
+        acc = acc + (T)D[3 * r + 2] * dw;
+        acc = acc + (T)c[r];
+        if (r == 0) ax = acc; else if (r == 1) ay = acc; else al = acc;
+    }
+}
+
+// diverged (:191-192)
+template <class T> inline bool diverged(T x, T y, T h, T vx, T vy, T w) {
+    return !isfin(x + y + h + vx + vy + w);
+}
+
+// step_state: forward Euler, all derivatives from the old state (:208-223).
+template <class T>
+inline void step_state(const pg_model& m, T s[6], T u1r, T u2r, T u3r, T dt, int32_t nsub) {
+    T hs = dt / (T)(float)nsub;
+    T u1, u2, u3;
+    saturate<T>(m, u1r, u2r, u3r, u1, u2, u3);
+    T x = s[0], y = s[1], h = s[2], vx = s[3], vy = s[4], w = s[5];
+    for (int32_t k = 0; k < nsub; ++k) {
+        if (diverged(x, y, h, vx, vy, w)) {
+            for (int j = 0; j < 6; ++j) s[j] = std::numeric_limits<T>::quiet_NaN();
+            return;
+        }
+        T ax, ay, al;
+        body_accel_sat<T>(m, u1, u2, u3, vx, vy, w, ax, ay, al);
+        T dvx = ax + w * vy;
+        T dvy = ay - w * vx;
+        T sh, ch;
+        sincos_t(h, sh, ch);
+        x = x + (ch * vx - sh * vy) * hs;
+        y = y + (sh * vx + ch * vy) * hs;
+        h = h + w * hs;
+        vx = vx + dvx * hs;
+        vy = vy + dvy * hs;
+        w = w + al * hs;
+    }
+    s[0] = x; s[1] = y; s[2] = h; s[3] = vx; s[4] = vy; s[5] = w;
+}
+
+// state_deriv (:226-236)
+template <class T>
+inline void state_deriv(const pg_model& m, const T s[6], T u1, T u2, T u3, T d[6]) {
+    T ax, ay, al;
+    body_accel_sat<T>(m, u1, u2, u3, s[3], s[4], s[5], ax, ay, al);
+    T sh, ch;
+    sincos_t(s[2], sh, ch);
+    d[0] = ch * s[3] - sh * s[4];
+    d[1] = sh * s[3] + ch * s[4];
+    d[2] = s[5];
+    d[3] = ax + s[5] * s[4];
+    d[4] = ay - s[5] * s[3];
+    d[5] = al;
+}
+
+// step_state_rk2: explicit midpoint (:259-279).
+template <class T>
+inline void step_state_rk2(const pg_model& m, T s[6], T u1r, T u2r, T u3r, T dt, int32_t nsub) {
+    T hs = dt / (T)(float)nsub;
+    T hh = (T)0.5f * hs;
+    T u1, u2, u3;
+    saturate<T>(m, u1r, u2r, u3r, u1, u2, u3);
+    for (int32_t k = 0; k < nsub; ++k) {
+        if (diverged(s[0], s[1], s[2], s[3], s[4], s[5])) {
+            for (int j = 0; j < 6; ++j) s[j] = std::numeric_limits<T>::quiet_NaN();
+            return;
+        }
+        T d1[6], mid[6], d2[6];
+        state_deriv<T>(m, s, u1, u2, u3, d1);
+        for (int j = 0; j < 6; ++j) mid[j] = s[j] + d1[j] * hh;
+        if (diverged(mid[0], mid[1], mid[2], mid[3], mid[4], mid[5])) {
+            for (int j = 0; j < 6; ++j) s[j] = std::numeric_limits<T>::quiet_NaN();
+            return;
+        }
+        state_deriv<T>(m, mid, u1, u2, u3, d2);
+        for (int j = 0; j < 6; ++j) s[j] = s[j] + d2[j] * hs;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Value reads: plain V, or the escape pass's view of it (:756-882)
+// ---------------------------------------------------------------------------
+
+template <class T> struct PlainV {
+    const T* V;
+    inline T operator()(int64_t i) const { return V[i]; }
+};
+
+// EscapeView getindex (:836-842): obstacle corners read cap unless the
+// updating cell is itself blocked; stored negatives read as their escape
+// time, terminal cells as zero, the rest as cap.
+template <class T> struct EscV {
+    const T* V;
+    const uint8_t* occ;
+    int64_t blk;
+    T cap;
+    bool ignore;
+    inline T operator()(int64_t i) const {
+        if (!ignore && occ[i / blk]) return cap;
+        T v = V[i];
+        return v < (T)0 ? -v : (v < cap ? (T)0 : cap);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Interpolation (:399-466, :571-665)
+// ---------------------------------------------------------------------------
+
+template <class T, class VR>
+inline T interp(const VR& V, const pg_grid& g, T x, T y, T h, T vx, T vy, T w, bool nearest,
+                T cap, bool vclamp) {
+    T f1 = (x - (T)g.lo[0]) / (T)g.step[0];
+    T f2 = (y - (T)g.lo[1]) / (T)g.step[1];
+    T n3 = (T)(float)g.n[2];
+    T f3 = (h - (T)g.lo[2]) / (T)g.step[2];
+    f3 = f3 - std::floor(f3 / n3) * n3;
+    T f4 = (vx - (T)g.lo[3]) / (T)g.step[3];
+    T f5 = (vy - (T)g.lo[4]) / (T)g.step[4];
+    T f6 = (w - (T)g.lo[5]) / (T)g.step[5];
+    if (!(isfin(f1) && isfin(f2) && isfin(f3) && isfin(f4) && isfin(f5) && isfin(f6))) return cap;
+    // A finite wrapped heading lies in [0, n3] for every heading the
+    // reference indexes correctly; outside it the reference would read a
+    // corner off the heading axis. Such a lookup rejects.
+    if (!(f3 >= (T)0 && f3 <= n3)) return cap;
+    if (f1 < (T)0 || f1 > (T)(float)(g.n[0] - 1)) return cap;
+    if (f2 < (T)0 || f2 > (T)(float)(g.n[1] - 1)) return cap;
+    if (vclamp) {
+        f4 = jclamp(f4, (T)0, (T)(float)(g.n[3] - 1));
+        f5 = jclamp(f5, (T)0, (T)(float)(g.n[4] - 1));
+        f6 = jclamp(f6, (T)0, (T)(float)(g.n[5] - 1));
+    } else {
+        if (f4 < (T)0 || f4 > (T)(float)(g.n[3] - 1)) return cap;
+        if (f5 < (T)0 || f5 > (T)(float)(g.n[4] - 1)) return cap;
+        if (f6 < (T)0 || f6 > (T)(float)(g.n[5] - 1)) return cap;
+    }
+    if (nearest) {
+        int32_t i1 = (int32_t)jround(f1), i2 = (int32_t)jround(f2);
+        int32_t i3 = (int32_t)jround(f3) % g.n[2];
+        int32_t i4 = (int32_t)jround(f4), i5 = (int32_t)jround(f5), i6 = (int32_t)jround(f6);
+        return V(flatten(g, i1, i2, i3, i4, i5, i6));
+    }
+    T b[6] = {std::floor(f1), std::floor(f2), std::floor(f3),
+              std::floor(f4), std::floor(f5), std::floor(f6)};
+    T f[6] = {f1, f2, f3, f4, f5, f6};
+    T t[6];
+    int32_t i[6];
+    for (int k = 0; k < 6; ++k) { t[k] = f[k] - b[k]; i[k] = (int32_t)b[k]; }
+    T acc = (T)0;
+    for (int d = 0; d < 64; ++d) {
+        int c[6];
+        for (int k = 0; k < 6; ++k) c[k] = (d >> k) & 1;
+        // weight = product over the six axes, left to right (:448-453)
+        T wgt = (c[0] ? t[0] : (T)1 - t[0]);
+        for (int k = 1; k < 6; ++k) wgt = wgt * (c[k] ? t[k] : (T)1 - t[k]);
+        if (wgt == (T)0) continue;
+        int32_t j1 = jmin<int32_t>(i[0] + c[0], g.n[0] - 1);
+        int32_t j2 = jmin<int32_t>(i[1] + c[1], g.n[1] - 1);
+        int32_t j3 = (i[2] + c[2]) % g.n[2];
+        int32_t j4 = jmin<int32_t>(i[3] + c[3], g.n[3] - 1);
+        int32_t j5 = jmin<int32_t>(i[4] + c[4], g.n[4] - 1);
+        int32_t j6 = jmin<int32_t>(i[5] + c[5], g.n[5] - 1);
+        acc = acc + wgt * V(flatten(g, j1, j2, j3, j4, j5, j6));
+    }
+    return jmin(acc, cap);
+}
+
+// Kuhn / Freudenthal simplex interpolation with gradient (:571-665).
+template <class T>
+inline void cmpx(T& ta, int32_t& ka, T& tb, int32_t& kb) {
+    if (!(ta >= tb)) { T t = ta; ta = tb; tb = t; int32_t k = ka; ka = kb; kb = k; }
+}
+
+template <class T, class VR>
+inline T interp_kuhn(const VR& V, const pg_grid& g, T x, T y, T h, T vx, T vy, T w, T cap,
+                     bool vclamp, T grad[6]) {
+    for (int k = 0; k < 6; ++k) grad[k] = (T)0;
+    T f1 = (x - (T)g.lo[0]) / (T)g.step[0];
+    T f2 = (y - (T)g.lo[1]) / (T)g.step[1];
+    T n3 = (T)(float)g.n[2];
+    T f3 = (h - (T)g.lo[2]) / (T)g.step[2];
+    f3 = f3 - std::floor(f3 / n3) * n3;
+    T f4 = (vx - (T)g.lo[3]) / (T)g.step[3];
+    T f5 = (vy - (T)g.lo[4]) / (T)g.step[4];
+    T f6 = (w - (T)g.lo[5]) / (T)g.step[5];
+    if (!(isfin(f1) && isfin(f2) && isfin(f3) && isfin(f4) && isfin(f5) && isfin(f6))) return cap;
+    if (f1 < (T)0 || f1 > (T)(float)(g.n[0] - 1)) return cap;
+    if (f2 < (T)0 || f2 > (T)(float)(g.n[1] - 1)) return cap;
+    if (vclamp) {
+        f4 = jclamp(f4, (T)0, (T)(float)(g.n[3] - 1));
+        f5 = jclamp(f5, (T)0, (T)(float)(g.n[4] - 1));
+        f6 = jclamp(f6, (T)0, (T)(float)(g.n[5] - 1));
+    } else {
+        if (f4 < (T)0 || f4 > (T)(float)(g.n[3] - 1)) return cap;
+        if (f5 < (T)0 || f5 > (T)(float)(g.n[4] - 1)) return cap;
+        if (f6 < (T)0 || f6 > (T)(float)(g.n[5] - 1)) return cap;
+    }
+    T f[6] = {f1, f2, f3, f4, f5, f6};
+    T b[6];
+    int32_t ii[6];
+    for (int k = 0; k < 6; ++k) { b[k] = std::floor(f[k]); ii[k] = (int32_t)b[k]; }
+    ii[2] = ii[2] % g.n[2];
+    T s[6];
+    int32_t a[6];
+    for (int k = 0; k < 6; ++k) { s[k] = f[k] - b[k]; a[k] = k + 1; }
+    // 12-comparator network, in the reference's order (:475-489)
+    cmpx(s[0], a[0], s[5], a[5]); cmpx(s[1], a[1], s[3], a[3]); cmpx(s[2], a[2], s[4], a[4]);
+    cmpx(s[1], a[1], s[2], a[2]); cmpx(s[3], a[3], s[4], a[4]); cmpx(s[0], a[0], s[3], a[3]);
+    cmpx(s[2], a[2], s[5], a[5]); cmpx(s[0], a[0], s[1], a[1]); cmpx(s[2], a[2], s[3], a[3]);
+    cmpx(s[4], a[4], s[5], a[5]); cmpx(s[1], a[1], s[2], a[2]); cmpx(s[3], a[3], s[4], a[4]);
+    T v0 = V(flatten(g, ii[0], ii[1], ii[2], ii[3], ii[4], ii[5]));
+    T acc = ((T)1 - s[0]) * v0;
+    T prev = v0;
+    T e[6];
+    for (int k = 0; k < 6; ++k) {
+        int ax = a[k] - 1;
+        if (ax == 2) ii[2] = (ii[2] + 1) % g.n[2];
+        else ii[ax] = jmin<int32_t>(ii[ax] + 1, g.n[ax] - 1);
+        T v = V(flatten(g, ii[0], ii[1], ii[2], ii[3], ii[4], ii[5]));
+        T wk = (k < 5) ? (s[k] - s[k + 1]) : s[5];
+        acc = acc + wk * v;
+        e[k] = (v - prev) / (T)g.step[ax];
+        prev = v;
+    }
+    for (int ax = 0; ax < 6; ++ax) {
+        // first k in order a1..a5 matching the axis, else e6 (:640-651)
+        T d = e[5];
+        for (int k = 4; k >= 0; --k) if (a[k] == ax + 1) d = e[k];
+        grad[ax] = d;
+    }
+    return jmin(acc, cap);
+}
+
+template <class T, class VR>
+inline T value_at(const VR& V, const pg_grid& g, T x, T y, T h, T vx, T vy, T w, const pg_params& p) {
+    if (p.simplex) {
+        T gr[6];
+        return interp_kuhn<T>(V, g, x, y, h, vx, vy, w, (T)p.cap, p.vclamp != 0, gr);
+    }
+    return interp<T>(V, g, x, y, h, vx, vy, w, p.nearest != 0, (T)p.cap, p.vclamp != 0);
+}
+
+// ---------------------------------------------------------------------------
+// Horizon and substeps (:954-989)
+// ---------------------------------------------------------------------------
+
+template <class T> inline T cfl_tau(const pg_grid& g, const pg_model& m, T vfx, T vfy, T w, const pg_params& p) {
+    if (p.cfl <= 0.0f) return (T)p.dt;
+    const float* B = m.B;
+    T ax = jmax(jmax(std::fabs((T)B[0]), std::fabs((T)B[1])), std::fabs((T)B[2]));
+    T ay = jmax(jmax(std::fabs((T)B[3]), std::fabs((T)B[4])), std::fabs((T)B[5]));
+    T aw = jmax(jmax(std::fabs((T)B[6]), std::fabs((T)B[7])), std::fabs((T)B[8]));
+    T t = (T)p.tau_max;
+    if (ax > (T)1.0e-6f) t = jmin(t, (T)p.cfl * (T)g.step[3] / ax);
+    if (ay > (T)1.0e-6f) t = jmin(t, (T)p.cfl * (T)g.step[4] / ay);
+    if (aw > (T)1.0e-6f) t = jmin(t, (T)p.cfl * (T)g.step[5] / aw);
+    T sp = std::sqrt(vfx * vfx + vfy * vfy);
+    if (sp > (T)1.0e-3f) t = jmin(t, (T)p.cfl * jmin((T)g.step[0], (T)g.step[1]) / sp);
+    T aw2 = std::fabs(w);
+    if (aw2 > (T)1.0e-4f) t = jmin(t, (T)p.cfl * (T)g.step[2] / aw2);
+    return jclamp(t, (T)p.tau_min, (T)p.tau_max);
+}
+
+template <class T> inline int32_t substeps_for(T tau, const pg_params& p) {
+    if (p.hmax <= 0.0f) return p.nsub;
+    int32_t k = (int32_t)jmin(tau / (T)p.hmax, (T)1.0e4f) + 1;   // unsafe_trunc: toward zero
+    return jmax<int32_t>(1, jmin<int32_t>(k, 64));
+}
+
+// ---------------------------------------------------------------------------
+// lookahead (:1100-1153)
+// ---------------------------------------------------------------------------
+
+template <class T, class VR>
+inline T lookahead(const VR& V, const uint8_t* occ, bool occ_ignore, const pg_grid& g,
+                   const pg_model& m, T x, T y, T h, T vx, T vy, T w, T u1, T u2, T u3,
+                   T tau, int32_t nsub, const pg_params& p) {
+    T s[6] = {x, y, h, vx, vy, w};
+    if (p.rk2) step_state_rk2<T>(m, s, u1, u2, u3, tau, nsub);
+    else step_state<T>(m, s, u1, u2, u3, tau, nsub);
+    T nx = s[0], ny = s[1], nh = s[2], nvx = s[3], nvy = s[4], nw = s[5];
+    T dx = nx - x, dy = ny - y, dh = nh - h;
+    const T cap = (T)p.cap;
+    if (!(isfin(dx) && isfin(dy) && isfin(dh) && isfin(nvx) && isfin(nvy) && isfin(nw))) return cap;
+
+    int32_t nchk = p.checks;
+    if (p.adaptive_checks) {
+        T cells_pos = std::sqrt(dx * dx + dy * dy) / ((T)0.5f * jmin((T)g.step[0], (T)g.step[1]));
+        T cells_rot = std::fabs(dh) / ((T)0.5f * (T)g.step[2]);
+        T cells = jmin(jmax(cells_pos, cells_rot), (T)1.0e4f);
+        int32_t need = (int32_t)cells + 1;
+        nchk = jmax<int32_t>(p.checks, jmin<int32_t>(need, 24));
+    }
+    for (int32_t sc = 1; sc <= nchk; ++sc) {
+        T a = (T)(float)sc / (T)(float)nchk;
+        T px = x + dx * a;
+        T py = y + dy * a;
+        T ph = h + dh * a;
+        T gx = (px - (T)g.lo[0]) / (T)g.step[0];
+        T gy = (py - (T)g.lo[1]) / (T)g.step[1];
+        if (gx < (T)0 || gy < (T)0 || gx > (T)(float)(g.n[0] - 1) || gy > (T)(float)(g.n[1] - 1))
+            return cap;
+        T n3f = (T)(float)g.n[2];
+        T gh = (ph - (T)g.lo[2]) / (T)g.step[2];
+        gh = gh - std::floor(gh / n3f) * n3f;
+        // The wrapped heading lies in [0, n3] for every heading the reference
+        // can index; outside it (non-finite or astronomically large headings)
+        // the reference raises at Int32(round(gh)). Such a probe rejects.
+        if (!(gh >= (T)0 && gh <= n3f)) return cap;
+        int32_t ci = (int32_t)jround(gx), cj = (int32_t)jround(gy);
+        int32_t ck = (int32_t)jround(gh) % g.n[2];
+        if (!occ_ignore && occ[((int64_t)ci * g.n[1] + cj) * g.n[2] + ck]) return cap;
+    }
+    T snh, cnh;
+    sincos_t(nh, snh, cnh);
+    T nvfx = cnh * nvx - snh * nvy;
+    T nvfy = snh * nvx + cnh * nvy;
+    T nv = value_at<T>(V, g, nx, ny, nh, nvfx, nvfy, nw, p);
+    return jmin(tau + nv, cap);
+}
+
+// ---------------------------------------------------------------------------
+// cell_update, plain scheme and the full-scan lattice (:1205-1344)
+// ---------------------------------------------------------------------------
+//
+// Returns the backed-up value and the index of the winning control (0 =
+// coast). The warm start (`pol`), pattern search and step-length ladder are
+// the reference's production scheme; this restatement covers the scan with
+// ncoarse >= 0 and rejects rounds > 0 / ntau > 1 at the API.
+template <class T>
+inline T cell_update(int64_t idx, const T* V, const uint8_t* occ, const pg_grid& g,
+                     const pg_model& m, const float* ctl, int32_t nctl, const pg_params& p,
+                     int32_t phase, bool esc, int32_t* best_k) {
+    int32_t i[6];
+    unflatten(g, idx, i);
+    bool blocked = occ[((int64_t)i[0] * g.n[1] + i[1]) * g.n[2] + i[2]] != 0;
+    const T cap = (T)p.cap;
+    *best_k = 0;
+    if (!esc && blocked) return cap;
+    T x = axisvalue<T>(g, 0, i[0]), y = axisvalue<T>(g, 1, i[1]);
+    T h = axisvalue<T>(g, 2, i[2]), vfx = axisvalue<T>(g, 3, i[3]);
+    T vfy = axisvalue<T>(g, 4, i[4]), w = axisvalue<T>(g, 5, i[5]);
+    T sh0, ch0;
+    sincos_t(h, sh0, ch0);
+    T vx = ch0 * vfx + sh0 * vfy;
+    T vy = -sh0 * vfx + ch0 * vfy;
+    T tau0 = cfl_tau<T>(g, m, vfx, vfy, w, p);
+    int32_t ns0 = substeps_for<T>(tau0, p);
+
+    int64_t blk = (int64_t)g.n[3] * g.n[4] * g.n[5];
+    auto run = [&](T u1, T u2, T u3) -> T {
+        if (esc) {
+            EscV<T> ev{V, occ, blk, cap, blocked};
+            return lookahead<T>(ev, occ, blocked, g, m, x, y, h, vx, vy, w, u1, u2, u3, tau0, ns0, p);
+        }
+        PlainV<T> pv{V};
+        return lookahead<T>(pv, occ, false, g, m, x, y, h, vx, vy, w, u1, u2, u3, tau0, ns0, p);
+    };
+
+    T best = run((T)0, (T)0, (T)0);
+    int32_t nnc = jmax<int32_t>(nctl - 1, 1);
+    int32_t span = p.ncoarse <= 0 ? nnc : jmin<int32_t>(p.ncoarse, nnc);
+    int32_t off = p.ncoarse <= 0 ? 0 : (int32_t)(((int64_t)phase * span) % nnc + nnc) % nnc;
+    for (int32_t j = 0; j < span; ++j) {
+        int32_t k = (int32_t)(((int64_t)off + j) % nnc) + 1;   // 0-based, skips coast
+        T c = run((T)ctl[k], (T)ctl[k + nctl], (T)ctl[k + 2 * nctl]);
+        if (c < best) { best = c; *best_k = k; }
+    }
+    return best;
+}
+
+}  // namespace orc

@@ -1,0 +1,455 @@
+// C-ABI implementation of libperegrine_hip (declared in include/peregrine_hip.h).
+//
+// Host side of the solver: argument validation, the per-solve constant
+// block, device allocations, the Jacobi buffer pair and the sweep loop. The
+// command saturation and the first three terms of each acceleration row
+// (B * u_sat, solver/src/PeregrineSolver.jl:139-161) are folded here once per
+// control in float, in the reference's order, so that the kernels start each
+// row's sum from the identical partial value.
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#include "../../include/peregrine_hip.h"
+#include "pg_kernels.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHECK(expr)                                                                 \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return fail(PG_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));   \
+    } while (0)
+
+int64_t grid_cells(const pg_grid& g) {
+    int64_t n = 1;
+    for (int k = 0; k < 6; ++k) n *= (int64_t)g.n[k];
+    return n;
+}
+
+// traction_gain (PeregrineSolver.jl:126-136)
+float traction_gain(float u1, float u2, float u3, float knee) {
+    if (std::isnan(knee) || knee <= 0.0f) return 1.0f;
+    float m = std::sqrt(u1 * u1 + u2 * u2 + u3 * u3);
+    float r = m / knee;
+    if (r < 1.0e-6f) return 1.0f;
+    return std::tanh(r) / r;
+}
+
+// substeps_for (PeregrineSolver.jl:985-989)
+int32_t substeps_for(float tau, const pg_params& p) {
+    if (p.hmax <= 0.0f) return p.nsub;
+    float q = tau / p.hmax;
+    if (!(q < 1.0e4f)) q = 1.0e4f;
+    int32_t k = (int32_t)q + 1;
+    if (k > 64) k = 64;
+    if (k < 1) k = 1;
+    return k;
+}
+
+}  // namespace
+
+struct pg_ctx {
+    int32_t device = 0;
+    hipStream_t stream = nullptr;
+};
+
+struct pg_solver {
+    pg_ctx* ctx = nullptr;
+    pg_grid g{};
+    pg_model m{};
+    pg_params p{};
+    pgk::Consts K{};
+    int32_t nctl = 0;
+    int32_t vstore = 0;
+    int64_t ncell = 0;
+    float* dV[2] = {nullptr, nullptr};
+    int cur = 0;
+    uint8_t* d_occ = nullptr;
+    float* d_bu = nullptr;
+    int64_t* d_seeds = nullptr;
+    int64_t nseeds = 0;
+    bool seeded = false;
+    double* d_part = nullptr;
+    int64_t part_cap = 0;
+    double* d_sum = nullptr;
+    unsigned* d_linf = nullptr;
+    uint8_t* d_pol = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    float last_ms = 0.0f;
+};
+
+namespace {
+
+void release(pg_solver* s) {
+    if (!s) return;
+    hipFree(s->dV[0]);
+    hipFree(s->dV[1]);
+    hipFree(s->d_occ);
+    hipFree(s->d_bu);
+    hipFree(s->d_seeds);
+    hipFree(s->d_part);
+    hipFree(s->d_sum);
+    hipFree(s->d_linf);
+    hipFree(s->d_pol);
+    if (s->ev0) hipEventDestroy(s->ev0);
+    if (s->ev1) hipEventDestroy(s->ev1);
+    delete s;
+}
+
+int check_grid(const pg_grid* g) {
+    for (int k = 0; k < 6; ++k) {
+        if (g->n[k] < 2) return fail(PG_ERR_ARG, "axis %d needs at least 2 samples, got %d", k, g->n[k]);
+        if (!(g->step[k] > 0.0f) || !std::isfinite(g->step[k]) || !std::isfinite(g->lo[k]))
+            return fail(PG_ERR_ARG, "axis %d has a non-positive or non-finite step", k);
+    }
+    return PG_OK;
+}
+
+int check_params(const pg_params* p, int32_t nctl) {
+    if (p->cfl > 0.0f) return fail(PG_ERR_UNSUPPORTED, "cfl > 0 (per-cell horizon) is not built");
+    if (p->ntau > 1) return fail(PG_ERR_UNSUPPORTED, "tau_levels > 1 (horizon ladder) is not built");
+    if (p->rounds > 0) return fail(PG_ERR_UNSUPPORTED, "refine_rounds > 0 (pattern search) is not built");
+    if (p->ncoarse > 0 && p->ncoarse < nctl - 1)
+        return fail(PG_ERR_UNSUPPORTED, "control_scan < nctl-1 (rotating slice) is not built");
+    if (p->rk2) return fail(PG_ERR_UNSUPPORTED, "rk2 integration is not built");
+    if (p->simplex) return fail(PG_ERR_UNSUPPORTED, "simplex (Kuhn) interpolation is not built");
+    if (p->nearest) return fail(PG_ERR_UNSUPPORTED, "nearest-cell lookup is not built");
+    if (p->vclamp) return fail(PG_ERR_UNSUPPORTED, "velocity_clamp is not built");
+    if (p->adaptive_checks) return fail(PG_ERR_UNSUPPORTED, "adaptive_checks is not built");
+    if (p->checks < 1 || p->checks > 24) return fail(PG_ERR_ARG, "sweep_checks must be in [1, 24], got %d", p->checks);
+    if (!(p->dt > 0.0f) || !std::isfinite(p->dt)) return fail(PG_ERR_ARG, "dt must be positive");
+    if (!(p->cap > 0.0f) || !std::isfinite(p->cap)) return fail(PG_ERR_ARG, "value_cap must be positive");
+    if (p->hmax <= 0.0f && p->nsub < 1) return fail(PG_ERR_ARG, "substeps must be >= 1");
+    return PG_OK;
+}
+
+void build_consts(pg_solver* s) {
+    pgk::Consts& K = s->K;
+    std::memset(&K, 0, sizeof K);
+    const pg_grid& g = s->g;
+    for (int k = 0; k < 6; ++k) {
+        K.n[k] = g.n[k];
+        K.lo[k] = g.lo[k];
+        K.step[k] = g.step[k];
+        K.rstep[k] = 1.0 / (double)g.step[k];
+        K.lim[k] = (float)(g.n[k] - 1);
+    }
+    K.n3f = (float)g.n[2];
+    K.lim[2] = K.n3f;
+    K.rn3 = 1.0 / (double)K.n3f;
+    K.ncell = s->ncell;
+    int64_t st = 1;
+    for (int k = 5; k >= 0; --k) { K.stride[k] = st; st *= (int64_t)g.n[k]; }
+    std::memcpy(K.A, s->m.A, sizeof K.A);
+    std::memcpy(K.q, s->m.q, sizeof K.q);
+    std::memcpy(K.S, s->m.S, sizeof K.S);
+    std::memcpy(K.D, s->m.D, sizeof K.D);
+    std::memcpy(K.c, s->m.c, sizeof K.c);
+    for (int k = 0; k < 3; ++k) K.reps[k] = 1.0 / (double)s->m.eps[k];
+    // plain scheme: cfl <= 0, so the horizon is dt (PeregrineSolver.jl:955)
+    K.tau = s->p.dt;
+    K.nsub = substeps_for(K.tau, s->p);
+    K.hs = K.tau / (float)K.nsub;
+    K.cap = s->p.cap;
+    K.nchk = s->p.checks;
+    for (int sc = 1; sc <= K.nchk; ++sc) K.probe_a[sc - 1] = (float)sc / (float)K.nchk;
+    K.nctl = s->nctl;
+}
+
+int ensure_partials(pg_solver* s, int64_t nb) {
+    if (nb <= s->part_cap) return PG_OK;
+    hipFree(s->d_part);
+    s->d_part = nullptr;
+    HIPCHECK(hipMalloc(&s->d_part, (size_t)nb * sizeof(double)));
+    s->part_cap = nb;
+    return PG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* pg_version(void) { return "peregrine_hip 0.1 (gfx950, plain FP32 Jacobi)"; }
+int pg_abi_version(void) { return PG_ABI_VERSION; }
+const char* pg_last_error(void) { return g_err.c_str(); }
+
+int pg_device_count(int32_t* n) {
+    if (!n) return fail(PG_ERR_ARG, "null output");
+    int c = 0;
+    HIPCHECK(hipGetDeviceCount(&c));
+    *n = c;
+    return PG_OK;
+}
+
+int pg_create(int32_t device, pg_ctx** out) {
+    if (!out) return fail(PG_ERR_ARG, "null output");
+    *out = nullptr;
+    int c = 0;
+    HIPCHECK(hipGetDeviceCount(&c));
+    if (device < 0 || device >= c) return fail(PG_ERR_ARG, "device %d out of range (%d visible)", device, c);
+    HIPCHECK(hipSetDevice(device));
+    pg_ctx* ctx = new pg_ctx;
+    ctx->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete ctx;
+        return fail(PG_ERR_HIP, "hipStreamCreate failed: %s", hipGetErrorString(e));
+    }
+    *out = ctx;
+    return PG_OK;
+}
+
+int pg_destroy(pg_ctx* ctx) {
+    if (!ctx) return PG_OK;
+    hipSetDevice(ctx->device);
+    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return PG_OK;
+}
+
+int pg_device_budget(pg_ctx* ctx, double headroom, int64_t* bytes) {
+    if (!ctx || !bytes) return fail(PG_ERR_ARG, "null argument");
+    HIPCHECK(hipSetDevice(ctx->device));
+    size_t fr = 0, tot = 0;
+    HIPCHECK(hipMemGetInfo(&fr, &tot));
+    *bytes = (int64_t)std::floor((double)fr * headroom);
+    return PG_OK;
+}
+
+int pg_grid_fits(pg_ctx* ctx, const pg_grid* g, int32_t vstore, double headroom, int32_t* fits,
+                 double* need_gib, double* free_gib) {
+    if (!ctx || !g || !fits) return fail(PG_ERR_ARG, "null argument");
+    int rc = check_grid(g);
+    if (rc) return rc;
+    HIPCHECK(hipSetDevice(ctx->device));
+    size_t fr = 0, tot = 0;
+    HIPCHECK(hipMemGetInfo(&fr, &tot));
+    const double per_cell = (vstore == PG_VSTORE_F16_TILE) ? 4.0 : 8.0;  // Jacobi pair
+    const double need = (double)grid_cells(*g) * per_cell / 1073741824.0;
+    const double free = (double)fr / 1073741824.0;
+    *fits = need <= free * headroom ? 1 : 0;
+    if (need_gib) *need_gib = need;
+    if (free_gib) *free_gib = free;
+    return PG_OK;
+}
+
+int pg_solver_create(pg_ctx* ctx, const pg_grid* g, const pg_model* m, const pg_params* p,
+                     const float* ctl_soa, int32_t nctl, const uint8_t* occ, int32_t vstore,
+                     pg_solver** out) {
+    if (!out) return fail(PG_ERR_ARG, "null output");
+    *out = nullptr;
+    if (!ctx || !g || !m || !p || !ctl_soa || !occ) return fail(PG_ERR_ARG, "null argument");
+    int rc = check_grid(g);
+    if (rc) return rc;
+    if (nctl < 2 || nctl > 255) return fail(PG_ERR_ARG, "nctl must be in [2, 255], got %d", nctl);
+    if (ctl_soa[0] != 0.0f || ctl_soa[nctl] != 0.0f || ctl_soa[2 * nctl] != 0.0f)
+        return fail(PG_ERR_ARG, "control 0 must be coast (0, 0, 0)");
+    rc = check_params(p, nctl);
+    if (rc) return rc;
+    if (vstore != PG_VSTORE_F32) return fail(PG_ERR_UNSUPPORTED, "vstore %d is not built", vstore);
+    for (int k = 0; k < 3; ++k)
+        if (!(m->eps[k] > 0.0f)) return fail(PG_ERR_ARG, "coulomb eps[%d] must be positive", k);
+    HIPCHECK(hipSetDevice(ctx->device));
+
+    pg_solver* s = new pg_solver;
+    s->ctx = ctx;
+    s->g = *g;
+    s->m = *m;
+    s->p = *p;
+    s->nctl = nctl;
+    s->vstore = vstore;
+    s->ncell = grid_cells(*g);
+    build_consts(s);
+
+    // B * u_sat per control, in the reference's summation order
+    std::vector<float> bu(3 * (size_t)nctl);
+    for (int k = 0; k < nctl; ++k) {
+        const float u1r = ctl_soa[k], u2r = ctl_soa[k + nctl], u3r = ctl_soa[k + 2 * nctl];
+        const float gn = traction_gain(u1r, u2r, u3r, m->knee);
+        const float u1 = u1r * gn, u2 = u2r * gn, u3 = u3r * gn;
+        for (int r = 0; r < 3; ++r) {
+            float a = m->B[3 * r] * u1;
+            a = a + m->B[3 * r + 1] * u2;
+            a = a + m->B[3 * r + 2] * u3;
+            bu[3 * k + r] = a;
+        }
+    }
+
+    const int64_t nocc = (int64_t)g->n[0] * g->n[1] * g->n[2];
+    hipError_t e = hipSuccess;
+    auto chk = [&](hipError_t x) { if (e == hipSuccess) e = x; };
+    chk(hipMalloc(&s->dV[0], (size_t)s->ncell * sizeof(float)));
+    chk(hipMalloc(&s->dV[1], (size_t)s->ncell * sizeof(float)));
+    chk(hipMalloc(&s->d_occ, (size_t)nocc));
+    chk(hipMalloc(&s->d_bu, bu.size() * sizeof(float)));
+    chk(hipMalloc(&s->d_sum, sizeof(double)));
+    chk(hipMalloc(&s->d_linf, sizeof(unsigned)));
+    if (e != hipSuccess) {
+        release(s);
+        return fail(e == hipErrorOutOfMemory ? PG_ERR_NOMEM : PG_ERR_HIP,
+                    "device allocation for %lld cells failed: %s", (long long)grid_cells(*g),
+                    hipGetErrorString(e));
+    }
+    std::vector<uint8_t> occ01(nocc);
+    for (int64_t i = 0; i < nocc; ++i) occ01[i] = occ[i] ? 1 : 0;
+    chk(hipMemcpy(s->d_occ, occ01.data(), (size_t)nocc, hipMemcpyHostToDevice));
+    chk(hipMemcpy(s->d_bu, bu.data(), bu.size() * sizeof(float), hipMemcpyHostToDevice));
+    chk(hipEventCreate(&s->ev0));
+    chk(hipEventCreate(&s->ev1));
+    if (e != hipSuccess) {
+        release(s);
+        return fail(PG_ERR_HIP, "solver setup failed: %s", hipGetErrorString(e));
+    }
+    *out = s;
+    return PG_OK;
+}
+
+int pg_solver_destroy(pg_solver* s) {
+    if (!s) return PG_OK;
+    hipSetDevice(s->ctx->device);
+    hipStreamSynchronize(s->ctx->stream);
+    release(s);
+    return PG_OK;
+}
+
+int pg_solver_ncells(pg_solver* s, int64_t* n) {
+    if (!s || !n) return fail(PG_ERR_ARG, "null argument");
+    *n = s->ncell;
+    return PG_OK;
+}
+
+int pg_fill_seed(pg_solver* s, const int64_t* seeds, int64_t nseeds) {
+    if (!s) return fail(PG_ERR_ARG, "null solver");
+    if (nseeds < 0 || (nseeds > 0 && !seeds)) return fail(PG_ERR_ARG, "bad seed list");
+    for (int64_t i = 0; i < nseeds; ++i)
+        if (seeds[i] < 0 || seeds[i] >= s->ncell)
+            return fail(PG_ERR_ARG, "seed %lld out of range [0, %lld)", (long long)seeds[i], (long long)s->ncell);
+    HIPCHECK(hipSetDevice(s->ctx->device));
+    hipStream_t st = s->ctx->stream;
+    hipFree(s->d_seeds);
+    s->d_seeds = nullptr;
+    s->nseeds = nseeds;
+    if (nseeds > 0) {
+        HIPCHECK(hipMalloc(&s->d_seeds, (size_t)nseeds * sizeof(int64_t)));
+        HIPCHECK(hipMemcpyAsync(s->d_seeds, seeds, (size_t)nseeds * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    }
+    s->cur = 0;
+    HIPCHECK(pgk_launch_fill(s->dV[0], s->ncell, s->p.cap, st));
+    HIPCHECK(pgk_launch_seed(s->dV[0], s->d_seeds, s->nseeds, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    s->seeded = true;
+    return PG_OK;
+}
+
+int pg_sweep(pg_solver* s, int32_t phase, int32_t rev, double* sum_decrease, float* linf) {
+    (void)phase;
+    (void)rev;
+    if (!s) return fail(PG_ERR_ARG, "null solver");
+    if (!s->seeded) return fail(PG_ERR_STATE, "pg_fill_seed (or pg_upload_V) must come before a sweep");
+    HIPCHECK(hipSetDevice(s->ctx->device));
+    hipStream_t st = s->ctx->stream;
+    const int64_t nb = (s->ncell + 255) / 256;
+    int rc = ensure_partials(s, nb);
+    if (rc) return rc;
+    HIPCHECK(hipMemsetAsync(s->d_linf, 0, sizeof(unsigned), st));
+    const float* Vin = s->dV[s->cur];
+    float* Vout = s->dV[1 - s->cur];
+    int64_t nblk = 0;
+    HIPCHECK(hipEventRecord(s->ev0, st));
+    HIPCHECK(pgk_launch_sweep(s->K, Vin, Vout, s->d_occ, s->d_bu, nullptr, s->d_part, s->d_linf, 1, st, &nblk));
+    HIPCHECK(hipEventRecord(s->ev1, st));
+    HIPCHECK(pgk_launch_reduce(s->d_part, nblk, s->d_sum, st));
+    HIPCHECK(pgk_launch_seed(Vout, s->d_seeds, s->nseeds, st));
+    double hsum = 0.0;
+    unsigned hl = 0;
+    HIPCHECK(hipMemcpyAsync(&hsum, s->d_sum, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(&hl, s->d_linf, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    HIPCHECK(hipEventElapsedTime(&s->last_ms, s->ev0, s->ev1));
+    s->cur = 1 - s->cur;
+    float lf;
+    std::memcpy(&lf, &hl, 4);
+    if (sum_decrease) *sum_decrease = hsum;
+    if (linf) *linf = lf;
+    return PG_OK;
+}
+
+int pg_solve(pg_solver* s, int32_t max_iters, double tol_sum, float tol_linf, int32_t* iters,
+             double* last_sum, float* last_linf) {
+    if (!s) return fail(PG_ERR_ARG, "null solver");
+    int32_t it = 0;
+    double d = INFINITY;
+    float l = INFINITY;
+    for (it = 1; it <= max_iters; ++it) {
+        int rc = pg_sweep(s, it - 1, it & 1, &d, &l);
+        if (rc) return rc;
+        const bool quiet_sum = tol_sum >= 0.0 && d <= tol_sum;
+        const bool quiet_linf = tol_linf > 0.0f && l < tol_linf;
+        if (quiet_sum || quiet_linf) break;
+    }
+    if (it > max_iters) it = max_iters;
+    if (iters) *iters = it;
+    if (last_sum) *last_sum = d;
+    if (last_linf) *last_linf = l;
+    return PG_OK;
+}
+
+int pg_policy(pg_solver* s, uint8_t* pol_host) {
+    if (!s || !pol_host) return fail(PG_ERR_ARG, "null argument");
+    if (!s->seeded) return fail(PG_ERR_STATE, "no value table yet");
+    HIPCHECK(hipSetDevice(s->ctx->device));
+    hipStream_t st = s->ctx->stream;
+    if (!s->d_pol) HIPCHECK(hipMalloc(&s->d_pol, (size_t)s->ncell));
+    int64_t nblk = 0;
+    HIPCHECK(pgk_launch_sweep(s->K, s->dV[s->cur], nullptr, s->d_occ, s->d_bu, s->d_pol, nullptr, nullptr, 0, st, &nblk));
+    HIPCHECK(hipMemcpyAsync(pol_host, s->d_pol, (size_t)s->ncell, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    return PG_OK;
+}
+
+int pg_download_V(pg_solver* s, float* dst, int64_t off, int64_t n) {
+    if (!s || (!dst && n > 0)) return fail(PG_ERR_ARG, "null argument");
+    if (off < 0 || n < 0 || off + n > s->ncell) return fail(PG_ERR_ARG, "range [%lld, %lld) outside the grid", (long long)off, (long long)(off + n));
+    if (n == 0) return PG_OK;
+    HIPCHECK(hipSetDevice(s->ctx->device));
+    HIPCHECK(hipMemcpyAsync(dst, s->dV[s->cur] + off, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, s->ctx->stream));
+    HIPCHECK(hipStreamSynchronize(s->ctx->stream));
+    return PG_OK;
+}
+
+int pg_upload_V(pg_solver* s, const float* src, int64_t off, int64_t n) {
+    if (!s || (!src && n > 0)) return fail(PG_ERR_ARG, "null argument");
+    if (off < 0 || n < 0 || off + n > s->ncell) return fail(PG_ERR_ARG, "range [%lld, %lld) outside the grid", (long long)off, (long long)(off + n));
+    HIPCHECK(hipSetDevice(s->ctx->device));
+    if (n > 0) {
+        HIPCHECK(hipMemcpyAsync(s->dV[s->cur] + off, src, (size_t)n * sizeof(float), hipMemcpyHostToDevice, s->ctx->stream));
+        HIPCHECK(hipStreamSynchronize(s->ctx->stream));
+    }
+    s->seeded = true;
+    return PG_OK;
+}
+
+int pg_last_sweep_ms(pg_solver* s, float* ms) {
+    if (!s || !ms) return fail(PG_ERR_ARG, "null argument");
+    *ms = s->last_ms;
+    return PG_OK;
+}
+
+}  // extern "C"

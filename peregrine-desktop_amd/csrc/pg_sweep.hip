@@ -1,0 +1,255 @@
+// Bellman sweep kernels for gfx950 (plain scheme: fixed horizon, full
+// control scan, forward Euler, fixed swept probes, 64-corner multilinear).
+//
+// One thread per cell; the control table is scanned in order with a strict
+// `<`, coast first, so ties resolve to the lowest index exactly as
+// cell_update does (solver/src/PeregrineSolver.jl:1245-1275). A sweep is
+// Jacobi: every backup reads Vin and writes min(Vin, TVin) to Vout, which
+// makes the result independent of scheduling and equal bit for bit to the
+// CPU restatement.
+#include "pg_device.hpp"
+#include "pg_kernels.h"
+
+namespace pgk {
+
+// 64-corner multilinear interpolation of V at a continuous state
+// (PeregrineSolver.jl:399-466). Rejections return cap. The corner weight is
+// the left-to-right product over the six axes and the corners are summed in
+// the reference's order d = 0..63 (bit 0 = x ... bit 5 = w); a zero weight
+// adds +0 to a non-negative running sum, so the reference's skip of those
+// corners needs no branch here.
+__device__ __forceinline__ float interp_global(const Consts& K, const float* __restrict__ V,
+                                               float x, float y, float h, float vx, float vy,
+                                               float w) {
+    float f[6];
+    f[0] = qdiv(x - K.lo[0], K.rstep[0]);
+    f[1] = qdiv(y - K.lo[1], K.rstep[1]);
+    f[2] = wrap_h(K, qdiv(h - K.lo[2], K.rstep[2]));
+    f[3] = qdiv(vx - K.lo[3], K.rstep[3]);
+    f[4] = qdiv(vy - K.lo[4], K.rstep[4]);
+    f[5] = qdiv(w - K.lo[5], K.rstep[5]);
+    bool in = true;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) in = in & (f[k] >= 0.0f) & (f[k] <= K.lim[k]);
+    if (!in) return K.cap;
+    float t[6], u[6];
+    int64_t o0[6], o1[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const float b = __builtin_floorf(f[k]);
+        t[k] = f[k] - b;
+        u[k] = 1.0f - t[k];
+        const int i = (int)b;
+        int j0, j1;
+        if (k == 2) {
+            j0 = (i >= K.n[2]) ? i - K.n[2] : i;
+            j1 = i + 1;
+            j1 = (j1 >= K.n[2]) ? j1 - K.n[2] : j1;
+        } else {
+            j0 = i;
+            j1 = (i + 1 < K.n[k] - 1) ? i + 1 : K.n[k] - 1;
+        }
+        o0[k] = (int64_t)j0 * K.stride[k];
+        o1[k] = (int64_t)j1 * K.stride[k];
+    }
+    // weights: p[d] for the first five axes, built level by level so that
+    // every weight is ((((w1*w2)*w3)*w4)*w5)*w6 in the reference's order
+    float p2[4], p3[8], p4[16], p5[32];
+    int64_t a2[4], a3[8], a4[16], a5[32];
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+        p2[d] = ((d & 1) ? t[0] : u[0]) * ((d & 2) ? t[1] : u[1]);
+        a2[d] = ((d & 1) ? o1[0] : o0[0]) + ((d & 2) ? o1[1] : o0[1]);
+    }
+#pragma unroll
+    for (int d = 0; d < 8; ++d) {
+        p3[d] = p2[d & 3] * ((d & 4) ? t[2] : u[2]);
+        a3[d] = a2[d & 3] + ((d & 4) ? o1[2] : o0[2]);
+    }
+#pragma unroll
+    for (int d = 0; d < 16; ++d) {
+        p4[d] = p3[d & 7] * ((d & 8) ? t[3] : u[3]);
+        a4[d] = a3[d & 7] + ((d & 8) ? o1[3] : o0[3]);
+    }
+#pragma unroll
+    for (int d = 0; d < 32; ++d) {
+        p5[d] = p4[d & 15] * ((d & 16) ? t[4] : u[4]);
+        a5[d] = a4[d & 15] + ((d & 16) ? o1[4] : o0[4]);
+    }
+    float vals[64];
+#pragma unroll
+    for (int d = 0; d < 64; ++d) vals[d] = V[a5[d & 31] + ((d & 32) ? o1[5] : o0[5])];
+    float acc = 0.0f;
+#pragma unroll
+    for (int d = 0; d < 64; ++d) {
+        const float wgt = p5[d & 31] * ((d & 32) ? t[5] : u[5]);
+        acc = acc + wgt * vals[d];
+    }
+    return (acc < K.cap) ? acc : K.cap;
+}
+
+struct OccGlobal {
+    const uint8_t* __restrict__ occ;
+    int32_t n2, n3;
+    __device__ __forceinline__ bool operator()(int ci, int cj, int ck) const {
+        return occ[((int64_t)ci * n2 + cj) * n3 + ck] != 0;
+    }
+};
+
+// lookahead (PeregrineSolver.jl:1100-1153) for one control.
+__device__ __forceinline__ float lookahead_global(const Consts& K, const float* __restrict__ V,
+                                                  const OccGlobal& occ, float x, float y,
+                                                  float h, float vx, float vy, float w,
+                                                  float bux, float buy, float buw) {
+    float nx = x, ny = y, nh = h, nvx = vx, nvy = vy, nw = w;
+    const bool ok = euler(K, bux, buy, buw, nx, ny, nh, nvx, nvy, nw);
+    const float dx = nx - x, dy = ny - y, dh = nh - h;
+    if (!(ok & finite_f(dx) & finite_f(dy) & finite_f(dh) & finite_f(nvx) & finite_f(nvy) &
+          finite_f(nw)))
+        return K.cap;
+    if (!probes_free(K, occ, x, y, h, dx, dy, dh)) return K.cap;
+    float snh, cnh;
+    sincos_ref(nh, snh, cnh);
+    const float nvfx = cnh * nvx - snh * nvy;
+    const float nvfy = snh * nvx + cnh * nvy;
+    const float nv = interp_global(K, V, nx, ny, nh, nvfx, nvfy, nw);
+    const float c = K.tau + nv;
+    return (c < K.cap) ? c : K.cap;
+}
+
+__device__ __forceinline__ void block_reduce(double dsum, float dmax, double* part_sum,
+                                             unsigned* linf_bits) {
+    __shared__ double ssum[4];
+    __shared__ float smax[4];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        dsum += __shfl_xor(dsum, off, 64);
+        dmax = fmaxf(dmax, __shfl_xor(dmax, off, 64));
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) { ssum[wv] = dsum; smax[wv] = dmax; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = ssum[0];
+        float m = smax[0];
+        const int nw = blockDim.x >> 6;
+        for (int i = 1; i < nw; ++i) { s += ssum[i]; m = fmaxf(m, smax[i]); }
+        part_sum[blockIdx.x] = s;
+        if (m > 0.0f) atomicMax(linf_bits, __float_as_uint(m));
+    }
+}
+
+template <bool WRITE_V, bool WRITE_POL>
+__global__ __launch_bounds__(256) void sweep_global_kernel(
+    const float* __restrict__ Vin, float* __restrict__ Vout, const uint8_t* __restrict__ occ,
+    const float* __restrict__ bu, Consts K, uint8_t* __restrict__ pol,
+    double* __restrict__ part_sum, unsigned* __restrict__ linf_bits) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    float dec = 0.0f;
+    if (idx < K.ncell) {
+        int64_t r = idx;
+        int32_t i[6];
+#pragma unroll
+        for (int k = 5; k >= 1; --k) {
+            i[k] = (int32_t)(r % K.n[k]);
+            r /= K.n[k];
+        }
+        i[0] = (int32_t)r;
+        const bool blocked = occ[((int64_t)i[0] * K.n[1] + i[1]) * K.n[2] + i[2]] != 0;
+        float best = K.cap;
+        int bk = 0;
+        if (!blocked) {
+            const float x = K.lo[0] + (float)i[0] * K.step[0];
+            const float y = K.lo[1] + (float)i[1] * K.step[1];
+            const float h = K.lo[2] + (float)i[2] * K.step[2];
+            const float vfx = K.lo[3] + (float)i[3] * K.step[3];
+            const float vfy = K.lo[4] + (float)i[4] * K.step[4];
+            const float w = K.lo[5] + (float)i[5] * K.step[5];
+            float sh0, ch0;
+            sincos_ref(h, sh0, ch0);
+            const float vx = ch0 * vfx + sh0 * vfy;
+            const float vy = -sh0 * vfx + ch0 * vfy;
+            const OccGlobal og{occ, K.n[1], K.n[2]};
+            best = lookahead_global(K, Vin, og, x, y, h, vx, vy, w, bu[0], bu[1], bu[2]);
+            for (int k = 1; k < K.nctl; ++k) {
+                const float c = lookahead_global(K, Vin, og, x, y, h, vx, vy, w, bu[3 * k],
+                                                 bu[3 * k + 1], bu[3 * k + 2]);
+                if (c < best) { best = c; bk = k; }
+            }
+        }
+        const float old = Vin[idx];
+        if (WRITE_V) {
+            float out = old;
+            if (best < old) { out = best; dec = old - best; }
+            Vout[idx] = out;
+        }
+        if (WRITE_POL) pol[idx] = (uint8_t)bk;
+    }
+    if (WRITE_V) block_reduce((double)dec, dec, part_sum, linf_bits);
+}
+
+// Deterministic sum of the per-block partials (fixed order per launch shape).
+__global__ __launch_bounds__(256) void reduce_partials_kernel(const double* __restrict__ part,
+                                                              int64_t n, double* __restrict__ out) {
+    __shared__ double s[256];
+    double acc = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += 256) acc += part[i];
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = 128; off >= 1; off >>= 1) {
+        if ((int)threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = s[0];
+}
+
+__global__ void fill_kernel(float* __restrict__ V, int64_t n, float v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) V[i] = v;
+}
+
+__global__ void seed_kernel(float* __restrict__ V, const int64_t* __restrict__ seeds, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) V[seeds[i]] = 0.0f;
+}
+
+}  // namespace pgk
+
+// ---------------------------------------------------------------------------
+// launch wrappers (host side)
+// ---------------------------------------------------------------------------
+
+hipError_t pgk_launch_sweep(const pgk::Consts& K, const float* Vin, float* Vout,
+                            const uint8_t* occ, const float* bu, uint8_t* pol, double* part,
+                            unsigned* linf_bits, int write_v, hipStream_t st, int64_t* nblocks) {
+    const int64_t nb = (K.ncell + 255) / 256;
+    *nblocks = nb;
+    if (write_v && pol)
+        hipLaunchKernelGGL((pgk::sweep_global_kernel<true, true>), dim3((unsigned)nb), dim3(256),
+                           0, st, Vin, Vout, occ, bu, K, pol, part, linf_bits);
+    else if (write_v)
+        hipLaunchKernelGGL((pgk::sweep_global_kernel<true, false>), dim3((unsigned)nb), dim3(256),
+                           0, st, Vin, Vout, occ, bu, K, pol, part, linf_bits);
+    else
+        hipLaunchKernelGGL((pgk::sweep_global_kernel<false, true>), dim3((unsigned)nb), dim3(256),
+                           0, st, Vin, Vout, occ, bu, K, pol, part, linf_bits);
+    return hipGetLastError();
+}
+
+hipError_t pgk_launch_reduce(const double* part, int64_t n, double* out, hipStream_t st) {
+    hipLaunchKernelGGL(pgk::reduce_partials_kernel, dim3(1), dim3(256), 0, st, part, n, out);
+    return hipGetLastError();
+}
+
+hipError_t pgk_launch_fill(float* V, int64_t n, float v, hipStream_t st) {
+    const int64_t nb = (n + 255) / 256;
+    hipLaunchKernelGGL(pgk::fill_kernel, dim3((unsigned)nb), dim3(256), 0, st, V, n, v);
+    return hipGetLastError();
+}
+
+hipError_t pgk_launch_seed(float* V, const int64_t* seeds, int64_t n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    const int64_t nb = (n + 255) / 256;
+    hipLaunchKernelGGL(pgk::seed_kernel, dim3((unsigned)nb), dim3(256), 0, st, V, seeds, n);
+    return hipGetLastError();
+}

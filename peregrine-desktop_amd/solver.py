@@ -1,0 +1,195 @@
+"""Host driver over libperegrine_hip, mirroring the reference's solve loop.
+
+`HipSolver` owns one resident grid on the device (the CuArray uploads of
+solver/src/Run.jl:1776-1784). `solve_value` keeps `solve_value!`'s meaning
+(solver/src/Run.jl:1434-1474): fill V with cap, seed the target cells with
+zero, sweep until quiet or out of iterations, return (iterations, last
+delta). `sweep_gpu` is one sweep (solver/src/Gpu.jl:27-37). `gpu_budget` and
+`gpu_fits` are solver/src/Gpu.jl:181-209. There is no CPU fallback: without
+the built library every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import abi
+from .model import Grid6, Model, Params, ctl_soa
+
+
+class Context:
+    """A bound device with its solver stream."""
+
+    def __init__(self, device: int = 0):
+        self.lib = abi.load()
+        self._h = ctypes.c_void_p()
+        abi.check(self.lib, "pg_create", self.lib.pg_create(int(device), ctypes.byref(self._h)))
+        self.device = device
+
+    def close(self):
+        if self._h:
+            self.lib.pg_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+_default_ctx = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+def gpu_budget(headroom: float = 0.85, ctx: Context | None = None) -> int:
+    """Device bytes this run may use (Gpu.jl:181-185)."""
+    ctx = ctx or default_context()
+    out = ctypes.c_int64()
+    abi.check(ctx.lib, "pg_device_budget", ctx.lib.pg_device_budget(ctx._h, headroom, ctypes.byref(out)))
+    return int(out.value)
+
+
+def gpu_fits(g: Grid6, headroom: float = 0.85, vstore: int = abi.PG_VSTORE_F32,
+             ctx: Context | None = None):
+    """(fits, need GiB, free GiB) for a whole-grid solve (Gpu.jl:204-209)."""
+    ctx = ctx or default_context()
+    fits = ctypes.c_int32()
+    need = ctypes.c_double()
+    free = ctypes.c_double()
+    gc = g.c()
+    abi.check(ctx.lib, "pg_grid_fits",
+              ctx.lib.pg_grid_fits(ctx._h, ctypes.byref(gc), vstore, headroom, ctypes.byref(fits),
+                                   ctypes.byref(need), ctypes.byref(free)))
+    return bool(fits.value), need.value, free.value
+
+
+@dataclass
+class SweepResult:
+    sum_decrease: float
+    linf: float
+    kernel_ms: float
+
+
+class HipSolver:
+    """One resident value table on the device."""
+
+    def __init__(self, g: Grid6, m: Model, p: Params, ctl: np.ndarray, occ: np.ndarray,
+                 vstore: int = abi.PG_VSTORE_F32, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        self.lib = self.ctx.lib
+        self.g, self.m, self.p = g, m, p
+        ctl = np.asarray(ctl, dtype=np.float32)
+        if ctl.ndim == 2:
+            soa = ctl_soa(ctl)
+            nctl = ctl.shape[0]
+        else:
+            soa = np.ascontiguousarray(ctl)
+            nctl = soa.size // 3
+        occ = np.ascontiguousarray(np.asarray(occ, dtype=np.uint8).reshape(-1))
+        if occ.size != g.n[0] * g.n[1] * g.n[2]:
+            raise ValueError(f"occupancy has {occ.size} cells, grid needs {g.n[0] * g.n[1] * g.n[2]}")
+        self.nctl = nctl
+        self._gc, self._mc, self._pc = g.c(), m.c_struct(), p.c()
+        self._soa, self._occ = soa, occ
+        self._h = ctypes.c_void_p()
+        abi.check(self.lib, "pg_solver_create",
+                  self.lib.pg_solver_create(self.ctx._h, ctypes.byref(self._gc), ctypes.byref(self._mc),
+                                            ctypes.byref(self._pc), soa.ctypes.data, nctl,
+                                            occ.ctypes.data, vstore, ctypes.byref(self._h)))
+        self.ncells = g.ncells
+
+    def close(self):
+        if self._h:
+            self.lib.pg_solver_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- the reference's loop pieces ---------------------------------------
+    def fill_seed(self, seeds) -> None:
+        s = np.ascontiguousarray(np.asarray(seeds, dtype=np.int64))
+        abi.check(self.lib, "pg_fill_seed", self.lib.pg_fill_seed(self._h, s.ctypes.data, s.size))
+
+    def sweep(self, phase: int = 0, rev: bool = False) -> SweepResult:
+        d = ctypes.c_double()
+        l = ctypes.c_float()
+        abi.check(self.lib, "pg_sweep",
+                  self.lib.pg_sweep(self._h, int(phase), int(bool(rev)), ctypes.byref(d), ctypes.byref(l)))
+        ms = ctypes.c_float()
+        self.lib.pg_last_sweep_ms(self._h, ctypes.byref(ms))
+        return SweepResult(d.value, l.value, ms.value)
+
+    def solve(self, iters: int, tol: float = -1.0, tol_linf: float = 0.0):
+        it = ctypes.c_int32()
+        d = ctypes.c_double()
+        l = ctypes.c_float()
+        abi.check(self.lib, "pg_solve",
+                  self.lib.pg_solve(self._h, int(iters), float(tol), float(tol_linf), ctypes.byref(it),
+                                    ctypes.byref(d), ctypes.byref(l)))
+        return it.value, d.value, l.value
+
+    def policy(self) -> np.ndarray:
+        out = np.empty(self.ncells, dtype=np.uint8)
+        abi.check(self.lib, "pg_policy", self.lib.pg_policy(self._h, out.ctypes.data))
+        return out
+
+    def download(self, off: int = 0, n: int | None = None) -> np.ndarray:
+        n = self.ncells - off if n is None else n
+        out = np.empty(n, dtype=np.float32)
+        abi.check(self.lib, "pg_download_V", self.lib.pg_download_V(self._h, out.ctypes.data, off, n))
+        return out
+
+    def upload(self, V: np.ndarray, off: int = 0) -> None:
+        V = np.ascontiguousarray(np.asarray(V, dtype=np.float32))
+        abi.check(self.lib, "pg_upload_V", self.lib.pg_upload_V(self._h, V.ctypes.data, off, V.size))
+
+
+def sweep_gpu(solver: HipSolver, phase: int = 0, rev: bool = False) -> float:
+    """One sweep; returns the total improvement (Gpu.jl:27-37)."""
+    return solver.sweep(phase, rev).sum_decrease
+
+
+def solve_value(solver: HipSolver, tidx, iters: int, tol: float, tol_linf: float = 0.0,
+                on_progress=None):
+    """solve_value! (Run.jl:1434-1474): returns (iterations_done, final_delta).
+
+    `tidx` holds 0-based seed indices. The quiet rule is the reference's
+    (summed decrease <= tol, one quiet sweep for a full control scan); with
+    tol_linf > 0 the sup-norm rule ||dV||_inf < tol_linf stops as well.
+    """
+    solver.fill_seed(tidx)
+    done, last = 0, float("inf")
+    for it in range(1, iters + 1):
+        r = solver.sweep(it - 1, it % 2 == 1)
+        done, last = it, r.sum_decrease
+        quiet = (tol >= 0 and r.sum_decrease <= tol) or (tol_linf > 0 and r.linf < tol_linf)
+        if on_progress is not None and (it % 5 == 0 or it == 1 or quiet):
+            on_progress(it, r)
+        if quiet:
+            break
+    return done, last
