@@ -1,0 +1,165 @@
+"""Benchmark: state-updates/s of the plain value-iteration sweep on one MI355X.
+
+A step is one Bellman sweep over the whole grid (every state backed up once
+against every control), inputs resident in HBM. The default workload is the
+564,854,784-state, 125-control problem (64x64x48x13x13x17 grid, six
+obstacles, plain FP32 scheme). Prints one JSON line with the throughput, the
+dominant kernel's roofline figure and a CPU baseline measured on a bounded
+sample of the same workload.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 2]
+                    [--converge] [--no-cpu]
+
+--converge additionally times a full solve from V = cap to the first sweep
+with ||dV||_inf < 1e-3 s (reported under "converge").
+"""
+from __future__ import annotations
+
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+HBM_PEAK_GBS = 8000.0
+
+
+def load_package():
+    name = "peregrine_desktop_amd"
+    if name in sys.modules:
+        return sys.modules[name]
+    pkg = ROOT / "peregrine-desktop_amd"
+    spec = importlib.util.spec_from_file_location(name, pkg / "__init__.py",
+                                                  submodule_search_locations=[str(pkg)])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def cpu_baseline(pg, prob, V, seconds_target: float = 15.0):
+    """The CPU restatement's backups (the reference's sweep_cpu! arithmetic,
+    OpenMP over cells) timed on a random sample of the same problem's cells
+    against the same V. Rank 0 only; bounded to ~seconds_target of work."""
+    sys.path.insert(0, str(ROOT))
+    from oracle import oracle as orc
+
+    nthreads = min(16, os.cpu_count() or 1)
+    op = orc.Problem(prob.grid.c(), prob.model.c_struct(), prob.params.c(), pg.ctl_soa(prob.ctl),
+                     prob.occ)
+    rng = np.random.default_rng(3)
+    n = 2048
+    cells = rng.integers(0, prob.grid.ncells, n)
+    t = time.perf_counter()
+    op.backup_cells_f32(V, cells, nthreads)
+    dt = time.perf_counter() - t
+    n = int(min(4_000_000, max(n, n * seconds_target / max(dt, 1e-6))))
+    cells = rng.integers(0, prob.grid.ncells, n)
+    t = time.perf_counter()
+    op.backup_cells_f32(V, cells, nthreads)
+    dt = time.perf_counter() - t
+    return {"value": n / dt, "unit": "state-updates/s", "cores": nthreads, "kind": "port",
+            "sample": f"{n} random cells of the same grid, full {prob.nctl}-control backup each, "
+                      f"against the post-warmup V; {dt:.1f} s"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--converge", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tol", type=float, default=1e-3)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the solve is single-GPU "
+              f"by design and no multi-GPU path is built", file=sys.stderr)
+        return 2
+
+    pg = load_package()
+    t0 = time.perf_counter()
+    prob = pg.configs.problem(args.config)
+    g = prob.grid
+    print(f"# problem {args.config}: grid {g.n} = {g.ncells} states, {prob.nctl} controls, "
+          f"occupied {float(prob.occ.mean()):.3f}, built in {time.perf_counter() - t0:.1f} s",
+          file=sys.stderr, flush=True)
+    s = pg.HipSolver(g, prob.model, prob.params, prob.ctl, prob.occ)
+    s.fill_seed(prob.seeds)
+    warm_ms = []
+    for i in range(args.warmup):
+        r = s.sweep(i, i % 2 == 1)
+        warm_ms.append(r.kernel_ms)
+        print(f"# warmup sweep {i}: {r.kernel_ms:.1f} ms kernel", file=sys.stderr, flush=True)
+    steps = args.steps
+    if steps is None:
+        per = (warm_ms[-1] / 1e3) if warm_ms else 5.0
+        steps = int(max(3, min(20, 45.0 / max(per, 1e-3))))
+    kms = []
+    t = time.perf_counter()
+    for i in range(steps):
+        r = s.sweep(args.warmup + i, (args.warmup + i) % 2 == 1)
+        kms.append(r.kernel_ms)
+        print(f"# sweep {i}: {r.kernel_ms:.1f} ms kernel, linf {r.linf:.4f}", file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t
+    ms_per_step = wall / steps * 1e3
+    value = g.ncells * steps / wall
+    kavg = float(np.mean(kms))
+    bytes_per_update = pg.configs.algorithmic_bytes_per_update(prob.nctl, 4)
+    alg_bytes = g.ncells * bytes_per_update
+    achieved = alg_bytes / (kavg / 1e3) / 1e9
+    out = {
+        "metric": "state-updates/s + time to ||dV||inf<1e-3 s: FP16-V vs FP32 plain, 1 GPU",
+        "value": value,
+        "unit": "state-updates/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"problem {args.config}: plain FP32 Jacobi value-iteration sweep",
+            "grid": list(g.n),
+            "states": g.ncells,
+            "controls": prob.nctl,
+            "obstacles": len(prob.rects),
+            "scheme": "fixed dt 0.05 s, forward Euler 5 substeps, 3 swept probes, 64-corner multilinear",
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel_ms": kavg,
+            "bytes_per_state_update": bytes_per_update,
+        },
+    }
+    if not args.no_cpu:
+        V = s.download()
+        out["cpu_baseline"] = cpu_baseline(pg, prob, V)
+    if args.converge:
+        s.fill_seed(prob.seeds)
+        t = time.perf_counter()
+        its, last, linf = s.solve(2000, -1.0, args.tol)
+        out["converge"] = {"seconds": time.perf_counter() - t, "sweeps": its, "linf": linf,
+                           "tol": args.tol}
+    s.close()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -38,6 +38,7 @@ SIG = {
     "orc_cell_update_f32": (_F, [_I64, _P, _P, _P, _P, _P, _I32, _P, _I32, _I32, _P]),
     "orc_cell_update_f64": (_D, [_I64, _P, _P, _P, _P, _P, _I32, _P, _I32, _I32, _P]),
     "orc_backup_cells_f64": (ctypes.c_int, [_P, _P, _P, _P, _P, _I32, _P, _P, _I64, _P, _P, _I32]),
+    "orc_backup_cells_f32": (ctypes.c_int, [_P, _P, _P, _P, _P, _I32, _P, _P, _I64, _P, _I32]),
     "orc_sweep_f32": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _I32, _P, _I32, _I32, _I32, _I32, _P, _P, _P, _I32]),
     "orc_sweep_f64": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _I32, _P, _I32, _I32, _I32, _I32, _P, _P, _P, _I32]),
     "orc_target_cells": (_I64, [_P, _P, _P, _P, _I64]),
@@ -115,6 +116,17 @@ class Problem:
         if rc != 0:
             raise RuntimeError(f"oracle backup failed ({rc})")
         return out, kout
+
+    def backup_cells_f32(self, V: np.ndarray, cells: np.ndarray, nthreads: int = 0):
+        L = lib()
+        cells = np.ascontiguousarray(cells, dtype=np.int64)
+        out = np.empty(cells.size, dtype=np.float32)
+        rc = L.orc_backup_cells_f32(_ptr(V), _ptr(self.occ), ctypes.byref(self.gc), ctypes.byref(self.mc),
+                                    _ptr(self.ctl), self.nctl, ctypes.byref(self.pc), _ptr(cells),
+                                    cells.size, _ptr(out), nthreads)
+        if rc != 0:
+            raise RuntimeError(f"oracle backup failed ({rc})")
+        return out
 
     def solve(self, seeds, iters: int, tol_linf: float = 1e-3, dtype=np.float32, nthreads: int = 0,
               mode: int = 0):

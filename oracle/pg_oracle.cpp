@@ -211,6 +211,21 @@ int orc_backup_cells_f64(const double* V, const uint8_t* occ, const pg_grid* g, 
     return 0;
 }
 
+// Backups of a list of cells against V in float (no write): the timed CPU
+// baseline on a bounded sample of a grid too large to sweep on the host.
+int orc_backup_cells_f32(const float* V, const uint8_t* occ, const pg_grid* g, const pg_model* m,
+                         const float* ctl, int32_t nctl, const pg_params* p, const int64_t* cells,
+                         int64_t ncell_list, float* out, int32_t nthreads) {
+    if (!params_supported(p) || nctl < 2) return -3;
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t j = 0; j < ncell_list; ++j) {
+        int32_t k = 0;
+        out[j] = cell_update<float>(cells[j], V, occ, *g, *m, ctl, nctl, *p, 0, false, &k);
+    }
+    return 0;
+}
+
 int orc_sweep_f32(const float* Vin, float* Vout, const uint8_t* occ, const pg_grid* g,
                   const pg_model* m, const float* ctl, int32_t nctl, const pg_params* p,
                   int32_t mode, int32_t rev, int32_t phase, int32_t nthreads, uint8_t* pol,
