@@ -111,6 +111,7 @@ def main() -> int:
     ms_per_step = wall / steps * 1e3
     value = g.ncells * steps / wall
     kavg = float(np.mean(kms))
+    kern, coverage = s.kernel_info()
     bytes_per_update = pg.configs.algorithmic_bytes_per_update(prob.nctl, 4)
     alg_bytes = g.ncells * bytes_per_update
     achieved = alg_bytes / (kavg / 1e3) / 1e9
@@ -135,6 +136,8 @@ def main() -> int:
             "obstacles": len(prob.rects),
             "scheme": "fixed dt 0.05 s, forward Euler 5 substeps, 3 swept probes, 64-corner multilinear",
             "parallelism": "single GPU",
+            "kernel": {1: "global-gather", 2: "LDS-staged"}.get(kern, str(kern)),
+            "window_coverage": coverage,
         },
         "roofline": {
             "bound": "hbm",

@@ -154,6 +154,19 @@ int pg_policy(pg_solver* s, uint8_t* pol_host);
 int pg_download_V(pg_solver* s, float* dst, int64_t off, int64_t n);
 int pg_upload_V(pg_solver* s, const float* src, int64_t off, int64_t n);
 
+/* Sweep kernel selection. AUTO (the default) takes the LDS-staged kernel
+ * whenever it applies (at most 128 controls) and the global-gather kernel
+ * otherwise; both produce identical bits. */
+#define PG_KERNEL_AUTO   0
+#define PG_KERNEL_GLOBAL 1
+#define PG_KERNEL_LDS    2
+int pg_solver_set_kernel(pg_solver* s, int32_t kernel);
+
+/* The kernel in use, and for the LDS-staged one the fraction of boxes whose
+ * successors all fall inside the staged window (0 before the first sweep;
+ * the rest read global memory for the corners outside). */
+int pg_solver_kernel_info(pg_solver* s, int32_t* kernel, double* window_coverage);
+
 /* Duration of the last sweep kernel, from HIP events on the solver stream. */
 int pg_last_sweep_ms(pg_solver* s, float* ms);
 

@@ -21,6 +21,10 @@ PG_ERR_UNSUPPORTED = -3
 PG_ERR_NOMEM = -4
 PG_ERR_STATE = -5
 
+PG_KERNEL_AUTO = 0
+PG_KERNEL_GLOBAL = 1
+PG_KERNEL_LDS = 2
+
 PG_VSTORE_F32 = 0
 PG_VSTORE_F16_TILE = 1
 
@@ -92,6 +96,8 @@ SIGNATURES = {
     "pg_policy": (ctypes.c_int, [_P, _P]),
     "pg_download_V": (ctypes.c_int, [_P, _P, _I64, _I64]),
     "pg_upload_V": (ctypes.c_int, [_P, _P, _I64, _I64]),
+    "pg_solver_set_kernel": (ctypes.c_int, [_P, _I32]),
+    "pg_solver_kernel_info": (ctypes.c_int, [_P, ctypes.POINTER(_I32), ctypes.POINTER(ctypes.c_double)]),
     "pg_last_sweep_ms": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_float)]),
     "pg_solver_ncells": (ctypes.c_int, [_P, ctypes.POINTER(_I64)]),
 }

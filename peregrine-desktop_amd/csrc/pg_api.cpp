@@ -93,6 +93,15 @@ struct pg_solver {
     uint8_t* d_pol = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     float last_ms = 0.0f;
+    // LDS-staged path
+    int32_t kernel_req = PG_KERNEL_AUTO;
+    int32_t kernel = PG_KERNEL_GLOBAL;
+    bool diag = false;
+    pgk::BoxGeom G{};
+    int16_t* d_worig = nullptr;
+    bool planned = false;
+    int shape = 0;
+    double coverage = 0.0;
 };
 
 namespace {
@@ -108,6 +117,7 @@ void release(pg_solver* s) {
     hipFree(s->d_sum);
     hipFree(s->d_linf);
     hipFree(s->d_pol);
+    hipFree(s->d_worig);
     if (s->ev0) hipEventDestroy(s->ev0);
     if (s->ev1) hipEventDestroy(s->ev1);
     delete s;
@@ -171,6 +181,70 @@ void build_consts(pg_solver* s) {
     K.nchk = s->p.checks;
     for (int sc = 1; sc <= K.nchk; ++sc) K.probe_a[sc - 1] = (float)sc / (float)K.nchk;
     K.nctl = s->nctl;
+}
+
+// A model whose A, S, D blocks are diagonal and q is zero (every other
+// coefficient exactly +-0) runs the kernels' diagonal specialisation.
+bool model_is_diagonal(const pg_model& m) {
+    for (int r = 0; r < 3; ++r) {
+        if (m.q[r] != 0.0f) return false;
+        for (int c = 0; c < 3; ++c) {
+            if (r == c) continue;
+            if (m.A[3 * r + c] != 0.0f || m.S[3 * r + c] != 0.0f || m.D[3 * r + c] != 0.0f) return false;
+        }
+    }
+    return true;
+}
+
+constexpr int kLdsMaxStates = 128;
+constexpr int kLdsMaxCtl = 128;
+
+void box_geometry(pg_solver* s) {
+    const int want[6] = {2, 2, 2, 2, 2, 4};
+    int64_t nbox = 1;
+    for (int k = 0; k < 6; ++k) {
+        s->G.b[k] = want[k] < s->g.n[k] ? want[k] : s->g.n[k];
+        s->G.nb[k] = (s->g.n[k] + s->G.b[k] - 1) / s->G.b[k];
+        nbox *= s->G.nb[k];
+    }
+    s->G.nbox = nbox;
+}
+
+int64_t lds_blocks(const pg_solver* s) { return ((s->G.nbox + 7) / 8) * 8; }
+
+int ensure_partials(pg_solver* s, int64_t nb);
+
+// One-time pass over every (state, control) pair: each box's window origin,
+// and the window shape that holds the most boxes' successors whole.
+int plan(pg_solver* s) {
+    if (s->planned) return PG_OK;
+    hipStream_t st = s->ctx->stream;
+    box_geometry(s);
+    const int64_t nbox = s->G.nbox;
+    uint8_t* d_ext = nullptr;
+    HIPCHECK(hipMalloc(&s->d_worig, (size_t)nbox * 6 * sizeof(int16_t)));
+    HIPCHECK(hipMalloc(&d_ext, (size_t)nbox * 6));
+    hipError_t e = pgk_launch_plan(s->K, s->G, s->d_occ, s->d_bu, s->d_worig, d_ext, s->diag ? 1 : 0, st);
+    std::vector<uint8_t> ext((size_t)nbox * 6);
+    if (e == hipSuccess) e = hipMemcpyAsync(ext.data(), d_ext, ext.size(), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    hipFree(d_ext);
+    if (e != hipSuccess) return fail(PG_ERR_HIP, "window plan failed: %s", hipGetErrorString(e));
+    const int (*shapes)[6] = nullptr;
+    const int nsh = pgk_window_shapes(&shapes);
+    int64_t best = -1;
+    for (int i = 0; i < nsh; ++i) {
+        int64_t cov = 0;
+        for (int64_t b = 0; b < nbox; ++b) {
+            bool ok = true;
+            for (int k = 0; k < 6; ++k) ok = ok && ext[(size_t)b * 6 + k] <= shapes[i][k];
+            cov += ok ? 1 : 0;
+        }
+        if (cov > best) { best = cov; s->shape = i; }
+    }
+    s->coverage = nbox > 0 ? (double)best / (double)nbox : 1.0;
+    s->planned = true;
+    return ensure_partials(s, lds_blocks(s));
 }
 
 int ensure_partials(pg_solver* s, int64_t nb) {
@@ -277,6 +351,8 @@ int pg_solver_create(pg_ctx* ctx, const pg_grid* g, const pg_model* m, const pg_
     s->vstore = vstore;
     s->ncell = grid_cells(*g);
     build_consts(s);
+    s->diag = model_is_diagonal(*m);
+    s->kernel = (nctl <= kLdsMaxCtl) ? PG_KERNEL_LDS : PG_KERNEL_GLOBAL;
 
     // B * u_sat per control, in the reference's summation order
     std::vector<float> bu(3 * (size_t)nctl);
@@ -365,15 +441,24 @@ int pg_sweep(pg_solver* s, int32_t phase, int32_t rev, double* sum_decrease, flo
     if (!s->seeded) return fail(PG_ERR_STATE, "pg_fill_seed (or pg_upload_V) must come before a sweep");
     HIPCHECK(hipSetDevice(s->ctx->device));
     hipStream_t st = s->ctx->stream;
-    const int64_t nb = (s->ncell + 255) / 256;
+    const bool lds = s->kernel == PG_KERNEL_LDS;
+    if (lds) {
+        int rc = plan(s);
+        if (rc) return rc;
+    }
+    const int64_t nb = lds ? lds_blocks(s) : (s->ncell + 255) / 256;
     int rc = ensure_partials(s, nb);
     if (rc) return rc;
     HIPCHECK(hipMemsetAsync(s->d_linf, 0, sizeof(unsigned), st));
     const float* Vin = s->dV[s->cur];
     float* Vout = s->dV[1 - s->cur];
-    int64_t nblk = 0;
+    int64_t nblk = nb;
     HIPCHECK(hipEventRecord(s->ev0, st));
-    HIPCHECK(pgk_launch_sweep(s->K, Vin, Vout, s->d_occ, s->d_bu, nullptr, s->d_part, s->d_linf, 1, st, &nblk));
+    if (lds)
+        HIPCHECK(pgk_launch_sweep_lds(s->shape, s->diag ? 1 : 0, s->K, s->G, Vin, Vout, s->d_occ, s->d_bu,
+                                      s->d_worig, nullptr, s->d_part, s->d_linf, 1, st));
+    else
+        HIPCHECK(pgk_launch_sweep(s->K, Vin, Vout, s->d_occ, s->d_bu, nullptr, s->d_part, s->d_linf, 1, st, &nblk));
     HIPCHECK(hipEventRecord(s->ev1, st));
     HIPCHECK(pgk_launch_reduce(s->d_part, nblk, s->d_sum, st));
     HIPCHECK(pgk_launch_seed(Vout, s->d_seeds, s->nseeds, st));
@@ -418,7 +503,14 @@ int pg_policy(pg_solver* s, uint8_t* pol_host) {
     hipStream_t st = s->ctx->stream;
     if (!s->d_pol) HIPCHECK(hipMalloc(&s->d_pol, (size_t)s->ncell));
     int64_t nblk = 0;
-    HIPCHECK(pgk_launch_sweep(s->K, s->dV[s->cur], nullptr, s->d_occ, s->d_bu, s->d_pol, nullptr, nullptr, 0, st, &nblk));
+    if (s->kernel == PG_KERNEL_LDS) {
+        int rc = plan(s);
+        if (rc) return rc;
+        HIPCHECK(pgk_launch_sweep_lds(s->shape, s->diag ? 1 : 0, s->K, s->G, s->dV[s->cur], nullptr, s->d_occ,
+                                      s->d_bu, s->d_worig, s->d_pol, nullptr, nullptr, 0, st));
+    } else {
+        HIPCHECK(pgk_launch_sweep(s->K, s->dV[s->cur], nullptr, s->d_occ, s->d_bu, s->d_pol, nullptr, nullptr, 0, st, &nblk));
+    }
     HIPCHECK(hipMemcpyAsync(pol_host, s->d_pol, (size_t)s->ncell, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
     return PG_OK;
@@ -443,6 +535,30 @@ int pg_upload_V(pg_solver* s, const float* src, int64_t off, int64_t n) {
         HIPCHECK(hipStreamSynchronize(s->ctx->stream));
     }
     s->seeded = true;
+    return PG_OK;
+}
+
+int pg_solver_set_kernel(pg_solver* s, int32_t kernel) {
+    if (!s) return fail(PG_ERR_ARG, "null solver");
+    if (kernel == PG_KERNEL_AUTO) {
+        s->kernel = (s->nctl <= kLdsMaxCtl) ? PG_KERNEL_LDS : PG_KERNEL_GLOBAL;
+    } else if (kernel == PG_KERNEL_GLOBAL) {
+        s->kernel = PG_KERNEL_GLOBAL;
+    } else if (kernel == PG_KERNEL_LDS) {
+        if (s->nctl > kLdsMaxCtl)
+            return fail(PG_ERR_UNSUPPORTED, "the LDS-staged sweep holds at most %d controls", kLdsMaxCtl);
+        s->kernel = PG_KERNEL_LDS;
+    } else {
+        return fail(PG_ERR_ARG, "unknown kernel %d", kernel);
+    }
+    s->kernel_req = kernel;
+    return PG_OK;
+}
+
+int pg_solver_kernel_info(pg_solver* s, int32_t* kernel, double* window_coverage) {
+    if (!s) return fail(PG_ERR_ARG, "null solver");
+    if (kernel) *kernel = s->kernel;
+    if (window_coverage) *window_coverage = (s->kernel == PG_KERNEL_LDS && s->planned) ? s->coverage : 0.0;
     return PG_OK;
 }
 

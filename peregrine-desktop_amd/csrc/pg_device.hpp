@@ -177,4 +177,238 @@ __device__ __forceinline__ bool probes_free(const Consts& K, const OccRead& occ,
     return ok;
 }
 
+
+// ---------------------------------------------------------------------------
+// Acceleration rows with the state-dependent terms precomputed. DIAG = true
+// drops every term whose coefficient is exactly +-0 (off-diagonal A, S, D and
+// q): adding a +-0 product to a partial sum changes at most the sign of a zero
+// partial sum, and no zero's sign reaches any V value (divisions are by
+// nonzero constants; comparisons, floor and rounding ignore the sign of
+// zero), so the V bits are those of the full left-to-right sum.
+// ---------------------------------------------------------------------------
+template <bool DIAG>
+__device__ __forceinline__ void accel_rows(const Consts& K, float bux, float buy, float buw,
+                                           float vx, float vy, float w, float w2, float sx,
+                                           float sy, float sw, float dx, float dy, float dw,
+                                           float& ax, float& ay, float& al) {
+    if (DIAG) {
+        ax = bux; ax = ax + K.A[0] * vx; ax = ax + K.S[0] * sx; ax = ax + K.D[0] * dx; ax = ax + K.c[0];
+        ay = buy; ay = ay + K.A[4] * vy; ay = ay + K.S[4] * sy; ay = ay + K.D[4] * dy; ay = ay + K.c[1];
+        al = buw; al = al + K.A[8] * w;  al = al + K.S[8] * sw; al = al + K.D[8] * dw; al = al + K.c[2];
+    } else {
+        ax = bux;
+        ax = ax + K.A[0] * vx; ax = ax + K.A[1] * vy; ax = ax + K.A[2] * w;
+        ax = ax + K.q[0] * w2;
+        ax = ax + K.S[0] * sx; ax = ax + K.S[1] * sy; ax = ax + K.S[2] * sw;
+        ax = ax + K.D[0] * dx; ax = ax + K.D[1] * dy; ax = ax + K.D[2] * dw;
+        ax = ax + K.c[0];
+        ay = buy;
+        ay = ay + K.A[3] * vx; ay = ay + K.A[4] * vy; ay = ay + K.A[5] * w;
+        ay = ay + K.q[1] * w2;
+        ay = ay + K.S[3] * sx; ay = ay + K.S[4] * sy; ay = ay + K.S[5] * sw;
+        ay = ay + K.D[3] * dx; ay = ay + K.D[4] * dy; ay = ay + K.D[5] * dw;
+        ay = ay + K.c[1];
+        al = buw;
+        al = al + K.A[6] * vx; al = al + K.A[7] * vy; al = al + K.A[8] * w;
+        al = al + K.q[2] * w2;
+        al = al + K.S[6] * sx; al = al + K.S[7] * sy; al = al + K.S[8] * sw;
+        al = al + K.D[6] * dx; al = al + K.D[7] * dy; al = al + K.D[8] * dw;
+        al = al + K.c[2];
+    }
+}
+
+// The per-cell quantities every lookahead of a cell starts from: the body
+// state and the state-only terms of the first substep (the heading's sincos
+// doubles as the field->body rotation, PeregrineSolver.jl:1234-1236).
+struct CellStart {
+    float x, y, h, vx, vy, w;
+    float sh, ch;
+    float sx, sy, sw, dx, dy, dw, w2;
+};
+
+__device__ __forceinline__ void cell_start(const Consts& K, int i0, int i1, int i2, int i3, int i4,
+                                           int i5, CellStart& c) {
+    c.x = K.lo[0] + (float)i0 * K.step[0];
+    c.y = K.lo[1] + (float)i1 * K.step[1];
+    c.h = K.lo[2] + (float)i2 * K.step[2];
+    const float vfx = K.lo[3] + (float)i3 * K.step[3];
+    const float vfy = K.lo[4] + (float)i4 * K.step[4];
+    c.w = K.lo[5] + (float)i5 * K.step[5];
+    sincos_ref(c.h, c.sh, c.ch);
+    c.vx = c.ch * vfx + c.sh * vfy;
+    c.vy = -c.sh * vfx + c.ch * vfy;
+    c.w2 = c.w * c.w;
+    c.sx = clamp1(qdiv(c.vx, K.reps[0]));
+    c.sy = clamp1(qdiv(c.vy, K.reps[1]));
+    c.sw = clamp1(qdiv(c.w, K.reps[2]));
+    c.dx = __builtin_fabsf(c.vx) * c.vx;
+    c.dy = __builtin_fabsf(c.vy) * c.vy;
+    c.dw = __builtin_fabsf(c.w) * c.w;
+}
+
+// Forward Euler from a cell start: substep 1 reuses the cell's state-only
+// terms (a grid state is finite, so the divergence test of substep 1 always
+// passes), substeps 2..nsub are PeregrineSolver.jl:208-221 verbatim.
+template <bool DIAG>
+__device__ __forceinline__ bool euler_from(const Consts& K, const CellStart& c, float bux, float buy,
+                                           float buw, float& x, float& y, float& h, float& vx,
+                                           float& vy, float& w) {
+    const float hs = K.hs;
+    bool ok = true;
+    {
+        float ax, ay, al;
+        accel_rows<DIAG>(K, bux, buy, buw, c.vx, c.vy, c.w, c.w2, c.sx, c.sy, c.sw, c.dx, c.dy, c.dw,
+                         ax, ay, al);
+        const float dvx = ax + c.w * c.vy;
+        const float dvy = ay - c.w * c.vx;
+        const float px = c.ch * c.vx - c.sh * c.vy;
+        const float py = c.sh * c.vx + c.ch * c.vy;
+        x = c.x + px * hs;
+        y = c.y + py * hs;
+        h = c.h + c.w * hs;
+        vx = c.vx + dvx * hs;
+        vy = c.vy + dvy * hs;
+        w = c.w + al * hs;
+    }
+    for (int k = 1; k < K.nsub; ++k) {
+        float sum = x + y;
+        sum = sum + h;
+        sum = sum + vx;
+        sum = sum + vy;
+        sum = sum + w;
+        ok = ok && finite_f(sum);
+        const float w2 = DIAG ? 0.0f : w * w;
+        const float sx = clamp1(qdiv(vx, K.reps[0]));
+        const float sy = clamp1(qdiv(vy, K.reps[1]));
+        const float sw = clamp1(qdiv(w, K.reps[2]));
+        const float dx = __builtin_fabsf(vx) * vx;
+        const float dy = __builtin_fabsf(vy) * vy;
+        const float dw = __builtin_fabsf(w) * w;
+        float ax, ay, al;
+        accel_rows<DIAG>(K, bux, buy, buw, vx, vy, w, w2, sx, sy, sw, dx, dy, dw, ax, ay, al);
+        const float dvx = ax + w * vy;
+        const float dvy = ay - w * vx;
+        float sh, ch;
+        sincos_ref(h, sh, ch);
+        const float px = ch * vx - sh * vy;
+        const float py = sh * vx + ch * vy;
+        x = x + px * hs;
+        y = y + py * hs;
+        h = h + w * hs;
+        vx = vx + dvx * hs;
+        vy = vy + dvy * hs;
+        w = w + al * hs;
+    }
+    return ok;
+}
+
+// Fractional coordinates of a successor, with the range tests of interp
+// (PeregrineSolver.jl:401-428). Returns false when the lookup rejects.
+__device__ __forceinline__ bool interp_coords(const Consts& K, float x, float y, float h, float vx,
+                                              float vy, float w, float f[6]) {
+    f[0] = qdiv(x - K.lo[0], K.rstep[0]);
+    f[1] = qdiv(y - K.lo[1], K.rstep[1]);
+    f[2] = wrap_h(K, qdiv(h - K.lo[2], K.rstep[2]));
+    f[3] = qdiv(vx - K.lo[3], K.rstep[3]);
+    f[4] = qdiv(vy - K.lo[4], K.rstep[4]);
+    f[5] = qdiv(w - K.lo[5], K.rstep[5]);
+    bool in = true;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) in = in & (f[k] >= 0.0f) & (f[k] <= K.lim[k]);
+    return in;
+}
+
+// Level-by-level corner weights of the first five axes; weight d of the
+// 64-corner sum is p5[d & 31] * (bit 5 of d ? t6 : u6), which is the
+// reference's left-to-right six-factor product.
+struct Weights {
+    float p5[32];
+    float t6, u6;
+};
+
+__device__ __forceinline__ void corner_weights(const float t[6], const float u[6], Weights& W) {
+    float p2[4], p3[8], p4[16];
+#pragma unroll
+    for (int d = 0; d < 4; ++d) p2[d] = ((d & 1) ? t[0] : u[0]) * ((d & 2) ? t[1] : u[1]);
+#pragma unroll
+    for (int d = 0; d < 8; ++d) p3[d] = p2[d & 3] * ((d & 4) ? t[2] : u[2]);
+#pragma unroll
+    for (int d = 0; d < 16; ++d) p4[d] = p3[d & 7] * ((d & 8) ? t[3] : u[3]);
+#pragma unroll
+    for (int d = 0; d < 32; ++d) W.p5[d] = p4[d & 15] * ((d & 16) ? t[4] : u[4]);
+    W.t6 = t[5];
+    W.u6 = u[5];
+}
+
+// The reference's accumulation order d = 0..63 (bit 0 = x ... bit 5 = w).
+__device__ __forceinline__ float corner_sum(const Weights& W, const float vals[64]) {
+    float acc = 0.0f;
+#pragma unroll
+    for (int d = 0; d < 64; ++d) {
+        const float wgt = W.p5[d & 31] * ((d & 32) ? W.t6 : W.u6);
+        acc = acc + wgt * vals[d];
+    }
+    return acc;
+}
+
+// Global-memory corner reads with the reference's clamped / periodic
+// indices (PeregrineSolver.jl:455-463).
+__device__ __forceinline__ void fetch_global(const Consts& K, const float* __restrict__ V,
+                                             const int i[6], float vals[64]) {
+    int64_t o0[6], o1[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        int j0, j1;
+        if (k == 2) {
+            j0 = (i[k] >= K.n[2]) ? i[k] - K.n[2] : i[k];
+            j1 = i[k] + 1;
+            j1 = (j1 >= K.n[2]) ? j1 - K.n[2] : j1;
+        } else {
+            j0 = i[k];
+            j1 = (i[k] + 1 < K.n[k] - 1) ? i[k] + 1 : K.n[k] - 1;
+        }
+        o0[k] = (int64_t)j0 * K.stride[k];
+        o1[k] = (int64_t)j1 * K.stride[k];
+    }
+#pragma unroll
+    for (int d = 0; d < 64; ++d) {
+        int64_t a = 0;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) a += ((d >> k) & 1) ? o1[k] : o0[k];
+        vals[d] = V[a];
+    }
+}
+
+// Box decomposition of the grid for the LDS-staged kernels.
+struct BoxGeom {
+    int32_t b[6];     // box dims
+    int32_t nb[6];    // boxes per axis
+    int64_t nbox;
+};
+
+// Block-wide sum (FP64) and max of the per-cell decreases: wave shuffles,
+// then one LDS slot per wave; thread 0 writes the block's partial sum and
+// folds the max into *linf_bits (non-negative floats order as their bits).
+__device__ __forceinline__ void block_reduce_sum_max(double dsum, float dmax, double* part_sum,
+                                                     unsigned* linf_bits) {
+    __shared__ double ssum[16];
+    __shared__ float smax[16];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        dsum += __shfl_xor(dsum, off, 64);
+        dmax = fmaxf(dmax, __shfl_xor(dmax, off, 64));
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) { ssum[wv] = dsum; smax[wv] = dmax; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = ssum[0];
+        float m = smax[0];
+        const int nw = (int)(blockDim.x >> 6);
+        for (int i = 1; i < nw; ++i) { s += ssum[i]; m = fmaxf(m, smax[i]); }
+        part_sum[blockIdx.x] = s;
+        if (m > 0.0f) atomicMax(linf_bits, __float_as_uint(m));
+    }
+}
+
 }  // namespace pgk

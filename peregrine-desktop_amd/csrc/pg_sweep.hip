@@ -117,28 +117,6 @@ __device__ __forceinline__ float lookahead_global(const Consts& K, const float* 
     return (c < K.cap) ? c : K.cap;
 }
 
-__device__ __forceinline__ void block_reduce(double dsum, float dmax, double* part_sum,
-                                             unsigned* linf_bits) {
-    __shared__ double ssum[4];
-    __shared__ float smax[4];
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        dsum += __shfl_xor(dsum, off, 64);
-        dmax = fmaxf(dmax, __shfl_xor(dmax, off, 64));
-    }
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    if (lane == 0) { ssum[wv] = dsum; smax[wv] = dmax; }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double s = ssum[0];
-        float m = smax[0];
-        const int nw = blockDim.x >> 6;
-        for (int i = 1; i < nw; ++i) { s += ssum[i]; m = fmaxf(m, smax[i]); }
-        part_sum[blockIdx.x] = s;
-        if (m > 0.0f) atomicMax(linf_bits, __float_as_uint(m));
-    }
-}
-
 template <bool WRITE_V, bool WRITE_POL>
 __global__ __launch_bounds__(256) void sweep_global_kernel(
     const float* __restrict__ Vin, float* __restrict__ Vout, const uint8_t* __restrict__ occ,
@@ -185,7 +163,7 @@ __global__ __launch_bounds__(256) void sweep_global_kernel(
         }
         if (WRITE_POL) pol[idx] = (uint8_t)bk;
     }
-    if (WRITE_V) block_reduce((double)dec, dec, part_sum, linf_bits);
+    if (WRITE_V) block_reduce_sum_max((double)dec, dec, part_sum, linf_bits);
 }
 
 // Deterministic sum of the per-block partials (fixed order per launch shape).

@@ -130,6 +130,17 @@ class HipSolver:
     def __exit__(self, *a):
         self.close()
 
+    def set_kernel(self, kernel: int) -> None:
+        """abi.PG_KERNEL_AUTO / _GLOBAL / _LDS (identical results)."""
+        abi.check(self.lib, "pg_solver_set_kernel", self.lib.pg_solver_set_kernel(self._h, int(kernel)))
+
+    def kernel_info(self):
+        k = ctypes.c_int32()
+        cov = ctypes.c_double()
+        abi.check(self.lib, "pg_solver_kernel_info",
+                  self.lib.pg_solver_kernel_info(self._h, ctypes.byref(k), ctypes.byref(cov)))
+        return k.value, cov.value
+
     # -- the reference's loop pieces ---------------------------------------
     def fill_seed(self, seeds) -> None:
         s = np.ascontiguousarray(np.asarray(seeds, dtype=np.int64))

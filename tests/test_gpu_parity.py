@@ -22,9 +22,17 @@ def diff_report(got, want):
     return f"{bad.size} cells differ; first {bad[:6].tolist()} got {got[bad[:6]].tolist()} want {want[bad[:6]].tolist()}"
 
 
-def step_parity(pg, orc, prob, sweeps, start=None):
+KERNELS = ["global", "lds"]
+
+
+def kernel_id(pg, name):
+    return {"global": pg.abi.PG_KERNEL_GLOBAL, "lds": pg.abi.PG_KERNEL_LDS}[name]
+
+
+def step_parity(pg, orc, prob, sweeps, start=None, kernel="lds"):
     op = oracle_problem(orc, prob)
     with pg.HipSolver(prob.grid, prob.model, prob.params, prob.ctl, prob.occ) as s:
+        s.set_kernel(kernel_id(pg, kernel))
         s.fill_seed(prob.seeds)
         if start is not None:
             s.upload(start)
@@ -46,28 +54,32 @@ def step_parity(pg, orc, prob, sweeps, start=None):
         return V, pol
 
 
-def test_config1_sweeps_bit_exact(pg, orc):
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_config1_sweeps_bit_exact(pg, orc, kernel):
     """537,600 states, 27 controls: three sweeps from the seeded start."""
     prob = pg.configs.problem(1)
-    step_parity(pg, orc, prob, 3)
+    step_parity(pg, orc, prob, 3, kernel=kernel)
 
 
-def test_config1_converged_table_bit_exact(pg, orc):
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_config1_converged_table_bit_exact(pg, orc, kernel):
     """A converged table (GPU solve) is a fixed point of the oracle's sweep
     too, and its argmin policy matches."""
     prob = pg.configs.problem(1)
     op = oracle_problem(orc, prob)
     with pg.HipSolver(prob.grid, prob.model, prob.params, prob.ctl, prob.occ) as s:
+        s.set_kernel(kernel_id(pg, kernel))
         s.fill_seed(prob.seeds)
         its, _, linf = s.solve(400, -1.0, 1e-3)
         V = s.download()
     assert linf < 1e-3 and its < 400
-    step_parity(pg, orc, prob, 2, start=V)
+    step_parity(pg, orc, prob, 2, start=V, kernel=kernel)
     reached = np.count_nonzero(V < prob.params.cap) / V.size
     assert reached > 0.5
 
 
-def test_obstacles_125_controls_bit_exact(pg, orc):
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_obstacles_125_controls_bit_exact(pg, orc, kernel):
     """Problem 2's model, obstacles and 125 controls on a reduced grid, from
     a partly propagated table (so obstacles and cap boundaries are active)."""
     prob = pg.configs.problem(2, shape=(16, 16, 12, 7, 7, 9))
@@ -76,17 +88,71 @@ def test_obstacles_125_controls_bit_exact(pg, orc):
         s.fill_seed(prob.seeds)
         s.solve(12, -1.0, 0.0)
         V = s.download()
-    step_parity(pg, orc, prob, 2, start=V)
+    step_parity(pg, orc, prob, 2, start=V, kernel=kernel)
 
 
-def test_full_solve_matches_oracle_solve(pg, orc):
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_full_solve_matches_oracle_solve(pg, orc, kernel):
     """Whole solve on a small grid: same iteration count, same table."""
     prob = pg.configs.problem(1, shape=(10, 10, 8, 5, 5, 5))
     op = oracle_problem(orc, prob)
     Vo, ito, lo = op.solve(prob.seeds, 300, 1e-3, nthreads=nthreads())
     with pg.HipSolver(prob.grid, prob.model, prob.params, prob.ctl, prob.occ) as s:
+        s.set_kernel(kernel_id(pg, kernel))
         s.fill_seed(prob.seeds)
         its, _, linf = s.solve(300, -1.0, 1e-3)
         Vg = s.download()
     assert its == ito
     assert bits_equal(Vg, Vo), diff_report(Vg, Vo)
+
+
+def realistic_model(pg):
+    """The reference's `_realistic()` fit (SelfTest.jl:51-61): dense blocks,
+    traction knee 0.45 -- the kernels' general (non-diagonal) path."""
+    B = (-849.0, -25.8, -69.5, 5.0, -846.8, 0.6, -0.2, -9.6, -64.9)
+    A = (-2.13, -0.007, -8.72, -0.064, -4.96, -0.013, 0.039, 0.078, -4.67)
+    S = (-11.6, -3.37, 3.29, 2.07, 6.33, 1.00, -0.56, -1.17, -0.28)
+    D = (0.021, 0.003, 0.729, 0.001, 0.049, 0.871, -0.0004, -0.0006, 0.312)
+    return pg.Model(B, A, (0.0, 0.0, 0.0), S, D, (0.0, 0.0, 0.0), (5.0, 5.0, 0.15), 0.45)
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_dense_model_with_knee_bit_exact(pg, orc, kernel):
+    import dataclasses
+    prob = pg.configs.problem(2, shape=(12, 12, 8, 7, 7, 9))
+    prob = dataclasses.replace(prob, model=realistic_model(pg), ctl=pg.control_set(4))
+    with pg.HipSolver(prob.grid, prob.model, prob.params, prob.ctl, prob.occ) as s:
+        s.fill_seed(prob.seeds)
+        s.solve(10, -1.0, 0.0)
+        V = s.download()
+    step_parity(pg, orc, prob, 2, start=V, kernel=kernel)
+
+
+def test_full_size_problem2_lds_equals_global_and_oracle(pg, orc):
+    """564,854,784 states x 125 controls: the LDS-staged sweep writes the
+    same table as the global-gather sweep, and random cells match the
+    oracle's backup of the same V."""
+    prob = pg.configs.problem(2)
+    op = oracle_problem(orc, prob)
+    with pg.HipSolver(prob.grid, prob.model, prob.params, prob.ctl, prob.occ) as s:
+        s.set_kernel(pg.abi.PG_KERNEL_LDS)
+        s.fill_seed(prob.seeds)
+        for _ in range(3):
+            s.sweep()
+        V3 = s.download()
+        r_lds = s.sweep()
+        V4_lds = s.download()
+        kern, cov = s.kernel_info()
+        assert kern == pg.abi.PG_KERNEL_LDS and cov > 0.9
+        s.upload(V3)
+        s.set_kernel(pg.abi.PG_KERNEL_GLOBAL)
+        r_glb = s.sweep()
+        V4_glb = s.download()
+    assert bits_equal(V4_lds, V4_glb), diff_report(V4_lds, V4_glb)
+    assert r_lds.linf == r_glb.linf
+    rng = np.random.default_rng(17)
+    cells = rng.integers(0, prob.grid.ncells, 20000)
+    c = op.backup_cells_f32(V3, cells, nthreads())
+    want = np.minimum(c, V3[cells])
+    want[np.isin(cells, prob.seeds)] = 0.0
+    assert bits_equal(V4_lds[cells], want), diff_report(V4_lds[cells], want)
