@@ -77,11 +77,26 @@ def main() -> int:
     ap.add_argument("--converge", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tol", type=float, default=1e-3)
+    ap.add_argument("--max-sweeps", type=int, default=2000)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run a device copy of 1 GiB (256 Mi floats) through torch, the "
+                         "reference point that scales FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the solve is single-GPU "
               f"by design and no multi-GPU path is built", file=sys.stderr)
         return 2
+
+    if args.calibrate_copy:
+        import torch
+        a = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print("# copy calibration: 3 x 1 GiB device-to-device copies", file=sys.stderr, flush=True)
+        del a, b
+        torch.cuda.empty_cache()
 
     pg = load_package()
     t0 = time.perf_counter()
@@ -156,9 +171,19 @@ def main() -> int:
     if args.converge:
         s.fill_seed(prob.seeds)
         t = time.perf_counter()
-        its, last, linf = s.solve(2000, -1.0, args.tol)
+        its, linf = 0, float("inf")
+        for its in range(1, args.max_sweeps + 1):
+            r = s.sweep(its - 1, its % 2 == 1)
+            linf = r.linf
+            if its % 10 == 0:
+                print(f"# converge sweep {its}: linf {linf:.5f}, {time.perf_counter() - t:.1f} s",
+                      file=sys.stderr, flush=True)
+            if linf < args.tol:
+                break
+        V = s.download()
         out["converge"] = {"seconds": time.perf_counter() - t, "sweeps": its, "linf": linf,
-                           "tol": args.tol}
+                           "tol": args.tol, "converged": bool(linf < args.tol),
+                           "reached_frac": float(np.count_nonzero(V < prob.params.cap) / V.size)}
     s.close()
     print(json.dumps(out), flush=True)
     return 0
