@@ -78,6 +78,10 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tol", type=float, default=1e-3)
     ap.add_argument("--max-sweeps", type=int, default=2000)
+    ap.add_argument("--kernel", choices=["auto", "global", "lds", "fast"], default="auto",
+                    help="sweep kernel of the FP32 run (exact: global, lds; contracted: fast)")
+    ap.add_argument("--no-fp16", action="store_true",
+                    help="skip the precision-form (FP16 store) throughput run")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run a device copy of 1 GiB (256 Mi floats) through torch, the "
                          "reference point that scales FETCH_SIZE / WRITE_SIZE counters")
@@ -105,24 +109,33 @@ def main() -> int:
     print(f"# problem {args.config}: grid {g.n} = {g.ncells} states, {prob.nctl} controls, "
           f"occupied {float(prob.occ.mean()):.3f}, built in {time.perf_counter() - t0:.1f} s",
           file=sys.stderr, flush=True)
-    s = pg.HipSolver(g, prob.model, prob.params, prob.ctl, prob.occ)
-    s.fill_seed(prob.seeds)
-    warm_ms = []
-    for i in range(args.warmup):
-        r = s.sweep(i, i % 2 == 1)
-        warm_ms.append(r.kernel_ms)
-        print(f"# warmup sweep {i}: {r.kernel_ms:.1f} ms kernel", file=sys.stderr, flush=True)
-    steps = args.steps
-    if steps is None:
-        per = (warm_ms[-1] / 1e3) if warm_ms else 5.0
-        steps = int(max(3, min(20, 45.0 / max(per, 1e-3))))
-    kms = []
-    t = time.perf_counter()
-    for i in range(steps):
-        r = s.sweep(args.warmup + i, (args.warmup + i) % 2 == 1)
-        kms.append(r.kernel_ms)
-        print(f"# sweep {i}: {r.kernel_ms:.1f} ms kernel, linf {r.linf:.4f}", file=sys.stderr, flush=True)
-    wall = time.perf_counter() - t
+    def timed(vstore, kernel):
+        s = pg.HipSolver(g, prob.model, prob.params, prob.ctl, prob.occ, vstore=vstore)
+        if kernel != "auto":
+            s.set_kernel({"global": pg.abi.PG_KERNEL_GLOBAL, "lds": pg.abi.PG_KERNEL_LDS,
+                          "fast": pg.abi.PG_KERNEL_FAST}[kernel])
+        s.fill_seed(prob.seeds)
+        tag = "fp32" if vstore == pg.abi.PG_VSTORE_F32 else "16bit"
+        warm_ms = []
+        for i in range(args.warmup):
+            r = s.sweep(i, i % 2 == 1)
+            warm_ms.append(r.kernel_ms)
+            print(f"# {tag} warmup sweep {i}: {r.kernel_ms:.1f} ms kernel", file=sys.stderr, flush=True)
+        steps = args.steps
+        if steps is None:
+            per = (warm_ms[-1] / 1e3) if warm_ms else 5.0
+            steps = int(max(3, min(20, 45.0 / max(per, 1e-3))))
+        kms = []
+        t = time.perf_counter()
+        for i in range(steps):
+            r = s.sweep(args.warmup + i, (args.warmup + i) % 2 == 1)
+            kms.append(r.kernel_ms)
+            print(f"# {tag} sweep {i}: {r.kernel_ms:.1f} ms kernel, linf {r.linf:.4f}", file=sys.stderr,
+                  flush=True)
+        wall = time.perf_counter() - t
+        return s, steps, wall, kms
+
+    s, steps, wall, kms = timed(pg.abi.PG_VSTORE_F32, args.kernel)
     ms_per_step = wall / steps * 1e3
     value = g.ncells * steps / wall
     kavg = float(np.mean(kms))
@@ -151,7 +164,7 @@ def main() -> int:
             "obstacles": len(prob.rects),
             "scheme": "fixed dt 0.05 s, forward Euler 5 substeps, 3 swept probes, 64-corner multilinear",
             "parallelism": "single GPU",
-            "kernel": {1: "global-gather", 2: "LDS-staged"}.get(kern, str(kern)),
+            "kernel": {1: "global-gather", 2: "LDS-staged", 3: "LDS-staged, contracted FP32"}.get(kern, str(kern)),
             "window_coverage": coverage,
         },
         "roofline": {
@@ -168,6 +181,20 @@ def main() -> int:
     if not args.no_cpu:
         V = s.download()
         out["cpu_baseline"] = cpu_baseline(pg, prob, V)
+    if not args.no_fp16 and prob.nctl <= 128:
+        s.close()
+        s16, st16, wall16, kms16 = timed(pg.abi.PG_VSTORE_U16_TILE, "auto")
+        v16 = g.ncells * st16 / wall16
+        out["fp16"] = {
+            "value": v16, "unit": "state-updates/s", "steps": st16,
+            "ms_per_step": wall16 / st16 * 1e3, "kernel_ms": float(np.mean(kms16)),
+            "speedup_vs_fp32": v16 / value,
+            "store": "16-bit fixed-point mirror for the gathers (per-4^6-tile FP32 offset/scale), FP32 own values; contracted FP32 backups with the diagonal solved",
+            "roofline_frac": g.ncells * pg.configs.algorithmic_bytes_per_update(prob.nctl, 2)
+                             / (float(np.mean(kms16)) / 1e3) / 1e9 / HBM_PEAK_GBS,
+        }
+        s16.close()
+        s = pg.HipSolver(g, prob.model, prob.params, prob.ctl, prob.occ)
     if args.converge:
         s.fill_seed(prob.seeds)
         t = time.perf_counter()

@@ -83,9 +83,12 @@ typedef struct pg_params {
     int32_t simplex;
 } pg_params;
 
-/* How V is held on the device. */
-#define PG_VSTORE_F32       0   /* plain FP32, Jacobi double buffer          */
-#define PG_VSTORE_F16_TILE  1   /* FP16 with per-tile FP32 scale/offset      */
+/* How V is held on the device. The 16-bit stores hold one code per cell and
+ * an FP32 (offset, scale) per 8^6-cell tile, value = offset + scale * code;
+ * unreached cells (value cap) hold an out-of-band code. */
+#define PG_VSTORE_F32       0   /* plain FP32, Jacobi double buffer             */
+#define PG_VSTORE_F16_TILE  1   /* IEEE binary16 codes, per-tile offset/scale    */
+#define PG_VSTORE_U16_TILE  2   /* 16-bit fixed-point codes, per-tile offset/scale */
 
 typedef struct pg_ctx pg_ctx;
 typedef struct pg_solver pg_solver;
@@ -144,22 +147,38 @@ int pg_sweep(pg_solver* s, int32_t phase, int32_t rev, double* sum_decrease,
 int pg_solve(pg_solver* s, int32_t max_iters, double tol_sum, float tol_linf,
              int32_t* iters, double* last_sum, float* last_linf);
 
+/* Finish a precision-form solve: decode the FP16 store into an FP32 Jacobi
+ * pair (allocated here) and run nsweeps exact FP32 sweeps on it; afterwards
+ * the solver behaves as an FP32 solver. With an FP32 store, runs nsweeps
+ * exact sweeps. No counterpart in the reference (its only FP16 is the card's
+ * output dtype, solver/src/Pipeline.jl:737-746). */
+int pg_polish(pg_solver* s, int32_t nsweeps, double* last_sum, float* last_linf);
+
 /* Argmin control index (0 = coast) of the Bellman backup of the current V,
  * one byte per cell; blocked cells report 0 (PeregrineSolver.jl:1226-1228).
  * Ties resolve to the lowest index, as the strict `<` scan does
  * (PeregrineSolver.jl:1245-1275). nctl must be <= 255. */
 int pg_policy(pg_solver* s, uint8_t* pol_host);
 
-/* Copy n values of the current V (FP32) starting at flat index off. */
+/* Copy n values of the current V (FP32) starting at flat index off. With a
+ * 16-bit store this is the FP32 value every state keeps of its own. */
 int pg_download_V(pg_solver* s, float* dst, int64_t off, int64_t n);
+/* No counterpart in the reference (diagnostics): the 16-bit mirror that the
+ * precision form's gathers read, decoded to FP32, for cells [off, off + n).
+ * PG_ERR_STATE unless a 16-bit store is active. */
+int pg_download_V16(pg_solver* s, float* dst, int64_t off, int64_t n);
 int pg_upload_V(pg_solver* s, const float* src, int64_t off, int64_t n);
 
-/* Sweep kernel selection. AUTO (the default) takes the LDS-staged kernel
- * whenever it applies (at most 128 controls) and the global-gather kernel
- * otherwise; both produce identical bits. */
+/* Sweep kernel selection. GLOBAL and LDS are the exact FP32 sweeps (the
+ * reference's unfused float arithmetic; identical bits). FAST is the
+ * precision form's contracted FP32 sweep (fused multiply-adds, reciprocal
+ * multiplies, hardware sin/cos, tensor-product interpolation). AUTO (the
+ * default) takes LDS for an FP32 store with at most 128 controls, GLOBAL
+ * otherwise; an FP16 store sweeps with FAST until pg_polish. */
 #define PG_KERNEL_AUTO   0
 #define PG_KERNEL_GLOBAL 1
 #define PG_KERNEL_LDS    2
+#define PG_KERNEL_FAST   3
 int pg_solver_set_kernel(pg_solver* s, int32_t kernel);
 
 /* The kernel in use, and for the LDS-staged one the fraction of boxes whose

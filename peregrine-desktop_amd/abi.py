@@ -24,9 +24,11 @@ PG_ERR_STATE = -5
 PG_KERNEL_AUTO = 0
 PG_KERNEL_GLOBAL = 1
 PG_KERNEL_LDS = 2
+PG_KERNEL_FAST = 3
 
 PG_VSTORE_F32 = 0
 PG_VSTORE_F16_TILE = 1
+PG_VSTORE_U16_TILE = 2
 
 
 class PgGrid(ctypes.Structure):
@@ -93,8 +95,10 @@ SIGNATURES = {
                                 ctypes.POINTER(ctypes.c_float)]),
     "pg_solve": (ctypes.c_int, [_P, _I32, ctypes.c_double, ctypes.c_float, ctypes.POINTER(_I32),
                                 ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_float)]),
+    "pg_polish": (ctypes.c_int, [_P, _I32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_float)]),
     "pg_policy": (ctypes.c_int, [_P, _P]),
     "pg_download_V": (ctypes.c_int, [_P, _P, _I64, _I64]),
+    "pg_download_V16": (ctypes.c_int, [_P, _P, _I64, _I64]),
     "pg_upload_V": (ctypes.c_int, [_P, _P, _I64, _I64]),
     "pg_solver_set_kernel": (ctypes.c_int, [_P, _I32]),
     "pg_solver_kernel_info": (ctypes.c_int, [_P, ctypes.POINTER(_I32), ctypes.POINTER(ctypes.c_double)]),

@@ -76,11 +76,22 @@ struct pg_solver {
     pg_model m{};
     pg_params p{};
     pgk::Consts K{};
+    pgk::FastConsts F{};
     int32_t nctl = 0;
     int32_t vstore = 0;
     int64_t ncell = 0;
+    // FP32 tables: the Jacobi pair (FP16 store: [0] is the scratch the fast
+    // sweep writes, [1] exists once polishing starts)
     float* dV[2] = {nullptr, nullptr};
     int cur = 0;
+    // FP16 store
+    bool f16_active = false;      // V lives in the 16-bit codes (until polished)
+    int store = PG_STORE_F32;     // code type of the 16-bit store
+    uint16_t* d_V16 = nullptr;
+    float2* d_tp = nullptr;
+    unsigned* d_tstat = nullptr;
+    uint8_t* d_trefit = nullptr;
+    pgk::QuantGeom Q{};
     uint8_t* d_occ = nullptr;
     float* d_bu = nullptr;
     int64_t* d_seeds = nullptr;
@@ -93,8 +104,7 @@ struct pg_solver {
     uint8_t* d_pol = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     float last_ms = 0.0f;
-    // LDS-staged path
-    int32_t kernel_req = PG_KERNEL_AUTO;
+    // kernel selection and the LDS window plan
     int32_t kernel = PG_KERNEL_GLOBAL;
     bool diag = false;
     pgk::BoxGeom G{};
@@ -106,26 +116,33 @@ struct pg_solver {
 
 namespace {
 
+constexpr int kLdsMaxCtl = 128;
+
 void release(pg_solver* s) {
     if (!s) return;
-    hipFree(s->dV[0]);
-    hipFree(s->dV[1]);
-    hipFree(s->d_occ);
-    hipFree(s->d_bu);
-    hipFree(s->d_seeds);
-    hipFree(s->d_part);
-    hipFree(s->d_sum);
-    hipFree(s->d_linf);
-    hipFree(s->d_pol);
-    hipFree(s->d_worig);
-    if (s->ev0) hipEventDestroy(s->ev0);
-    if (s->ev1) hipEventDestroy(s->ev1);
+    (void)hipFree(s->dV[0]);
+    (void)hipFree(s->dV[1]);
+    (void)hipFree(s->d_V16);
+    (void)hipFree(s->d_tp);
+    (void)hipFree(s->d_tstat);
+    (void)hipFree(s->d_trefit);
+    (void)hipFree(s->d_occ);
+    (void)hipFree(s->d_bu);
+    (void)hipFree(s->d_seeds);
+    (void)hipFree(s->d_part);
+    (void)hipFree(s->d_sum);
+    (void)hipFree(s->d_linf);
+    (void)hipFree(s->d_pol);
+    (void)hipFree(s->d_worig);
+    if (s->ev0) (void)hipEventDestroy(s->ev0);
+    if (s->ev1) (void)hipEventDestroy(s->ev1);
     delete s;
 }
 
 int check_grid(const pg_grid* g) {
     for (int k = 0; k < 6; ++k) {
         if (g->n[k] < 2) return fail(PG_ERR_ARG, "axis %d needs at least 2 samples, got %d", k, g->n[k]);
+        if (g->n[k] > 32000) return fail(PG_ERR_ARG, "axis %d has %d samples (at most 32000)", k, g->n[k]);
         if (!(g->step[k] > 0.0f) || !std::isfinite(g->step[k]) || !std::isfinite(g->lo[k]))
             return fail(PG_ERR_ARG, "axis %d has a non-positive or non-finite step", k);
     }
@@ -160,10 +177,12 @@ void build_consts(pg_solver* s) {
         K.step[k] = g.step[k];
         K.rstep[k] = 1.0 / (double)g.step[k];
         K.lim[k] = (float)(g.n[k] - 1);
+        s->F.rstep[k] = 1.0f / g.step[k];
     }
     K.n3f = (float)g.n[2];
     K.lim[2] = K.n3f;
     K.rn3 = 1.0 / (double)K.n3f;
+    s->F.rn3 = 1.0f / K.n3f;
     K.ncell = s->ncell;
     int64_t st = 1;
     for (int k = 5; k >= 0; --k) { K.stride[k] = st; st *= (int64_t)g.n[k]; }
@@ -172,7 +191,10 @@ void build_consts(pg_solver* s) {
     std::memcpy(K.S, s->m.S, sizeof K.S);
     std::memcpy(K.D, s->m.D, sizeof K.D);
     std::memcpy(K.c, s->m.c, sizeof K.c);
-    for (int k = 0; k < 3; ++k) K.reps[k] = 1.0 / (double)s->m.eps[k];
+    for (int k = 0; k < 3; ++k) {
+        K.reps[k] = 1.0 / (double)s->m.eps[k];
+        s->F.reps[k] = 1.0f / s->m.eps[k];
+    }
     // plain scheme: cfl <= 0, so the horizon is dt (PeregrineSolver.jl:955)
     K.tau = s->p.dt;
     K.nsub = substeps_for(K.tau, s->p);
@@ -181,6 +203,12 @@ void build_consts(pg_solver* s) {
     K.nchk = s->p.checks;
     for (int sc = 1; sc <= K.nchk; ++sc) K.probe_a[sc - 1] = (float)sc / (float)K.nchk;
     K.nctl = s->nctl;
+    int64_t nt = 1;
+    for (int k = 0; k < 6; ++k) {
+        s->Q.nt[k] = (g.n[k] + (1 << pgk::kQuantShift) - 1) >> pgk::kQuantShift;
+        nt *= s->Q.nt[k];
+    }
+    s->Q.ntile = nt;
 }
 
 // A model whose A, S, D blocks are diagonal and q is zero (every other
@@ -196,11 +224,10 @@ bool model_is_diagonal(const pg_model& m) {
     return true;
 }
 
-constexpr int kLdsMaxStates = 128;
-constexpr int kLdsMaxCtl = 128;
-
 void box_geometry(pg_solver* s) {
     const int want[6] = {2, 2, 2, 2, 2, 4};
+    // a box must lie inside one quantization tile (its statistics are per tile)
+    static_assert((1 << pgk::kQuantShift) % 4 == 0, "tile edge must be a multiple of the box edges");
     int64_t nbox = 1;
     for (int k = 0; k < 6; ++k) {
         s->G.b[k] = want[k] < s->g.n[k] ? want[k] : s->g.n[k];
@@ -212,7 +239,14 @@ void box_geometry(pg_solver* s) {
 
 int64_t lds_blocks(const pg_solver* s) { return ((s->G.nbox + 7) / 8) * 8; }
 
-int ensure_partials(pg_solver* s, int64_t nb);
+int ensure_partials(pg_solver* s, int64_t nb) {
+    if (nb <= s->part_cap) return PG_OK;
+    (void)hipFree(s->d_part);
+    s->d_part = nullptr;
+    HIPCHECK(hipMalloc(&s->d_part, (size_t)nb * sizeof(double)));
+    s->part_cap = nb;
+    return PG_OK;
+}
 
 // One-time pass over every (state, control) pair: each box's window origin,
 // and the window shape that holds the most boxes' successors whole.
@@ -228,7 +262,7 @@ int plan(pg_solver* s) {
     std::vector<uint8_t> ext((size_t)nbox * 6);
     if (e == hipSuccess) e = hipMemcpyAsync(ext.data(), d_ext, ext.size(), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    hipFree(d_ext);
+    (void)hipFree(d_ext);
     if (e != hipSuccess) return fail(PG_ERR_HIP, "window plan failed: %s", hipGetErrorString(e));
     const int (*shapes)[6] = nullptr;
     const int nsh = pgk_window_shapes(&shapes);
@@ -247,20 +281,31 @@ int plan(pg_solver* s) {
     return ensure_partials(s, lds_blocks(s));
 }
 
-int ensure_partials(pg_solver* s, int64_t nb) {
-    if (nb <= s->part_cap) return PG_OK;
-    hipFree(s->d_part);
-    s->d_part = nullptr;
-    HIPCHECK(hipMalloc(&s->d_part, (size_t)nb * sizeof(double)));
-    s->part_cap = nb;
+int auto_kernel(const pg_solver* s) {
+    return (s->nctl <= kLdsMaxCtl) ? PG_KERNEL_LDS : PG_KERNEL_GLOBAL;
+}
+
+// FP32 table in dV[0] -> 16-bit mirror. A whole-table load (force) fits
+// every tile afresh; after a sweep, tiles are refit only when their values
+// left the representable range and only changed codes are written.
+int encode_from_scratch(pg_solver* s, bool force) {
+    hipStream_t st = s->ctx->stream;
+    if (force) HIPCHECK(pgk_launch_tile_stats(s->dV[0], s->K, s->Q, s->d_tstat, st));
+    else HIPCHECK(pgk_launch_seed_stats(s->d_seeds, s->nseeds, s->K, s->Q, s->d_tstat, st));
+    HIPCHECK(pgk_launch_tile_refit(s->store, s->d_tstat, s->d_tp, s->d_trefit, s->Q.ntile, force ? 1 : 0, st));
+    HIPCHECK(pgk_launch_encode(s->store, s->dV[0], s->K, s->Q, s->d_tp, s->d_trefit, s->d_V16, st));
     return PG_OK;
 }
+
+// The current FP32 table on the device. With a 16-bit store, dV[0] holds
+// every state's own value in FP32 and the codes mirror it for the gathers.
+float* current_f32(pg_solver* s) { return s->f16_active ? s->dV[0] : s->dV[s->cur]; }
 
 }  // namespace
 
 extern "C" {
 
-const char* pg_version(void) { return "peregrine_hip 0.1 (gfx950, plain FP32 Jacobi)"; }
+const char* pg_version(void) { return "peregrine_hip 0.2 (gfx950: exact FP32 Jacobi sweeps, FP16-tile precision form)"; }
 int pg_abi_version(void) { return PG_ABI_VERSION; }
 const char* pg_last_error(void) { return g_err.c_str(); }
 
@@ -292,8 +337,8 @@ int pg_create(int32_t device, pg_ctx** out) {
 
 int pg_destroy(pg_ctx* ctx) {
     if (!ctx) return PG_OK;
-    hipSetDevice(ctx->device);
-    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return PG_OK;
 }
@@ -315,7 +360,8 @@ int pg_grid_fits(pg_ctx* ctx, const pg_grid* g, int32_t vstore, double headroom,
     HIPCHECK(hipSetDevice(ctx->device));
     size_t fr = 0, tot = 0;
     HIPCHECK(hipMemGetInfo(&fr, &tot));
-    const double per_cell = (vstore == PG_VSTORE_F16_TILE) ? 4.0 : 8.0;  // Jacobi pair
+    // FP32: the Jacobi pair; 16-bit: the codes plus the FP32 sweep scratch
+    const double per_cell = (vstore == PG_VSTORE_F32) ? 8.0 : 6.0;
     const double need = (double)grid_cells(*g) * per_cell / 1073741824.0;
     const double free = (double)fr / 1073741824.0;
     *fits = need <= free * headroom ? 1 : 0;
@@ -337,7 +383,10 @@ int pg_solver_create(pg_ctx* ctx, const pg_grid* g, const pg_model* m, const pg_
         return fail(PG_ERR_ARG, "control 0 must be coast (0, 0, 0)");
     rc = check_params(p, nctl);
     if (rc) return rc;
-    if (vstore != PG_VSTORE_F32) return fail(PG_ERR_UNSUPPORTED, "vstore %d is not built", vstore);
+    if (vstore != PG_VSTORE_F32 && vstore != PG_VSTORE_F16_TILE && vstore != PG_VSTORE_U16_TILE)
+        return fail(PG_ERR_ARG, "unknown vstore %d", vstore);
+    if (vstore != PG_VSTORE_F32 && nctl > kLdsMaxCtl)
+        return fail(PG_ERR_UNSUPPORTED, "the 16-bit stores sweep at most %d controls", kLdsMaxCtl);
     for (int k = 0; k < 3; ++k)
         if (!(m->eps[k] > 0.0f)) return fail(PG_ERR_ARG, "coulomb eps[%d] must be positive", k);
     HIPCHECK(hipSetDevice(ctx->device));
@@ -352,7 +401,9 @@ int pg_solver_create(pg_ctx* ctx, const pg_grid* g, const pg_model* m, const pg_
     s->ncell = grid_cells(*g);
     build_consts(s);
     s->diag = model_is_diagonal(*m);
-    s->kernel = (nctl <= kLdsMaxCtl) ? PG_KERNEL_LDS : PG_KERNEL_GLOBAL;
+    s->store = vstore == PG_VSTORE_F16_TILE ? PG_STORE_F16
+             : vstore == PG_VSTORE_U16_TILE ? PG_STORE_U16 : PG_STORE_F32;
+    s->kernel = (vstore != PG_VSTORE_F32) ? PG_KERNEL_FAST : auto_kernel(s);
 
     // B * u_sat per control, in the reference's summation order
     std::vector<float> bu(3 * (size_t)nctl);
@@ -372,7 +423,14 @@ int pg_solver_create(pg_ctx* ctx, const pg_grid* g, const pg_model* m, const pg_
     hipError_t e = hipSuccess;
     auto chk = [&](hipError_t x) { if (e == hipSuccess) e = x; };
     chk(hipMalloc(&s->dV[0], (size_t)s->ncell * sizeof(float)));
-    chk(hipMalloc(&s->dV[1], (size_t)s->ncell * sizeof(float)));
+    if (vstore == PG_VSTORE_F32) {
+        chk(hipMalloc(&s->dV[1], (size_t)s->ncell * sizeof(float)));
+    } else {
+        chk(hipMalloc(&s->d_V16, (size_t)s->ncell * sizeof(uint16_t)));
+        chk(hipMalloc(&s->d_tp, (size_t)s->Q.ntile * sizeof(float2)));
+        chk(hipMalloc(&s->d_tstat, (size_t)s->Q.ntile * 2 * sizeof(unsigned)));
+        chk(hipMalloc(&s->d_trefit, (size_t)s->Q.ntile));
+    }
     chk(hipMalloc(&s->d_occ, (size_t)nocc));
     chk(hipMalloc(&s->d_bu, bu.size() * sizeof(float)));
     chk(hipMalloc(&s->d_sum, sizeof(double)));
@@ -389,6 +447,13 @@ int pg_solver_create(pg_ctx* ctx, const pg_grid* g, const pg_model* m, const pg_
     chk(hipMemcpy(s->d_bu, bu.data(), bu.size() * sizeof(float), hipMemcpyHostToDevice));
     chk(hipEventCreate(&s->ev0));
     chk(hipEventCreate(&s->ev1));
+    if (vstore != PG_VSTORE_F32) {
+        // tile statistics start empty and every tile unset (refit before any use)
+        chk(hipMemsetAsync(s->d_tp, 0, (size_t)s->Q.ntile * sizeof(float2), ctx->stream));
+        chk(hipMemsetAsync(s->d_tstat, 0, (size_t)s->Q.ntile * 2 * sizeof(unsigned), ctx->stream));
+        chk(pgk_launch_tile_refit(s->store, s->d_tstat, s->d_tp, s->d_trefit, s->Q.ntile, 0, ctx->stream));
+        chk(hipStreamSynchronize(ctx->stream));
+    }
     if (e != hipSuccess) {
         release(s);
         return fail(PG_ERR_HIP, "solver setup failed: %s", hipGetErrorString(e));
@@ -399,8 +464,8 @@ int pg_solver_create(pg_ctx* ctx, const pg_grid* g, const pg_model* m, const pg_
 
 int pg_solver_destroy(pg_solver* s) {
     if (!s) return PG_OK;
-    hipSetDevice(s->ctx->device);
-    hipStreamSynchronize(s->ctx->stream);
+    (void)hipSetDevice(s->ctx->device);
+    (void)hipStreamSynchronize(s->ctx->stream);
     release(s);
     return PG_OK;
 }
@@ -419,7 +484,7 @@ int pg_fill_seed(pg_solver* s, const int64_t* seeds, int64_t nseeds) {
             return fail(PG_ERR_ARG, "seed %lld out of range [0, %lld)", (long long)seeds[i], (long long)s->ncell);
     HIPCHECK(hipSetDevice(s->ctx->device));
     hipStream_t st = s->ctx->stream;
-    hipFree(s->d_seeds);
+    (void)hipFree(s->d_seeds);
     s->d_seeds = nullptr;
     s->nseeds = nseeds;
     if (nseeds > 0) {
@@ -429,6 +494,12 @@ int pg_fill_seed(pg_solver* s, const int64_t* seeds, int64_t nseeds) {
     s->cur = 0;
     HIPCHECK(pgk_launch_fill(s->dV[0], s->ncell, s->p.cap, st));
     HIPCHECK(pgk_launch_seed(s->dV[0], s->d_seeds, s->nseeds, st));
+    if (s->vstore != PG_VSTORE_F32) {
+        s->f16_active = true;
+        s->kernel = PG_KERNEL_FAST;
+        int rc = encode_from_scratch(s, true);
+        if (rc) return rc;
+    }
     HIPCHECK(hipStreamSynchronize(st));
     s->seeded = true;
     return PG_OK;
@@ -441,34 +512,48 @@ int pg_sweep(pg_solver* s, int32_t phase, int32_t rev, double* sum_decrease, flo
     if (!s->seeded) return fail(PG_ERR_STATE, "pg_fill_seed (or pg_upload_V) must come before a sweep");
     HIPCHECK(hipSetDevice(s->ctx->device));
     hipStream_t st = s->ctx->stream;
-    const bool lds = s->kernel == PG_KERNEL_LDS;
-    if (lds) {
+    const int kern = s->f16_active ? PG_KERNEL_FAST : s->kernel;
+    if (kern != PG_KERNEL_GLOBAL) {
         int rc = plan(s);
         if (rc) return rc;
     }
-    const int64_t nb = lds ? lds_blocks(s) : (s->ncell + 255) / 256;
+    const int64_t nb = (kern == PG_KERNEL_GLOBAL) ? (s->ncell + 255) / 256 : lds_blocks(s);
     int rc = ensure_partials(s, nb);
     if (rc) return rc;
     HIPCHECK(hipMemsetAsync(s->d_linf, 0, sizeof(unsigned), st));
-    const float* Vin = s->dV[s->cur];
-    float* Vout = s->dV[1 - s->cur];
+    float* Vout = s->f16_active ? s->dV[0] : s->dV[1 - s->cur];
     int64_t nblk = nb;
     HIPCHECK(hipEventRecord(s->ev0, st));
-    if (lds)
-        HIPCHECK(pgk_launch_sweep_lds(s->shape, s->diag ? 1 : 0, s->K, s->G, Vin, Vout, s->d_occ, s->d_bu,
-                                      s->d_worig, nullptr, s->d_part, s->d_linf, 1, st));
-    else
-        HIPCHECK(pgk_launch_sweep(s->K, Vin, Vout, s->d_occ, s->d_bu, nullptr, s->d_part, s->d_linf, 1, st, &nblk));
+    if (s->f16_active) {
+        HIPCHECK(pgk_launch_sweep_fast(s->shape, s->diag ? 1 : 0, s->store, s->K, s->F, s->G, s->Q,
+                                       s->d_V16, s->d_tp, s->dV[0], Vout, s->d_occ, s->d_bu, s->d_worig,
+                                       nullptr, s->d_part, s->d_linf, s->d_tstat, 1, st));
+    } else if (kern == PG_KERNEL_FAST) {
+        HIPCHECK(pgk_launch_sweep_fast(s->shape, s->diag ? 1 : 0, PG_STORE_F32, s->K, s->F, s->G, s->Q,
+                                       s->dV[s->cur], nullptr, s->dV[s->cur], Vout, s->d_occ, s->d_bu,
+                                       s->d_worig, nullptr, s->d_part, s->d_linf, nullptr, 1, st));
+    } else if (kern == PG_KERNEL_LDS) {
+        HIPCHECK(pgk_launch_sweep_lds(s->shape, s->diag ? 1 : 0, s->K, s->G, s->dV[s->cur], Vout, s->d_occ,
+                                      s->d_bu, s->d_worig, nullptr, s->d_part, s->d_linf, 1, st));
+    } else {
+        HIPCHECK(pgk_launch_sweep(s->K, s->dV[s->cur], Vout, s->d_occ, s->d_bu, nullptr, s->d_part,
+                                  s->d_linf, 1, st, &nblk));
+    }
     HIPCHECK(hipEventRecord(s->ev1, st));
     HIPCHECK(pgk_launch_reduce(s->d_part, nblk, s->d_sum, st));
     HIPCHECK(pgk_launch_seed(Vout, s->d_seeds, s->nseeds, st));
+    if (s->f16_active) {
+        rc = encode_from_scratch(s, false);
+        if (rc) return rc;
+    } else {
+        s->cur = 1 - s->cur;
+    }
     double hsum = 0.0;
     unsigned hl = 0;
     HIPCHECK(hipMemcpyAsync(&hsum, s->d_sum, sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHECK(hipMemcpyAsync(&hl, s->d_linf, sizeof(unsigned), hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
     HIPCHECK(hipEventElapsedTime(&s->last_ms, s->ev0, s->ev1));
-    s->cur = 1 - s->cur;
     float lf;
     std::memcpy(&lf, &hl, 4);
     if (sum_decrease) *sum_decrease = hsum;
@@ -496,6 +581,35 @@ int pg_solve(pg_solver* s, int32_t max_iters, double tol_sum, float tol_linf, in
     return PG_OK;
 }
 
+int pg_polish(pg_solver* s, int32_t nsweeps, double* last_sum, float* last_linf) {
+    if (!s) return fail(PG_ERR_ARG, "null solver");
+    if (!s->seeded) return fail(PG_ERR_STATE, "no value table yet");
+    if (nsweeps < 0) return fail(PG_ERR_ARG, "nsweeps must be >= 0");
+    HIPCHECK(hipSetDevice(s->ctx->device));
+    hipStream_t st = s->ctx->stream;
+    if (s->f16_active) {
+        if (!s->dV[1]) {
+            hipError_t e = hipMalloc(&s->dV[1], (size_t)s->ncell * sizeof(float));
+            if (e != hipSuccess)
+                return fail(e == hipErrorOutOfMemory ? PG_ERR_NOMEM : PG_ERR_HIP,
+                            "polish buffer allocation failed: %s", hipGetErrorString(e));
+        }
+        // dV[0] already holds the table in FP32; the codes are dropped
+        s->cur = 0;
+        s->f16_active = false;
+    }
+    s->kernel = auto_kernel(s);
+    double d = 0.0;
+    float l = 0.0f;
+    for (int32_t i = 0; i < nsweeps; ++i) {
+        int rc = pg_sweep(s, i, i & 1, &d, &l);
+        if (rc) return rc;
+    }
+    if (last_sum) *last_sum = d;
+    if (last_linf) *last_linf = l;
+    return PG_OK;
+}
+
 int pg_policy(pg_solver* s, uint8_t* pol_host) {
     if (!s || !pol_host) return fail(PG_ERR_ARG, "null argument");
     if (!s->seeded) return fail(PG_ERR_STATE, "no value table yet");
@@ -503,9 +617,20 @@ int pg_policy(pg_solver* s, uint8_t* pol_host) {
     hipStream_t st = s->ctx->stream;
     if (!s->d_pol) HIPCHECK(hipMalloc(&s->d_pol, (size_t)s->ncell));
     int64_t nblk = 0;
-    if (s->kernel == PG_KERNEL_LDS) {
+    const int kern = s->f16_active ? PG_KERNEL_FAST : s->kernel;
+    if (kern != PG_KERNEL_GLOBAL) {
         int rc = plan(s);
         if (rc) return rc;
+    }
+    if (s->f16_active) {
+        HIPCHECK(pgk_launch_sweep_fast(s->shape, s->diag ? 1 : 0, s->store, s->K, s->F, s->G, s->Q,
+                                       s->d_V16, s->d_tp, s->dV[0], nullptr, s->d_occ, s->d_bu, s->d_worig,
+                                       s->d_pol, nullptr, nullptr, nullptr, 0, st));
+    } else if (kern == PG_KERNEL_FAST) {
+        HIPCHECK(pgk_launch_sweep_fast(s->shape, s->diag ? 1 : 0, PG_STORE_F32, s->K, s->F, s->G, s->Q,
+                                       s->dV[s->cur], nullptr, s->dV[s->cur], nullptr, s->d_occ, s->d_bu,
+                                       s->d_worig, s->d_pol, nullptr, nullptr, nullptr, 0, st));
+    } else if (kern == PG_KERNEL_LDS) {
         HIPCHECK(pgk_launch_sweep_lds(s->shape, s->diag ? 1 : 0, s->K, s->G, s->dV[s->cur], nullptr, s->d_occ,
                                       s->d_bu, s->d_worig, s->d_pol, nullptr, nullptr, 0, st));
     } else {
@@ -521,8 +646,27 @@ int pg_download_V(pg_solver* s, float* dst, int64_t off, int64_t n) {
     if (off < 0 || n < 0 || off + n > s->ncell) return fail(PG_ERR_ARG, "range [%lld, %lld) outside the grid", (long long)off, (long long)(off + n));
     if (n == 0) return PG_OK;
     HIPCHECK(hipSetDevice(s->ctx->device));
-    HIPCHECK(hipMemcpyAsync(dst, s->dV[s->cur] + off, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, s->ctx->stream));
+    const float* src = current_f32(s);
+    HIPCHECK(hipMemcpyAsync(dst, src + off, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, s->ctx->stream));
     HIPCHECK(hipStreamSynchronize(s->ctx->stream));
+    return PG_OK;
+}
+
+int pg_download_V16(pg_solver* s, float* dst, int64_t off, int64_t n) {
+    if (!s || (!dst && n > 0)) return fail(PG_ERR_ARG, "null argument");
+    if (off < 0 || n < 0 || off + n > s->ncell) return fail(PG_ERR_ARG, "range [%lld, %lld) outside the grid", (long long)off, (long long)(off + n));
+    if (!s->f16_active) return fail(PG_ERR_STATE, "no 16-bit mirror is active (FP32 store, or polished)");
+    if (n == 0) return PG_OK;
+    HIPCHECK(hipSetDevice(s->ctx->device));
+    hipStream_t st = s->ctx->stream;
+    float* tmp = nullptr;
+    hipError_t e = hipMalloc(&tmp, (size_t)n * sizeof(float));
+    if (e != hipSuccess) return fail(PG_ERR_NOMEM, "decode buffer allocation failed: %s", hipGetErrorString(e));
+    e = pgk_launch_decode(s->store, s->d_V16, s->K, s->Q, s->d_tp, off, n, tmp, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(dst, tmp, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return fail(PG_ERR_HIP, "decode failed: %s", hipGetErrorString(e));
     return PG_OK;
 }
 
@@ -530,10 +674,20 @@ int pg_upload_V(pg_solver* s, const float* src, int64_t off, int64_t n) {
     if (!s || (!src && n > 0)) return fail(PG_ERR_ARG, "null argument");
     if (off < 0 || n < 0 || off + n > s->ncell) return fail(PG_ERR_ARG, "range [%lld, %lld) outside the grid", (long long)off, (long long)(off + n));
     HIPCHECK(hipSetDevice(s->ctx->device));
-    if (n > 0) {
-        HIPCHECK(hipMemcpyAsync(s->dV[s->cur] + off, src, (size_t)n * sizeof(float), hipMemcpyHostToDevice, s->ctx->stream));
-        HIPCHECK(hipStreamSynchronize(s->ctx->stream));
+    hipStream_t st = s->ctx->stream;
+    if (s->vstore != PG_VSTORE_F32 && (s->f16_active || !s->seeded)) {
+        if (!s->seeded && (off > 0 || n < s->ncell)) {
+            HIPCHECK(pgk_launch_fill(s->dV[0], s->ncell, s->p.cap, st));
+        }
+        if (n > 0) HIPCHECK(hipMemcpyAsync(s->dV[0] + off, src, (size_t)n * sizeof(float), hipMemcpyHostToDevice, st));
+        s->f16_active = true;
+        s->kernel = PG_KERNEL_FAST;
+        int rc = encode_from_scratch(s, true);
+        if (rc) return rc;
+    } else if (n > 0) {
+        HIPCHECK(hipMemcpyAsync(s->dV[s->cur] + off, src, (size_t)n * sizeof(float), hipMemcpyHostToDevice, st));
     }
+    HIPCHECK(hipStreamSynchronize(st));
     s->seeded = true;
     return PG_OK;
 }
@@ -541,24 +695,24 @@ int pg_upload_V(pg_solver* s, const float* src, int64_t off, int64_t n) {
 int pg_solver_set_kernel(pg_solver* s, int32_t kernel) {
     if (!s) return fail(PG_ERR_ARG, "null solver");
     if (kernel == PG_KERNEL_AUTO) {
-        s->kernel = (s->nctl <= kLdsMaxCtl) ? PG_KERNEL_LDS : PG_KERNEL_GLOBAL;
-    } else if (kernel == PG_KERNEL_GLOBAL) {
-        s->kernel = PG_KERNEL_GLOBAL;
-    } else if (kernel == PG_KERNEL_LDS) {
-        if (s->nctl > kLdsMaxCtl)
-            return fail(PG_ERR_UNSUPPORTED, "the LDS-staged sweep holds at most %d controls", kLdsMaxCtl);
-        s->kernel = PG_KERNEL_LDS;
-    } else {
-        return fail(PG_ERR_ARG, "unknown kernel %d", kernel);
+        s->kernel = s->f16_active ? PG_KERNEL_FAST : auto_kernel(s);
+        return PG_OK;
     }
-    s->kernel_req = kernel;
+    if (kernel != PG_KERNEL_GLOBAL && kernel != PG_KERNEL_LDS && kernel != PG_KERNEL_FAST)
+        return fail(PG_ERR_ARG, "unknown kernel %d", kernel);
+    if (kernel != PG_KERNEL_GLOBAL && s->nctl > kLdsMaxCtl)
+        return fail(PG_ERR_UNSUPPORTED, "the LDS-staged sweeps hold at most %d controls", kLdsMaxCtl);
+    if (s->f16_active && kernel != PG_KERNEL_FAST)
+        return fail(PG_ERR_STATE, "a 16-bit store sweeps with the fast kernel until pg_polish");
+    s->kernel = kernel;
     return PG_OK;
 }
 
 int pg_solver_kernel_info(pg_solver* s, int32_t* kernel, double* window_coverage) {
     if (!s) return fail(PG_ERR_ARG, "null solver");
-    if (kernel) *kernel = s->kernel;
-    if (window_coverage) *window_coverage = (s->kernel == PG_KERNEL_LDS && s->planned) ? s->coverage : 0.0;
+    const int kern = s->f16_active ? PG_KERNEL_FAST : s->kernel;
+    if (kernel) *kernel = kern;
+    if (window_coverage) *window_coverage = (kern != PG_KERNEL_GLOBAL && s->planned) ? s->coverage : 0.0;
     return PG_OK;
 }
 

@@ -379,6 +379,18 @@ __device__ __forceinline__ void fetch_global(const Consts& K, const float* __res
     }
 }
 
+// Quantization tiles of the FP16 store: 8^6 index cells, edge tiles partial.
+#define PG_STORE_F32 0
+#define PG_STORE_F16 1   // IEEE binary16 codes of (v - off) / scale
+#define PG_STORE_U16 2   // 16-bit fixed-point codes of (v - off) / scale
+// Quantization tiles of the 16-bit stores: 2^kQuantShift cells per axis (a
+// multiple of every box edge, so each box lies in one tile).
+constexpr int kQuantShift = 2;
+struct QuantGeom {
+    int32_t nt[6];    // tiles per axis, ceil(n / 2^kQuantShift)
+    int64_t ntile;
+};
+
 // Box decomposition of the grid for the LDS-staged kernels.
 struct BoxGeom {
     int32_t b[6];     // box dims

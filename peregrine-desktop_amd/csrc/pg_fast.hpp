@@ -1,0 +1,156 @@
+// Contracted FP32 arithmetic of the Bellman backup, for the precision form.
+//
+// Same scheme as pg_device.hpp (forward Euler, swept probes, 64-corner
+// multilinear lookup, the reference's rejection rules) evaluated the fast
+// way: fused multiply-adds, reciprocal multiplies in place of divisions by
+// grid constants, the hardware sin/cos, and the multilinear interpolant as a
+// tensor product of 63 lerps (identical weights in exact arithmetic,
+// PeregrineSolver.jl:399-466). Results agree with the exact path to FP32
+// rounding, not bit for bit; the precision form is measured against the
+// FP64 restatement instead.
+#pragma once
+#include "pg_device.hpp"
+
+namespace pgk {
+
+// Launch-uniform reciprocals of the fast path.
+struct FastConsts {
+    float rstep[6];   // 1 / step[k]
+    float rn3;        // 1 / n3
+    float reps[3];    // 1 / eps[k]
+};
+
+#pragma clang fp contract(fast)
+
+__device__ __forceinline__ void sincos_fast(float x, float& s, float& c) {
+    // v_sin_f32 / v_cos_f32 take the angle in revolutions
+    const float r = x * 0.15915494309189535f;
+    s = __builtin_amdgcn_sinf(r);
+    c = __builtin_amdgcn_cosf(r);
+}
+
+template <bool DIAG>
+__device__ __forceinline__ void accel_fast(const Consts& K, const FastConsts& F, float bux, float buy,
+                                           float buw, float vx, float vy, float w, float& ax,
+                                           float& ay, float& al) {
+    const float sx = clamp1(vx * F.reps[0]);
+    const float sy = clamp1(vy * F.reps[1]);
+    const float sw = clamp1(w * F.reps[2]);
+    const float dx = __builtin_fabsf(vx) * vx;
+    const float dy = __builtin_fabsf(vy) * vy;
+    const float dw = __builtin_fabsf(w) * w;
+    if (DIAG) {
+        ax = bux + K.A[0] * vx + K.S[0] * sx + K.D[0] * dx + K.c[0];
+        ay = buy + K.A[4] * vy + K.S[4] * sy + K.D[4] * dy + K.c[1];
+        al = buw + K.A[8] * w + K.S[8] * sw + K.D[8] * dw + K.c[2];
+    } else {
+        const float w2 = w * w;
+        ax = bux + K.A[0] * vx + K.A[1] * vy + K.A[2] * w + K.q[0] * w2 + K.S[0] * sx + K.S[1] * sy +
+             K.S[2] * sw + K.D[0] * dx + K.D[1] * dy + K.D[2] * dw + K.c[0];
+        ay = buy + K.A[3] * vx + K.A[4] * vy + K.A[5] * w + K.q[1] * w2 + K.S[3] * sx + K.S[4] * sy +
+             K.S[5] * sw + K.D[3] * dx + K.D[4] * dy + K.D[5] * dw + K.c[1];
+        al = buw + K.A[6] * vx + K.A[7] * vy + K.A[8] * w + K.q[2] * w2 + K.S[6] * sx + K.S[7] * sy +
+             K.S[8] * sw + K.D[6] * dx + K.D[7] * dy + K.D[8] * dw + K.c[2];
+    }
+}
+
+// Forward Euler, all derivatives from the old state (PeregrineSolver.jl:208-221).
+// Divergence is caught by the caller's finiteness test on the successor.
+template <bool DIAG>
+__device__ __forceinline__ void euler_fast(const Consts& K, const FastConsts& F, const CellStart& c,
+                                           float bux, float buy, float buw, float& x, float& y,
+                                           float& h, float& vx, float& vy, float& w) {
+    const float hs = K.hs;
+    x = c.x; y = c.y; h = c.h; vx = c.vx; vy = c.vy; w = c.w;
+    float sh = c.sh, ch = c.ch;
+    for (int k = 0; k < K.nsub; ++k) {
+        if (k > 0) sincos_fast(h, sh, ch);
+        float ax, ay, al;
+        accel_fast<DIAG>(K, F, bux, buy, buw, vx, vy, w, ax, ay, al);
+        const float dvx = ax + w * vy;
+        const float dvy = ay - w * vx;
+        const float px = ch * vx - sh * vy;
+        const float py = sh * vx + ch * vy;
+        x = x + px * hs;
+        y = y + py * hs;
+        h = h + w * hs;
+        vx = vx + dvx * hs;
+        vy = vy + dvy * hs;
+        w = w + al * hs;
+    }
+}
+
+__device__ __forceinline__ float wrap_fast(const Consts& K, const FastConsts& F, float f) {
+    return f - __builtin_floorf(f * F.rn3) * K.n3f;
+}
+
+template <class OccRead>
+__device__ __forceinline__ bool probes_fast(const Consts& K, const FastConsts& F, const OccRead& occ,
+                                            float x, float y, float h, float dx, float dy, float dh) {
+    for (int sc = 0; sc < K.nchk; ++sc) {
+        const float a = K.probe_a[sc];
+        const float gx = (x + dx * a - K.lo[0]) * F.rstep[0];
+        const float gy = (y + dy * a - K.lo[1]) * F.rstep[1];
+        const float gh = wrap_fast(K, F, (h + dh * a - K.lo[2]) * F.rstep[2]);
+        const bool in = (gx >= 0.0f) & (gy >= 0.0f) & (gx <= K.lim[0]) & (gy <= K.lim[1]) &
+                        (gh >= 0.0f) & (gh <= K.n3f);
+        if (!in) return false;
+        int ck = (int)__builtin_rintf(gh);
+        ck = (ck >= K.n[2]) ? ck - K.n[2] : ck;
+        if (occ((int)__builtin_rintf(gx), (int)__builtin_rintf(gy), ck)) return false;
+    }
+    return true;
+}
+
+template <bool DIAG, class OccRead>
+__device__ __forceinline__ bool successor_fast(const Consts& K, const FastConsts& F,
+                                               const OccRead& og, const CellStart& c, float bux,
+                                               float buy, float buw, float f[6]) {
+    float nx, ny, nh, nvx, nvy, nw;
+    euler_fast<DIAG>(K, F, c, bux, buy, buw, nx, ny, nh, nvx, nvy, nw);
+    const float dx = nx - c.x, dy = ny - c.y, dh = nh - c.h;
+    if (!(finite_f(dx) & finite_f(dy) & finite_f(dh) & finite_f(nvx) & finite_f(nvy) & finite_f(nw)))
+        return false;
+    if (!probes_fast(K, F, og, c.x, c.y, c.h, dx, dy, dh)) return false;
+    float snh, cnh;
+    sincos_fast(nh, snh, cnh);
+    const float nvfx = cnh * nvx - snh * nvy;
+    const float nvfy = snh * nvx + cnh * nvy;
+    f[0] = (nx - K.lo[0]) * F.rstep[0];
+    f[1] = (ny - K.lo[1]) * F.rstep[1];
+    f[2] = wrap_fast(K, F, (nh - K.lo[2]) * F.rstep[2]);
+    f[3] = (nvfx - K.lo[3]) * F.rstep[3];
+    f[4] = (nvfy - K.lo[4]) * F.rstep[4];
+    f[5] = (nw - K.lo[5]) * F.rstep[5];
+    bool in = true;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) in = in & (f[k] >= 0.0f) & (f[k] <= K.lim[k]);
+    return in;
+}
+
+__device__ __forceinline__ float lerp_fast(float a, float b, float t) { return a + t * (b - a); }
+
+// Multilinear interpolation as a tensor product: w first (the corner pairs
+// are adjacent in memory), then vy, vx, h, y, x. vals[d] holds corner d with
+// bit 0 = x ... bit 5 = w.
+__device__ __forceinline__ float tensor_interp(const float t[6], const float vals[64]) {
+    float l5[32];
+#pragma unroll
+    for (int d = 0; d < 32; ++d) l5[d] = lerp_fast(vals[d], vals[d + 32], t[5]);
+    float l4[16];
+#pragma unroll
+    for (int d = 0; d < 16; ++d) l4[d] = lerp_fast(l5[d], l5[d + 16], t[4]);
+    float l3[8];
+#pragma unroll
+    for (int d = 0; d < 8; ++d) l3[d] = lerp_fast(l4[d], l4[d + 8], t[3]);
+    float l2[4];
+#pragma unroll
+    for (int d = 0; d < 4; ++d) l2[d] = lerp_fast(l3[d], l3[d + 4], t[2]);
+    const float l1a = lerp_fast(l2[0], l2[2], t[1]);
+    const float l1b = lerp_fast(l2[1], l2[3], t[1]);
+    return lerp_fast(l1a, l1b, t[0]);
+}
+
+#pragma clang fp contract(off)
+
+}  // namespace pgk

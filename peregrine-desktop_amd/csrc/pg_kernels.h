@@ -2,7 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
-#include "pg_device.hpp"
+#include "pg_fast.hpp"
 
 hipError_t pgk_launch_sweep(const pgk::Consts& K, const float* Vin, float* Vout,
                             const uint8_t* occ, const float* bu, uint8_t* pol, double* part,
@@ -20,3 +20,24 @@ hipError_t pgk_launch_sweep_lds(int shape, int diag, const pgk::Consts& K, const
                                 const float* Vin, float* Vout, const uint8_t* occ, const float* bu,
                                 const int16_t* worig, uint8_t* pol, double* part, unsigned* linf,
                                 int write_v, hipStream_t st);
+
+// Precision form (pg_sweep_fast.hip).
+hipError_t pgk_launch_sweep_fast(int shape, int diag, int store, const pgk::Consts& K,
+                                 const pgk::FastConsts& F, const pgk::BoxGeom& G,
+                                 const pgk::QuantGeom& Q, const void* Vin, const float2* tp,
+                                 const float* vself, float* Vout, const uint8_t* occ, const float* bu,
+                                 const int16_t* worig, uint8_t* pol, double* part, unsigned* linf,
+                                 unsigned* tstat, int write_v, hipStream_t st);
+hipError_t pgk_launch_tile_stats(const float* V, const pgk::Consts& K, const pgk::QuantGeom& Q,
+                                 unsigned* tstat, hipStream_t st);
+hipError_t pgk_launch_seed_stats(const int64_t* seeds, int64_t nseeds, const pgk::Consts& K,
+                                 const pgk::QuantGeom& Q, unsigned* tstat, hipStream_t st);
+// Refit tiles whose reached values left their range (every tile with force).
+hipError_t pgk_launch_tile_refit(int store, unsigned* tstat, float2* tp, uint8_t* trefit, int64_t ntile,
+                                 int force, hipStream_t st);
+// FP32 table -> 16-bit mirror (changed codes and refit tiles only).
+hipError_t pgk_launch_encode(int store, const float* V32, const pgk::Consts& K, const pgk::QuantGeom& Q,
+                             const float2* tp, const uint8_t* trefit, uint16_t* codes, hipStream_t st);
+// Codes of cells [off, off + n) -> V32[0, n).
+hipError_t pgk_launch_decode(int store, const uint16_t* codes, const pgk::Consts& K, const pgk::QuantGeom& Q,
+                             const float2* tp, int64_t off, int64_t n, float* V32, hipStream_t st);

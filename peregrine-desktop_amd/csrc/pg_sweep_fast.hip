@@ -1,0 +1,616 @@
+// Precision-form sweep: contracted FP32 backups over an LDS window, with V
+// held in FP32 or as 16-bit codes with a per-tile FP32 offset/scale.
+//
+// 16-bit stores. The grid is cut into quantization tiles of 4^6 index cells
+// (edge tiles partial). A cell's value is off_t + scale_t * q with q a 16-bit
+// code -- IEEE binary16 (11 significant bits of (v - off)) or fixed point
+// (65,535 uniform levels over the tile's range) -- and unreached cells (value
+// cap) hold an out-of-band code, so the cap never widens a tile's range.
+// A sweep decodes into the FP32 LDS window, writes its FP32 results to a
+// scratch buffer and each box's min/max of reached values into its tile's
+// statistics. A tile's offset/scale are refit only when its values leave the
+// representable range (with headroom below, since values decrease);
+// otherwise only the cells whose value changed are re-encoded, so a settled
+// cell keeps its code bit for bit.
+#include "pg_fast.hpp"
+#include "pg_kernels.h"
+
+namespace pgk {
+
+namespace fastk {
+
+constexpr int kThreads = 256;
+constexpr int kMaxCtl = 128;
+constexpr uint16_t kCapF16 = 0x7c00;   // binary16 +inf
+constexpr uint16_t kCapU16 = 0xffff;
+constexpr float kNcodeF16 = 1000.0f;   // fitted range maps to codes [0, 1000] (< 1024: one binade spacing)
+constexpr float kNcodeU16 = 65534.0f;  // fitted range maps to codes [0, 65534]
+
+template <int STORE>
+struct Codec;
+
+template <>
+struct Codec<PG_STORE_F16> {
+    static constexpr float ncode = kNcodeF16;
+    static __device__ __forceinline__ float decode(uint16_t q, float2 p, float cap) {
+        if (q == kCapF16) return cap;
+        const float v = p.x + p.y * (float)__builtin_bit_cast(_Float16, q);
+        return v < cap ? v : cap;
+    }
+    static __device__ __forceinline__ uint16_t encode(float v, float2 p, float cap) {
+        if (!(v < cap)) return kCapF16;
+        return __builtin_bit_cast(uint16_t, (_Float16)((v - p.x) / p.y));
+    }
+};
+
+template <>
+struct Codec<PG_STORE_U16> {
+    static constexpr float ncode = kNcodeU16;
+    static __device__ __forceinline__ float decode(uint16_t q, float2 p, float cap) {
+        if (q == kCapU16) return cap;
+        const float v = p.x + p.y * (float)q;
+        return v < cap ? v : cap;
+    }
+    static __device__ __forceinline__ uint16_t encode(float v, float2 p, float cap) {
+        if (!(v < cap)) return kCapU16;
+        float u = __builtin_rintf((v - p.x) / p.y);
+        u = u < 0.0f ? 0.0f : (u > kNcodeU16 ? kNcodeU16 : u);
+        return (uint16_t)u;
+    }
+};
+
+__device__ __forceinline__ int64_t tile_of_coords(const QuantGeom& Q, const int* gi) {
+    int64_t t = 0;
+#pragma unroll
+    for (int a = 0; a < 6; ++a) t = t * Q.nt[a] + (gi[a] >> kQuantShift);
+    return t;
+}
+
+template <int E1, int E2, int E3, int E4, int E5, int E6>
+struct Win {
+    static constexpr int E[6] = {E1, E2, E3, E4, E5, E6};
+    static constexpr int S6 = 1;
+    static constexpr int S5 = E6 * S6;
+    static constexpr int S4 = (E5 * S5) | 1;
+    static constexpr int S3 = (E4 * S4) | 1;
+    static constexpr int S2 = (E3 * S3) | 1;
+    static constexpr int S1 = (E2 * S2) | 1;
+    static constexpr int SIZE = E1 * S1;
+    static constexpr int CELLS = E1 * E2 * E3 * E4 * E5 * E6;
+    static constexpr int S[6] = {S1, S2, S3, S4, S5, S6};
+};
+
+__device__ __forceinline__ int64_t box_of_block(const BoxGeom& G) {
+    const int64_t per = (G.nbox + 7) / 8;
+    return (int64_t)(blockIdx.x % 8) * per + (int64_t)(blockIdx.x / 8);
+}
+
+struct BoxInfo {
+    int32_t s0[6];
+    int32_t vb[6];
+    int32_t ns;
+};
+
+__device__ __forceinline__ void box_info(const Consts& K, const BoxGeom& G, int64_t box, BoxInfo& B) {
+    int64_t r = box;
+    int32_t bi[6];
+#pragma unroll
+    for (int k = 5; k >= 1; --k) {
+        bi[k] = (int32_t)(r % G.nb[k]);
+        r /= G.nb[k];
+    }
+    bi[0] = (int32_t)r;
+    B.ns = 1;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        B.s0[k] = bi[k] * G.b[k];
+        const int32_t rem = K.n[k] - B.s0[k];
+        B.vb[k] = rem < G.b[k] ? rem : G.b[k];
+        B.ns *= B.vb[k];
+    }
+}
+
+__device__ __forceinline__ void state_coords(const BoxInfo& B, int s, int i[6]) {
+    int r = s;
+#pragma unroll
+    for (int k = 5; k >= 1; --k) {
+        i[k] = B.s0[k] + r % B.vb[k];
+        r /= B.vb[k];
+    }
+    i[0] = B.s0[0] + r;
+}
+
+struct OccG {
+    const uint8_t* __restrict__ occ;
+    int32_t n2, n3;
+    __device__ __forceinline__ bool operator()(int ci, int cj, int ck) const {
+        return occ[((int64_t)ci * n2 + cj) * n3 + ck] != 0;
+    }
+};
+
+// Reads of V in either store.
+template <int STORE>
+struct VRead {
+    const uint16_t* __restrict__ V;
+    const float2* __restrict__ tp;
+    __device__ __forceinline__ float operator()(const Consts& K, const QuantGeom& Q, const int* gi,
+                                                int64_t fl) const {
+        return Codec<STORE>::decode(V[fl], tp[tile_of_coords(Q, gi)], K.cap);
+    }
+};
+
+template <>
+struct VRead<PG_STORE_F32> {
+    const float* __restrict__ V;
+    __device__ __forceinline__ float operator()(const Consts&, const QuantGeom&, const int*, int64_t fl) const {
+        return V[fl];
+    }
+};
+
+template <int MAXS>
+struct BoxShared {
+    float x[MAXS], y[MAXS], h[MAXS], vx[MAXS], vy[MAXS], w[MAXS];
+    float sh[MAXS], ch[MAXS];
+    float vold[MAXS];          // the state's value before the sweep (FP32)
+    float vmir[MAXS];          // the same value as the gathers see it
+    uint16_t loc[MAXS];        // its coordinates inside the box, 3 bits per axis
+    unsigned long long best[MAXS];
+    uint8_t blocked[MAXS];
+    float bu[3 * kMaxCtl];
+    unsigned bmin, bmax;
+};
+
+}  // namespace fastk
+
+template <class W, bool DIAG, int STORE, int MAXS, bool WRITE_V, bool WRITE_POL>
+__global__ __launch_bounds__(256) void sweep_fast_kernel(
+    fastk::VRead<STORE> vin, const float* vself, float* Vout, const uint8_t* __restrict__ occ,
+    const float* __restrict__ bu_g, Consts K, FastConsts F, BoxGeom G, QuantGeom Q,
+    const int16_t* __restrict__ worig, uint8_t* __restrict__ pol, double* __restrict__ part,
+    unsigned* __restrict__ linf_bits, unsigned* __restrict__ tstat) {
+    using namespace fastk;
+    // the 16-bit stores solve each backup's diagonal (see below); with the
+    // FP32 store the kernel is the contracted twin of the exact backup
+    constexpr bool SELF = STORE != PG_STORE_F32;
+    __shared__ float win[W::SIZE];
+    __shared__ BoxShared<MAXS> sm;
+    const int64_t box = box_of_block(G);
+    const bool live = box < G.nbox;
+    BoxInfo B;
+    int o[6];
+    if (threadIdx.x == 0) { sm.bmin = 0x7f800000u; sm.bmax = 0u; }
+    if (live) {
+        box_info(K, G, box, B);
+#pragma unroll
+        for (int a = 0; a < 6; ++a) o[a] = worig[box * 6 + a];
+        for (int j = threadIdx.x; j < W::CELLS; j += kThreads) {
+            int l[6];
+            int r = j;
+            l[5] = r % W::E[5]; r /= W::E[5];
+            l[4] = r % W::E[4]; r /= W::E[4];
+            l[3] = r % W::E[3]; r /= W::E[3];
+            l[2] = r % W::E[2]; r /= W::E[2];
+            l[1] = r % W::E[1]; r /= W::E[1];
+            l[0] = r;
+            int gi[6];
+            bool on = true;
+#pragma unroll
+            for (int a = 0; a < 6; ++a) {
+                int gv = o[a] + l[a];
+                if (a == 2) {
+                    while (gv >= K.n[2]) gv -= K.n[2];
+                } else {
+                    on = on & (gv >= 0) & (gv < K.n[a]);
+                }
+                gi[a] = gv;
+            }
+            float v = K.cap;
+            if (on) {
+                int64_t fl = 0;
+#pragma unroll
+                for (int a = 0; a < 6; ++a) fl += (int64_t)gi[a] * K.stride[a];
+                v = vin(K, Q, gi, fl);
+            }
+            int lo = 0;
+#pragma unroll
+            for (int a = 0; a < 6; ++a) lo += l[a] * W::S[a];
+            win[lo] = v;
+        }
+        for (int s = threadIdx.x; s < B.ns; s += kThreads) {
+            int i[6];
+            state_coords(B, s, i);
+            CellStart c;
+            cell_start(K, i[0], i[1], i[2], i[3], i[4], i[5], c);
+            sm.x[s] = c.x; sm.y[s] = c.y; sm.h[s] = c.h; sm.vx[s] = c.vx; sm.vy[s] = c.vy; sm.w[s] = c.w;
+            sm.sh[s] = c.sh; sm.ch[s] = c.ch;
+            {
+                int64_t fl = 0;
+                unsigned pk = 0;
+#pragma unroll
+                for (int a = 0; a < 6; ++a) {
+                    fl += (int64_t)i[a] * K.stride[a];
+                    pk |= (unsigned)(i[a] - B.s0[a]) << (3 * a);
+                }
+                sm.vold[s] = vself[fl];
+                sm.vmir[s] = STORE == PG_STORE_F32 ? sm.vold[s] : vin(K, Q, i, fl);
+                sm.loc[s] = (uint16_t)pk;
+            }
+            const bool blk = occ[((int64_t)i[0] * K.n[1] + i[1]) * K.n[2] + i[2]] != 0;
+            sm.blocked[s] = blk ? 1 : 0;
+            sm.best[s] = blk ? ((unsigned long long)__float_as_uint(K.cap) << 32)
+                             : ((unsigned long long)0x7f800000u << 32);
+        }
+        for (int j = threadIdx.x; j < 3 * K.nctl; j += kThreads) sm.bu[j] = bu_g[j];
+    }
+    __syncthreads();
+    if (live) {
+        const OccG og{occ, K.n[1], K.n[2]};
+        const int ns = B.ns;
+        const int total = ns * K.nctl;
+        int s = threadIdx.x % ns, k = threadIdx.x / ns;
+        const int ds = kThreads % ns, dk = kThreads / ns;
+        for (int p = threadIdx.x; p < total; p += kThreads) {
+            if (!sm.blocked[s]) {
+                CellStart c;
+                c.x = sm.x[s]; c.y = sm.y[s]; c.h = sm.h[s]; c.vx = sm.vx[s]; c.vy = sm.vy[s];
+                c.w = sm.w[s]; c.sh = sm.sh[s]; c.ch = sm.ch[s];
+                float f[6];
+                float val = K.cap;
+                if (successor_fast<DIAG>(K, F, og, c, sm.bu[3 * k], sm.bu[3 * k + 1], sm.bu[3 * k + 2], f)) {
+                    float t[6];
+                    int i[6];
+                    int l[6];
+                    bool inwin = true;
+#pragma unroll
+                    for (int a = 0; a < 6; ++a) {
+                        const float b = __builtin_floorf(f[a]);
+                        t[a] = f[a] - b;
+                        i[a] = (int)b;
+                        int d = i[a] - o[a];
+                        if (a == 2) {
+                            d = d < 0 ? d + K.n[2] : d;
+                            d = d >= K.n[2] ? d - K.n[2] : d;
+                        }
+                        l[a] = d;
+                        inwin = inwin & ((unsigned)d <= (unsigned)(W::E[a] - 2));
+                    }
+                    // Weight of the state's own cell in the successor's stencil.
+                    float wself = 0.0f;
+                    if (SELF) {
+                        const unsigned pk = sm.loc[s];
+                        wself = 1.0f;
+#pragma unroll
+                        for (int a = 0; a < 6; ++a) {
+                            int d = B.s0[a] + (int)((pk >> (3 * a)) & 7u) - i[a];
+                            if (a == 2) d = d < 0 ? d + K.n[2] : d;
+                            wself *= d == 0 ? 1.0f - t[a] : (d == 1 ? t[a] : 0.0f);
+                        }
+                    }
+                    float vals[64];
+                    if (inwin) {
+                        int base = 0;
+#pragma unroll
+                        for (int a = 0; a < 6; ++a) base += l[a] * W::S[a];
+                        const float* wp = win + base;
+#pragma unroll
+                        for (int d = 0; d < 64; ++d) {
+                            const int off = ((d >> 0) & 1) * W::S1 + ((d >> 1) & 1) * W::S2 +
+                                            ((d >> 2) & 1) * W::S3 + ((d >> 3) & 1) * W::S4 +
+                                            ((d >> 4) & 1) * W::S5 + ((d >> 5) & 1) * W::S6;
+                            vals[d] = wp[off];
+                        }
+                    } else {
+#pragma unroll
+                        for (int d = 0; d < 64; ++d) {
+                            int gi[6];
+                            int64_t fl = 0;
+#pragma unroll
+                            for (int a = 0; a < 6; ++a) {
+                                int j = i[a] + ((d >> a) & 1);
+                                if (a == 2) {
+                                    j = j >= K.n[2] ? j - K.n[2] : j;
+                                    j = j >= K.n[2] ? j - K.n[2] : j;
+                                } else {
+                                    j = j < K.n[a] - 1 ? j : K.n[a] - 1;
+                                }
+                                gi[a] = j;
+                                fl += (int64_t)j * K.stride[a];
+                            }
+                            vals[d] = vin(K, Q, gi, fl);
+                        }
+                    }
+                    const float acc = tensor_interp(t, vals);
+                    if (SELF && wself > 0.0f) {
+                        // V = tau + w V + rest solved for V (Jacobi on the
+                        // policy system with its diagonal): same fixed point
+                        // as tau + min(acc, cap), without the 1/(1-w)-times
+                        // slower approach through the cell's own value.
+                        const float rest = acc - wself * sm.vmir[s];
+                        const float cc = (K.tau + rest) * __builtin_amdgcn_rcpf(1.0f - wself);
+                        val = cc < K.cap ? cc : K.cap;
+                    } else {
+                        const float nv = acc < K.cap ? acc : K.cap;
+                        const float cc = K.tau + nv;
+                        val = cc < K.cap ? cc : K.cap;
+                    }
+                }
+                const unsigned long long key = ((unsigned long long)__float_as_uint(val) << 32) | (unsigned)k;
+                atomicMin(&sm.best[s], key);
+            }
+            s += ds; k += dk;
+            if (s >= ns) { s -= ns; ++k; }
+        }
+    }
+    __syncthreads();
+    double dsum = 0.0;
+    float dmax = 0.0f;
+    unsigned lmin = 0x7f800000u, lmax = 0u;
+    if (live) {
+        for (int s = threadIdx.x; s < B.ns; s += kThreads) {
+            int i[6];
+            state_coords(B, s, i);
+            int64_t idx = 0;
+#pragma unroll
+            for (int a = 0; a < 6; ++a) idx += (int64_t)i[a] * K.stride[a];
+            const unsigned long long key = sm.best[s];
+            const float best = __uint_as_float((unsigned)(key >> 32));
+            if (WRITE_V) {
+                const float old = sm.vold[s];
+                float out = old;
+                if (best < old) {
+                    out = best;
+                    const float d = old - best;
+                    dsum += (double)d;
+                    dmax = fmaxf(dmax, d);
+                }
+                Vout[idx] = out;
+                if (out < K.cap) {
+                    lmin = min(lmin, __float_as_uint(out));
+                    lmax = max(lmax, __float_as_uint(out));
+                }
+            }
+            if (WRITE_POL) pol[idx] = (uint8_t)(key & 0xffffffffu);
+        }
+    }
+    if (WRITE_V) {
+        if (STORE != PG_STORE_F32) {
+            atomicMin(&sm.bmin, lmin);
+            atomicMax(&sm.bmax, lmax);
+        }
+        block_reduce_sum_max(dsum, dmax, part, linf_bits);
+        if (STORE != PG_STORE_F32 && live && threadIdx.x == 0 && sm.bmin <= sm.bmax) {
+            int64_t t = 0;
+#pragma unroll
+            for (int a = 0; a < 6; ++a) t = t * Q.nt[a] + (B.s0[a] >> kQuantShift);
+            atomicMin(&tstat[2 * t], sm.bmin);
+            atomicMax(&tstat[2 * t + 1], sm.bmax);
+        }
+    }
+}
+
+// The seeds are reset to 0 after the sweep kernel took its statistics; fold
+// that 0 into their tiles so that every seed stays exactly on the code grid.
+__global__ __launch_bounds__(256) void seed_stats_kernel(const int64_t* __restrict__ seeds, int64_t nseeds,
+                                                         Consts K, QuantGeom Q, unsigned* __restrict__ tstat) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nseeds) return;
+    int64_t r = seeds[j], t = 0, mul = 1;
+    for (int a = 5; a >= 0; --a) {
+        const int i = (int)(r % K.n[a]);
+        r /= K.n[a];
+        t += (int64_t)(i >> kQuantShift) * mul;
+        mul *= Q.nt[a];
+    }
+    atomicMin(&tstat[2 * t], 0u);
+}
+
+// Per-tile statistics of an FP32 table (used when a table is loaded whole).
+__global__ __launch_bounds__(256) void tile_stats_kernel(const float* __restrict__ V, Consts K,
+                                                         QuantGeom Q, unsigned* __restrict__ tstat) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= K.ncell) return;
+    const float v = V[idx];
+    if (!(v < K.cap)) return;
+    int64_t r = idx, t = 0, mul = 1;
+    for (int a = 5; a >= 0; --a) {
+        const int i = (int)(r % K.n[a]);
+        r /= K.n[a];
+        t += (int64_t)(i >> kQuantShift) * mul;
+        mul *= Q.nt[a];
+    }
+    atomicMin(&tstat[2 * t], __float_as_uint(v));
+    atomicMax(&tstat[2 * t + 1], __float_as_uint(v));
+}
+
+// Refit each tile's offset/scale when its reached values leave the current
+// representable range [off, off + ncode*scale] (or always, with force), and
+// flag the refit tiles; clear the statistics. A refit leaves headroom of a
+// quarter of the span below the minimum, since values only decrease.
+__global__ __launch_bounds__(256) void tile_refit_kernel(unsigned* __restrict__ tstat,
+                                                         float2* __restrict__ tp,
+                                                         uint8_t* __restrict__ trefit, int64_t ntile,
+                                                         float ncode, int force) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntile) return;
+    const unsigned mnb = tstat[2 * t], mxb = tstat[2 * t + 1];
+    tstat[2 * t] = 0x7f800000u;
+    tstat[2 * t + 1] = 0u;
+    uint8_t flag = 0;
+    if (mnb <= mxb) {
+        const float mn = __uint_as_float(mnb), mx = __uint_as_float(mxb);
+        const float2 p = tp[t];
+        const bool unset = !(p.y > 0.0f);
+        const float top = p.x + p.y * ncode;
+        if (force || unset || mn < p.x || mx > top) {
+            const float span = mx - mn;
+            // values are >= 0, so the headroom stops at 0 (a seed's 0 stays
+            // exactly on the code grid)
+            float off, scale;
+            if (span > 0.0f) {
+                off = fmaxf(mn - 0.25f * span, 0.0f);
+                scale = (mx - off) / ncode;
+            } else {
+                const float mag = fmaxf(fabsf(mn), 1.0f);
+                off = fmaxf(mn - mag * (1.0f / 1024.0f), 0.0f);
+                scale = (mn + mag * (1.0f / 1024.0f) - off) / ncode;
+            }
+            tp[t] = make_float2(off, scale);
+            flag = 1;
+        }
+    } else if (force) {
+        tp[t] = make_float2(0.0f, 0.0f);   // nothing reached: unset
+        flag = 1;
+    }
+    trefit[t] = flag;
+}
+
+// FP32 table -> codes. In a refit tile every cell is encoded afresh; in
+// any other tile only a cell whose code changed is written.
+template <int STORE>
+__global__ __launch_bounds__(256) void encode_kernel(const float* __restrict__ V32, Consts K,
+                                                     QuantGeom Q, const float2* __restrict__ tp,
+                                                     const uint8_t* __restrict__ trefit,
+                                                     uint16_t* __restrict__ codes) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= K.ncell) return;
+    int64_t r = idx;
+    int i[6];
+    for (int a = 5; a >= 0; --a) {
+        i[a] = (int)(r % K.n[a]);
+        r /= K.n[a];
+    }
+    const int64_t t = fastk::tile_of_coords(Q, i);
+    const uint16_t qn = fastk::Codec<STORE>::encode(V32[idx], tp[t], K.cap);
+    if (trefit[t] || qn != codes[idx]) codes[idx] = qn;
+}
+
+// Codes -> FP32 table.
+template <int STORE>
+__global__ __launch_bounds__(256) void decode_kernel(const uint16_t* __restrict__ codes, Consts K,
+                                                     QuantGeom Q, const float2* __restrict__ tp,
+                                                     int64_t off, int64_t n, float* __restrict__ V32) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const int64_t idx = off + j;
+    int64_t r = idx;
+    int i[6];
+    for (int a = 5; a >= 0; --a) {
+        i[a] = (int)(r % K.n[a]);
+        r /= K.n[a];
+    }
+    V32[j] = fastk::Codec<STORE>::decode(codes[idx], tp[fastk::tile_of_coords(Q, i)], K.cap);
+}
+
+}  // namespace pgk
+
+namespace {
+
+template <class W, bool DIAG, int STORE>
+hipError_t launch_fast_w(const pgk::Consts& K, const pgk::FastConsts& F, const pgk::BoxGeom& G,
+                         const pgk::QuantGeom& Q, const void* Vin, const float2* tp, const float* vself, float* Vout,
+                         const uint8_t* occ, const float* bu, const int16_t* worig, uint8_t* pol,
+                         double* part, unsigned* linf, unsigned* tstat, int write_v, hipStream_t st) {
+    const unsigned nb = (unsigned)(((G.nbox + 7) / 8) * 8);
+    pgk::fastk::VRead<STORE> rd;
+    if constexpr (STORE == PG_STORE_F32) {
+        rd.V = (const float*)Vin;
+    } else {
+        rd.V = (const uint16_t*)Vin;
+        rd.tp = tp;
+    }
+    if (write_v && pol)
+        hipLaunchKernelGGL((pgk::sweep_fast_kernel<W, DIAG, STORE, 128, true, true>), dim3(nb), dim3(256), 0, st,
+                           rd, vself, Vout, occ, bu, K, F, G, Q, worig, pol, part, linf, tstat);
+    else if (write_v)
+        hipLaunchKernelGGL((pgk::sweep_fast_kernel<W, DIAG, STORE, 128, true, false>), dim3(nb), dim3(256), 0, st,
+                           rd, vself, Vout, occ, bu, K, F, G, Q, worig, pol, part, linf, tstat);
+    else
+        hipLaunchKernelGGL((pgk::sweep_fast_kernel<W, DIAG, STORE, 128, false, true>), dim3(nb), dim3(256), 0, st,
+                           rd, vself, Vout, occ, bu, K, F, G, Q, worig, pol, part, linf, tstat);
+    return hipGetLastError();
+}
+
+template <bool DIAG, int STORE>
+hipError_t launch_fast_shape(int shape, const pgk::Consts& K, const pgk::FastConsts& F,
+                             const pgk::BoxGeom& G, const pgk::QuantGeom& Q, const void* Vin,
+                             const float2* tp, const float* vself, float* Vout, const uint8_t* occ, const float* bu,
+                             const int16_t* worig, uint8_t* pol, double* part, unsigned* linf,
+                             unsigned* tstat, int write_v, hipStream_t st) {
+    using W0 = pgk::fastk::Win<4, 4, 4, 4, 4, 6>;
+    using W1 = pgk::fastk::Win<4, 4, 4, 4, 4, 4>;
+    using W2 = pgk::fastk::Win<4, 4, 4, 5, 5, 5>;
+    switch (shape) {
+        case 0: return launch_fast_w<W0, DIAG, STORE>(K, F, G, Q, Vin, tp, vself, Vout, occ, bu, worig, pol, part, linf, tstat, write_v, st);
+        case 1: return launch_fast_w<W1, DIAG, STORE>(K, F, G, Q, Vin, tp, vself, Vout, occ, bu, worig, pol, part, linf, tstat, write_v, st);
+        case 2: return launch_fast_w<W2, DIAG, STORE>(K, F, G, Q, Vin, tp, vself, Vout, occ, bu, worig, pol, part, linf, tstat, write_v, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace
+
+hipError_t pgk_launch_sweep_fast(int shape, int diag, int store, const pgk::Consts& K,
+                                 const pgk::FastConsts& F, const pgk::BoxGeom& G,
+                                 const pgk::QuantGeom& Q, const void* Vin, const float2* tp,
+                                 const float* vself, float* Vout, const uint8_t* occ, const float* bu,
+                                 const int16_t* worig, uint8_t* pol, double* part, unsigned* linf,
+                                 unsigned* tstat, int write_v, hipStream_t st) {
+    switch (store * 2 + (diag ? 1 : 0)) {
+        case PG_STORE_F32 * 2 + 0: return launch_fast_shape<false, PG_STORE_F32>(shape, K, F, G, Q, Vin, tp, vself, Vout, occ, bu, worig, pol, part, linf, tstat, write_v, st);
+        case PG_STORE_F32 * 2 + 1: return launch_fast_shape<true, PG_STORE_F32>(shape, K, F, G, Q, Vin, tp, vself, Vout, occ, bu, worig, pol, part, linf, tstat, write_v, st);
+        case PG_STORE_F16 * 2 + 0: return launch_fast_shape<false, PG_STORE_F16>(shape, K, F, G, Q, Vin, tp, vself, Vout, occ, bu, worig, pol, part, linf, tstat, write_v, st);
+        case PG_STORE_F16 * 2 + 1: return launch_fast_shape<true, PG_STORE_F16>(shape, K, F, G, Q, Vin, tp, vself, Vout, occ, bu, worig, pol, part, linf, tstat, write_v, st);
+        case PG_STORE_U16 * 2 + 0: return launch_fast_shape<false, PG_STORE_U16>(shape, K, F, G, Q, Vin, tp, vself, Vout, occ, bu, worig, pol, part, linf, tstat, write_v, st);
+        case PG_STORE_U16 * 2 + 1: return launch_fast_shape<true, PG_STORE_U16>(shape, K, F, G, Q, Vin, tp, vself, Vout, occ, bu, worig, pol, part, linf, tstat, write_v, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t pgk_launch_tile_stats(const float* V, const pgk::Consts& K, const pgk::QuantGeom& Q,
+                                 unsigned* tstat, hipStream_t st) {
+    const int64_t nb = (K.ncell + 255) / 256;
+    hipLaunchKernelGGL(pgk::tile_stats_kernel, dim3((unsigned)nb), dim3(256), 0, st, V, K, Q, tstat);
+    return hipGetLastError();
+}
+
+hipError_t pgk_launch_seed_stats(const int64_t* seeds, int64_t nseeds, const pgk::Consts& K,
+                                 const pgk::QuantGeom& Q, unsigned* tstat, hipStream_t st) {
+    if (nseeds <= 0) return hipSuccess;
+    const int64_t nb = (nseeds + 255) / 256;
+    hipLaunchKernelGGL(pgk::seed_stats_kernel, dim3((unsigned)nb), dim3(256), 0, st, seeds, nseeds, K, Q, tstat);
+    return hipGetLastError();
+}
+
+hipError_t pgk_launch_tile_refit(int store, unsigned* tstat, float2* tp, uint8_t* trefit, int64_t ntile,
+                                 int force, hipStream_t st) {
+    const int64_t nb = (ntile + 255) / 256;
+    const float ncode = store == PG_STORE_U16 ? pgk::fastk::kNcodeU16 : pgk::fastk::kNcodeF16;
+    hipLaunchKernelGGL(pgk::tile_refit_kernel, dim3((unsigned)nb), dim3(256), 0, st, tstat, tp, trefit,
+                       ntile, ncode, force);
+    return hipGetLastError();
+}
+
+hipError_t pgk_launch_encode(int store, const float* V32, const pgk::Consts& K, const pgk::QuantGeom& Q,
+                             const float2* tp, const uint8_t* trefit, uint16_t* codes, hipStream_t st) {
+    const int64_t nb = (K.ncell + 255) / 256;
+    if (store == PG_STORE_U16)
+        hipLaunchKernelGGL(pgk::encode_kernel<PG_STORE_U16>, dim3((unsigned)nb), dim3(256), 0, st, V32, K, Q, tp,
+                           trefit, codes);
+    else
+        hipLaunchKernelGGL(pgk::encode_kernel<PG_STORE_F16>, dim3((unsigned)nb), dim3(256), 0, st, V32, K, Q, tp,
+                           trefit, codes);
+    return hipGetLastError();
+}
+
+hipError_t pgk_launch_decode(int store, const uint16_t* codes, const pgk::Consts& K, const pgk::QuantGeom& Q,
+                             const float2* tp, int64_t off, int64_t n, float* V32, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    const int64_t nb = (n + 255) / 256;
+    if (store == PG_STORE_U16)
+        hipLaunchKernelGGL(pgk::decode_kernel<PG_STORE_U16>, dim3((unsigned)nb), dim3(256), 0, st, codes, K, Q, tp,
+                           off, n, V32);
+    else
+        hipLaunchKernelGGL(pgk::decode_kernel<PG_STORE_F16>, dim3((unsigned)nb), dim3(256), 0, st, codes, K, Q, tp,
+                           off, n, V32);
+    return hipGetLastError();
+}

@@ -164,6 +164,13 @@ class HipSolver:
                                     ctypes.byref(d), ctypes.byref(l)))
         return it.value, d.value, l.value
 
+    def polish(self, nsweeps: int):
+        """Decode an FP16 store to FP32 and run `nsweeps` exact sweeps."""
+        d = ctypes.c_double()
+        l = ctypes.c_float()
+        abi.check(self.lib, "pg_polish", self.lib.pg_polish(self._h, int(nsweeps), ctypes.byref(d), ctypes.byref(l)))
+        return d.value, l.value
+
     def policy(self) -> np.ndarray:
         out = np.empty(self.ncells, dtype=np.uint8)
         abi.check(self.lib, "pg_policy", self.lib.pg_policy(self._h, out.ctypes.data))
@@ -173,6 +180,13 @@ class HipSolver:
         n = self.ncells - off if n is None else n
         out = np.empty(n, dtype=np.float32)
         abi.check(self.lib, "pg_download_V", self.lib.pg_download_V(self._h, out.ctypes.data, off, n))
+        return out
+
+    def download_mirror(self, off: int = 0, n: int | None = None) -> np.ndarray:
+        """The 16-bit mirror the precision form's gathers read, decoded."""
+        n = self.ncells - off if n is None else n
+        out = np.empty(n, dtype=np.float32)
+        abi.check(self.lib, "pg_download_V16", self.lib.pg_download_V16(self._h, out.ctypes.data, off, n))
         return out
 
     def upload(self, V: np.ndarray, off: int = 0) -> None:
