@@ -192,6 +192,34 @@ int pg_last_sweep_ms(pg_solver* s, float* ms);
 /* Number of cells of the solver's grid. */
 int pg_solver_ncells(pg_solver* s, int64_t* n);
 
+/* FP64 tables. No counterpart in the reference, whose tables are Float32
+ * only: these are the FP64 yardstick the FP32 and 16-bit tables are measured
+ * against, and the FP64 residual check of a finished table. The arithmetic
+ * is the plain scheme carried in double (grid values, forward Euler, probes,
+ * 64-corner multilinear lookup, PeregrineSolver.jl:208-223, 399-466,
+ * 1100-1153); two extra FP64 tables live on the device once used.
+ *
+ * pg_solve_f64 starts from cap with the pg_fill_seed targets at 0 and runs
+ * FP64 sweeps until ||dV||_inf < tol_linf or max_iters: PLAIN sweeps are
+ * V' = min(V, T V); DIAGONAL sweeps solve each control's own-cell term of the
+ * interpolation (V = tau + w V + rest  =>  V = (tau + rest) / (1 - w)),
+ * which has T's fixed point and reaches it in fewer sweeps. At most 128
+ * controls. */
+#define PG_F64_SWEEP_PLAIN    0
+#define PG_F64_SWEEP_DIAGONAL 1
+int pg_solve_f64(pg_solver* s, int32_t mode, int32_t max_iters, double tol_linf, int32_t* iters,
+                 double* last_linf);
+/* max and sum over states of |T V - V| with T one FP64 backup, over the
+ * states where T V or V is below cap; which = 0: the FP64 table, 1: the
+ * current FP32 table (widened to double). */
+int pg_residual_f64(pg_solver* s, int32_t which, double* linf, double* sum);
+int pg_download_V64(pg_solver* s, double* dst, int64_t off, int64_t n);
+/* argmin control of one FP64 backup of the FP64 table (ties within 2^-44
+ * relative resolve to the lowest index); blocked cells report 0. */
+int pg_policy_f64(pg_solver* s, uint8_t* pol_host);
+/* Frees the FP64 tables. */
+int pg_release_f64(pg_solver* s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -29,6 +29,8 @@ PG_KERNEL_FAST = 3
 PG_VSTORE_F32 = 0
 PG_VSTORE_F16_TILE = 1
 PG_VSTORE_U16_TILE = 2
+PG_F64_SWEEP_PLAIN = 0
+PG_F64_SWEEP_DIAGONAL = 1
 
 
 class PgGrid(ctypes.Structure):
@@ -99,6 +101,13 @@ SIGNATURES = {
     "pg_policy": (ctypes.c_int, [_P, _P]),
     "pg_download_V": (ctypes.c_int, [_P, _P, _I64, _I64]),
     "pg_download_V16": (ctypes.c_int, [_P, _P, _I64, _I64]),
+    "pg_solve_f64": (ctypes.c_int, [_P, _I32, _I32, ctypes.c_double, ctypes.POINTER(_I32),
+                                    ctypes.POINTER(ctypes.c_double)]),
+    "pg_residual_f64": (ctypes.c_int, [_P, _I32, ctypes.POINTER(ctypes.c_double),
+                                       ctypes.POINTER(ctypes.c_double)]),
+    "pg_download_V64": (ctypes.c_int, [_P, _P, _I64, _I64]),
+    "pg_policy_f64": (ctypes.c_int, [_P, _P]),
+    "pg_release_f64": (ctypes.c_int, [_P]),
     "pg_upload_V": (ctypes.c_int, [_P, _P, _I64, _I64]),
     "pg_solver_set_kernel": (ctypes.c_int, [_P, _I32]),
     "pg_solver_kernel_info": (ctypes.c_int, [_P, ctypes.POINTER(_I32), ctypes.POINTER(ctypes.c_double)]),

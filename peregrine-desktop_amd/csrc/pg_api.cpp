@@ -63,6 +63,25 @@ int32_t substeps_for(float tau, const pg_params& p) {
     return k;
 }
 
+// traction_gain and substeps_for carried in double, as the FP64 tables are.
+double traction_gain64(double u1, double u2, double u3, float knee) {
+    if (std::isnan(knee) || knee <= 0.0f) return 1.0;
+    const double m = std::sqrt(u1 * u1 + u2 * u2 + u3 * u3);
+    const double r = m / (double)knee;
+    if (r < (double)1.0e-6f) return 1.0;
+    return std::tanh(r) / r;
+}
+
+int32_t substeps_for64(double tau, const pg_params& p) {
+    if (p.hmax <= 0.0f) return p.nsub;
+    double q = tau / (double)p.hmax;
+    if (!(q < (double)1.0e4f)) q = (double)1.0e4f;
+    int32_t k = (int32_t)q + 1;
+    if (k > 64) k = 64;
+    if (k < 1) k = 1;
+    return k;
+}
+
 }  // namespace
 
 struct pg_ctx {
@@ -112,6 +131,15 @@ struct pg_solver {
     bool planned = false;
     int shape = 0;
     double coverage = 0.0;
+    // FP64 tables (allocated on first use)
+    pgk::Consts64 K64{};
+    std::vector<double> bu64;
+    double* d_bu64 = nullptr;
+    double* dV64[2] = {nullptr, nullptr};
+    uint32_t* d_seedmask = nullptr;   // one bit per cell: the pg_fill_seed targets
+    bool seedmask_ok = false;
+    int cur64 = 0;
+    bool f64_valid = false;       // dV64[cur64] holds an FP64 table
 };
 
 namespace {
@@ -124,6 +152,10 @@ void release(pg_solver* s) {
     (void)hipFree(s->dV[1]);
     (void)hipFree(s->d_V16);
     (void)hipFree(s->d_tp);
+    (void)hipFree(s->d_bu64);
+    (void)hipFree(s->dV64[0]);
+    (void)hipFree(s->dV64[1]);
+    (void)hipFree(s->d_seedmask);
     (void)hipFree(s->d_tstat);
     (void)hipFree(s->d_trefit);
     (void)hipFree(s->d_occ);
@@ -209,6 +241,32 @@ void build_consts(pg_solver* s) {
         nt *= s->Q.nt[k];
     }
     s->Q.ntile = nt;
+
+    pgk::Consts64& L = s->K64;
+    std::memset(&L, 0, sizeof L);
+    for (int k = 0; k < 6; ++k) {
+        L.n[k] = g.n[k];
+        L.lo[k] = (double)g.lo[k];
+        L.step[k] = (double)g.step[k];
+        L.lim[k] = (double)(float)(g.n[k] - 1);
+        L.stride[k] = K.stride[k];
+    }
+    L.n3 = (double)(float)g.n[2];
+    L.lim[2] = L.n3;
+    L.ncell = s->ncell;
+    for (int j = 0; j < 9; ++j) {
+        L.A[j] = s->m.A[j]; L.S[j] = s->m.S[j]; L.D[j] = s->m.D[j];
+    }
+    for (int j = 0; j < 3; ++j) {
+        L.q[j] = s->m.q[j]; L.c[j] = s->m.c[j]; L.eps[j] = s->m.eps[j];
+    }
+    L.tau = (double)s->p.dt;
+    L.nsub = substeps_for64(L.tau, s->p);
+    L.hs = L.tau / (double)(float)L.nsub;
+    L.cap = (double)s->p.cap;
+    L.nchk = s->p.checks;
+    for (int sc = 1; sc <= L.nchk; ++sc) L.probe_a[sc - 1] = (double)(float)sc / (double)(float)L.nchk;
+    L.nctl = s->nctl;
 }
 
 // A model whose A, S, D blocks are diagonal and q is zero (every other
@@ -407,6 +465,7 @@ int pg_solver_create(pg_ctx* ctx, const pg_grid* g, const pg_model* m, const pg_
 
     // B * u_sat per control, in the reference's summation order
     std::vector<float> bu(3 * (size_t)nctl);
+    s->bu64.assign(3 * (size_t)nctl, 0.0);
     for (int k = 0; k < nctl; ++k) {
         const float u1r = ctl_soa[k], u2r = ctl_soa[k + nctl], u3r = ctl_soa[k + 2 * nctl];
         const float gn = traction_gain(u1r, u2r, u3r, m->knee);
@@ -416,6 +475,15 @@ int pg_solver_create(pg_ctx* ctx, const pg_grid* g, const pg_model* m, const pg_
             a = a + m->B[3 * r + 1] * u2;
             a = a + m->B[3 * r + 2] * u3;
             bu[3 * k + r] = a;
+        }
+        // the same in double for the FP64 tables
+        const double g64 = traction_gain64(u1r, u2r, u3r, m->knee);
+        const double w1 = u1r * g64, w2 = u2r * g64, w3 = u3r * g64;
+        for (int r = 0; r < 3; ++r) {
+            double a = (double)m->B[3 * r] * w1;
+            a = a + (double)m->B[3 * r + 1] * w2;
+            a = a + (double)m->B[3 * r + 2] * w3;
+            s->bu64[3 * k + r] = a;
         }
     }
 
@@ -487,6 +555,7 @@ int pg_fill_seed(pg_solver* s, const int64_t* seeds, int64_t nseeds) {
     (void)hipFree(s->d_seeds);
     s->d_seeds = nullptr;
     s->nseeds = nseeds;
+    s->seedmask_ok = false;
     if (nseeds > 0) {
         HIPCHECK(hipMalloc(&s->d_seeds, (size_t)nseeds * sizeof(int64_t)));
         HIPCHECK(hipMemcpyAsync(s->d_seeds, seeds, (size_t)nseeds * sizeof(int64_t), hipMemcpyHostToDevice, st));
@@ -689,6 +758,150 @@ int pg_upload_V(pg_solver* s, const float* src, int64_t off, int64_t n) {
     }
     HIPCHECK(hipStreamSynchronize(st));
     s->seeded = true;
+    return PG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 tables
+// ---------------------------------------------------------------------------
+
+namespace {
+
+int f64_setup(pg_solver* s) {
+    if (s->nctl > kLdsMaxCtl)
+        return fail(PG_ERR_UNSUPPORTED, "the FP64 sweeps hold at most %d controls", kLdsMaxCtl);
+    int rc = plan(s);
+    if (rc) return rc;
+    hipError_t e = hipSuccess;
+    auto chk = [&](hipError_t x) { if (e == hipSuccess) e = x; };
+    if (!s->d_bu64) {
+        chk(hipMalloc(&s->d_bu64, s->bu64.size() * sizeof(double)));
+        chk(hipMemcpy(s->d_bu64, s->bu64.data(), s->bu64.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    for (int j = 0; j < 2; ++j)
+        if (!s->dV64[j]) chk(hipMalloc(&s->dV64[j], (size_t)s->ncell * sizeof(double)));
+    const size_t mask_words = (size_t)((s->ncell + 31) / 32);
+    if (!s->d_seedmask) chk(hipMalloc(&s->d_seedmask, mask_words * sizeof(uint32_t)));
+    if (e == hipSuccess && !s->seedmask_ok) {
+        chk(hipMemsetAsync(s->d_seedmask, 0, mask_words * sizeof(uint32_t), s->ctx->stream));
+        chk(pgk_launch_seed_mask(s->d_seedmask, s->d_seeds, s->nseeds, s->ctx->stream));
+        if (e == hipSuccess) s->seedmask_ok = true;
+    }
+    if (e != hipSuccess)
+        return fail(e == hipErrorOutOfMemory ? PG_ERR_NOMEM : PG_ERR_HIP, "FP64 table allocation failed: %s",
+                    hipGetErrorString(e));
+    return PG_OK;
+}
+
+// One FP64 sweep (or residual pass) of Vin; results in *linf / *sum.
+int f64_pass(pg_solver* s, int mode, const double* Vin, double* Vout, uint8_t* pol, float* linf, double* sum) {
+    hipStream_t st = s->ctx->stream;
+    HIPCHECK(hipMemsetAsync(s->d_linf, 0, sizeof(unsigned), st));
+    HIPCHECK(pgk_launch_sweep_f64(mode, s->shape, s->K64, s->G, Vin, Vout, s->d_occ, s->d_bu64, s->d_worig,
+                                  s->d_seedmask, pol, s->d_part, s->d_linf, st));
+    HIPCHECK(pgk_launch_reduce(s->d_part, lds_blocks(s), s->d_sum, st));
+    unsigned hl = 0;
+    double hs = 0.0;
+    HIPCHECK(hipMemcpyAsync(&hs, s->d_sum, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(&hl, s->d_linf, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    std::memcpy(linf, &hl, 4);
+    *sum = hs;
+    return PG_OK;
+}
+
+}  // namespace
+
+int pg_solve_f64(pg_solver* s, int32_t mode, int32_t max_iters, double tol_linf, int32_t* iters,
+                 double* last_linf) {
+    if (!s) return fail(PG_ERR_ARG, "null solver");
+    if (mode != PG_F64_SWEEP_PLAIN && mode != PG_F64_SWEEP_DIAGONAL) return fail(PG_ERR_ARG, "unknown FP64 sweep mode %d", mode);
+    if (!s->seeded || (s->nseeds > 0 && !s->d_seeds)) return fail(PG_ERR_STATE, "pg_fill_seed must come first");
+    HIPCHECK(hipSetDevice(s->ctx->device));
+    int rc = f64_setup(s);
+    if (rc) return rc;
+    hipStream_t st = s->ctx->stream;
+    HIPCHECK(pgk_launch_fill_f64(s->dV64[0], s->ncell, s->K64.cap, st));
+    HIPCHECK(pgk_launch_seed_f64(s->dV64[0], s->d_seeds, s->nseeds, st));
+    s->cur64 = 0;
+    s->f64_valid = true;
+    const int kmode = mode == PG_F64_SWEEP_DIAGONAL ? PG_F64_DIAGONAL : PG_F64_PLAIN;
+    float l = INFINITY;
+    double d = 0.0;
+    int32_t it = 0;
+    for (it = 1; it <= max_iters; ++it) {
+        double* in = s->dV64[s->cur64];
+        double* out = s->dV64[1 - s->cur64];
+        rc = f64_pass(s, kmode, in, out, nullptr, &l, &d);
+        if (rc) return rc;
+        HIPCHECK(pgk_launch_seed_f64(out, s->d_seeds, s->nseeds, st));
+        s->cur64 = 1 - s->cur64;
+        if ((double)l < tol_linf) break;
+    }
+    if (it > max_iters) it = max_iters;
+    if (iters) *iters = it;
+    if (last_linf) *last_linf = l;
+    return PG_OK;
+}
+
+int pg_residual_f64(pg_solver* s, int32_t which, double* linf, double* sum) {
+    if (!s) return fail(PG_ERR_ARG, "null solver");
+    if (which != 0 && which != 1) return fail(PG_ERR_ARG, "which must be 0 (FP64 table) or 1 (current table)");
+    if (which == 0 && !s->f64_valid) return fail(PG_ERR_STATE, "no FP64 table yet (pg_solve_f64)");
+    if (which == 1 && !s->seeded) return fail(PG_ERR_STATE, "no value table yet");
+    HIPCHECK(hipSetDevice(s->ctx->device));
+    int rc = f64_setup(s);
+    if (rc) return rc;
+    const double* V = s->dV64[s->cur64];
+    if (which == 1) {
+        double* tmp = s->dV64[s->f64_valid ? 1 - s->cur64 : 0];
+        HIPCHECK(pgk_launch_widen(current_f32(s), tmp, s->ncell, s->ctx->stream));
+        V = tmp;
+    }
+    float l = 0.0f;
+    double d = 0.0;
+    rc = f64_pass(s, PG_F64_RESIDUAL, V, nullptr, nullptr, &l, &d);
+    if (rc) return rc;
+    if (linf) *linf = l;
+    if (sum) *sum = d;
+    return PG_OK;
+}
+
+int pg_download_V64(pg_solver* s, double* dst, int64_t off, int64_t n) {
+    if (!s || (!dst && n > 0)) return fail(PG_ERR_ARG, "null argument");
+    if (off < 0 || n < 0 || off + n > s->ncell) return fail(PG_ERR_ARG, "range [%lld, %lld) outside the grid", (long long)off, (long long)(off + n));
+    if (!s->f64_valid) return fail(PG_ERR_STATE, "no FP64 table yet (pg_solve_f64)");
+    if (n == 0) return PG_OK;
+    HIPCHECK(hipSetDevice(s->ctx->device));
+    HIPCHECK(hipMemcpyAsync(dst, s->dV64[s->cur64] + off, (size_t)n * sizeof(double), hipMemcpyDeviceToHost,
+                            s->ctx->stream));
+    HIPCHECK(hipStreamSynchronize(s->ctx->stream));
+    return PG_OK;
+}
+
+int pg_policy_f64(pg_solver* s, uint8_t* pol_host) {
+    if (!s || !pol_host) return fail(PG_ERR_ARG, "null argument");
+    if (!s->f64_valid) return fail(PG_ERR_STATE, "no FP64 table yet (pg_solve_f64)");
+    HIPCHECK(hipSetDevice(s->ctx->device));
+    int rc = f64_setup(s);
+    if (rc) return rc;
+    if (!s->d_pol) HIPCHECK(hipMalloc(&s->d_pol, (size_t)s->ncell));
+    float l = 0.0f;
+    double d = 0.0;
+    rc = f64_pass(s, PG_F64_RESIDUAL, s->dV64[s->cur64], nullptr, s->d_pol, &l, &d);
+    if (rc) return rc;
+    HIPCHECK(hipMemcpy(pol_host, s->d_pol, (size_t)s->ncell, hipMemcpyDeviceToHost));
+    return PG_OK;
+}
+
+int pg_release_f64(pg_solver* s) {
+    if (!s) return fail(PG_ERR_ARG, "null solver");
+    HIPCHECK(hipSetDevice(s->ctx->device));
+    for (int j = 0; j < 2; ++j) {
+        (void)hipFree(s->dV64[j]);
+        s->dV64[j] = nullptr;
+    }
+    s->f64_valid = false;
     return PG_OK;
 }
 

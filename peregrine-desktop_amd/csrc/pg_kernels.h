@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include "pg_fast.hpp"
+#include "pg_f64.hpp"
 
 hipError_t pgk_launch_sweep(const pgk::Consts& K, const float* Vin, float* Vout,
                             const uint8_t* occ, const float* bu, uint8_t* pol, double* part,
@@ -41,3 +42,17 @@ hipError_t pgk_launch_encode(int store, const float* V32, const pgk::Consts& K, 
 // Codes of cells [off, off + n) -> V32[0, n).
 hipError_t pgk_launch_decode(int store, const uint16_t* codes, const pgk::Consts& K, const pgk::QuantGeom& Q,
                              const float2* tp, int64_t off, int64_t n, float* V32, hipStream_t st);
+
+// FP64 sweeps (pg_sweep_f64.hip).
+#define PG_F64_PLAIN 0      // V' = min(V, T V)
+#define PG_F64_DIAGONAL 1   // V' = min(V, C V), each control's own-cell term solved
+#define PG_F64_RESIDUAL 2   // no write: |T V - V| where either is below cap
+hipError_t pgk_launch_sweep_f64(int mode, int shape, const pgk::Consts64& K, const pgk::BoxGeom& G,
+                                const double* Vin, double* Vout, const uint8_t* occ, const double* bu,
+                                const int16_t* worig, const uint32_t* seedmask, uint8_t* pol, double* part,
+                                unsigned* linf, hipStream_t st);
+// Sets bit seeds[i] of mask (one bit per cell; the mask must start cleared).
+hipError_t pgk_launch_seed_mask(uint32_t* mask, const int64_t* seeds, int64_t n, hipStream_t st);
+hipError_t pgk_launch_fill_f64(double* V, int64_t n, double v, hipStream_t st);
+hipError_t pgk_launch_seed_f64(double* V, const int64_t* seeds, int64_t n, hipStream_t st);
+hipError_t pgk_launch_widen(const float* V32, double* V64, int64_t n, hipStream_t st);

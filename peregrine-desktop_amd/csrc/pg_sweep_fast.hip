@@ -12,6 +12,7 @@
 // representable range (with headroom below, since values decrease);
 // otherwise only the cells whose value changed are re-encoded, so a settled
 // cell keeps its code bit for bit.
+#include "pg_box.hpp"
 #include "pg_fast.hpp"
 #include "pg_kernels.h"
 
@@ -19,7 +20,6 @@ namespace pgk {
 
 namespace fastk {
 
-constexpr int kThreads = 256;
 constexpr int kMaxCtl = 128;
 constexpr uint16_t kCapF16 = 0x7c00;   // binary16 +inf
 constexpr uint16_t kCapU16 = 0xffff;
@@ -65,68 +65,6 @@ __device__ __forceinline__ int64_t tile_of_coords(const QuantGeom& Q, const int*
     for (int a = 0; a < 6; ++a) t = t * Q.nt[a] + (gi[a] >> kQuantShift);
     return t;
 }
-
-template <int E1, int E2, int E3, int E4, int E5, int E6>
-struct Win {
-    static constexpr int E[6] = {E1, E2, E3, E4, E5, E6};
-    static constexpr int S6 = 1;
-    static constexpr int S5 = E6 * S6;
-    static constexpr int S4 = (E5 * S5) | 1;
-    static constexpr int S3 = (E4 * S4) | 1;
-    static constexpr int S2 = (E3 * S3) | 1;
-    static constexpr int S1 = (E2 * S2) | 1;
-    static constexpr int SIZE = E1 * S1;
-    static constexpr int CELLS = E1 * E2 * E3 * E4 * E5 * E6;
-    static constexpr int S[6] = {S1, S2, S3, S4, S5, S6};
-};
-
-__device__ __forceinline__ int64_t box_of_block(const BoxGeom& G) {
-    const int64_t per = (G.nbox + 7) / 8;
-    return (int64_t)(blockIdx.x % 8) * per + (int64_t)(blockIdx.x / 8);
-}
-
-struct BoxInfo {
-    int32_t s0[6];
-    int32_t vb[6];
-    int32_t ns;
-};
-
-__device__ __forceinline__ void box_info(const Consts& K, const BoxGeom& G, int64_t box, BoxInfo& B) {
-    int64_t r = box;
-    int32_t bi[6];
-#pragma unroll
-    for (int k = 5; k >= 1; --k) {
-        bi[k] = (int32_t)(r % G.nb[k]);
-        r /= G.nb[k];
-    }
-    bi[0] = (int32_t)r;
-    B.ns = 1;
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-        B.s0[k] = bi[k] * G.b[k];
-        const int32_t rem = K.n[k] - B.s0[k];
-        B.vb[k] = rem < G.b[k] ? rem : G.b[k];
-        B.ns *= B.vb[k];
-    }
-}
-
-__device__ __forceinline__ void state_coords(const BoxInfo& B, int s, int i[6]) {
-    int r = s;
-#pragma unroll
-    for (int k = 5; k >= 1; --k) {
-        i[k] = B.s0[k] + r % B.vb[k];
-        r /= B.vb[k];
-    }
-    i[0] = B.s0[0] + r;
-}
-
-struct OccG {
-    const uint8_t* __restrict__ occ;
-    int32_t n2, n3;
-    __device__ __forceinline__ bool operator()(int ci, int cj, int ck) const {
-        return occ[((int64_t)ci * n2 + cj) * n3 + ck] != 0;
-    }
-};
 
 // Reads of V in either store.
 template <int STORE>
@@ -180,30 +118,12 @@ __global__ __launch_bounds__(256) void sweep_fast_kernel(
     int o[6];
     if (threadIdx.x == 0) { sm.bmin = 0x7f800000u; sm.bmax = 0u; }
     if (live) {
-        box_info(K, G, box, B);
+        box_info(K.n, G, box, B);
 #pragma unroll
         for (int a = 0; a < 6; ++a) o[a] = worig[box * 6 + a];
         for (int j = threadIdx.x; j < W::CELLS; j += kThreads) {
-            int l[6];
-            int r = j;
-            l[5] = r % W::E[5]; r /= W::E[5];
-            l[4] = r % W::E[4]; r /= W::E[4];
-            l[3] = r % W::E[3]; r /= W::E[3];
-            l[2] = r % W::E[2]; r /= W::E[2];
-            l[1] = r % W::E[1]; r /= W::E[1];
-            l[0] = r;
-            int gi[6];
-            bool on = true;
-#pragma unroll
-            for (int a = 0; a < 6; ++a) {
-                int gv = o[a] + l[a];
-                if (a == 2) {
-                    while (gv >= K.n[2]) gv -= K.n[2];
-                } else {
-                    on = on & (gv >= 0) & (gv < K.n[a]);
-                }
-                gi[a] = gv;
-            }
+            int l[6], gi[6];
+            const bool on = window_cell<W>(K.n, o, j, l, gi);
             float v = K.cap;
             if (on) {
                 int64_t fl = 0;
@@ -536,9 +456,9 @@ hipError_t launch_fast_shape(int shape, const pgk::Consts& K, const pgk::FastCon
                              const float2* tp, const float* vself, float* Vout, const uint8_t* occ, const float* bu,
                              const int16_t* worig, uint8_t* pol, double* part, unsigned* linf,
                              unsigned* tstat, int write_v, hipStream_t st) {
-    using W0 = pgk::fastk::Win<4, 4, 4, 4, 4, 6>;
-    using W1 = pgk::fastk::Win<4, 4, 4, 4, 4, 4>;
-    using W2 = pgk::fastk::Win<4, 4, 4, 5, 5, 5>;
+    using W0 = pgk::Win0;
+    using W1 = pgk::Win1;
+    using W2 = pgk::Win2;
     switch (shape) {
         case 0: return launch_fast_w<W0, DIAG, STORE>(K, F, G, Q, Vin, tp, vself, Vout, occ, bu, worig, pol, part, linf, tstat, write_v, st);
         case 1: return launch_fast_w<W1, DIAG, STORE>(K, F, G, Q, Vin, tp, vself, Vout, occ, bu, worig, pol, part, linf, tstat, write_v, st);

@@ -1,91 +1,19 @@
 // LDS-staged Bellman sweep for gfx950.
 //
-// The grid is cut into boxes of states (b1 x ... x b6). One workgroup owns a
-// box: it stages the window of V that the box's successors can read -- a
-// 6-D block of E1 x ... x E6 cells whose origin comes from a one-time plan
-// pass -- into LDS, precomputes each state's starting terms, then spreads
+// One workgroup owns a box of states (pg_box.hpp): it stages the window of V
+// that the box's successors can read into LDS, precomputes each state's starting terms, then spreads
 // the box's (state, control) pairs over its 256 threads. Every 64-corner
 // gather inside the window is 64 ds_read_b32 with immediate offsets from one
 // base address; a successor outside the window (rare: the plan sizes the
 // window per box) reads global memory with the reference's clamped indices.
 // The arithmetic is pg_device.hpp's, so the results equal the global-path
 // kernel's, bit for bit.
-#include "pg_device.hpp"
+#include "pg_box.hpp"
 #include "pg_kernels.h"
 
 namespace pgk {
 
-// Padded LDS strides of an E1..E6 window, omega fastest. Each stride is
-// above omega's rows is rounded up to an odd number of dwords so that
-// neighbouring rows start in different banks.
-template <int E1, int E2, int E3, int E4, int E5, int E6>
-struct Win {
-    static constexpr int E[6] = {E1, E2, E3, E4, E5, E6};
-    static constexpr int S6 = 1;
-    static constexpr int S5 = E6 * S6;
-    static constexpr int S4 = (E5 * S5) | 1;
-    static constexpr int S3 = (E4 * S4) | 1;
-    static constexpr int S2 = (E3 * S3) | 1;
-    static constexpr int S1 = (E2 * S2) | 1;
-    static constexpr int SIZE = E1 * S1;
-    static constexpr int CELLS = E1 * E2 * E3 * E4 * E5 * E6;
-    static constexpr int S[6] = {S1, S2, S3, S4, S5, S6};
-};
-
-constexpr int kThreads = 256;
 constexpr int kMaxCtlLds = 128;   // control table held in LDS (nctl <= 128)
-
-// The box a workgroup owns, with XCD-contiguous numbering: consecutive
-// workgroup ids land on different XCDs, so box ranges are dealt out so that
-// each XCD walks neighbouring boxes (whose windows overlap) through its L2.
-__device__ __forceinline__ int64_t box_of_block(const BoxGeom& G) {
-    const int64_t per = (G.nbox + 7) / 8;
-    return (int64_t)(blockIdx.x % 8) * per + (int64_t)(blockIdx.x / 8);
-}
-
-struct BoxInfo {
-    int32_t s0[6];   // first state of the box, per axis
-    int32_t vb[6];   // states of the box per axis (edge boxes are partial)
-    int32_t ns;
-};
-
-__device__ __forceinline__ void box_info(const Consts& K, const BoxGeom& G, int64_t box,
-                                         BoxInfo& B) {
-    int64_t r = box;
-    int32_t bi[6];
-#pragma unroll
-    for (int k = 5; k >= 1; --k) {
-        bi[k] = (int32_t)(r % G.nb[k]);
-        r /= G.nb[k];
-    }
-    bi[0] = (int32_t)r;
-    B.ns = 1;
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-        B.s0[k] = bi[k] * G.b[k];
-        const int32_t rem = K.n[k] - B.s0[k];
-        B.vb[k] = rem < G.b[k] ? rem : G.b[k];
-        B.ns *= B.vb[k];
-    }
-}
-
-__device__ __forceinline__ void state_coords(const BoxInfo& B, int s, int i[6]) {
-    int r = s;
-#pragma unroll
-    for (int k = 5; k >= 1; --k) {
-        i[k] = B.s0[k] + r % B.vb[k];
-        r /= B.vb[k];
-    }
-    i[0] = B.s0[0] + r;
-}
-
-struct OccG {
-    const uint8_t* __restrict__ occ;
-    int32_t n2, n3;
-    __device__ __forceinline__ bool operator()(int ci, int cj, int ck) const {
-        return occ[((int64_t)ci * n2 + cj) * n3 + ck] != 0;
-    }
-};
 
 // Per-box shared state: starting terms of each state, the running
 // (value, control) minimum per state, and the control table.
@@ -160,7 +88,7 @@ __global__ __launch_bounds__(256) void plan_kernel(const uint8_t* __restrict__ o
     __shared__ BoxShared<MAXS> sm;
     __shared__ int mn[6], mx[6];
     BoxInfo B;
-    box_info(K, G, box, B);
+    box_info(K.n, G, box, B);
     if (threadIdx.x < 6) { mn[threadIdx.x] = 1 << 30; mx[threadIdx.x] = -(1 << 30); }
     setup_box<MAXS>(K, B, occ, bu_g, sm);
     __syncthreads();
@@ -233,31 +161,13 @@ __global__ __launch_bounds__(256) void sweep_lds_kernel(
     BoxInfo B;
     int o[6];
     if (live) {
-        box_info(K, G, box, B);
+        box_info(K.n, G, box, B);
 #pragma unroll
         for (int a = 0; a < 6; ++a) o[a] = worig[box * 6 + a];
         // stage the window; cells off the grid hold cap (read only with zero weight)
         for (int j = threadIdx.x; j < W::CELLS; j += kThreads) {
-            int l[6];
-            int r = j;
-            l[5] = r % W::E[5]; r /= W::E[5];
-            l[4] = r % W::E[4]; r /= W::E[4];
-            l[3] = r % W::E[3]; r /= W::E[3];
-            l[2] = r % W::E[2]; r /= W::E[2];
-            l[1] = r % W::E[1]; r /= W::E[1];
-            l[0] = r;
-            int gi[6];
-            bool on = true;
-#pragma unroll
-            for (int a = 0; a < 6; ++a) {
-                int gv = o[a] + l[a];
-                if (a == 2) {
-                    while (gv >= K.n[2]) gv -= K.n[2];
-                } else {
-                    on = on & (gv >= 0) & (gv < K.n[a]);
-                }
-                gi[a] = gv;
-            }
+            int l[6], gi[6];
+            const bool on = window_cell<W>(K.n, o, j, l, gi);
             float v = K.cap;
             if (on) {
                 int64_t fl = 0;
@@ -425,9 +335,9 @@ hipError_t pgk_launch_sweep_lds(int shape, int diag, const pgk::Consts& K, const
                                 const float* Vin, float* Vout, const uint8_t* occ, const float* bu,
                                 const int16_t* worig, uint8_t* pol, double* part, unsigned* linf,
                                 int write_v, hipStream_t st) {
-    using W0 = pgk::Win<4, 4, 4, 4, 4, 6>;
-    using W1 = pgk::Win<4, 4, 4, 4, 4, 4>;
-    using W2 = pgk::Win<4, 4, 4, 5, 5, 5>;
+    using W0 = pgk::Win0;
+    using W1 = pgk::Win1;
+    using W2 = pgk::Win2;
     switch (shape * 2 + (diag ? 1 : 0)) {
         case 0: return launch_w<W0, false, 128>(K, G, Vin, Vout, occ, bu, worig, pol, part, linf, write_v, st);
         case 1: return launch_w<W0, true, 128>(K, G, Vin, Vout, occ, bu, worig, pol, part, linf, write_v, st);

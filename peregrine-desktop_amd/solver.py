@@ -189,6 +189,39 @@ class HipSolver:
         abi.check(self.lib, "pg_download_V16", self.lib.pg_download_V16(self._h, out.ctypes.data, off, n))
         return out
 
+    # FP64 tables ---------------------------------------------------------
+    def solve_f64(self, iters: int, tol_linf: float, diagonal: bool = True):
+        """FP64 table from cap and the seeds; returns (sweeps, last ||dV||_inf)."""
+        it = ctypes.c_int32()
+        l = ctypes.c_double()
+        mode = abi.PG_F64_SWEEP_DIAGONAL if diagonal else abi.PG_F64_SWEEP_PLAIN
+        abi.check(self.lib, "pg_solve_f64", self.lib.pg_solve_f64(self._h, mode, int(iters), float(tol_linf),
+                                                                   ctypes.byref(it), ctypes.byref(l)))
+        return it.value, l.value
+
+    def residual_f64(self, which: int = 0):
+        """(max, sum) of |T V - V| with T one FP64 backup; which 0 = FP64
+        table, 1 = the current FP32 table."""
+        l = ctypes.c_double()
+        d = ctypes.c_double()
+        abi.check(self.lib, "pg_residual_f64", self.lib.pg_residual_f64(self._h, int(which), ctypes.byref(l),
+                                                                         ctypes.byref(d)))
+        return l.value, d.value
+
+    def download_f64(self, off: int = 0, n: int | None = None) -> np.ndarray:
+        n = self.ncells - off if n is None else n
+        out = np.empty(n, dtype=np.float64)
+        abi.check(self.lib, "pg_download_V64", self.lib.pg_download_V64(self._h, out.ctypes.data, off, n))
+        return out
+
+    def policy_f64(self) -> np.ndarray:
+        out = np.empty(self.ncells, dtype=np.uint8)
+        abi.check(self.lib, "pg_policy_f64", self.lib.pg_policy_f64(self._h, out.ctypes.data))
+        return out
+
+    def release_f64(self) -> None:
+        abi.check(self.lib, "pg_release_f64", self.lib.pg_release_f64(self._h))
+
     def upload(self, V: np.ndarray, off: int = 0) -> None:
         V = np.ascontiguousarray(np.asarray(V, dtype=np.float32))
         abi.check(self.lib, "pg_upload_V", self.lib.pg_upload_V(self._h, V.ctypes.data, off, V.size))
