@@ -8,10 +8,17 @@ dominant kernel's roofline figure and a CPU baseline measured on a bounded
 sample of the same workload.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 2]
-                    [--converge] [--no-cpu]
+                    [--converge] [--accuracy] [--no-cpu]
 
---converge additionally times a full solve from V = cap to the first sweep
-with ||dV||_inf < 1e-3 s (reported under "converge").
+--converge additionally times, for the plain FP32 form and for the
+precision form, a full solve from V = cap to the first sweep with
+||dV||_inf < 1e-3 s (the precision form then runs its 3 exact FP32 polish
+sweeps); reported under "converge". --accuracy also solves the FP64 table of
+the same problem and reports both forms' errors against it on a seeded
+1M-state subsample, with argmin-policy agreement (under "accuracy").
+--f64-only solves just the FP64 table and writes the subsample (indices,
+FP64 values, FP64 policy) to --f64-out; --f64-sample reads such a file in
+place of the FP64 solve.
 """
 from __future__ import annotations
 
@@ -68,6 +75,107 @@ def cpu_baseline(pg, prob, V, seconds_target: float = 15.0):
                       f"against the post-warmup V; {dt:.1f} s"}
 
 
+def solve_timed(pg, prob, vstore, args, tag):
+    """From V = cap (targets 0) to the first sweep with ||dV||_inf < tol;
+    wall time of the sweeps (host loop included), the sweep count, and the
+    table and policy at the end (after the polish sweeps for a 16-bit store)."""
+    g = prob.grid
+    s = pg.HipSolver(g, prob.model, prob.params, prob.ctl, prob.occ, vstore=vstore)
+    s.fill_seed(prob.seeds)
+    t = time.perf_counter()
+    its, linf = 0, float("inf")
+    for its in range(1, args.max_sweeps + 1):
+        r = s.sweep(its - 1, its % 2 == 1)
+        linf = r.linf
+        if its % 10 == 0:
+            print(f"# converge {tag} sweep {its}: linf {linf:.5f}, {time.perf_counter() - t:.1f} s",
+                  file=sys.stderr, flush=True)
+        if linf < args.tol:
+            break
+    t_conv = time.perf_counter() - t
+    res = {"seconds": t_conv, "sweeps": its, "linf": linf, "tol": args.tol, "converged": bool(linf < args.tol)}
+    if vstore != pg.abi.PG_VSTORE_F32:
+        t = time.perf_counter()
+        _, lp = s.polish(args.polish)
+        res["polish_sweeps"] = args.polish
+        res["polish_seconds"] = time.perf_counter() - t
+        res["polish_linf"] = lp
+        res["seconds_with_polish"] = t_conv + res["polish_seconds"]
+    V = s.download()
+    pol = s.policy()
+    res["reached_frac"] = float(np.count_nonzero(V < prob.params.cap) / V.size)
+    print(f"# converge {tag}: {json.dumps(res)}", file=sys.stderr, flush=True)
+    s.close()
+    return res, V, pol
+
+
+def converge_both(pg, prob, args):
+    r32, V32, p32 = solve_timed(pg, prob, pg.abi.PG_VSTORE_F32, args, "fp32")
+    r16, V16, p16 = solve_timed(pg, prob, pg.abi.PG_VSTORE_U16_TILE, args, "16bit")
+    r16["speedup_vs_fp32"] = r32["seconds"] / r16["seconds_with_polish"]
+    return {"fp32": r32, "16bit": r16}, {"fp32": (V32, p32), "16bit": (V16, p16)}
+
+
+def f64_sample(pg, prob, args):
+    """The FP64 table of the problem (diagonal-solved FP64 Jacobi from cap to
+    ||dV||_inf < f64_tol) at a seeded 1M-state subsample: numpy
+    PCG64(3 + 100 * config), uniform over the grid."""
+    g = prob.grid
+    s = pg.HipSolver(g, prob.model, prob.params, prob.ctl, prob.occ)
+    s.fill_seed(prob.seeds)
+    t = time.perf_counter()
+    s.solve_f64(0, 0.0)
+    its, l = 0, float("inf")
+    for its in range(1, args.f64_max_sweeps + 1):
+        l, _ = s.sweep_f64(diagonal=True)
+        if its % 10 == 0:
+            print(f"# FP64 sweep {its}: linf {l:.3e}, {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
+        if l < args.f64_tol:
+            break
+    t64 = time.perf_counter() - t
+    r64, _ = s.residual_f64(0)
+    rng = np.random.Generator(np.random.PCG64(3 + 100 * args.config))
+    idx = np.sort(rng.integers(0, g.ncells, 1_000_000))
+    V64 = s.download_f64()[idx]
+    P64 = s.policy_f64()[idx]
+    s.close()
+    info = {"sweeps": its, "linf": l, "tol": args.f64_tol, "seconds": t64, "residual_linf": r64,
+            "mode": "diagonal-solved FP64 Jacobi from cap",
+            "sample": "1,000,000 states, numpy PCG64(3 + 100 * config), uniform over the grid"}
+    print(f"# FP64 table: {json.dumps(info)}", file=sys.stderr, flush=True)
+    return idx, V64, P64, info
+
+
+def accuracy(pg, prob, args, tables):
+    """Both forms' tables against the FP64 table of the same problem on the
+    seeded 1M-state subsample; errors where both tables are below cap,
+    policy agreement over those states that are not blocked."""
+    g = prob.grid
+    cap = prob.params.cap
+    if args.f64_sample:
+        z = np.load(args.f64_sample, allow_pickle=False)
+        idx, V64, P64 = z["idx"], z["v64"], z["p64"]
+        info = json.loads(str(z["info"]))
+        if int(z["config"]) != args.config:
+            raise SystemExit(f"{args.f64_sample} holds problem {int(z['config'])}, not {args.config}")
+    else:
+        idx, V64, P64, info = f64_sample(pg, prob, args)
+    out = {"f64": info}
+    blocked = np.asarray(prob.occ, np.uint8).reshape(-1)[idx // (g.n[3] * g.n[4] * g.n[5])] != 0
+    for tag, (V, P) in tables.items():
+        a, b = V[idx].astype(np.float64), V64
+        both = (a < cap) & (b < cap)
+        e = np.abs(a - b)[both]
+        q = np.quantile(e, [0.5, 0.99, 0.999])
+        live = both & ~blocked
+        out[tag] = {"max": float(e.max()), "p50": float(q[0]), "p99": float(q[1]), "p999": float(q[2]),
+                    "mean_signed": float(np.mean((a - b)[both])), "n": int(both.sum()),
+                    "reach_mismatch": float(np.mean((a < cap) != (b < cap))),
+                    "policy_agreement": float(np.mean(P[idx][live] == P64[live]))}
+        print(f"# accuracy {tag}: {json.dumps(out[tag])}", file=sys.stderr, flush=True)
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,6 +183,14 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", type=int, default=2)
     ap.add_argument("--converge", action="store_true")
+    ap.add_argument("--accuracy", action="store_true",
+                    help="with --converge: FP64 table and subsample errors of both forms")
+    ap.add_argument("--f64-tol", type=float, default=1e-7)
+    ap.add_argument("--f64-max-sweeps", type=int, default=1500)
+    ap.add_argument("--polish", type=int, default=3)
+    ap.add_argument("--f64-only", action="store_true")
+    ap.add_argument("--f64-out", default=None)
+    ap.add_argument("--f64-sample", default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tol", type=float, default=1e-3)
     ap.add_argument("--max-sweeps", type=int, default=2000)
@@ -109,6 +225,13 @@ def main() -> int:
     print(f"# problem {args.config}: grid {g.n} = {g.ncells} states, {prob.nctl} controls, "
           f"occupied {float(prob.occ.mean()):.3f}, built in {time.perf_counter() - t0:.1f} s",
           file=sys.stderr, flush=True)
+    if args.f64_only:
+        idx, V64, P64, info = f64_sample(pg, prob, args)
+        if args.f64_out:
+            np.savez(args.f64_out, idx=idx, v64=V64, p64=P64, info=json.dumps(info), config=args.config)
+        print(json.dumps({"f64": info, "config": {"workload": f"problem {args.config}", "grid": list(g.n)}}),
+              flush=True)
+        return 0
     def timed(vstore, kernel):
         s = pg.HipSolver(g, prob.model, prob.params, prob.ctl, prob.occ, vstore=vstore)
         if kernel != "auto":
@@ -195,23 +318,14 @@ def main() -> int:
         }
         s16.close()
         s = pg.HipSolver(g, prob.model, prob.params, prob.ctl, prob.occ)
-    if args.converge:
-        s.fill_seed(prob.seeds)
-        t = time.perf_counter()
-        its, linf = 0, float("inf")
-        for its in range(1, args.max_sweeps + 1):
-            r = s.sweep(its - 1, its % 2 == 1)
-            linf = r.linf
-            if its % 10 == 0:
-                print(f"# converge sweep {its}: linf {linf:.5f}, {time.perf_counter() - t:.1f} s",
-                      file=sys.stderr, flush=True)
-            if linf < args.tol:
-                break
-        V = s.download()
-        out["converge"] = {"seconds": time.perf_counter() - t, "sweeps": its, "linf": linf,
-                           "tol": args.tol, "converged": bool(linf < args.tol),
-                           "reached_frac": float(np.count_nonzero(V < prob.params.cap) / V.size)}
-    s.close()
+    if args.converge or args.accuracy:
+        s.close()
+        out["converge"], tables = converge_both(pg, prob, args)
+        if args.accuracy:
+            out["accuracy"] = accuracy(pg, prob, args, tables)
+        s = None
+    if s is not None:
+        s.close()
     print(json.dumps(out), flush=True)
     return 0
 

@@ -192,6 +192,29 @@ int pg_last_sweep_ms(pg_solver* s, float* ms);
 /* Number of cells of the solver's grid. */
 int pg_solver_ncells(pg_solver* s, int64_t* n);
 
+/* Out-of-core residency (replaces solve_value_ooc!, OutOfCore.jl:412-584,
+ * and the tile planning of Tiles.jl:311-512). Both FP32 tables of the
+ * Jacobi pair live in pinned host memory; a sweep streams one tile's window
+ * of the old table to the device (side stream), sweeps the tile's interior
+ * with the LDS-staged kernel and streams the interior back (second side
+ * stream), overlapping copies with the sweep of the neighbouring tiles.
+ * Tiles cut x, y and heading; the velocity axes stay whole. A tile's window
+ * is the union of the cells its backups read, taken from the one-time
+ * window plan, so each sweep equals the in-HBM sweep bit for bit.
+ *
+ * tile = {nx, ny, nheading} interior cells (rounded up to the box edge), or
+ * {0, 0, 0} for the largest x-y tile with the whole heading axis whose
+ * buffers fit budget_bytes of device memory. PG_ERR_NOMEM if they do not.
+ * The solver then takes pg_fill_seed, pg_sweep, pg_solve, pg_download_V,
+ * pg_upload_V, pg_last_sweep_ms as usual (at most 128 controls; no 16-bit
+ * store, policy or FP64 tables). */
+int pg_solver_create_ooc(pg_ctx* ctx, const pg_grid* g, const pg_model* m, const pg_params* p,
+                         const float* ctl_soa, int32_t nctl, const uint8_t* occ, int64_t budget_bytes,
+                         const int32_t tile[3], pg_solver** out);
+/* The tile interior used, the tile count, the device bytes the tiled sweep
+ * holds and the cells loaded per cell updated. */
+int pg_ooc_info(pg_solver* s, int32_t tile[3], int64_t* ntiles, int64_t* device_bytes, double* amplification);
+
 /* FP64 tables. No counterpart in the reference, whose tables are Float32
  * only: these are the FP64 yardstick the FP32 and 16-bit tables are measured
  * against, and the FP64 residual check of a finished table. The arithmetic
@@ -200,7 +223,7 @@ int pg_solver_ncells(pg_solver* s, int64_t* n);
  * 1100-1153); two extra FP64 tables live on the device once used.
  *
  * pg_solve_f64 starts from cap with the pg_fill_seed targets at 0 and runs
- * FP64 sweeps until ||dV||_inf < tol_linf or max_iters: PLAIN sweeps are
+ * FP64 sweeps until ||dV||_inf < tol_linf or max_iters sweeps: PLAIN sweeps are
  * V' = min(V, T V); DIAGONAL sweeps solve each control's own-cell term of the
  * interpolation (V = tau + w V + rest  =>  V = (tau + rest) / (1 - w)),
  * which has T's fixed point and reaches it in fewer sweeps. At most 128
@@ -209,6 +232,9 @@ int pg_solver_ncells(pg_solver* s, int64_t* n);
 #define PG_F64_SWEEP_DIAGONAL 1
 int pg_solve_f64(pg_solver* s, int32_t mode, int32_t max_iters, double tol_linf, int32_t* iters,
                  double* last_linf);
+/* One more FP64 sweep of the FP64 table (pg_solve_f64 with max_iters = 0
+ * only sets up the seeded start). */
+int pg_sweep_f64(pg_solver* s, int32_t mode, double* linf, double* sum_decrease);
 /* max and sum over states of |T V - V| with T one FP64 backup, over the
  * states where T V or V is below cap; which = 0: the FP64 table, 1: the
  * current FP32 table (widened to double). */

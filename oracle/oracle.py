@@ -43,6 +43,8 @@ SIG = {
     "orc_sweep_f64": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _I32, _P, _I32, _I32, _I32, _I32, _P, _P, _P, _I32]),
     "orc_target_cells": (_I64, [_P, _P, _P, _P, _I64]),
     "orc_check_quotient": (_I64, [_F, _I32]),
+    "orc_adjoint_f64": (_I32, [_P, _P, _P, _P, _P, _I32, _P, _P, _I64, _P, _I64, _I32, ctypes.c_double, _I32,
+                               _P, _P, _P]),
 }
 
 _lib = None
@@ -127,6 +129,24 @@ class Problem:
         if rc != 0:
             raise RuntimeError(f"oracle backup failed ({rc})")
         return out
+
+    def adjoint_f64(self, V: np.ndarray, seeds, starts: np.ndarray, max_iters: int = 100000, tol: float = 1e-12,
+                    nthreads: int = 0):
+        """dL/dtheta (30) and L = mean interp(V, start) at the FP64 table V,
+        with the argmin policy held fixed; returns (grad, L, iterations,
+        policy). iterations is -1 if the adjoint did not converge."""
+        L = lib()
+        V = np.ascontiguousarray(V, dtype=np.float64)
+        seeds = np.ascontiguousarray(seeds, dtype=np.int64)
+        starts = np.ascontiguousarray(starts, dtype=np.float64).reshape(-1, 6)
+        grad = np.zeros(30, np.float64)
+        lv = np.zeros(1, np.float64)
+        pol = np.zeros(self.ncell, np.int32)
+        it = L.orc_adjoint_f64(_ptr(V), _ptr(self.occ), ctypes.byref(self.gc), ctypes.byref(self.mc),
+                               _ptr(self.ctl), self.nctl, ctypes.byref(self.pc), _ptr(seeds), seeds.size,
+                               _ptr(starts), starts.shape[0], int(max_iters), float(tol), int(nthreads),
+                               _ptr(grad), _ptr(lv), _ptr(pol))
+        return grad, float(lv[0]), int(it), pol
 
     def solve(self, seeds, iters: int, tol_linf: float = 1e-3, dtype=np.float32, nthreads: int = 0,
               mode: int = 0):

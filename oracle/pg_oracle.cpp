@@ -105,6 +105,94 @@ int sweep_impl(const T* Vin, T* Vout, const uint8_t* occ, const pg_grid* g, cons
 
 }  // namespace
 
+namespace {
+
+struct Row {
+    int64_t base[6];   // corner indices j0 per axis (j1 derived)
+    double t[6];       // fractional parts
+    double self;       // weight of the state's own cell
+    bool live;
+};
+
+// The successor of grid state idx under control k in double, as lookahead
+// computes it (:1100-1153); false when the control is rejected.
+bool successor_f64(int64_t idx, int32_t k, const uint8_t* occ, const pg_grid& g, const pg_model& m,
+                   const float* ctl, int32_t nctl, const pg_params& p, double s0[6], double f[6]) {
+    int32_t i[6];
+    unflatten(g, idx, i);
+    double h = axisvalue<double>(g, 2, i[2]);
+    double vfx = axisvalue<double>(g, 3, i[3]), vfy = axisvalue<double>(g, 4, i[4]);
+    double sh0, ch0;
+    sincos_t(h, sh0, ch0);
+    s0[0] = axisvalue<double>(g, 0, i[0]);
+    s0[1] = axisvalue<double>(g, 1, i[1]);
+    s0[2] = h;
+    s0[3] = ch0 * vfx + sh0 * vfy;
+    s0[4] = -sh0 * vfx + ch0 * vfy;
+    s0[5] = axisvalue<double>(g, 5, i[5]);
+    double s[6];
+    for (int q = 0; q < 6; ++q) s[q] = s0[q];
+    const double tau = (double)p.dt;
+    step_state<double>(m, s, (double)ctl[k], (double)ctl[k + nctl], (double)ctl[k + 2 * nctl], tau,
+                       substeps_for<double>(tau, p));
+    const double dx = s[0] - s0[0], dy = s[1] - s0[1], dh = s[2] - s0[2];
+    if (!(isfin(dx) && isfin(dy) && isfin(dh) && isfin(s[3]) && isfin(s[4]) && isfin(s[5]))) return false;
+    const double n3 = (double)(float)g.n[2];
+    for (int32_t sc = 1; sc <= p.checks; ++sc) {
+        const double a = (double)(float)sc / (double)(float)p.checks;
+        const double gx = (s0[0] + dx * a - (double)g.lo[0]) / (double)g.step[0];
+        const double gy = (s0[1] + dy * a - (double)g.lo[1]) / (double)g.step[1];
+        if (gx < 0 || gy < 0 || gx > (double)(float)(g.n[0] - 1) || gy > (double)(float)(g.n[1] - 1)) return false;
+        double gh = (s0[2] + dh * a - (double)g.lo[2]) / (double)g.step[2];
+        gh = gh - std::floor(gh / n3) * n3;
+        if (!(gh >= 0 && gh <= n3)) return false;
+        const int32_t ci = (int32_t)jround(gx), cj = (int32_t)jround(gy), ck = (int32_t)jround(gh) % g.n[2];
+        if (occ[((int64_t)ci * g.n[1] + cj) * g.n[2] + ck]) return false;
+    }
+    double snh, cnh;
+    sincos_t(s[2], snh, cnh);
+    const double sv[6] = {s[0], s[1], s[2], cnh * s[3] - snh * s[4], snh * s[3] + cnh * s[4], s[5]};
+    for (int a = 0; a < 6; ++a) f[a] = (sv[a] - (double)g.lo[a]) / (double)g.step[a];
+    f[2] = f[2] - std::floor(f[2] / n3) * n3;
+    if (!(f[2] >= 0 && f[2] <= n3)) return false;
+    for (int a = 0; a < 6; ++a) {
+        if (a == 2) continue;
+        if (!(f[a] >= 0 && f[a] <= (double)(float)(g.n[a] - 1))) return false;
+    }
+    return true;
+}
+
+void row_of(const pg_grid& g, const double f[6], int64_t idx, Row& r) {
+    int32_t own[6];
+    unflatten(g, idx, own);
+    r.self = 1.0;
+    for (int a = 0; a < 6; ++a) {
+        const double b = std::floor(f[a]);
+        r.t[a] = f[a] - b;
+        r.base[a] = (int64_t)b;
+        int32_t d = own[a] - (int32_t)b;
+        if (a == 2) d = ((d % g.n[2]) + g.n[2]) % g.n[2];
+        r.self *= d == 0 ? 1.0 - r.t[a] : (d == 1 ? r.t[a] : 0.0);
+    }
+}
+
+template <class Fn>
+void for_corners(const pg_grid& g, const Row& r, Fn fn) {
+    for (int d = 0; d < 64; ++d) {
+        int32_t j[6];
+        double w = 1.0;
+        for (int k = 0; k < 6; ++k) {
+            const int c = (d >> k) & 1;
+            const int32_t b = (int32_t)r.base[k];
+            j[k] = (k == 2) ? (b + c) % g.n[2] : jmin<int32_t>(b + c, g.n[k] - 1);
+            w *= c ? r.t[k] : 1.0 - r.t[k];
+        }
+        fn(flatten(g, j[0], j[1], j[2], j[3], j[4], j[5]), w);
+    }
+}
+
+}  // namespace
+
 extern "C" {
 
 int orc_abi(void) { return PG_ABI_VERSION; }
@@ -294,6 +382,118 @@ int64_t orc_target_cells(const pg_grid* g, const float* s, const float* tol, int
     int64_t nout = (int64_t)idx.size();
     for (int64_t i = 0; i < nout && i < cap_out; ++i) out[i] = idx[i];
     return nout;
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint of L = mean over starts of interp(V, start) with respect to the 30
+// drivetrain parameters at a fixed point V of the plain backup (no
+// counterpart in the reference). With the argmin control held fixed, every
+// reached state's value is tau + P_i . V (P_i: its successor's 64 corner
+// weights); targets, blocked and capped states have no row. Then
+//   dL/dtheta = sum_i lambda_i  d(P_i . V)/dtheta,   (I - P)^T lambda = g,
+// g = mean of the starts' corner weights, and d(P_i . V)/dtheta is the
+// interpolant's gradient at the successor times the successor's tangent.
+// lambda is iterated as lambda_j <- (g_j + sum_{i != j} P_ij lambda_i) / (1 - P_jj).
+// Returns the iterations used (or -1 if max_iters ran out).
+// ---------------------------------------------------------------------------
+
+int32_t orc_adjoint_f64(const double* V, const uint8_t* occ, const pg_grid* g, const pg_model* m,
+                        const float* ctl, int32_t nctl, const pg_params* p, const int64_t* seeds,
+                        int64_t nseeds, const double* starts, int64_t nstarts, int32_t max_iters, double tol,
+                        int32_t nthreads, double* grad, double* L, int32_t* pol_out) {
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+    const int64_t n = ncells(*g);
+    const double cap = (double)p->cap;
+    std::vector<uint8_t> seed(n, 0);
+    for (int64_t i = 0; i < nseeds; ++i) seed[seeds[i]] = 1;
+    std::vector<Row> rows(n);
+    std::vector<int32_t> pol(n, 0);
+    const int64_t blk = (int64_t)g->n[3] * g->n[4] * g->n[5];
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < n; ++i) {
+        Row& r = rows[i];
+        r.live = false;
+        if (seed[i] || occ[i / blk] || !(V[i] < cap)) continue;
+        int32_t k = 0;
+        const double best = cell_update<double>(i, V, occ, *g, *m, ctl, nctl, *p, 0, false, &k);
+        pol[i] = k;
+        if (!(best < cap)) continue;
+        double s0[6], f[6];
+        if (!successor_f64(i, k, occ, *g, *m, ctl, nctl, *p, s0, f)) continue;
+        row_of(*g, f, i, r);
+        r.live = true;
+    }
+    // g and L from the starts
+    std::vector<double> gv(n, 0.0), lam(n, 0.0), nxt(n, 0.0);
+    double Ls = 0.0;
+    PlainV<double> pv{V};
+    for (int64_t k = 0; k < nstarts; ++k) {
+        const double* st = starts + 6 * k;
+        double f[6];
+        for (int a = 0; a < 6; ++a) f[a] = (st[a] - (double)g->lo[a]) / (double)g->step[a];
+        const double n3 = (double)(float)g->n[2];
+        f[2] = f[2] - std::floor(f[2] / n3) * n3;
+        const double v = interp<double>(pv, *g, st[0], st[1], st[2], st[3], st[4], st[5], false, cap, false);
+        Ls += v;
+        if (!(v < cap)) continue;
+        Row r;
+        row_of(*g, f, 0, r);
+        for_corners(*g, r, [&](int64_t j, double w) { gv[j] += w / (double)nstarts; });
+    }
+    *L = Ls / (double)nstarts;
+    for (int64_t j = 0; j < n; ++j) lam[j] = gv[j];
+    int32_t it = 0;
+    bool done = false;
+    for (it = 1; it <= max_iters; ++it) {
+        std::fill(nxt.begin(), nxt.end(), 0.0);
+        for (int64_t i = 0; i < n; ++i) {
+            if (!rows[i].live || lam[i] == 0.0) continue;
+            const double li = lam[i];
+            for_corners(*g, rows[i], [&](int64_t j, double w) { nxt[j] += w * li; });
+        }
+        double dmax = 0.0, lmax = 0.0;
+        for (int64_t j = 0; j < n; ++j) {
+            const double self = rows[j].live ? rows[j].self : 0.0;
+            const double v = (gv[j] + nxt[j] - self * lam[j]) / (1.0 - self);
+            dmax = std::max(dmax, std::fabs(v - lam[j]));
+            lmax = std::max(lmax, std::fabs(v));
+            lam[j] = v;
+        }
+        if (dmax <= tol * std::max(lmax, 1e-300)) { done = true; break; }
+    }
+    for (int j = 0; j < 30; ++j) grad[j] = 0.0;
+    std::vector<double> acc(30 * (size_t)std::max(1, omp_get_max_threads()), 0.0);
+#pragma omp parallel
+    {
+        double* my = acc.data() + 30 * (size_t)omp_get_thread_num();
+#pragma omp for schedule(dynamic, 256)
+        for (int64_t i = 0; i < n; ++i) {
+            if (!rows[i].live || lam[i] == 0.0) continue;
+            double s0[6], f[6], gi[6];
+            successor_f64(i, pol[i], occ, *g, *m, ctl, nctl, *p, s0, f);
+            interp_grad(pv, *g, f, gi);
+            const int32_t k = pol[i];
+            for (int j = 0; j < 30; ++j) {
+                double s[6], ds[6];
+                for (int q = 0; q < 6; ++q) s[q] = s0[q];
+                const double tau = (double)p->dt;
+                step_state_tangent(*m, s, ds, (double)ctl[k], (double)ctl[k + nctl], (double)ctl[k + 2 * nctl],
+                                   tau, substeps_for<double>(tau, *p), j);
+                double snh, cnh;
+                sincos_t(s[2], snh, cnh);
+                const double a1 = ds[3] - s[4] * ds[2], a2 = ds[4] + s[3] * ds[2];
+                const double dsv[6] = {ds[0], ds[1], ds[2], cnh * a1 - snh * a2, snh * a1 + cnh * a2, ds[5]};
+                double r = 0.0;
+                for (int a = 0; a < 6; ++a) r += gi[a] * dsv[a] / (double)g->step[a];
+                my[j] += lam[i] * r;
+            }
+        }
+    }
+    for (size_t t = 0; t < acc.size() / 30; ++t)
+        for (int j = 0; j < 30; ++j) grad[j] += acc[30 * t + j];
+    if (pol_out)
+        for (int64_t i = 0; i < n; ++i) pol_out[i] = pol[i];
+    return done ? it : -1;
 }
 
 // Property check of the quotient the HIP kernels use in place of a float

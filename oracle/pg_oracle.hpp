@@ -552,4 +552,101 @@ inline T cell_update(int64_t idx, const T* V, const uint8_t* occ, const pg_grid&
     return best;
 }
 
+// ---------------------------------------------------------------------------
+// Adjoint support (no counterpart in the reference): the successor's
+// derivative with respect to one drivetrain parameter, and the gradient of
+// the multilinear interpolant. Parameter index j of theta (30 entries):
+// B (0-8, row-major), A (9-17), q (18-20), S diagonal (21-23), D diagonal
+// (24-26), c (27-29), as in body_accel_sat (:150-162).
+// ---------------------------------------------------------------------------
+
+// d a_r / d theta_j holding the state fixed.
+inline double accel_param_partial(int j, int r, double u1, double u2, double u3, double vx, double vy,
+                                  double w, const pg_model& m) {
+    const double v[3] = {vx, vy, w};
+    const double u[3] = {u1, u2, u3};
+    if (j < 9) return (j / 3 == r) ? u[j % 3] : 0.0;
+    if (j < 18) return ((j - 9) / 3 == r) ? v[(j - 9) % 3] : 0.0;
+    if (j < 21) return (j - 18 == r) ? w * w : 0.0;
+    if (j < 24) return (j - 21 == r) ? csign<double>(v[r], m.eps[r]) : 0.0;
+    if (j < 27) return (j - 24 == r) ? std::fabs(v[r]) * v[r] : 0.0;
+    return (j - 27 == r) ? 1.0 : 0.0;
+}
+
+// Forward Euler (:208-223) in double from s[6] under the raw command u,
+// carrying the tangent ds[6] = d state / d theta_j (ds starts at 0).
+inline void step_state_tangent(const pg_model& m, double s[6], double ds[6], double u1r, double u2r,
+                               double u3r, double dt, int32_t nsub, int j) {
+    const double hs = dt / (double)(float)nsub;
+    double u1, u2, u3;
+    saturate<double>(m, u1r, u2r, u3r, u1, u2, u3);
+    for (int k = 0; k < 6; ++k) ds[k] = 0.0;
+    for (int32_t k = 0; k < nsub; ++k) {
+        const double h = s[2], vx = s[3], vy = s[4], w = s[5];
+        const double dh = ds[2], dvx = ds[3], dvy = ds[4], dw = ds[5];
+        double ax, ay, al;
+        body_accel_sat<double>(m, u1, u2, u3, vx, vy, w, ax, ay, al);
+        // tangent of the acceleration rows: state part + explicit part
+        const double v[3] = {vx, vy, w}, dv[3] = {dvx, dvy, dw};
+        double dcs[3], dqd[3];
+        for (int c = 0; c < 3; ++c) {
+            const double e = (double)m.eps[c];
+            dcs[c] = std::fabs(v[c] / e) < 1.0 ? dv[c] / e : 0.0;   // d csign
+            dqd[c] = 2.0 * std::fabs(v[c]) * dv[c];                   // d |v| v
+        }
+        double da[3];
+        for (int r = 0; r < 3; ++r) {
+            double t = 0.0;
+            for (int c = 0; c < 3; ++c)
+                t += (double)m.A[3 * r + c] * dv[c] + (double)m.S[3 * r + c] * dcs[c] +
+                     (double)m.D[3 * r + c] * dqd[c];
+            t += 2.0 * (double)m.q[r] * w * dw;
+            da[r] = t + accel_param_partial(j, r, u1, u2, u3, vx, vy, w, m);
+        }
+        double sh, ch;
+        sincos_t(h, sh, ch);
+        const double a1 = dvx - vy * dh, a2 = dvy + vx * dh;
+        const double dF[6] = {ch * a1 - sh * a2, sh * a1 + ch * a2, dw,
+                              da[0] + dw * vy + w * dvy, da[1] - dw * vx - w * dvx, da[2]};
+        const double F[6] = {ch * vx - sh * vy, sh * vx + ch * vy, w, ax + w * vy, ay - w * vx, al};
+        for (int q = 0; q < 6; ++q) {
+            s[q] = s[q] + F[q] * hs;
+            ds[q] = ds[q] + dF[q] * hs;
+        }
+    }
+}
+
+// Multilinear interpolant's value and gradient with respect to the
+// fractional coordinates f (cells), corners clamped / periodic as in interp.
+template <class VR>
+inline double interp_grad(const VR& V, const pg_grid& g, const double f[6], double grad[6]) {
+    double t[6];
+    int32_t i[6];
+    for (int k = 0; k < 6; ++k) {
+        const double b = std::floor(f[k]);
+        t[k] = f[k] - b;
+        i[k] = (int32_t)b;
+    }
+    double acc = 0.0;
+    for (int k = 0; k < 6; ++k) grad[k] = 0.0;
+    for (int d = 0; d < 64; ++d) {
+        int32_t j[6];
+        for (int k = 0; k < 6; ++k) {
+            const int c = (d >> k) & 1;
+            j[k] = (k == 2) ? (i[k] + c) % g.n[2] : jmin<int32_t>(i[k] + c, g.n[k] - 1);
+        }
+        const double v = V(flatten(g, j[0], j[1], j[2], j[3], j[4], j[5]));
+        double wall = 1.0;
+        for (int k = 0; k < 6; ++k) wall *= ((d >> k) & 1) ? t[k] : 1.0 - t[k];
+        acc += wall * v;
+        for (int a = 0; a < 6; ++a) {
+            double wa = ((d >> a) & 1) ? 1.0 : -1.0;
+            for (int k = 0; k < 6; ++k)
+                if (k != a) wa *= ((d >> k) & 1) ? t[k] : 1.0 - t[k];
+            grad[a] += wa * v;
+        }
+    }
+    return acc;
+}
+
 }  // namespace orc

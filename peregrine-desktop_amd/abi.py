@@ -101,8 +101,14 @@ SIGNATURES = {
     "pg_policy": (ctypes.c_int, [_P, _P]),
     "pg_download_V": (ctypes.c_int, [_P, _P, _I64, _I64]),
     "pg_download_V16": (ctypes.c_int, [_P, _P, _I64, _I64]),
+    "pg_solver_create_ooc": (ctypes.c_int, [_P, ctypes.POINTER(PgGrid), ctypes.POINTER(PgModel),
+                                            ctypes.POINTER(PgParams), _P, _I32, _P, _I64, _P,
+                                            ctypes.POINTER(_P)]),
+    "pg_ooc_info": (ctypes.c_int, [_P, _P, ctypes.POINTER(_I64), ctypes.POINTER(_I64),
+                                   ctypes.POINTER(ctypes.c_double)]),
     "pg_solve_f64": (ctypes.c_int, [_P, _I32, _I32, ctypes.c_double, ctypes.POINTER(_I32),
                                     ctypes.POINTER(ctypes.c_double)]),
+    "pg_sweep_f64": (ctypes.c_int, [_P, _I32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     "pg_residual_f64": (ctypes.c_int, [_P, _I32, ctypes.POINTER(ctypes.c_double),
                                        ctypes.POINTER(ctypes.c_double)]),
     "pg_download_V64": (ctypes.c_int, [_P, _P, _I64, _I64]),

@@ -13,10 +13,9 @@
 #include <cstring>
 #include <string>
 #include <vector>
-#include "../../include/peregrine_hip.h"
-#include "pg_kernels.h"
+#include "pg_internal.hpp"
 
-namespace {
+namespace pgh {
 
 thread_local std::string g_err;
 
@@ -30,18 +29,19 @@ int fail(int code, const char* fmt, ...) {
     return code;
 }
 
-#define HIPCHECK(expr)                                                                 \
-    do {                                                                               \
-        hipError_t e_ = (expr);                                                        \
-        if (e_ != hipSuccess)                                                          \
-            return fail(PG_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));   \
-    } while (0)
-
 int64_t grid_cells(const pg_grid& g) {
     int64_t n = 1;
     for (int k = 0; k < 6; ++k) n *= (int64_t)g.n[k];
     return n;
 }
+
+}  // namespace pgh
+
+using pgh::fail;
+using pgh::g_err;
+using pgh::grid_cells;
+
+namespace {
 
 // traction_gain (PeregrineSolver.jl:126-136)
 float traction_gain(float u1, float u2, float u3, float knee) {
@@ -84,70 +84,13 @@ int32_t substeps_for64(double tau, const pg_params& p) {
 
 }  // namespace
 
-struct pg_ctx {
-    int32_t device = 0;
-    hipStream_t stream = nullptr;
-};
-
-struct pg_solver {
-    pg_ctx* ctx = nullptr;
-    pg_grid g{};
-    pg_model m{};
-    pg_params p{};
-    pgk::Consts K{};
-    pgk::FastConsts F{};
-    int32_t nctl = 0;
-    int32_t vstore = 0;
-    int64_t ncell = 0;
-    // FP32 tables: the Jacobi pair (FP16 store: [0] is the scratch the fast
-    // sweep writes, [1] exists once polishing starts)
-    float* dV[2] = {nullptr, nullptr};
-    int cur = 0;
-    // FP16 store
-    bool f16_active = false;      // V lives in the 16-bit codes (until polished)
-    int store = PG_STORE_F32;     // code type of the 16-bit store
-    uint16_t* d_V16 = nullptr;
-    float2* d_tp = nullptr;
-    unsigned* d_tstat = nullptr;
-    uint8_t* d_trefit = nullptr;
-    pgk::QuantGeom Q{};
-    uint8_t* d_occ = nullptr;
-    float* d_bu = nullptr;
-    int64_t* d_seeds = nullptr;
-    int64_t nseeds = 0;
-    bool seeded = false;
-    double* d_part = nullptr;
-    int64_t part_cap = 0;
-    double* d_sum = nullptr;
-    unsigned* d_linf = nullptr;
-    uint8_t* d_pol = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    float last_ms = 0.0f;
-    // kernel selection and the LDS window plan
-    int32_t kernel = PG_KERNEL_GLOBAL;
-    bool diag = false;
-    pgk::BoxGeom G{};
-    int16_t* d_worig = nullptr;
-    bool planned = false;
-    int shape = 0;
-    double coverage = 0.0;
-    // FP64 tables (allocated on first use)
-    pgk::Consts64 K64{};
-    std::vector<double> bu64;
-    double* d_bu64 = nullptr;
-    double* dV64[2] = {nullptr, nullptr};
-    uint32_t* d_seedmask = nullptr;   // one bit per cell: the pg_fill_seed targets
-    bool seedmask_ok = false;
-    int cur64 = 0;
-    bool f64_valid = false;       // dV64[cur64] holds an FP64 table
-};
-
 namespace {
 
-constexpr int kLdsMaxCtl = 128;
+using pgh::kLdsMaxCtl;
 
 void release(pg_solver* s) {
     if (!s) return;
+    pgh::ooc_release(s);
     (void)hipFree(s->dV[0]);
     (void)hipFree(s->dV[1]);
     (void)hipFree(s->d_V16);
@@ -290,12 +233,19 @@ void box_geometry(pg_solver* s) {
     for (int k = 0; k < 6; ++k) {
         s->G.b[k] = want[k] < s->g.n[k] ? want[k] : s->g.n[k];
         s->G.nb[k] = (s->g.n[k] + s->G.b[k] - 1) / s->G.b[k];
+        s->G.nbg[k] = s->G.nb[k];
+        s->G.org[k] = 0;
+        s->G.rn[k] = s->g.n[k];
         nbox *= s->G.nb[k];
     }
     s->G.nbox = nbox;
 }
 
 int64_t lds_blocks(const pg_solver* s) { return ((s->G.nbox + 7) / 8) * 8; }
+
+}  // namespace
+
+namespace pgh {
 
 int ensure_partials(pg_solver* s, int64_t nb) {
     if (nb <= s->part_cap) return PG_OK;
@@ -307,18 +257,26 @@ int ensure_partials(pg_solver* s, int64_t nb) {
 }
 
 // One-time pass over every (state, control) pair: each box's window origin,
-// and the window shape that holds the most boxes' successors whole.
-int plan(pg_solver* s) {
-    if (s->planned) return PG_OK;
+// and the window shape that holds the most boxes' successors whole. With
+// keep_host the origins and extents are also returned (the out-of-core
+// tiler builds its tile windows from them).
+int plan(pg_solver* s, bool keep_host, std::vector<int16_t>* h_orig, std::vector<uint8_t>* h_ext) {
+    if (s->planned && !keep_host) return PG_OK;
     hipStream_t st = s->ctx->stream;
     box_geometry(s);
     const int64_t nbox = s->G.nbox;
     uint8_t* d_ext = nullptr;
-    HIPCHECK(hipMalloc(&s->d_worig, (size_t)nbox * 6 * sizeof(int16_t)));
+    if (!s->d_worig) HIPCHECK(hipMalloc(&s->d_worig, (size_t)nbox * 6 * sizeof(int16_t)));
     HIPCHECK(hipMalloc(&d_ext, (size_t)nbox * 6));
     hipError_t e = pgk_launch_plan(s->K, s->G, s->d_occ, s->d_bu, s->d_worig, d_ext, s->diag ? 1 : 0, st);
-    std::vector<uint8_t> ext((size_t)nbox * 6);
+    std::vector<uint8_t> ext_local;
+    std::vector<uint8_t>& ext = h_ext ? *h_ext : ext_local;
+    ext.resize((size_t)nbox * 6);
     if (e == hipSuccess) e = hipMemcpyAsync(ext.data(), d_ext, ext.size(), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess && keep_host && h_orig) {
+        h_orig->resize((size_t)nbox * 6);
+        e = hipMemcpyAsync(h_orig->data(), s->d_worig, h_orig->size() * sizeof(int16_t), hipMemcpyDeviceToHost, st);
+    }
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     (void)hipFree(d_ext);
     if (e != hipSuccess) return fail(PG_ERR_HIP, "window plan failed: %s", hipGetErrorString(e));
@@ -338,6 +296,13 @@ int plan(pg_solver* s) {
     s->planned = true;
     return ensure_partials(s, lds_blocks(s));
 }
+
+}  // namespace pgh
+
+using pgh::ensure_partials;
+using pgh::plan;
+
+namespace {
 
 int auto_kernel(const pg_solver* s) {
     return (s->nctl <= kLdsMaxCtl) ? PG_KERNEL_LDS : PG_KERNEL_GLOBAL;
@@ -428,9 +393,15 @@ int pg_grid_fits(pg_ctx* ctx, const pg_grid* g, int32_t vstore, double headroom,
     return PG_OK;
 }
 
-int pg_solver_create(pg_ctx* ctx, const pg_grid* g, const pg_model* m, const pg_params* p,
-                     const float* ctl_soa, int32_t nctl, const uint8_t* occ, int32_t vstore,
-                     pg_solver** out) {
+}  // extern "C"
+
+namespace {
+
+// pg_solver_create and pg_solver_create_ooc: with host_tables the Jacobi
+// pair is not allocated on the device (the out-of-core driver keeps it in
+// pinned host memory).
+int create_solver(pg_ctx* ctx, const pg_grid* g, const pg_model* m, const pg_params* p, const float* ctl_soa,
+                  int32_t nctl, const uint8_t* occ, int32_t vstore, bool host_tables, pg_solver** out) {
     if (!out) return fail(PG_ERR_ARG, "null output");
     *out = nullptr;
     if (!ctx || !g || !m || !p || !ctl_soa || !occ) return fail(PG_ERR_ARG, "null argument");
@@ -490,9 +461,9 @@ int pg_solver_create(pg_ctx* ctx, const pg_grid* g, const pg_model* m, const pg_
     const int64_t nocc = (int64_t)g->n[0] * g->n[1] * g->n[2];
     hipError_t e = hipSuccess;
     auto chk = [&](hipError_t x) { if (e == hipSuccess) e = x; };
-    chk(hipMalloc(&s->dV[0], (size_t)s->ncell * sizeof(float)));
+    if (!host_tables) chk(hipMalloc(&s->dV[0], (size_t)s->ncell * sizeof(float)));
     if (vstore == PG_VSTORE_F32) {
-        chk(hipMalloc(&s->dV[1], (size_t)s->ncell * sizeof(float)));
+        if (!host_tables) chk(hipMalloc(&s->dV[1], (size_t)s->ncell * sizeof(float)));
     } else {
         chk(hipMalloc(&s->d_V16, (size_t)s->ncell * sizeof(uint16_t)));
         chk(hipMalloc(&s->d_tp, (size_t)s->Q.ntile * sizeof(float2)));
@@ -530,6 +501,40 @@ int pg_solver_create(pg_ctx* ctx, const pg_grid* g, const pg_model* m, const pg_
     return PG_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
+int pg_solver_create(pg_ctx* ctx, const pg_grid* g, const pg_model* m, const pg_params* p,
+                     const float* ctl_soa, int32_t nctl, const uint8_t* occ, int32_t vstore,
+                     pg_solver** out) {
+    return create_solver(ctx, g, m, p, ctl_soa, nctl, occ, vstore, false, out);
+}
+
+int pg_solver_create_ooc(pg_ctx* ctx, const pg_grid* g, const pg_model* m, const pg_params* p,
+                         const float* ctl_soa, int32_t nctl, const uint8_t* occ, int64_t budget_bytes,
+                         const int32_t tile[3], pg_solver** out) {
+    if (nctl > kLdsMaxCtl)
+        return fail(PG_ERR_UNSUPPORTED, "the out-of-core sweep holds at most %d controls", kLdsMaxCtl);
+    if (!tile) return fail(PG_ERR_ARG, "null tile");
+    int rc = create_solver(ctx, g, m, p, ctl_soa, nctl, occ, PG_VSTORE_F32, true, out);
+    if (rc) return rc;
+    rc = pgh::ooc_create(*out, budget_bytes, tile);
+    if (rc) {
+        const std::string msg = g_err;
+        pg_solver_destroy(*out);
+        *out = nullptr;
+        g_err = msg;
+    }
+    return rc;
+}
+
+int pg_ooc_info(pg_solver* s, int32_t tile[3], int64_t* ntiles, int64_t* device_bytes, double* amplification) {
+    if (!s) return fail(PG_ERR_ARG, "null solver");
+    if (!s->ooc) return fail(PG_ERR_STATE, "not an out-of-core solver");
+    return pgh::ooc_info(s, tile, ntiles, device_bytes, amplification);
+}
+
 int pg_solver_destroy(pg_solver* s) {
     if (!s) return PG_OK;
     (void)hipSetDevice(s->ctx->device);
@@ -560,7 +565,15 @@ int pg_fill_seed(pg_solver* s, const int64_t* seeds, int64_t nseeds) {
         HIPCHECK(hipMalloc(&s->d_seeds, (size_t)nseeds * sizeof(int64_t)));
         HIPCHECK(hipMemcpyAsync(s->d_seeds, seeds, (size_t)nseeds * sizeof(int64_t), hipMemcpyHostToDevice, st));
     }
+    s->h_seeds.assign(seeds, seeds + nseeds);
     s->cur = 0;
+    if (s->ooc) {
+        HIPCHECK(hipStreamSynchronize(st));
+        int rc = pgh::ooc_fill_seed(s);
+        if (rc) return rc;
+        s->seeded = true;
+        return PG_OK;
+    }
     HIPCHECK(pgk_launch_fill(s->dV[0], s->ncell, s->p.cap, st));
     HIPCHECK(pgk_launch_seed(s->dV[0], s->d_seeds, s->nseeds, st));
     if (s->vstore != PG_VSTORE_F32) {
@@ -580,6 +593,7 @@ int pg_sweep(pg_solver* s, int32_t phase, int32_t rev, double* sum_decrease, flo
     if (!s) return fail(PG_ERR_ARG, "null solver");
     if (!s->seeded) return fail(PG_ERR_STATE, "pg_fill_seed (or pg_upload_V) must come before a sweep");
     HIPCHECK(hipSetDevice(s->ctx->device));
+    if (s->ooc) return pgh::ooc_sweep(s, sum_decrease, linf);
     hipStream_t st = s->ctx->stream;
     const int kern = s->f16_active ? PG_KERNEL_FAST : s->kernel;
     if (kern != PG_KERNEL_GLOBAL) {
@@ -652,6 +666,7 @@ int pg_solve(pg_solver* s, int32_t max_iters, double tol_sum, float tol_linf, in
 
 int pg_polish(pg_solver* s, int32_t nsweeps, double* last_sum, float* last_linf) {
     if (!s) return fail(PG_ERR_ARG, "null solver");
+    if (s->ooc) return fail(PG_ERR_UNSUPPORTED, "an out-of-core solver has no 16-bit store to polish");
     if (!s->seeded) return fail(PG_ERR_STATE, "no value table yet");
     if (nsweeps < 0) return fail(PG_ERR_ARG, "nsweeps must be >= 0");
     HIPCHECK(hipSetDevice(s->ctx->device));
@@ -681,6 +696,7 @@ int pg_polish(pg_solver* s, int32_t nsweeps, double* last_sum, float* last_linf)
 
 int pg_policy(pg_solver* s, uint8_t* pol_host) {
     if (!s || !pol_host) return fail(PG_ERR_ARG, "null argument");
+    if (s->ooc) return fail(PG_ERR_UNSUPPORTED, "pg_policy is not built for an out-of-core solver");
     if (!s->seeded) return fail(PG_ERR_STATE, "no value table yet");
     HIPCHECK(hipSetDevice(s->ctx->device));
     hipStream_t st = s->ctx->stream;
@@ -714,6 +730,7 @@ int pg_download_V(pg_solver* s, float* dst, int64_t off, int64_t n) {
     if (!s || (!dst && n > 0)) return fail(PG_ERR_ARG, "null argument");
     if (off < 0 || n < 0 || off + n > s->ncell) return fail(PG_ERR_ARG, "range [%lld, %lld) outside the grid", (long long)off, (long long)(off + n));
     if (n == 0) return PG_OK;
+    if (s->ooc) return pgh::ooc_download(s, dst, off, n);
     HIPCHECK(hipSetDevice(s->ctx->device));
     const float* src = current_f32(s);
     HIPCHECK(hipMemcpyAsync(dst, src + off, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, s->ctx->stream));
@@ -742,6 +759,11 @@ int pg_download_V16(pg_solver* s, float* dst, int64_t off, int64_t n) {
 int pg_upload_V(pg_solver* s, const float* src, int64_t off, int64_t n) {
     if (!s || (!src && n > 0)) return fail(PG_ERR_ARG, "null argument");
     if (off < 0 || n < 0 || off + n > s->ncell) return fail(PG_ERR_ARG, "range [%lld, %lld) outside the grid", (long long)off, (long long)(off + n));
+    if (s->ooc) {
+        int rc = pgh::ooc_upload(s, src, off, n);
+        if (rc == PG_OK) s->seeded = true;
+        return rc;
+    }
     HIPCHECK(hipSetDevice(s->ctx->device));
     hipStream_t st = s->ctx->stream;
     if (s->vstore != PG_VSTORE_F32 && (s->f16_active || !s->seeded)) {
@@ -768,6 +790,7 @@ int pg_upload_V(pg_solver* s, const float* src, int64_t off, int64_t n) {
 namespace {
 
 int f64_setup(pg_solver* s) {
+    if (s->ooc) return fail(PG_ERR_UNSUPPORTED, "the FP64 tables are not built for an out-of-core solver");
     if (s->nctl > kLdsMaxCtl)
         return fail(PG_ERR_UNSUPPORTED, "the FP64 sweeps hold at most %d controls", kLdsMaxCtl);
     int rc = plan(s);
@@ -844,6 +867,26 @@ int pg_solve_f64(pg_solver* s, int32_t mode, int32_t max_iters, double tol_linf,
     return PG_OK;
 }
 
+int pg_sweep_f64(pg_solver* s, int32_t mode, double* linf, double* sum_decrease) {
+    if (!s) return fail(PG_ERR_ARG, "null solver");
+    if (mode != PG_F64_SWEEP_PLAIN && mode != PG_F64_SWEEP_DIAGONAL) return fail(PG_ERR_ARG, "unknown FP64 sweep mode %d", mode);
+    if (!s->f64_valid) return fail(PG_ERR_STATE, "no FP64 table yet (pg_solve_f64)");
+    HIPCHECK(hipSetDevice(s->ctx->device));
+    int rc = f64_setup(s);
+    if (rc) return rc;
+    float l = 0.0f;
+    double d = 0.0;
+    double* out = s->dV64[1 - s->cur64];
+    rc = f64_pass(s, mode == PG_F64_SWEEP_DIAGONAL ? PG_F64_DIAGONAL : PG_F64_PLAIN, s->dV64[s->cur64], out,
+                  nullptr, &l, &d);
+    if (rc) return rc;
+    HIPCHECK(pgk_launch_seed_f64(out, s->d_seeds, s->nseeds, s->ctx->stream));
+    s->cur64 = 1 - s->cur64;
+    if (linf) *linf = l;
+    if (sum_decrease) *sum_decrease = d;
+    return PG_OK;
+}
+
 int pg_residual_f64(pg_solver* s, int32_t which, double* linf, double* sum) {
     if (!s) return fail(PG_ERR_ARG, "null solver");
     if (which != 0 && which != 1) return fail(PG_ERR_ARG, "which must be 0 (FP64 table) or 1 (current table)");
@@ -917,6 +960,8 @@ int pg_solver_set_kernel(pg_solver* s, int32_t kernel) {
         return fail(PG_ERR_UNSUPPORTED, "the LDS-staged sweeps hold at most %d controls", kLdsMaxCtl);
     if (s->f16_active && kernel != PG_KERNEL_FAST)
         return fail(PG_ERR_STATE, "a 16-bit store sweeps with the fast kernel until pg_polish");
+    if (s->ooc && kernel != PG_KERNEL_LDS)
+        return fail(PG_ERR_UNSUPPORTED, "an out-of-core solver sweeps with the LDS-staged kernel");
     s->kernel = kernel;
     return PG_OK;
 }

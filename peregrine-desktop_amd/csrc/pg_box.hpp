@@ -41,12 +41,13 @@ __device__ __forceinline__ int64_t box_of_block(const BoxGeom& G) {
 }
 
 struct BoxInfo {
-    int32_t s0[6];   // first state of the box, per axis
+    int32_t s0[6];   // first state of the box, per axis (grid coordinates)
     int32_t vb[6];   // states of the box per axis (edge boxes are partial)
     int32_t ns;
+    int64_t gbox;    // the box's number over the whole grid
 };
 
-__device__ __forceinline__ void box_info(const int32_t n[6], const BoxGeom& G, int64_t box, BoxInfo& B) {
+__device__ __forceinline__ void box_info(const BoxGeom& G, int64_t box, BoxInfo& B) {
     int64_t r = box;
     int32_t bi[6];
 #pragma unroll
@@ -56,12 +57,14 @@ __device__ __forceinline__ void box_info(const int32_t n[6], const BoxGeom& G, i
     }
     bi[0] = (int32_t)r;
     B.ns = 1;
+    B.gbox = 0;
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
-        B.s0[k] = bi[k] * G.b[k];
-        const int32_t rem = n[k] - B.s0[k];
+        B.s0[k] = G.org[k] + bi[k] * G.b[k];
+        const int32_t rem = G.org[k] + G.rn[k] - B.s0[k];
         B.vb[k] = rem < G.b[k] ? rem : G.b[k];
         B.ns *= B.vb[k];
+        B.gbox = B.gbox * G.nbg[k] + (G.org[k] / G.b[k] + bi[k]);
     }
 }
 

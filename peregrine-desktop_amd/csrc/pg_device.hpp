@@ -392,10 +392,16 @@ struct QuantGeom {
 };
 
 // Box decomposition of the grid for the LDS-staged kernels.
+// Boxes of a region of the grid (the whole grid, or one tile of an
+// out-of-core sweep). The plan's per-box arrays are numbered over the whole
+// grid's boxes, so a region carries that numbering too.
 struct BoxGeom {
     int32_t b[6];     // box dims
-    int32_t nb[6];    // boxes per axis
-    int64_t nbox;
+    int32_t nb[6];    // boxes per axis over the region
+    int64_t nbox;     // boxes of the region
+    int32_t org[6];   // first cell of the region per axis (a multiple of b)
+    int32_t rn[6];    // cells of the region per axis
+    int32_t nbg[6];   // boxes per axis over the whole grid
 };
 
 // Block-wide sum (FP64) and max of the per-cell decreases: wave shuffles,

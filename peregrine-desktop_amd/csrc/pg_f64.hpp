@@ -6,8 +6,7 @@
 // derivative from the old state (:208-223), the traction gain and B * u
 // folded per control on the host in double, swept probes with
 // round-to-nearest-even cells (:1127-1144), 64-corner multilinear lookup with
-// the reference's left-to-right weight products and clamped / periodic
-// corners (:399-466), value min(tau + V, cap). Divisions are IEEE divisions
+// clamped / periodic corners (:399-466), value min(tau + V, cap). Divisions are IEEE divisions
 // and sin/cos the device library's double functions; no result here is meant
 // to match the FP32 path bit for bit.
 #pragma once
@@ -47,7 +46,8 @@ __device__ __forceinline__ void cell_start64(const Consts64& K, const int i[6], 
     const double vfx = K.lo[3] + (double)(float)i[3] * K.step[3];
     const double vfy = K.lo[4] + (double)(float)i[4] * K.step[4];
     c.w = K.lo[5] + (double)(float)i[5] * K.step[5];
-    const double sh = sin(c.h), ch = cos(c.h);
+    double sh, ch;
+    sincos(c.h, &sh, &ch);
     c.vx = ch * vfx + sh * vfy;
     c.vy = -sh * vfx + ch * vfy;
 }
@@ -99,7 +99,8 @@ __device__ __forceinline__ bool successor64(const Consts64& K, const OccRead& oc
         accel64(K, bux, buy, buw, vx, vy, w, ax, ay, al);
         const double dvx = ax + w * vy;
         const double dvy = ay - w * vx;
-        const double sh = sin(h), ch = cos(h);
+        double sh, ch;
+        sincos(h, &sh, &ch);
         x = x + (ch * vx - sh * vy) * hs;
         y = y + (sh * vx + ch * vy) * hs;
         h = h + w * hs;
@@ -123,7 +124,8 @@ __device__ __forceinline__ bool successor64(const Consts64& K, const OccRead& oc
         ck = ck >= K.n[2] ? ck - K.n[2] : ck;
         if (occ((int)rint(gx), (int)rint(gy), ck)) return false;
     }
-    const double snh = sin(h), cnh = cos(h);
+    double snh, cnh;
+    sincos(h, &snh, &cnh);
     const double nvfx = cnh * vx - snh * vy;
     const double nvfy = snh * vx + cnh * vy;
     f[0] = (x - K.lo[0]) / K.step[0];
@@ -139,13 +141,16 @@ __device__ __forceinline__ bool successor64(const Consts64& K, const OccRead& oc
     return in;
 }
 
-// 64-corner weights in the reference's order d = 0..63, bit 0 = x ... bit
-// 5 = w; each weight is the left-to-right product of the six factors.
-__device__ __forceinline__ double corner_weight64(const double t[6], int d) {
-    double wgt = (d & 1) ? t[0] : 1.0 - t[0];
+// Weights of the 64 corners (d = 0..63, bit 0 = x ... bit 5 = w) as the
+// product lo[d & 7] * hi[d >> 3] of the first three axes' factors and the
+// last three's (the FP64 tables need the value, not the FP32 path's exact
+// left-to-right product order).
+__device__ __forceinline__ void corner_weights64(const double t[6], double lo[8], double hi[8]) {
 #pragma unroll
-    for (int k = 1; k < 6; ++k) wgt = wgt * (((d >> k) & 1) ? t[k] : 1.0 - t[k]);
-    return wgt;
+    for (int d = 0; d < 8; ++d) {
+        lo[d] = ((d & 1) ? t[0] : 1.0 - t[0]) * ((d & 2) ? t[1] : 1.0 - t[1]) * ((d & 4) ? t[2] : 1.0 - t[2]);
+        hi[d] = ((d & 1) ? t[3] : 1.0 - t[3]) * ((d & 2) ? t[4] : 1.0 - t[4]) * ((d & 4) ? t[5] : 1.0 - t[5]);
+    }
 }
 
 }  // namespace pgk

@@ -1,0 +1,109 @@
+// Internals of libperegrine_hip shared by its host source files.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <string>
+#include <vector>
+#include "../../include/peregrine_hip.h"
+#include "pg_kernels.h"
+
+namespace pgh {
+
+extern thread_local std::string g_err;
+// Records a formatted message for pg_last_error and returns code.
+int fail(int code, const char* fmt, ...);
+int64_t grid_cells(const pg_grid& g);
+
+constexpr int kLdsMaxCtl = 128;   // controls the LDS-staged sweeps hold
+
+struct Ooc;
+
+}  // namespace pgh
+
+#define HIPCHECK(expr)                                                                       \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return pgh::fail(PG_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));    \
+    } while (0)
+
+struct pg_ctx {
+    int32_t device = 0;
+    hipStream_t stream = nullptr;
+};
+
+struct pg_solver {
+    pg_ctx* ctx = nullptr;
+    pg_grid g{};
+    pg_model m{};
+    pg_params p{};
+    pgk::Consts K{};
+    pgk::FastConsts F{};
+    int32_t nctl = 0;
+    int32_t vstore = 0;
+    int64_t ncell = 0;
+    // FP32 tables: the Jacobi pair (FP16 store: [0] is the scratch the fast
+    // sweep writes, [1] exists once polishing starts)
+    float* dV[2] = {nullptr, nullptr};
+    int cur = 0;
+    // FP16 store
+    bool f16_active = false;      // V lives in the 16-bit codes (until polished)
+    int store = PG_STORE_F32;     // code type of the 16-bit store
+    uint16_t* d_V16 = nullptr;
+    float2* d_tp = nullptr;
+    unsigned* d_tstat = nullptr;
+    uint8_t* d_trefit = nullptr;
+    pgk::QuantGeom Q{};
+    uint8_t* d_occ = nullptr;
+    float* d_bu = nullptr;
+    int64_t* d_seeds = nullptr;
+    int64_t nseeds = 0;
+    std::vector<int64_t> h_seeds;
+    bool seeded = false;
+    double* d_part = nullptr;
+    int64_t part_cap = 0;
+    double* d_sum = nullptr;
+    unsigned* d_linf = nullptr;
+    uint8_t* d_pol = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    float last_ms = 0.0f;
+    // kernel selection and the LDS window plan
+    int32_t kernel = PG_KERNEL_GLOBAL;
+    bool diag = false;
+    pgk::BoxGeom G{};
+    int16_t* d_worig = nullptr;
+    bool planned = false;
+    int shape = 0;
+    double coverage = 0.0;
+    // FP64 tables (allocated on first use)
+    pgk::Consts64 K64{};
+    std::vector<double> bu64;
+    double* d_bu64 = nullptr;
+    double* dV64[2] = {nullptr, nullptr};
+    uint32_t* d_seedmask = nullptr;   // one bit per cell: the pg_fill_seed targets
+    bool seedmask_ok = false;
+    int cur64 = 0;
+    bool f64_valid = false;       // dV64[cur64] holds an FP64 table
+    // out-of-core: both tables of the Jacobi pair in pinned host memory,
+    // swept tile by tile (pg_ooc.cpp)
+    pgh::Ooc* ooc = nullptr;
+};
+
+
+namespace pgh {
+
+// The LDS window plan of the whole grid (pg_api.cpp): per-box window origins
+// on the device; with keep_host, origins and extents also on the host.
+int plan(pg_solver* s, bool keep_host = false, std::vector<int16_t>* h_orig = nullptr,
+         std::vector<uint8_t>* h_ext = nullptr);
+int ensure_partials(pg_solver* s, int64_t nb);
+
+// Out-of-core residency (pg_ooc.cpp).
+int ooc_create(pg_solver* s, int64_t budget_bytes, const int32_t tile[3]);
+void ooc_release(pg_solver* s);
+int ooc_fill_seed(pg_solver* s);
+int ooc_sweep(pg_solver* s, double* sum_decrease, float* linf);
+int ooc_download(pg_solver* s, float* dst, int64_t off, int64_t n);
+int ooc_upload(pg_solver* s, const float* src, int64_t off, int64_t n);
+int ooc_info(pg_solver* s, int32_t tile[3], int64_t* ntiles, int64_t* device_bytes, double* amplification);
+
+}  // namespace pgh

@@ -21,6 +21,15 @@ hipError_t pgk_launch_sweep_lds(int shape, int diag, const pgk::Consts& K, const
                                 const float* Vin, float* Vout, const uint8_t* occ, const float* bu,
                                 const int16_t* worig, uint8_t* pol, double* part, unsigned* linf,
                                 int write_v, hipStream_t st);
+// One tile of an out-of-core sweep: G is the tile's interior region; the
+// window (cells wo[a] .. wo[a] + wl[a] - 1 on x, y and heading modulo n3,
+// every velocity cell) holds the old V; results go to the interior buffer
+// `out` (and the argmin to `pol`, when given). *miss is raised if a backup
+// needed a cell outside the window.
+hipError_t pgk_launch_sweep_tile(int shape, int diag, const pgk::Consts& K, const pgk::BoxGeom& G,
+                                 const float* win, const int32_t wo[3], const int32_t wl[3], float* out,
+                                 uint8_t* pol, const uint8_t* occ, const float* bu, const int16_t* worig,
+                                 double* part, unsigned* linf, unsigned* miss, hipStream_t st);
 
 // Precision form (pg_sweep_fast.hip).
 hipError_t pgk_launch_sweep_fast(int shape, int diag, int store, const pgk::Consts& K,

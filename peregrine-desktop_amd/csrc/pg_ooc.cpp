@@ -1,0 +1,385 @@
+// Out-of-core residency: the Jacobi pair in pinned host memory, swept tile
+// by tile through a device window.
+//
+// Replaces the reference's tiled driver (solver/src/OutOfCore.jl:412-584,
+// tile planning solver/src/Tiles.jl:311-512) with a Jacobi design whose
+// sweep equals the in-HBM sweep bit for bit:
+//   * tiles cut x, y and heading (heading periodic); the velocity axes stay
+//     whole, as in Tiles.jl:330-333. A tile's interior is a multiple of the
+//     box edge on each cut axis, so its boxes are boxes of the whole grid and
+//     reuse the whole grid's window plan.
+//   * a tile's window is the union of its boxes' planned corner ranges -- the
+//     exact set of cells its backups read, from the same arithmetic that does
+//     the backups -- so no reach estimate or safety margin is involved, and
+//     the kernel raises a flag if a read ever falls outside it.
+//   * every backup reads the old table and writes the new one, so tile order
+//     and overlap change nothing: tile k's window is copied in (side stream)
+//     while tile k-1 is swept, and tile k-1's interior is copied out (a second
+//     side stream) behind it. Two window and two interior buffers alternate.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include "pg_internal.hpp"
+
+namespace pgh {
+
+struct OocTile {
+    int32_t t0[3], tn[3];   // interior origin and extent: x, y, heading
+    int32_t wo[3], wl[3];   // window origin (heading modulo n3) and extent
+    pgk::BoxGeom G;         // the interior's boxes
+};
+
+struct Ooc {
+    int32_t tile[3] = {0, 0, 0};
+    std::vector<OocTile> tiles;
+    float* hV[2] = {nullptr, nullptr};
+    int cur = 0;
+    float* dwin[2] = {nullptr, nullptr};
+    float* dout[2] = {nullptr, nullptr};
+    hipStream_t h2d = nullptr, d2h = nullptr;
+    hipEvent_t ev_h2d[2] = {nullptr, nullptr}, ev_k[2] = {nullptr, nullptr}, ev_d2h[2] = {nullptr, nullptr};
+    unsigned* d_miss = nullptr;
+    double* d_tsum = nullptr;
+    int64_t device_bytes = 0;
+    double amplification = 0.0;
+    int64_t colv = 0;        // cells per (x, y, heading): the velocity block
+};
+
+namespace {
+
+int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Boxes and window of every tile, from the whole grid's plan.
+void build_tiles(const pg_solver* s, Ooc& o, const std::vector<int16_t>& orig, const std::vector<uint8_t>& ext) {
+    const pg_grid& g = s->g;
+    const pgk::BoxGeom& GG = s->G;
+    const int n3 = g.n[2];
+    o.tiles.clear();
+    for (int32_t x0 = 0; x0 < g.n[0]; x0 += o.tile[0])
+        for (int32_t y0 = 0; y0 < g.n[1]; y0 += o.tile[1])
+            for (int32_t h0 = 0; h0 < n3; h0 += o.tile[2]) {
+                OocTile t{};
+                const int32_t start[3] = {x0, y0, h0};
+                for (int a = 0; a < 3; ++a) {
+                    t.t0[a] = start[a];
+                    t.tn[a] = std::min(o.tile[a], g.n[a] - start[a]);
+                }
+                pgk::BoxGeom& G = t.G;
+                G = GG;
+                int64_t nbox = 1;
+                for (int a = 0; a < 6; ++a) {
+                    G.org[a] = a < 3 ? t.t0[a] : 0;
+                    G.rn[a] = a < 3 ? t.tn[a] : g.n[a];
+                    G.nb[a] = (int32_t)ceil_div(G.rn[a], G.b[a]);
+                    nbox *= G.nb[a];
+                }
+                G.nbox = nbox;
+                // union of the boxes' corner ranges; heading relative to h0
+                int32_t lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
+                bool full[3] = {false, false, false};
+                const int32_t b0 = G.org[0] / G.b[0], b1 = G.org[1] / G.b[1], b2 = G.org[2] / G.b[2];
+                for (int32_t i0 = 0; i0 < G.nb[0]; ++i0)
+                    for (int32_t i1 = 0; i1 < G.nb[1]; ++i1)
+                        for (int32_t i2 = 0; i2 < G.nb[2]; ++i2) {
+                            const int64_t head = (((int64_t)(b0 + i0) * GG.nb[1] + (b1 + i1)) * GG.nb[2] + (b2 + i2));
+                            const int32_t sh = (b2 + i2) * G.b[2];   // the box's first heading
+                            const int64_t per = (int64_t)GG.nb[3] * GG.nb[4] * GG.nb[5];
+                            for (int64_t v = 0; v < per; ++v) {
+                                const size_t bx = (size_t)(head * per + v) * 6;
+                                for (int a = 0; a < 3; ++a) {
+                                    const int e = ext[bx + a];
+                                    if (e >= 255) full[a] = true;
+                                    int32_t l = orig[bx + a];
+                                    if (a == 2) {
+                                        // the plan measures a box's heading range from
+                                        // the box's own first heading, in (-n3/2, n3/2]
+                                        l = (l - sh) % n3;
+                                        if (l < 0) l += n3;
+                                        if (2 * l > n3) l -= n3;
+                                        l += sh - h0;
+                                    }
+                                    lo[a] = std::min(lo[a], l);
+                                    hi[a] = std::max(hi[a], l + e);
+                                }
+                            }
+                        }
+                for (int a = 0; a < 2; ++a) {
+                    if (full[a] || lo[a] > hi[a]) { lo[a] = 0; hi[a] = g.n[a]; }
+                    lo[a] = std::max(lo[a], 0);
+                    hi[a] = std::min(hi[a], g.n[a]);
+                    t.wo[a] = lo[a];
+                    t.wl[a] = hi[a] - lo[a];
+                }
+                if (full[2] || lo[2] > hi[2] || hi[2] - lo[2] >= n3) {
+                    t.wo[2] = 0;
+                    t.wl[2] = n3;
+                } else {
+                    t.wo[2] = ((h0 + lo[2]) % n3 + n3) % n3;
+                    t.wl[2] = hi[2] - lo[2];
+                }
+                o.tiles.push_back(t);
+            }
+}
+
+int64_t window_cells(const OocTile& t, int64_t colv) { return (int64_t)t.wl[0] * t.wl[1] * t.wl[2] * colv; }
+int64_t interior_cells(const OocTile& t, int64_t colv) { return (int64_t)t.tn[0] * t.tn[1] * t.tn[2] * colv; }
+
+// Host table rows -> window buffer, x plane by x plane; a heading range that
+// wraps is two copies.
+hipError_t copy_window_in(const pg_solver* s, const Ooc& o, const OocTile& t, const float* hV, float* dw,
+                          hipStream_t st) {
+    const pg_grid& g = s->g;
+    const int64_t colv = o.colv;
+    const int n3 = g.n[2];
+    const int32_t h_first = std::min(t.wl[2], n3 - t.wo[2]);
+    for (int32_t lx = 0; lx < t.wl[0]; ++lx) {
+        const int64_t x = t.wo[0] + lx;
+        for (int piece = 0; piece < 2; ++piece) {
+            const int32_t hs = piece == 0 ? t.wo[2] : 0;
+            const int32_t hl = piece == 0 ? h_first : t.wl[2] - h_first;
+            const int32_t lh = piece == 0 ? 0 : h_first;
+            if (hl <= 0) continue;
+            const float* src = hV + ((x * g.n[1] + t.wo[1]) * n3 + hs) * colv;
+            float* dst = dw + (((int64_t)lx * t.wl[1]) * t.wl[2] + lh) * colv;
+            hipError_t e = hipMemcpy2DAsync(dst, (size_t)t.wl[2] * colv * sizeof(float), src,
+                                            (size_t)n3 * colv * sizeof(float), (size_t)hl * colv * sizeof(float),
+                                            (size_t)t.wl[1], hipMemcpyHostToDevice, st);
+            if (e != hipSuccess) return e;
+        }
+    }
+    return hipSuccess;
+}
+
+// Interior buffer -> host table, x plane by x plane.
+hipError_t copy_interior_out(const pg_solver* s, const Ooc& o, const OocTile& t, const float* dout, float* hV,
+                             hipStream_t st) {
+    const pg_grid& g = s->g;
+    const int64_t colv = o.colv;
+    const int n3 = g.n[2];
+    for (int32_t lx = 0; lx < t.tn[0]; ++lx) {
+        const int64_t x = t.t0[0] + lx;
+        float* dst = hV + ((x * g.n[1] + t.t0[1]) * n3 + t.t0[2]) * colv;
+        const float* src = dout + ((int64_t)lx * t.tn[1] * t.tn[2]) * colv;
+        hipError_t e = hipMemcpy2DAsync(dst, (size_t)n3 * colv * sizeof(float), src,
+                                        (size_t)t.tn[2] * colv * sizeof(float), (size_t)t.tn[2] * colv * sizeof(float),
+                                        (size_t)t.tn[1], hipMemcpyDeviceToHost, st);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+int64_t fixed_device_bytes(const pg_solver* s) {
+    return s->G.nbox * 6 * (int64_t)sizeof(int16_t)                  // window plan
+           + (int64_t)s->g.n[0] * s->g.n[1] * s->g.n[2]              // occupancy
+           + 3 * (int64_t)s->nctl * (int64_t)sizeof(float);          // B u per control
+}
+
+}  // namespace
+
+int ooc_create(pg_solver* s, int64_t budget_bytes, const int32_t tile_req[3]) {
+    const pg_grid& g = s->g;
+    if (budget_bytes <= 0) return fail(PG_ERR_ARG, "budget_bytes must be positive");
+    std::vector<int16_t> orig;
+    std::vector<uint8_t> ext;
+    int rc = plan(s, true, &orig, &ext);
+    if (rc) return rc;
+    // the tiles need partial sums for the largest tile only
+    (void)hipFree(s->d_part);
+    s->d_part = nullptr;
+    s->part_cap = 0;
+    s->kernel = PG_KERNEL_LDS;
+    Ooc* o = new Ooc;
+    s->ooc = o;
+    o->colv = (int64_t)g.n[3] * g.n[4] * g.n[5];
+    const int64_t fixed = fixed_device_bytes(s);
+    // partial sums: one per workgroup of the largest tile, one total per tile
+    auto need = [&](Ooc& oc) {
+        int64_t win = 0, in = 0, blocks = 0;
+        for (const OocTile& t : oc.tiles) {
+            win = std::max(win, window_cells(t, oc.colv));
+            in = std::max(in, interior_cells(t, oc.colv));
+            blocks = std::max(blocks, ceil_div(t.G.nbox, 8) * 8);
+        }
+        return fixed + 2 * (win + in) * (int64_t)sizeof(float) + blocks * (int64_t)sizeof(double) +
+               (int64_t)oc.tiles.size() * (int64_t)sizeof(double);
+    };
+    auto even = [&](int32_t v, int a) {
+        const int32_t b = s->G.b[a];
+        v = std::max(v, b);
+        v = (v + b - 1) / b * b;
+        return std::min(v, g.n[a]);
+    };
+    if (tile_req[0] > 0 && tile_req[1] > 0 && tile_req[2] > 0) {
+        for (int a = 0; a < 3; ++a) o->tile[a] = even(tile_req[a], a);
+        build_tiles(s, *o, orig, ext);
+        o->device_bytes = need(*o);
+        if (o->device_bytes > budget_bytes)
+            return fail(PG_ERR_NOMEM, "tiles %dx%dx%d need %lld device bytes, over the budget of %lld", o->tile[0],
+                        o->tile[1], o->tile[2], (long long)o->device_bytes, (long long)budget_bytes);
+    } else {
+        // largest roughly square (in cells) x-y tile with the whole heading
+        // axis that fits, by bisection on the fraction of each axis
+        o->tile[2] = g.n[2];
+        double lo = 0.0, hi = 1.0;
+        bool any = false;
+        for (int it = 0; it < 24; ++it) {
+            const double f = it == 0 ? 1.0 : 0.5 * (lo + hi);
+            Ooc trial;
+            trial.colv = o->colv;
+            trial.tile[0] = even((int32_t)std::lround(f * g.n[0]), 0);
+            trial.tile[1] = even((int32_t)std::lround(f * g.n[1]), 1);
+            trial.tile[2] = g.n[2];
+            build_tiles(s, trial, orig, ext);
+            const int64_t b = need(trial);
+            if (b <= budget_bytes) {
+                any = true;
+                lo = f;
+                o->tile[0] = trial.tile[0];
+                o->tile[1] = trial.tile[1];
+                if (it == 0) break;
+            } else {
+                hi = f;
+            }
+        }
+        if (!any) return fail(PG_ERR_NOMEM, "no tile fits a device budget of %lld bytes", (long long)budget_bytes);
+        build_tiles(s, *o, orig, ext);
+        o->device_bytes = need(*o);
+    }
+    int64_t win = 0, in = 0, loaded = 0;
+    for (const OocTile& t : o->tiles) {
+        win = std::max(win, window_cells(t, o->colv));
+        in = std::max(in, interior_cells(t, o->colv));
+        loaded += window_cells(t, o->colv);
+    }
+    o->amplification = (double)loaded / (double)s->ncell;
+    hipError_t e = hipSuccess;
+    auto chk = [&](hipError_t x) { if (e == hipSuccess) e = x; };
+    for (int j = 0; j < 2; ++j) {
+        chk(hipHostMalloc((void**)&o->hV[j], (size_t)s->ncell * sizeof(float), hipHostMallocDefault));
+        chk(hipMalloc(&o->dwin[j], (size_t)win * sizeof(float)));
+        chk(hipMalloc(&o->dout[j], (size_t)in * sizeof(float)));
+        chk(hipEventCreateWithFlags(&o->ev_h2d[j], hipEventDisableTiming));
+        chk(hipEventCreateWithFlags(&o->ev_k[j], hipEventDisableTiming));
+        chk(hipEventCreateWithFlags(&o->ev_d2h[j], hipEventDisableTiming));
+    }
+    chk(hipStreamCreate(&o->h2d));
+    chk(hipStreamCreate(&o->d2h));
+    chk(hipMalloc(&o->d_miss, sizeof(unsigned)));
+    chk(hipMalloc(&o->d_tsum, o->tiles.size() * sizeof(double)));
+    int64_t blocks = 0;
+    for (const OocTile& t : o->tiles) blocks = std::max(blocks, ceil_div(t.G.nbox, 8) * 8);
+    if (e != hipSuccess)
+        return fail(e == hipErrorOutOfMemory ? PG_ERR_NOMEM : PG_ERR_HIP, "out-of-core buffers: %s",
+                    hipGetErrorString(e));
+    return ensure_partials(s, blocks);
+}
+
+void ooc_release(pg_solver* s) {
+    Ooc* o = s->ooc;
+    if (!o) return;
+    if (o->h2d) (void)hipStreamSynchronize(o->h2d);
+    if (o->d2h) (void)hipStreamSynchronize(o->d2h);
+    for (int j = 0; j < 2; ++j) {
+        (void)hipHostFree(o->hV[j]);
+        (void)hipFree(o->dwin[j]);
+        (void)hipFree(o->dout[j]);
+        if (o->ev_h2d[j]) (void)hipEventDestroy(o->ev_h2d[j]);
+        if (o->ev_k[j]) (void)hipEventDestroy(o->ev_k[j]);
+        if (o->ev_d2h[j]) (void)hipEventDestroy(o->ev_d2h[j]);
+    }
+    if (o->h2d) (void)hipStreamDestroy(o->h2d);
+    if (o->d2h) (void)hipStreamDestroy(o->d2h);
+    (void)hipFree(o->d_miss);
+    (void)hipFree(o->d_tsum);
+    delete o;
+    s->ooc = nullptr;
+}
+
+int ooc_fill_seed(pg_solver* s) {
+    Ooc& o = *s->ooc;
+    o.cur = 0;
+    std::fill(o.hV[0], o.hV[0] + s->ncell, s->p.cap);
+    for (int64_t i : s->h_seeds) o.hV[0][i] = 0.0f;
+    return PG_OK;
+}
+
+int ooc_sweep(pg_solver* s, double* sum_decrease, float* linf) {
+    Ooc& o = *s->ooc;
+    hipStream_t st = s->ctx->stream;
+    const float* hin = o.hV[o.cur];
+    float* hout = o.hV[1 - o.cur];
+    const int64_t nt = (int64_t)o.tiles.size();
+    HIPCHECK(hipMemsetAsync(s->d_linf, 0, sizeof(unsigned), st));
+    HIPCHECK(hipMemsetAsync(o.d_miss, 0, sizeof(unsigned), st));
+    HIPCHECK(hipEventRecord(s->ev0, st));
+    // the first window; later ones are issued one tile ahead
+    HIPCHECK(hipStreamWaitEvent(o.h2d, s->ev0, 0));
+    HIPCHECK(copy_window_in(s, o, o.tiles[0], hin, o.dwin[0], o.h2d));
+    HIPCHECK(hipEventRecord(o.ev_h2d[0], o.h2d));
+    for (int64_t k = 0; k < nt; ++k) {
+        const int j = (int)(k & 1);
+        const OocTile& t = o.tiles[(size_t)k];
+        // sweep tile k once its window is in and interior buffer j is free
+        HIPCHECK(hipStreamWaitEvent(st, o.ev_h2d[j], 0));
+        if (k >= 2) HIPCHECK(hipStreamWaitEvent(st, o.ev_d2h[j], 0));
+        HIPCHECK(pgk_launch_sweep_tile(s->shape, s->diag ? 1 : 0, s->K, t.G, o.dwin[j], t.wo, t.wl, o.dout[j],
+                                       nullptr, s->d_occ, s->d_bu, s->d_worig, s->d_part, s->d_linf, o.d_miss, st));
+        HIPCHECK(pgk_launch_reduce(s->d_part, ceil_div(t.G.nbox, 8) * 8, o.d_tsum + k, st));
+        HIPCHECK(hipEventRecord(o.ev_k[j], st));
+        // tile k+1's window, into the buffer tile k-1 has finished with
+        if (k + 1 < nt) {
+            const int jn = 1 - j;
+            if (k >= 1) HIPCHECK(hipStreamWaitEvent(o.h2d, o.ev_k[jn], 0));
+            HIPCHECK(copy_window_in(s, o, o.tiles[(size_t)k + 1], hin, o.dwin[jn], o.h2d));
+            HIPCHECK(hipEventRecord(o.ev_h2d[jn], o.h2d));
+        }
+        // tile k's interior out
+        HIPCHECK(hipStreamWaitEvent(o.d2h, o.ev_k[j], 0));
+        HIPCHECK(copy_interior_out(s, o, t, o.dout[j], hout, o.d2h));
+        HIPCHECK(hipEventRecord(o.ev_d2h[j], o.d2h));
+    }
+    std::vector<double> tsum((size_t)nt);
+    unsigned hl = 0, miss = 0;
+    HIPCHECK(hipMemcpyAsync(tsum.data(), o.d_tsum, (size_t)nt * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(&hl, s->d_linf, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(&miss, o.d_miss, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    HIPCHECK(hipStreamSynchronize(o.d2h));
+    HIPCHECK(hipEventRecord(s->ev1, st));
+    HIPCHECK(hipEventSynchronize(s->ev1));
+    HIPCHECK(hipEventElapsedTime(&s->last_ms, s->ev0, s->ev1));
+    if (miss) return fail(PG_ERR_STATE, "a backup read a cell outside its tile's window");
+    for (int64_t i : s->h_seeds) hout[i] = 0.0f;
+    o.cur = 1 - o.cur;
+    double d = 0.0;
+    for (double v : tsum) d += v;
+    float lf;
+    std::memcpy(&lf, &hl, 4);
+    if (sum_decrease) *sum_decrease = d;
+    if (linf) *linf = lf;
+    return PG_OK;
+}
+
+int ooc_download(pg_solver* s, float* dst, int64_t off, int64_t n) {
+    std::memcpy(dst, s->ooc->hV[s->ooc->cur] + off, (size_t)n * sizeof(float));
+    return PG_OK;
+}
+
+int ooc_upload(pg_solver* s, const float* src, int64_t off, int64_t n) {
+    Ooc& o = *s->ooc;
+    if (!s->seeded) std::fill(o.hV[o.cur], o.hV[o.cur] + s->ncell, s->p.cap);
+    std::memcpy(o.hV[o.cur] + off, src, (size_t)n * sizeof(float));
+    return PG_OK;
+}
+
+int ooc_info(pg_solver* s, int32_t tile[3], int64_t* ntiles, int64_t* device_bytes, double* amplification) {
+    const Ooc& o = *s->ooc;
+    if (tile)
+        for (int a = 0; a < 3; ++a) tile[a] = o.tile[a];
+    if (ntiles) *ntiles = (int64_t)o.tiles.size();
+    if (device_bytes) *device_bytes = o.device_bytes;
+    if (amplification) *amplification = o.amplification;
+    return PG_OK;
+}
+
+}  // namespace pgh

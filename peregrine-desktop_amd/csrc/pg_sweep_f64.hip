@@ -75,9 +75,9 @@ __global__ __launch_bounds__(256) void sweep_f64_kernel(
     BoxInfo B;
     int o[6];
     if (live) {
-        box_info(K.n, G, box, B);
+        box_info(G, box, B);
 #pragma unroll
-        for (int a = 0; a < 6; ++a) o[a] = worig[box * 6 + a];
+        for (int a = 0; a < 6; ++a) o[a] = worig[B.gbox * 6 + a];
         for (int j = threadIdx.x; j < W::CELLS; j += kThreads) {
             int l[6], gi[6];
             const bool on = window_cell<W>(K.n, o, j, l, gi);
@@ -146,23 +146,25 @@ __global__ __launch_bounds__(256) void sweep_f64_kernel(
                         l[a] = d;
                         inwin = inwin & ((unsigned)d <= (unsigned)(W::E[a] - 2));
                     }
-                    int base = 0;
-#pragma unroll
-                    for (int a = 0; a < 6; ++a) base += l[a] * W::S[a];
+                    double wl[8], wh[8];
+                    corner_weights64(t, wl, wh);
+                    // every corner value is finite (<= cap), so a zero weight
+                    // adds +0 and the reference's skip needs no branch
                     double acc = 0.0;
-                    for (int d = 0; d < 64; ++d) {
-                        const double wgt = corner_weight64(t, d);
-                        if (wgt == 0.0) continue;   // the reference skips zero weights
-                        double v;
-                        if (inwin) {
+                    if (inwin) {
+                        int base = 0;
+#pragma unroll
+                        for (int a = 0; a < 6; ++a) base += l[a] * W::S[a];
+#pragma unroll
+                        for (int d = 0; d < 64; ++d) {
                             const int off = ((d >> 0) & 1) * W::S1 + ((d >> 1) & 1) * W::S2 +
                                             ((d >> 2) & 1) * W::S3 + ((d >> 3) & 1) * W::S4 +
                                             ((d >> 4) & 1) * W::S5 + ((d >> 5) & 1) * W::S6;
-                            v = win[base + off];
-                        } else {
-                            v = corner_global64(K, Vin, i, d);
+                            acc = acc + wl[d & 7] * wh[d >> 3] * win[base + off];
                         }
-                        acc = acc + wgt * v;
+                    } else {
+                        for (int d = 0; d < 64; ++d)
+                            acc = acc + wl[d & 7] * wh[d >> 3] * corner_global64(K, Vin, i, d);
                     }
                     double wself = 0.0;
                     if (MODE == PG_F64_DIAGONAL) {

@@ -118,9 +118,9 @@ __global__ __launch_bounds__(256) void sweep_fast_kernel(
     int o[6];
     if (threadIdx.x == 0) { sm.bmin = 0x7f800000u; sm.bmax = 0u; }
     if (live) {
-        box_info(K.n, G, box, B);
+        box_info(G, box, B);
 #pragma unroll
-        for (int a = 0; a < 6; ++a) o[a] = worig[box * 6 + a];
+        for (int a = 0; a < 6; ++a) o[a] = worig[B.gbox * 6 + a];
         for (int j = threadIdx.x; j < W::CELLS; j += kThreads) {
             int l[6], gi[6];
             const bool on = window_cell<W>(K.n, o, j, l, gi);

@@ -88,7 +88,7 @@ __global__ __launch_bounds__(256) void plan_kernel(const uint8_t* __restrict__ o
     __shared__ BoxShared<MAXS> sm;
     __shared__ int mn[6], mx[6];
     BoxInfo B;
-    box_info(K.n, G, box, B);
+    box_info(G, box, B);
     if (threadIdx.x < 6) { mn[threadIdx.x] = 1 << 30; mx[threadIdx.x] = -(1 << 30); }
     setup_box<MAXS>(K, B, occ, bu_g, sm);
     __syncthreads();
@@ -139,21 +139,121 @@ __global__ __launch_bounds__(256) void plan_kernel(const uint8_t* __restrict__ o
                 o %= K.n[2];
                 if (o < 0) o += K.n[2];
             }
-            worig[box * 6 + a] = (int16_t)o;
+            worig[B.gbox * 6 + a] = (int16_t)o;
             const int ext = hi - lo + 2;
-            wext[box * 6 + a] = (uint8_t)(ext > 255 ? 255 : ext);
+            wext[B.gbox * 6 + a] = (uint8_t)(ext > 255 ? 255 : ext);
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Where a sweep reads V and writes its results: the whole table in HBM, or
+// (out-of-core) the window of one tile and that tile's interior.
+// ---------------------------------------------------------------------------
+struct DevSrc {
+    const float* __restrict__ V;
+    // a window cell to stage (may lie anywhere on the grid)
+    __device__ __forceinline__ float stage(const Consts& K, const int gi[6]) const { return read(K, gi); }
+    // a value the backup uses
+    __device__ __forceinline__ float read(const Consts& K, const int gi[6]) const {
+        int64_t fl = 0;
+#pragma unroll
+        for (int a = 0; a < 6; ++a) fl += (int64_t)gi[a] * K.stride[a];
+        return V[fl];
+    }
+};
+
+struct DevDst {
+    float* __restrict__ V;
+    uint8_t* __restrict__ pol;
+    __device__ __forceinline__ void put(const Consts& K, const int i[6], float v, int k) const {
+        int64_t fl = 0;
+#pragma unroll
+        for (int a = 0; a < 6; ++a) fl += (int64_t)i[a] * K.stride[a];
+        if (V) V[fl] = v;
+        if (pol) pol[fl] = (uint8_t)k;
+    }
+};
+
+// One tile's window: cells x in [o0, o0 + L0), y in [o1, o1 + L1), heading in
+// [o2, o2 + L2) modulo n3, every velocity cell; flat, omega fastest. The
+// window holds every corner any backup of the tile reads (it is the union
+// of the plan's per-box ranges), so a read outside it is a defect: it
+// returns cap and raises *miss. Staged cells outside it are never read as
+// corners and stage cap quietly.
+struct TileSrc {
+    const float* __restrict__ W;
+    int32_t o[3], L[3];
+    int64_t ws[6];
+    unsigned* miss;
+    __device__ __forceinline__ bool local(const Consts& K, const int gi[6], int64_t& fl) const {
+        int l2 = gi[2] - o[2];
+        l2 = l2 < 0 ? l2 + K.n[2] : l2;
+        const int l0 = gi[0] - o[0], l1 = gi[1] - o[1];
+        fl = (int64_t)l0 * ws[0] + (int64_t)l1 * ws[1] + (int64_t)l2 * ws[2] + (int64_t)gi[3] * ws[3] +
+             (int64_t)gi[4] * ws[4] + gi[5];
+        return ((unsigned)l0 < (unsigned)L[0]) & ((unsigned)l1 < (unsigned)L[1]) & ((unsigned)l2 < (unsigned)L[2]);
+    }
+    __device__ __forceinline__ float stage(const Consts& K, const int gi[6]) const {
+        int64_t fl;
+        return local(K, gi, fl) ? W[fl] : K.cap;
+    }
+    __device__ __forceinline__ float read(const Consts& K, const int gi[6]) const {
+        int64_t fl;
+        if (local(K, gi, fl)) return W[fl];
+        atomicOr(miss, 1u);
+        return K.cap;
+    }
+};
+
+// The tile's interior: cells from t0 (x, y, heading) over r cells, every
+// velocity cell; flat, omega fastest.
+struct TileDst {
+    float* __restrict__ V;
+    uint8_t* __restrict__ pol;
+    int32_t t0[3];
+    int64_t os[6];
+    __device__ __forceinline__ void put(const Consts& K, const int i[6], float v, int k) const {
+        const int64_t fl = (int64_t)(i[0] - t0[0]) * os[0] + (int64_t)(i[1] - t0[1]) * os[1] +
+                           (int64_t)(i[2] - t0[2]) * os[2] + (int64_t)i[3] * os[3] + (int64_t)i[4] * os[4] + i[5];
+        if (V) V[fl] = v;
+        if (pol) pol[fl] = (uint8_t)k;
+    }
+};
+
+// The 64 corners of the successor cell i[] with the reference's clamped /
+// periodic indices (PeregrineSolver.jl:455-463), read through src.
+template <class SRC>
+__device__ __forceinline__ void fetch_src(const Consts& K, const SRC& src, const int i[6], float vals[64]) {
+    int j0[6], j1[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        if (k == 2) {
+            j0[k] = (i[k] >= K.n[2]) ? i[k] - K.n[2] : i[k];
+            const int jj = i[k] + 1;
+            j1[k] = (jj >= K.n[2]) ? jj - K.n[2] : jj;
+        } else {
+            j0[k] = i[k];
+            j1[k] = (i[k] + 1 < K.n[k] - 1) ? i[k] + 1 : K.n[k] - 1;
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < 64; ++d) {
+        int c[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) c[k] = ((d >> k) & 1) ? j1[k] : j0[k];
+        vals[d] = src.read(K, c);
     }
 }
 
 // ---------------------------------------------------------------------------
 // Sweep
 // ---------------------------------------------------------------------------
-template <class W, bool DIAG, int MAXS, bool WRITE_V, bool WRITE_POL>
+template <class W, bool DIAG, int MAXS, bool WRITE_V, bool WRITE_POL, class SRC, class DST>
 __global__ __launch_bounds__(256) void sweep_lds_kernel(
-    const float* __restrict__ Vin, float* __restrict__ Vout, const uint8_t* __restrict__ occ,
+    SRC src, DST dst, const uint8_t* __restrict__ occ,
     const float* __restrict__ bu_g, Consts K, BoxGeom G, const int16_t* __restrict__ worig,
-    uint8_t* __restrict__ pol, double* __restrict__ part, unsigned* __restrict__ linf_bits) {
+    double* __restrict__ part, unsigned* __restrict__ linf_bits) {
     __shared__ float win[W::SIZE];
     __shared__ BoxShared<MAXS> sm;
     const int64_t box = box_of_block(G);
@@ -161,20 +261,14 @@ __global__ __launch_bounds__(256) void sweep_lds_kernel(
     BoxInfo B;
     int o[6];
     if (live) {
-        box_info(K.n, G, box, B);
+        box_info(G, box, B);
 #pragma unroll
-        for (int a = 0; a < 6; ++a) o[a] = worig[box * 6 + a];
+        for (int a = 0; a < 6; ++a) o[a] = worig[B.gbox * 6 + a];
         // stage the window; cells off the grid hold cap (read only with zero weight)
         for (int j = threadIdx.x; j < W::CELLS; j += kThreads) {
             int l[6], gi[6];
             const bool on = window_cell<W>(K.n, o, j, l, gi);
-            float v = K.cap;
-            if (on) {
-                int64_t fl = 0;
-#pragma unroll
-                for (int a = 0; a < 6; ++a) fl += (int64_t)gi[a] * K.stride[a];
-                v = Vin[fl];
-            }
+            const float v = on ? src.stage(K, gi) : K.cap;
             int lo = 0;
 #pragma unroll
             for (int a = 0; a < 6; ++a) lo += l[a] * W::S[a];
@@ -233,7 +327,7 @@ __global__ __launch_bounds__(256) void sweep_lds_kernel(
                             vals[d] = wp[off];
                         }
                     } else {
-                        fetch_global(K, Vin, i, vals);
+                        fetch_src(K, src, i, vals);
                     }
                     const float acc = corner_sum(Wt, vals);
                     const float nv = (acc < K.cap) ? acc : K.cap;
@@ -255,24 +349,21 @@ __global__ __launch_bounds__(256) void sweep_lds_kernel(
         for (int s = threadIdx.x; s < B.ns; s += kThreads) {
             int i[6];
             state_coords(B, s, i);
-            int64_t idx = 0;
-#pragma unroll
-            for (int a = 0; a < 6; ++a) idx += (int64_t)i[a] * K.stride[a];
             const unsigned long long key = sm.best[s];
             const float best = __uint_as_float((unsigned)(key >> 32));
             const int bk = (int)(key & 0xffffffffu);
+            float out = 0.0f;
             if (WRITE_V) {
-                const float old = Vin[idx];
-                float out = old;
+                const float old = src.read(K, i);
+                out = old;
                 if (best < old) {
                     out = best;
                     const float d = old - best;
                     dsum += (double)d;
                     dmax = fmaxf(dmax, d);
                 }
-                Vout[idx] = out;
             }
-            if (WRITE_POL) pol[idx] = (uint8_t)bk;
+            dst.put(K, i, out, bk);
         }
     }
     if (WRITE_V) block_reduce_sum_max(dsum, dmax, part, linf_bits);
@@ -286,21 +377,36 @@ __global__ __launch_bounds__(256) void sweep_lds_kernel(
 
 namespace {
 
-template <class W, bool DIAG, int MAXS>
-hipError_t launch_w(const pgk::Consts& K, const pgk::BoxGeom& G, const float* Vin, float* Vout,
-                    const uint8_t* occ, const float* bu, const int16_t* worig, uint8_t* pol,
-                    double* part, unsigned* linf, int write_v, hipStream_t st) {
+template <class W, bool DIAG, int MAXS, class SRC, class DST>
+hipError_t launch_w(const pgk::Consts& K, const pgk::BoxGeom& G, const SRC& src, const DST& dst,
+                    const uint8_t* occ, const float* bu, const int16_t* worig, double* part, unsigned* linf,
+                    int write_v, int write_pol, hipStream_t st) {
     const unsigned nb = (unsigned)(((G.nbox + 7) / 8) * 8);
-    if (write_v && pol)
-        hipLaunchKernelGGL((pgk::sweep_lds_kernel<W, DIAG, MAXS, true, true>), dim3(nb), dim3(256), 0,
-                           st, Vin, Vout, occ, bu, K, G, worig, pol, part, linf);
+    if (write_v && write_pol)
+        hipLaunchKernelGGL((pgk::sweep_lds_kernel<W, DIAG, MAXS, true, true, SRC, DST>), dim3(nb), dim3(256), 0,
+                           st, src, dst, occ, bu, K, G, worig, part, linf);
     else if (write_v)
-        hipLaunchKernelGGL((pgk::sweep_lds_kernel<W, DIAG, MAXS, true, false>), dim3(nb), dim3(256),
-                           0, st, Vin, Vout, occ, bu, K, G, worig, pol, part, linf);
+        hipLaunchKernelGGL((pgk::sweep_lds_kernel<W, DIAG, MAXS, true, false, SRC, DST>), dim3(nb), dim3(256),
+                           0, st, src, dst, occ, bu, K, G, worig, part, linf);
     else
-        hipLaunchKernelGGL((pgk::sweep_lds_kernel<W, DIAG, MAXS, false, true>), dim3(nb), dim3(256),
-                           0, st, Vin, Vout, occ, bu, K, G, worig, pol, part, linf);
+        hipLaunchKernelGGL((pgk::sweep_lds_kernel<W, DIAG, MAXS, false, true, SRC, DST>), dim3(nb), dim3(256),
+                           0, st, src, dst, occ, bu, K, G, worig, part, linf);
     return hipGetLastError();
+}
+
+template <class SRC, class DST>
+hipError_t launch_shape(int shape, int diag, const pgk::Consts& K, const pgk::BoxGeom& G, const SRC& src,
+                        const DST& dst, const uint8_t* occ, const float* bu, const int16_t* worig, double* part,
+                        unsigned* linf, int write_v, int write_pol, hipStream_t st) {
+    switch (shape * 2 + (diag ? 1 : 0)) {
+        case 0: return launch_w<pgk::Win0, false, 128>(K, G, src, dst, occ, bu, worig, part, linf, write_v, write_pol, st);
+        case 1: return launch_w<pgk::Win0, true, 128>(K, G, src, dst, occ, bu, worig, part, linf, write_v, write_pol, st);
+        case 2: return launch_w<pgk::Win1, false, 128>(K, G, src, dst, occ, bu, worig, part, linf, write_v, write_pol, st);
+        case 3: return launch_w<pgk::Win1, true, 128>(K, G, src, dst, occ, bu, worig, part, linf, write_v, write_pol, st);
+        case 4: return launch_w<pgk::Win2, false, 128>(K, G, src, dst, occ, bu, worig, part, linf, write_v, write_pol, st);
+        case 5: return launch_w<pgk::Win2, true, 128>(K, G, src, dst, occ, bu, worig, part, linf, write_v, write_pol, st);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 }  // namespace
@@ -335,16 +441,39 @@ hipError_t pgk_launch_sweep_lds(int shape, int diag, const pgk::Consts& K, const
                                 const float* Vin, float* Vout, const uint8_t* occ, const float* bu,
                                 const int16_t* worig, uint8_t* pol, double* part, unsigned* linf,
                                 int write_v, hipStream_t st) {
-    using W0 = pgk::Win0;
-    using W1 = pgk::Win1;
-    using W2 = pgk::Win2;
-    switch (shape * 2 + (diag ? 1 : 0)) {
-        case 0: return launch_w<W0, false, 128>(K, G, Vin, Vout, occ, bu, worig, pol, part, linf, write_v, st);
-        case 1: return launch_w<W0, true, 128>(K, G, Vin, Vout, occ, bu, worig, pol, part, linf, write_v, st);
-        case 2: return launch_w<W1, false, 128>(K, G, Vin, Vout, occ, bu, worig, pol, part, linf, write_v, st);
-        case 3: return launch_w<W1, true, 128>(K, G, Vin, Vout, occ, bu, worig, pol, part, linf, write_v, st);
-        case 4: return launch_w<W2, false, 128>(K, G, Vin, Vout, occ, bu, worig, pol, part, linf, write_v, st);
-        case 5: return launch_w<W2, true, 128>(K, G, Vin, Vout, occ, bu, worig, pol, part, linf, write_v, st);
-        default: return hipErrorInvalidValue;
+    const pgk::DevSrc src{Vin};
+    const pgk::DevDst dst{write_v ? Vout : nullptr, pol};
+    return launch_shape(shape, diag, K, G, src, dst, occ, bu, worig, part, linf, write_v, pol != nullptr, st);
+}
+
+hipError_t pgk_launch_sweep_tile(int shape, int diag, const pgk::Consts& K, const pgk::BoxGeom& G,
+                                 const float* win, const int32_t wo[3], const int32_t wl[3], float* out,
+                                 uint8_t* pol, const uint8_t* occ, const float* bu, const int16_t* worig,
+                                 double* part, unsigned* linf, unsigned* miss, hipStream_t st) {
+    pgk::TileSrc src;
+    src.W = win;
+    src.miss = miss;
+    pgk::TileDst dst;
+    dst.V = out;
+    dst.pol = pol;
+    for (int a = 0; a < 3; ++a) {
+        src.o[a] = wo[a];
+        src.L[a] = wl[a];
+        dst.t0[a] = G.org[a];
     }
+    // window strides: L0 x L1 x L2 x n4 x n5 x n6; interior: rn0 x rn1 x rn2 x ...
+    src.ws[5] = 1;
+    src.ws[4] = K.n[5];
+    src.ws[3] = (int64_t)K.n[4] * K.n[5];
+    src.ws[2] = src.ws[3] * K.n[3];
+    src.ws[1] = src.ws[2] * wl[2];
+    src.ws[0] = src.ws[1] * wl[1];
+    dst.os[5] = 1;
+    dst.os[4] = K.n[5];
+    dst.os[3] = (int64_t)K.n[4] * K.n[5];
+    dst.os[2] = dst.os[3] * K.n[3];
+    dst.os[1] = dst.os[2] * G.rn[2];
+    dst.os[0] = dst.os[1] * G.rn[1];
+    return launch_shape(shape, diag, K, G, src, dst, occ, bu, worig, part, linf, out != nullptr, pol != nullptr,
+                        st);
 }

@@ -89,7 +89,12 @@ class HipSolver:
     """One resident value table on the device."""
 
     def __init__(self, g: Grid6, m: Model, p: Params, ctl: np.ndarray, occ: np.ndarray,
-                 vstore: int = abi.PG_VSTORE_F32, ctx: Context | None = None):
+                 vstore: int = abi.PG_VSTORE_F32, ctx: Context | None = None,
+                 ooc_budget: int | None = None, ooc_tile=(0, 0, 0)):
+        """A solver with its tables in HBM, or with ooc_budget (device bytes)
+        an out-of-core one: tables in pinned host memory, swept by tiles of
+        ooc_tile = (nx, ny, nheading) interior cells ((0, 0, 0): the largest
+        that fits the budget)."""
         self.ctx = ctx or default_context()
         self.lib = self.ctx.lib
         self.g, self.m, self.p = g, m, p
@@ -107,10 +112,18 @@ class HipSolver:
         self._gc, self._mc, self._pc = g.c(), m.c_struct(), p.c()
         self._soa, self._occ = soa, occ
         self._h = ctypes.c_void_p()
-        abi.check(self.lib, "pg_solver_create",
-                  self.lib.pg_solver_create(self.ctx._h, ctypes.byref(self._gc), ctypes.byref(self._mc),
-                                            ctypes.byref(self._pc), soa.ctypes.data, nctl,
-                                            occ.ctypes.data, vstore, ctypes.byref(self._h)))
+        if ooc_budget is None:
+            abi.check(self.lib, "pg_solver_create",
+                      self.lib.pg_solver_create(self.ctx._h, ctypes.byref(self._gc), ctypes.byref(self._mc),
+                                                ctypes.byref(self._pc), soa.ctypes.data, nctl,
+                                                occ.ctypes.data, vstore, ctypes.byref(self._h)))
+        else:
+            tile = np.ascontiguousarray(ooc_tile, dtype=np.int32)
+            abi.check(self.lib, "pg_solver_create_ooc",
+                      self.lib.pg_solver_create_ooc(self.ctx._h, ctypes.byref(self._gc), ctypes.byref(self._mc),
+                                                    ctypes.byref(self._pc), soa.ctypes.data, nctl,
+                                                    occ.ctypes.data, int(ooc_budget), tile.ctypes.data,
+                                                    ctypes.byref(self._h)))
         self.ncells = g.ncells
 
     def close(self):
@@ -189,6 +202,17 @@ class HipSolver:
         abi.check(self.lib, "pg_download_V16", self.lib.pg_download_V16(self._h, out.ctypes.data, off, n))
         return out
 
+    def ooc_info(self):
+        """(tile interior, tile count, device bytes, cells loaded per cell
+        updated) of an out-of-core solver."""
+        tile = np.zeros(3, np.int32)
+        nt = ctypes.c_int64()
+        nb = ctypes.c_int64()
+        amp = ctypes.c_double()
+        abi.check(self.lib, "pg_ooc_info", self.lib.pg_ooc_info(self._h, tile.ctypes.data, ctypes.byref(nt),
+                                                                 ctypes.byref(nb), ctypes.byref(amp)))
+        return tuple(int(v) for v in tile), nt.value, nb.value, amp.value
+
     # FP64 tables ---------------------------------------------------------
     def solve_f64(self, iters: int, tol_linf: float, diagonal: bool = True):
         """FP64 table from cap and the seeds; returns (sweeps, last ||dV||_inf)."""
@@ -198,6 +222,14 @@ class HipSolver:
         abi.check(self.lib, "pg_solve_f64", self.lib.pg_solve_f64(self._h, mode, int(iters), float(tol_linf),
                                                                    ctypes.byref(it), ctypes.byref(l)))
         return it.value, l.value
+
+    def sweep_f64(self, diagonal: bool = True):
+        """One more FP64 sweep; returns (||dV||_inf, summed decrease)."""
+        l = ctypes.c_double()
+        d = ctypes.c_double()
+        mode = abi.PG_F64_SWEEP_DIAGONAL if diagonal else abi.PG_F64_SWEEP_PLAIN
+        abi.check(self.lib, "pg_sweep_f64", self.lib.pg_sweep_f64(self._h, mode, ctypes.byref(l), ctypes.byref(d)))
+        return l.value, d.value
 
     def residual_f64(self, which: int = 0):
         """(max, sum) of |T V - V| with T one FP64 backup; which 0 = FP64
