@@ -246,6 +246,21 @@ int pg_policy_f64(pg_solver* s, uint8_t* pol_host);
 /* Frees the FP64 tables. */
 int pg_release_f64(pg_solver* s);
 
+/* Adjoint. No counterpart in the reference (its only derivative is the Kuhn
+ * interpolant's gradient in the state, PeregrineSolver.jl:533-570). For
+ * L = mean over the nstarts states (row-major, 6 doubles each: x, y,
+ * heading, field vx, vy, omega) of the multilinear V at the state, with V
+ * the FP64 table of pg_solve_f64, returns L and dL/dtheta for the 30
+ * drivetrain parameters theta = B (9, row-major), A (9), q (3), diagonal of
+ * S (3), diagonal of D (3), c (3), the argmin control held fixed: the
+ * transposed policy-fixed interpolation system (I - P)^T lambda = g is
+ * iterated (FP64) until max |change| <= tol * max |lambda| (iters = -1 if
+ * max_iters ran out), then lambda is contracted with the successors'
+ * forward-mode derivatives. A start whose V is cap contributes cap to L and
+ * nothing to the gradient. */
+int pg_adjoint(pg_solver* s, const double* starts, int64_t nstarts, int32_t max_iters, double tol,
+               double* dL_dtheta, double* L, int32_t* iters);
+
 #ifdef __cplusplus
 }
 #endif

@@ -114,6 +114,8 @@ SIGNATURES = {
     "pg_download_V64": (ctypes.c_int, [_P, _P, _I64, _I64]),
     "pg_policy_f64": (ctypes.c_int, [_P, _P]),
     "pg_release_f64": (ctypes.c_int, [_P]),
+    "pg_adjoint": (ctypes.c_int, [_P, _P, _I64, _I32, ctypes.c_double, _P, ctypes.POINTER(ctypes.c_double),
+                                  ctypes.POINTER(_I32)]),
     "pg_upload_V": (ctypes.c_int, [_P, _P, _I64, _I64]),
     "pg_solver_set_kernel": (ctypes.c_int, [_P, _I32]),
     "pg_solver_kernel_info": (ctypes.c_int, [_P, ctypes.POINTER(_I32), ctypes.POINTER(ctypes.c_double)]),

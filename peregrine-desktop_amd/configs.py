@@ -129,3 +129,26 @@ def problem(cfg: int, shape=None, with_occ: bool = True) -> Problem:
 def algorithmic_bytes_per_update(nctl: int, bv: int) -> int:
     """Gathered bytes per state-update with no reuse: nctl*64*bV + 2*bV + 1."""
     return nctl * 64 * bv + 2 * bv + 1
+
+
+def adjoint_starts(prob: Problem, n: int = 4096) -> np.ndarray:
+    """The adjoint objective's start states: x, y uniform over the field,
+    heading uniform, velocity and spin zero, redrawn while the start's
+    nearest (x, y, heading) cell is blocked; numpy PCG64(4 + 100 * config).
+    Rows of (x, y, heading, vx, vy, omega)."""
+    r = rng_for(4, prob.cfg)
+    g = prob.grid
+    occ = np.asarray(prob.occ, np.uint8).reshape(g.n[0], g.n[1], g.n[2])
+    out = np.zeros((n, 6), np.float64)
+    k = 0
+    while k < n:
+        x, y = r.uniform(0.0, FIELD), r.uniform(0.0, FIELD)
+        h = r.uniform(-math.pi, math.pi)
+        ix = int(round((x - g.lo[0]) / g.step[0]))
+        iy = int(round((y - g.lo[1]) / g.step[1]))
+        ih = int(round((h - g.lo[2]) / g.step[2])) % g.n[2]
+        if occ[ix, iy, ih]:
+            continue
+        out[k, :3] = (x, y, h)
+        k += 1
+    return out

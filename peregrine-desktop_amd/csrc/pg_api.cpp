@@ -437,6 +437,7 @@ int create_solver(pg_ctx* ctx, const pg_grid* g, const pg_model* m, const pg_par
     // B * u_sat per control, in the reference's summation order
     std::vector<float> bu(3 * (size_t)nctl);
     s->bu64.assign(3 * (size_t)nctl, 0.0);
+    s->usat64.assign(3 * (size_t)nctl, 0.0);
     for (int k = 0; k < nctl; ++k) {
         const float u1r = ctl_soa[k], u2r = ctl_soa[k + nctl], u3r = ctl_soa[k + 2 * nctl];
         const float gn = traction_gain(u1r, u2r, u3r, m->knee);
@@ -450,6 +451,9 @@ int create_solver(pg_ctx* ctx, const pg_grid* g, const pg_model* m, const pg_par
         // the same in double for the FP64 tables
         const double g64 = traction_gain64(u1r, u2r, u3r, m->knee);
         const double w1 = u1r * g64, w2 = u2r * g64, w3 = u3r * g64;
+        s->usat64[3 * k] = w1;
+        s->usat64[3 * k + 1] = w2;
+        s->usat64[3 * k + 2] = w3;
         for (int r = 0; r < 3; ++r) {
             double a = (double)m->B[3 * r] * w1;
             a = a + (double)m->B[3 * r + 1] * w2;
@@ -934,6 +938,82 @@ int pg_policy_f64(pg_solver* s, uint8_t* pol_host) {
     rc = f64_pass(s, PG_F64_RESIDUAL, s->dV64[s->cur64], nullptr, s->d_pol, &l, &d);
     if (rc) return rc;
     HIPCHECK(hipMemcpy(pol_host, s->d_pol, (size_t)s->ncell, hipMemcpyDeviceToHost));
+    return PG_OK;
+}
+
+int pg_adjoint(pg_solver* s, const double* starts, int64_t nstarts, int32_t max_iters, double tol,
+               double* dL_dtheta, double* L, int32_t* iters) {
+    if (!s || !dL_dtheta || !L || (!starts && nstarts > 0)) return fail(PG_ERR_ARG, "null argument");
+    if (nstarts <= 0) return fail(PG_ERR_ARG, "nstarts must be positive");
+    if (!s->f64_valid) return fail(PG_ERR_STATE, "no FP64 table yet (pg_solve_f64)");
+    HIPCHECK(hipSetDevice(s->ctx->device));
+    int rc = f64_setup(s);
+    if (rc) return rc;
+    hipStream_t st = s->ctx->stream;
+    const int64_t n = s->ncell;
+    const double* V = s->dV64[s->cur64];
+    if (!s->d_pol) HIPCHECK(hipMalloc(&s->d_pol, (size_t)n));
+    float l32 = 0.0f;
+    double d = 0.0;
+    rc = f64_pass(s, PG_F64_RESIDUAL, V, nullptr, s->d_pol, &l32, &d);   // the FP64 argmin
+    if (rc) return rc;
+    const int64_t nbg = pgk_adj_grad_blocks(s->K64);
+    double *tcoord = nullptr, *self = nullptr, *g = nullptr, *lam = nullptr, *nxt = nullptr, *dst = nullptr,
+           *vst = nullptr, *part = nullptr, *usat = nullptr;
+    int16_t* bcoord = nullptr;
+    unsigned long long* stat = nullptr;
+    hipError_t e = hipSuccess;
+    auto chk = [&](hipError_t x) { if (e == hipSuccess) e = x; };
+    chk(hipMalloc(&tcoord, (size_t)n * 6 * sizeof(double)));
+    chk(hipMalloc(&bcoord, (size_t)n * 6 * sizeof(int16_t)));
+    chk(hipMalloc(&self, (size_t)n * sizeof(double)));
+    chk(hipMalloc(&g, (size_t)n * sizeof(double)));
+    chk(hipMalloc(&lam, (size_t)n * sizeof(double)));
+    chk(hipMalloc(&nxt, (size_t)n * sizeof(double)));
+    chk(hipMalloc(&dst, (size_t)nstarts * 6 * sizeof(double)));
+    chk(hipMalloc(&vst, (size_t)nstarts * sizeof(double)));
+    chk(hipMalloc(&part, (size_t)nbg * 30 * sizeof(double)));
+    chk(hipMalloc(&usat, s->usat64.size() * sizeof(double)));
+    chk(hipMalloc(&stat, 2 * sizeof(unsigned long long)));
+    int32_t it = 0;
+    bool done = false;
+    std::vector<double> vstart((size_t)nstarts), parts((size_t)nbg * 30);
+    if (e == hipSuccess) {
+        chk(hipMemcpyAsync(dst, starts, (size_t)nstarts * 6 * sizeof(double), hipMemcpyHostToDevice, st));
+        chk(hipMemcpyAsync(usat, s->usat64.data(), s->usat64.size() * sizeof(double), hipMemcpyHostToDevice, st));
+        chk(hipMemsetAsync(g, 0, (size_t)n * sizeof(double), st));
+        chk(hipMemsetAsync(nxt, 0, (size_t)n * sizeof(double), st));
+        chk(pgk_launch_adj_linearize(V, s->d_pol, s->d_seedmask, s->d_occ, s->d_bu64, s->K64, tcoord, bcoord, self, st));
+        chk(pgk_launch_adj_starts(V, dst, nstarts, s->K64, g, vst, st));
+        chk(hipMemcpyAsync(lam, g, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, st));
+        for (it = 1; it <= max_iters && e == hipSuccess; ++it) {
+            unsigned long long h[2] = {0, 0};
+            chk(hipMemsetAsync(stat, 0, sizeof h, st));
+            chk(pgk_launch_adj_iteration(lam, nxt, g, self, tcoord, bcoord, s->K64, stat, st));
+            chk(hipMemcpyAsync(h, stat, sizeof h, hipMemcpyDeviceToHost, st));
+            chk(hipStreamSynchronize(st));
+            double dch, mag;
+            std::memcpy(&dch, &h[0], 8);
+            std::memcpy(&mag, &h[1], 8);
+            if (dch <= tol * mag) { done = true; break; }
+        }
+        chk(pgk_launch_adj_grad(V, lam, s->d_pol, tcoord, s->d_occ, s->d_bu64, usat, s->K64, part, st));
+        chk(hipMemcpyAsync(parts.data(), part, parts.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+        chk(hipMemcpyAsync(vstart.data(), vst, vstart.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+        chk(hipStreamSynchronize(st));
+    }
+    (void)hipFree(tcoord); (void)hipFree(bcoord); (void)hipFree(self); (void)hipFree(g); (void)hipFree(lam);
+    (void)hipFree(nxt); (void)hipFree(dst); (void)hipFree(vst); (void)hipFree(part); (void)hipFree(usat);
+    (void)hipFree(stat);
+    if (e != hipSuccess)
+        return fail(e == hipErrorOutOfMemory ? PG_ERR_NOMEM : PG_ERR_HIP, "adjoint failed: %s", hipGetErrorString(e));
+    for (int j = 0; j < 30; ++j) dL_dtheta[j] = 0.0;
+    for (int64_t b = 0; b < nbg; ++b)
+        for (int j = 0; j < 30; ++j) dL_dtheta[j] += parts[(size_t)b * 30 + j];
+    double ls = 0.0;
+    for (double v : vstart) ls += v;
+    *L = ls / (double)nstarts;
+    if (iters) *iters = done ? (it > max_iters ? max_iters : it) : -1;
     return PG_OK;
 }
 

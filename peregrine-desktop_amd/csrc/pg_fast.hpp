@@ -151,6 +151,31 @@ __device__ __forceinline__ float tensor_interp(const float t[6], const float val
     return lerp_fast(l1a, l1b, t[0]);
 }
 
+// The same interpolant with the corners fetched on demand: fetch(d) returns
+// corner d (bit 0 = x ... bit 5 = w). Eight groups of eight corners (x, y,
+// h fixed within a group) are each reduced over omega, vy, vx as they are
+// loaded, then the eight results over h, y, x. The lerp tree is
+// tensor_interp's, so the value is identical; only eight corner values are
+// live at a time.
+template <class Fetch>
+__device__ __forceinline__ float tensor_interp_grouped(const Fetch& fetch, const float t[6]) {
+    float r[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const float a0 = lerp_fast(fetch(c), fetch(c | 32), t[5]);
+        const float a1 = lerp_fast(fetch(c | 16), fetch(c | 48), t[5]);
+        const float a2 = lerp_fast(fetch(c | 8), fetch(c | 40), t[5]);
+        const float a3 = lerp_fast(fetch(c | 24), fetch(c | 56), t[5]);
+        const float b0 = lerp_fast(a0, a1, t[4]);
+        const float b1 = lerp_fast(a2, a3, t[4]);
+        r[c] = lerp_fast(b0, b1, t[3]);
+    }
+    const float h0 = lerp_fast(r[0], r[4], t[2]), h1 = lerp_fast(r[1], r[5], t[2]);
+    const float h2 = lerp_fast(r[2], r[6], t[2]), h3 = lerp_fast(r[3], r[7], t[2]);
+    const float y0 = lerp_fast(h0, h2, t[1]), y1 = lerp_fast(h1, h3, t[1]);
+    return lerp_fast(y0, y1, t[0]);
+}
+
 #pragma clang fp contract(off)
 
 }  // namespace pgk

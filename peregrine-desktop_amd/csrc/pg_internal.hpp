@@ -77,6 +77,7 @@ struct pg_solver {
     // FP64 tables (allocated on first use)
     pgk::Consts64 K64{};
     std::vector<double> bu64;
+    std::vector<double> usat64;   // saturated commands per control, in double
     double* d_bu64 = nullptr;
     double* dV64[2] = {nullptr, nullptr};
     uint32_t* d_seedmask = nullptr;   // one bit per cell: the pg_fill_seed targets

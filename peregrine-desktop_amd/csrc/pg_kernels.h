@@ -65,3 +65,19 @@ hipError_t pgk_launch_seed_mask(uint32_t* mask, const int64_t* seeds, int64_t n,
 hipError_t pgk_launch_fill_f64(double* V, int64_t n, double v, hipStream_t st);
 hipError_t pgk_launch_seed_f64(double* V, const int64_t* seeds, int64_t n, hipStream_t st);
 hipError_t pgk_launch_widen(const float* V32, double* V64, int64_t n, hipStream_t st);
+
+// Adjoint (pg_adjoint.hip): rows of the policy-fixed system, the starts'
+// weights, one iteration of the transposed solve, and per-block gradient
+// partials (30 per block).
+hipError_t pgk_launch_adj_linearize(const double* V, const uint8_t* pol, const uint32_t* seedmask, const uint8_t* occ,
+                                    const double* bu, const pgk::Consts64& K, double* tcoord, int16_t* bcoord,
+                                    double* self, hipStream_t st);
+hipError_t pgk_launch_adj_starts(const double* V, const double* starts, int64_t n, const pgk::Consts64& K, double* g,
+                                 double* vstart, hipStream_t st);
+hipError_t pgk_launch_adj_iteration(double* lam, double* nxt, const double* g, const double* self, const double* tcoord,
+                                    const int16_t* bcoord, const pgk::Consts64& K, unsigned long long* stat,
+                                    hipStream_t st);
+int64_t pgk_adj_grad_blocks(const pgk::Consts64& K);
+hipError_t pgk_launch_adj_grad(const double* V, const double* lam, const uint8_t* pol, const double* tcoord,
+                               const uint8_t* occ, const double* bu, const double* usat, const pgk::Consts64& K,
+                               double* part, hipStream_t st);

@@ -101,7 +101,7 @@ struct BoxShared {
 }  // namespace fastk
 
 template <class W, bool DIAG, int STORE, int MAXS, bool WRITE_V, bool WRITE_POL>
-__global__ __launch_bounds__(256) void sweep_fast_kernel(
+__global__ __launch_bounds__(256, 4) void sweep_fast_kernel(
     fastk::VRead<STORE> vin, const float* vself, float* Vout, const uint8_t* __restrict__ occ,
     const float* __restrict__ bu_g, Consts K, FastConsts F, BoxGeom G, QuantGeom Q,
     const int16_t* __restrict__ worig, uint8_t* __restrict__ pol, double* __restrict__ part,
@@ -206,40 +206,41 @@ __global__ __launch_bounds__(256) void sweep_fast_kernel(
                             wself *= d == 0 ? 1.0f - t[a] : (d == 1 ? t[a] : 0.0f);
                         }
                     }
-                    float vals[64];
+                    float acc;
                     if (inwin) {
                         int base = 0;
 #pragma unroll
                         for (int a = 0; a < 6; ++a) base += l[a] * W::S[a];
                         const float* wp = win + base;
-#pragma unroll
-                        for (int d = 0; d < 64; ++d) {
-                            const int off = ((d >> 0) & 1) * W::S1 + ((d >> 1) & 1) * W::S2 +
-                                            ((d >> 2) & 1) * W::S3 + ((d >> 3) & 1) * W::S4 +
-                                            ((d >> 4) & 1) * W::S5 + ((d >> 5) & 1) * W::S6;
-                            vals[d] = wp[off];
-                        }
+                        acc = tensor_interp_grouped(
+                            [&](int d) {
+                                return wp[((d >> 0) & 1) * W::S1 + ((d >> 1) & 1) * W::S2 + ((d >> 2) & 1) * W::S3 +
+                                          ((d >> 3) & 1) * W::S4 + ((d >> 4) & 1) * W::S5 + ((d >> 5) & 1) * W::S6];
+                            },
+                            t);
                     } else {
+                        // corners outside the window: global memory, the
+                        // reference's clamped / periodic indices
+                        acc = tensor_interp_grouped(
+                            [&](int d) {
+                                int gi[6];
+                                int64_t fl = 0;
 #pragma unroll
-                        for (int d = 0; d < 64; ++d) {
-                            int gi[6];
-                            int64_t fl = 0;
-#pragma unroll
-                            for (int a = 0; a < 6; ++a) {
-                                int j = i[a] + ((d >> a) & 1);
-                                if (a == 2) {
-                                    j = j >= K.n[2] ? j - K.n[2] : j;
-                                    j = j >= K.n[2] ? j - K.n[2] : j;
-                                } else {
-                                    j = j < K.n[a] - 1 ? j : K.n[a] - 1;
+                                for (int a = 0; a < 6; ++a) {
+                                    int j = i[a] + ((d >> a) & 1);
+                                    if (a == 2) {
+                                        j = j >= K.n[2] ? j - K.n[2] : j;
+                                        j = j >= K.n[2] ? j - K.n[2] : j;
+                                    } else {
+                                        j = j < K.n[a] - 1 ? j : K.n[a] - 1;
+                                    }
+                                    gi[a] = j;
+                                    fl += (int64_t)j * K.stride[a];
                                 }
-                                gi[a] = j;
-                                fl += (int64_t)j * K.stride[a];
-                            }
-                            vals[d] = vin(K, Q, gi, fl);
-                        }
+                                return vin(K, Q, gi, fl);
+                            },
+                            t);
                     }
-                    const float acc = tensor_interp(t, vals);
                     if (SELF && wself > 0.0f) {
                         // V = tau + w V + rest solved for V (Jacobi on the
                         // policy system with its diagonal): same fixed point

@@ -250,7 +250,7 @@ __device__ __forceinline__ void fetch_src(const Consts& K, const SRC& src, const
 // Sweep
 // ---------------------------------------------------------------------------
 template <class W, bool DIAG, int MAXS, bool WRITE_V, bool WRITE_POL, class SRC, class DST>
-__global__ __launch_bounds__(256) void sweep_lds_kernel(
+__global__ __launch_bounds__(256, 4) void sweep_lds_kernel(
     SRC src, DST dst, const uint8_t* __restrict__ occ,
     const float* __restrict__ bu_g, Consts K, BoxGeom G, const int16_t* __restrict__ worig,
     double* __restrict__ part, unsigned* __restrict__ linf_bits) {

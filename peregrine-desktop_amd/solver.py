@@ -251,6 +251,18 @@ class HipSolver:
         abi.check(self.lib, "pg_policy_f64", self.lib.pg_policy_f64(self._h, out.ctypes.data))
         return out
 
+    def adjoint(self, starts: np.ndarray, max_iters: int = 100000, tol: float = 1e-12):
+        """(L, dL/dtheta (30), iterations) at the FP64 table; L is the mean
+        multilinear V over the starts (rows of x, y, heading, vx, vy, omega)."""
+        st = np.ascontiguousarray(starts, dtype=np.float64).reshape(-1, 6)
+        grad = np.zeros(30, np.float64)
+        L = ctypes.c_double()
+        it = ctypes.c_int32()
+        abi.check(self.lib, "pg_adjoint", self.lib.pg_adjoint(self._h, st.ctypes.data, st.shape[0], int(max_iters),
+                                                               float(tol), grad.ctypes.data, ctypes.byref(L),
+                                                               ctypes.byref(it)))
+        return L.value, grad, it.value
+
     def release_f64(self) -> None:
         abi.check(self.lib, "pg_release_f64", self.lib.pg_release_f64(self._h))
 
