@@ -139,6 +139,25 @@ class Unsupported(PeregrineError):
 _lib = None
 
 
+def _share_torch_runtime() -> None:
+    """Let torch (when it is installed) bring up its HIP runtime first.
+
+    torch ships its own libamdhip64 with the same soname as the system one.
+    Loaded first, this library binds the system runtime and torch then loads
+    a second runtime that sees no GPU; loaded second, it binds torch's, and
+    both share one runtime. Without torch nothing changes.
+    """
+    try:
+        import torch
+    except Exception:
+        return
+    try:
+        if getattr(torch.version, "hip", None):
+            torch.cuda.is_available()
+    except Exception:
+        pass
+
+
 def load(path: os.PathLike | str | None = None) -> ctypes.CDLL:
     """Load libperegrine_hip.so from the package directory (or `path`)."""
     global _lib
@@ -148,6 +167,7 @@ def load(path: os.PathLike | str | None = None) -> ctypes.CDLL:
     if not p.exists():
         raise FileNotFoundError(
             f"{p} is not built; run __graft_entry__.build() (make -C csrc) first")
+    _share_torch_runtime()
     lib = ctypes.CDLL(str(p))
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)

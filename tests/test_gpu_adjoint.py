@@ -45,7 +45,7 @@ def test_device_adjoint_against_central_differences(pg):
     L, grad, it, _ = fp64_L(pg, prob, prob.model, starts)
     th = prob.model.theta()
     for j in (0, 4, 8, 9, 13, 17, 19, 21, 25, 27, 29):
-        h = 1e-3 * max(abs(th[j]), 1e-2)
+        h = 1e-4 * max(abs(th[j]), 1e-2)
         tp, tm = th.copy(), th.copy()
         tp[j] += h
         tm[j] -= h
