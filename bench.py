@@ -19,6 +19,11 @@ the same problem and reports both forms' errors against it on a seeded
 --f64-only solves just the FP64 table and writes the subsample (indices,
 FP64 values, FP64 policy) to --f64-out; --f64-sample reads such a file in
 place of the FP64 solve.
+
+--ooc runs the out-of-core sweep instead (default workload: problem 4,
+96x96x64x17x17x21, device budget 8 GB, 16x16x8 tiles): --steps tiled sweeps
+from the seeded start, timed, each compared bit for bit with the in-HBM sweep
+of the same grid (--ooc-compare, on by default when it fits).
 """
 from __future__ import annotations
 
@@ -176,6 +181,69 @@ def accuracy(pg, prob, args, tables):
     return out
 
 
+def run_ooc(pg, prob, args) -> dict:
+    """Tiled sweeps from pinned host memory against in-HBM sweeps."""
+    g = prob.grid
+    tile = tuple(int(v) for v in args.ooc_tile.split(","))
+    t = time.perf_counter()
+    s = pg.HipSolver(g, prob.model, prob.params, prob.ctl, prob.occ, ooc_budget=int(args.ooc_budget), ooc_tile=tile)
+    info = s.ooc_info()
+    print(f"# ooc: tiles {info[0]} x {info[1]}, {info[2] / 1e9:.2f} GB on the device, amplification {info[3]:.2f}, "
+          f"set up in {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
+    tl = s.ooc_tiles()
+    wl = tl[:, 9:12]
+    print(f"# ooc windows: x {wl[:, 0].min()}..{wl[:, 0].max()} (median {int(np.median(wl[:, 0]))}), "
+          f"y {wl[:, 1].min()}..{wl[:, 1].max()}, heading {wl[:, 2].min()}..{wl[:, 2].max()} "
+          f"(median {int(np.median(wl[:, 2]))})", file=sys.stderr, flush=True)
+    big = np.argsort(-wl.prod(axis=1))[:3]
+    for k in big:
+        print(f"#   tile {k}: interior {tl[k, 0:3].tolist()}+{tl[k, 3:6].tolist()} window {tl[k, 6:9].tolist()}+"
+              f"{tl[k, 9:12].tolist()}", file=sys.stderr, flush=True)
+    if args.steps == 0:
+        s.close()
+        return {"ooc_windows_only": True}
+    def note(msg):
+        print(f"# ooc {msg} ({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+
+    t0 = time.perf_counter()
+    s.fill_seed(prob.seeds)
+    note("seeded")
+    ref = None
+    if not args.no_ooc_compare:
+        ref = pg.HipSolver(g, prob.model, prob.params, prob.ctl, prob.occ)
+        ref.set_kernel(pg.abi.PG_KERNEL_LDS)
+        ref.fill_seed(prob.seeds)
+        note("in-HBM reference seeded")
+    steps = args.steps if args.steps is not None else 3
+    ms, equal = [], []
+    for i in range(steps):
+        t = time.perf_counter()
+        r = s.sweep(i, i % 2 == 1)
+        ms.append((time.perf_counter() - t) * 1e3)
+        line = f"# ooc sweep {i}: {ms[-1]:.0f} ms, linf {r.linf:.4f}"
+        print(line, file=sys.stderr, flush=True)
+        if ref is not None:
+            rr = ref.sweep(i, i % 2 == 1)
+            note(f"in-HBM sweep {i}: {rr.kernel_ms:.0f} ms")
+            a = s.download().view(np.uint32)
+            note("tiled table downloaded")
+            b = ref.download().view(np.uint32)
+            note("in-HBM table downloaded")
+            same = bool(np.array_equal(a, b))
+            del a, b
+            equal.append(same and rr.linf == r.linf)
+            note(f"sweep {i}: tables {'bit-identical' if equal[-1] else 'DIFFER'}")
+    s.close()
+    if ref is not None:
+        ref.close()
+    return {"metric": "out-of-core tiled sweep", "value": g.ncells / (float(np.mean(ms)) / 1e3),
+            "unit": "state-updates/s", "ms_per_step": float(np.mean(ms)), "steps": steps,
+            "config": {"workload": f"problem {args.config}", "grid": list(g.n), "states": g.ncells,
+                       "controls": prob.nctl, "budget_bytes": int(args.ooc_budget), "tile": list(info[0]),
+                       "tiles": info[1], "device_bytes": info[2], "amplification": info[3]},
+            "bit_identical_to_in_hbm": (all(equal) if equal else None)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -189,6 +257,10 @@ def main() -> int:
     ap.add_argument("--f64-max-sweeps", type=int, default=1500)
     ap.add_argument("--polish", type=int, default=3)
     ap.add_argument("--f64-only", action="store_true")
+    ap.add_argument("--ooc", action="store_true")
+    ap.add_argument("--ooc-budget", type=float, default=8e9)
+    ap.add_argument("--ooc-tile", default="16,16,8")
+    ap.add_argument("--no-ooc-compare", action="store_true")
     ap.add_argument("--f64-out", default=None)
     ap.add_argument("--f64-sample", default=None)
     ap.add_argument("--no-cpu", action="store_true")
@@ -225,6 +297,9 @@ def main() -> int:
     print(f"# problem {args.config}: grid {g.n} = {g.ncells} states, {prob.nctl} controls, "
           f"occupied {float(prob.occ.mean()):.3f}, built in {time.perf_counter() - t0:.1f} s",
           file=sys.stderr, flush=True)
+    if args.ooc:
+        print(json.dumps(run_ooc(pg, prob, args)), flush=True)
+        return 0
     if args.f64_only:
         idx, V64, P64, info = f64_sample(pg, prob, args)
         if args.f64_out:

@@ -214,6 +214,9 @@ int pg_solver_create_ooc(pg_ctx* ctx, const pg_grid* g, const pg_model* m, const
 /* The tile interior used, the tile count, the device bytes the tiled sweep
  * holds and the cells loaded per cell updated. */
 int pg_ooc_info(pg_solver* s, int32_t tile[3], int64_t* ntiles, int64_t* device_bytes, double* amplification);
+/* Tile k: interior origin t0 and extent tn, window origin wo (heading modulo
+ * n3) and extent wl, each on x, y, heading. */
+int pg_ooc_tile(pg_solver* s, int64_t k, int32_t* t0, int32_t* tn, int32_t* wo, int32_t* wl);
 
 /* FP64 tables. No counterpart in the reference, whose tables are Float32
  * only: these are the FP64 yardstick the FP32 and 16-bit tables are measured

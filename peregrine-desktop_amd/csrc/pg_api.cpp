@@ -418,6 +418,11 @@ int create_solver(pg_ctx* ctx, const pg_grid* g, const pg_model* m, const pg_par
         return fail(PG_ERR_UNSUPPORTED, "the 16-bit stores sweep at most %d controls", kLdsMaxCtl);
     for (int k = 0; k < 3; ++k)
         if (!(m->eps[k] > 0.0f)) return fail(PG_ERR_ARG, "coulomb eps[%d] must be positive", k);
+    // element-wise kernels run one work-item per cell, and a dispatch counts
+    // work-items in 32 bits; larger grids are solved out of core
+    if (!host_tables && grid_cells(*g) >= ((int64_t)1 << 32))
+        return fail(PG_ERR_UNSUPPORTED, "%lld cells: an in-HBM solver holds fewer than 2^32 cells "
+                    "(use pg_solver_create_ooc)", (long long)grid_cells(*g));
     HIPCHECK(hipSetDevice(ctx->device));
 
     pg_solver* s = new pg_solver;
@@ -531,6 +536,12 @@ int pg_solver_create_ooc(pg_ctx* ctx, const pg_grid* g, const pg_model* m, const
         g_err = msg;
     }
     return rc;
+}
+
+int pg_ooc_tile(pg_solver* s, int64_t k, int32_t* t0, int32_t* tn, int32_t* wo, int32_t* wl) {
+    if (!s || !t0 || !tn || !wo || !wl) return fail(PG_ERR_ARG, "null argument");
+    if (!s->ooc) return fail(PG_ERR_STATE, "not an out-of-core solver");
+    return pgh::ooc_tile(s, k, t0, tn, wo, wl);
 }
 
 int pg_ooc_info(pg_solver* s, int32_t tile[3], int64_t* ntiles, int64_t* device_bytes, double* amplification) {

@@ -37,7 +37,27 @@ using Win2 = Win<4, 4, 4, 5, 5, 5>;
 // each XCD walks neighbouring boxes (whose windows overlap) through its L2.
 __device__ __forceinline__ int64_t box_of_block(const BoxGeom& G) {
     const int64_t per = (G.nbox + 7) / 8;
-    return (int64_t)(blockIdx.x % 8) * per + (int64_t)(blockIdx.x / 8);
+    const int64_t b = G.blk0 + (int64_t)blockIdx.x;
+    return (b % 8) * per + b / 8;
+}
+
+// A dispatch counts its work-items in 32 bits, so a grid of boxes is
+// launched in chunks of at most 2^23 workgroups (2^31 work-items); each
+// chunk's geometry carries its first workgroup in blk0.
+constexpr int64_t kMaxBoxBlocks = (int64_t)1 << 23;
+
+template <class Launch>
+inline hipError_t for_box_chunks(const BoxGeom& G, Launch launch) {
+    const int64_t nb = ((G.nbox + 7) / 8) * 8;
+    for (int64_t b0 = 0; b0 < nb; b0 += kMaxBoxBlocks) {
+        BoxGeom g = G;
+        g.blk0 = b0;
+        const int64_t n = nb - b0 < kMaxBoxBlocks ? nb - b0 : kMaxBoxBlocks;
+        launch(g, (unsigned)n);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 struct BoxInfo {

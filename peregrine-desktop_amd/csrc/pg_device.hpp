@@ -402,6 +402,7 @@ struct BoxGeom {
     int32_t org[6];   // first cell of the region per axis (a multiple of b)
     int32_t rn[6];    // cells of the region per axis
     int32_t nbg[6];   // boxes per axis over the whole grid
+    int64_t blk0;     // first workgroup of this launch (launches are chunked)
 };
 
 // Block-wide sum (FP64) and max of the per-cell decreases: wave shuffles,

@@ -382,4 +382,17 @@ int ooc_info(pg_solver* s, int32_t tile[3], int64_t* ntiles, int64_t* device_byt
     return PG_OK;
 }
 
+int ooc_tile(pg_solver* s, int64_t k, int32_t t0[3], int32_t tn[3], int32_t wo[3], int32_t wl[3]) {
+    const Ooc& o = *s->ooc;
+    if (k < 0 || k >= (int64_t)o.tiles.size()) return fail(PG_ERR_ARG, "tile %lld out of range", (long long)k);
+    const OocTile& t = o.tiles[(size_t)k];
+    for (int a = 0; a < 3; ++a) {
+        t0[a] = t.t0[a];
+        tn[a] = t.tn[a];
+        wo[a] = t.wo[a];
+        wl[a] = t.wl[a];
+    }
+    return PG_OK;
+}
+
 }  // namespace pgh

@@ -224,7 +224,7 @@ __global__ __launch_bounds__(256) void sweep_f64_kernel(
             if (WRITE_POL) pol[idx] = (uint8_t)(sm.key[s] & 0xffu);
         }
     }
-    if (part) block_reduce_sum_max(dsum, dmax, part, linf_bits);
+    if (part) block_reduce_sum_max(dsum, dmax, part + G.blk0, linf_bits);
 }
 
 __global__ void fill_f64_kernel(double* __restrict__ V, int64_t n, double v) {
@@ -255,14 +255,14 @@ template <class W, int MODE>
 hipError_t launch_f64_w(const pgk::Consts64& K, const pgk::BoxGeom& G, const double* Vin, double* Vout,
                         const uint8_t* occ, const double* bu, const int16_t* worig, const uint32_t* seedmask,
                         uint8_t* pol, double* part, unsigned* linf, hipStream_t st) {
-    const unsigned nb = (unsigned)(((G.nbox + 7) / 8) * 8);
-    if (pol)
-        hipLaunchKernelGGL((pgk::sweep_f64_kernel<W, MODE, 128, true>), dim3(nb), dim3(256), 0, st, Vin, Vout,
-                           occ, bu, K, G, worig, seedmask, pol, part, linf);
-    else
-        hipLaunchKernelGGL((pgk::sweep_f64_kernel<W, MODE, 128, false>), dim3(nb), dim3(256), 0, st, Vin, Vout,
-                           occ, bu, K, G, worig, seedmask, pol, part, linf);
-    return hipGetLastError();
+    return pgk::for_box_chunks(G, [&](const pgk::BoxGeom& g, unsigned nb) {
+        if (pol)
+            hipLaunchKernelGGL((pgk::sweep_f64_kernel<W, MODE, 128, true>), dim3(nb), dim3(256), 0, st, Vin, Vout,
+                               occ, bu, K, g, worig, seedmask, pol, part, linf);
+        else
+            hipLaunchKernelGGL((pgk::sweep_f64_kernel<W, MODE, 128, false>), dim3(nb), dim3(256), 0, st, Vin, Vout,
+                               occ, bu, K, g, worig, seedmask, pol, part, linf);
+    });
 }
 
 template <int MODE>

@@ -298,7 +298,7 @@ __global__ __launch_bounds__(256, 4) void sweep_fast_kernel(
             atomicMin(&sm.bmin, lmin);
             atomicMax(&sm.bmax, lmax);
         }
-        block_reduce_sum_max(dsum, dmax, part, linf_bits);
+        block_reduce_sum_max(dsum, dmax, part + G.blk0, linf_bits);
         if (STORE != PG_STORE_F32 && live && threadIdx.x == 0 && sm.bmin <= sm.bmax) {
             int64_t t = 0;
 #pragma unroll
@@ -431,7 +431,6 @@ hipError_t launch_fast_w(const pgk::Consts& K, const pgk::FastConsts& F, const p
                          const pgk::QuantGeom& Q, const void* Vin, const float2* tp, const float* vself, float* Vout,
                          const uint8_t* occ, const float* bu, const int16_t* worig, uint8_t* pol,
                          double* part, unsigned* linf, unsigned* tstat, int write_v, hipStream_t st) {
-    const unsigned nb = (unsigned)(((G.nbox + 7) / 8) * 8);
     pgk::fastk::VRead<STORE> rd;
     if constexpr (STORE == PG_STORE_F32) {
         rd.V = (const float*)Vin;
@@ -439,16 +438,17 @@ hipError_t launch_fast_w(const pgk::Consts& K, const pgk::FastConsts& F, const p
         rd.V = (const uint16_t*)Vin;
         rd.tp = tp;
     }
-    if (write_v && pol)
-        hipLaunchKernelGGL((pgk::sweep_fast_kernel<W, DIAG, STORE, 128, true, true>), dim3(nb), dim3(256), 0, st,
-                           rd, vself, Vout, occ, bu, K, F, G, Q, worig, pol, part, linf, tstat);
-    else if (write_v)
-        hipLaunchKernelGGL((pgk::sweep_fast_kernel<W, DIAG, STORE, 128, true, false>), dim3(nb), dim3(256), 0, st,
-                           rd, vself, Vout, occ, bu, K, F, G, Q, worig, pol, part, linf, tstat);
-    else
-        hipLaunchKernelGGL((pgk::sweep_fast_kernel<W, DIAG, STORE, 128, false, true>), dim3(nb), dim3(256), 0, st,
-                           rd, vself, Vout, occ, bu, K, F, G, Q, worig, pol, part, linf, tstat);
-    return hipGetLastError();
+    return pgk::for_box_chunks(G, [&](const pgk::BoxGeom& g, unsigned nb) {
+        if (write_v && pol)
+            hipLaunchKernelGGL((pgk::sweep_fast_kernel<W, DIAG, STORE, 128, true, true>), dim3(nb), dim3(256), 0, st,
+                               rd, vself, Vout, occ, bu, K, F, g, Q, worig, pol, part, linf, tstat);
+        else if (write_v)
+            hipLaunchKernelGGL((pgk::sweep_fast_kernel<W, DIAG, STORE, 128, true, false>), dim3(nb), dim3(256), 0, st,
+                               rd, vself, Vout, occ, bu, K, F, g, Q, worig, pol, part, linf, tstat);
+        else
+            hipLaunchKernelGGL((pgk::sweep_fast_kernel<W, DIAG, STORE, 128, false, true>), dim3(nb), dim3(256), 0, st,
+                               rd, vself, Vout, occ, bu, K, F, g, Q, worig, pol, part, linf, tstat);
+    });
 }
 
 template <bool DIAG, int STORE>

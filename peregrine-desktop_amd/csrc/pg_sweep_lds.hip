@@ -366,7 +366,7 @@ __global__ __launch_bounds__(256, 4) void sweep_lds_kernel(
             dst.put(K, i, out, bk);
         }
     }
-    if (WRITE_V) block_reduce_sum_max(dsum, dmax, part, linf_bits);
+    if (WRITE_V) block_reduce_sum_max(dsum, dmax, part + G.blk0, linf_bits);
 }
 
 }  // namespace pgk
@@ -381,17 +381,17 @@ template <class W, bool DIAG, int MAXS, class SRC, class DST>
 hipError_t launch_w(const pgk::Consts& K, const pgk::BoxGeom& G, const SRC& src, const DST& dst,
                     const uint8_t* occ, const float* bu, const int16_t* worig, double* part, unsigned* linf,
                     int write_v, int write_pol, hipStream_t st) {
-    const unsigned nb = (unsigned)(((G.nbox + 7) / 8) * 8);
-    if (write_v && write_pol)
-        hipLaunchKernelGGL((pgk::sweep_lds_kernel<W, DIAG, MAXS, true, true, SRC, DST>), dim3(nb), dim3(256), 0,
-                           st, src, dst, occ, bu, K, G, worig, part, linf);
-    else if (write_v)
-        hipLaunchKernelGGL((pgk::sweep_lds_kernel<W, DIAG, MAXS, true, false, SRC, DST>), dim3(nb), dim3(256),
-                           0, st, src, dst, occ, bu, K, G, worig, part, linf);
-    else
-        hipLaunchKernelGGL((pgk::sweep_lds_kernel<W, DIAG, MAXS, false, true, SRC, DST>), dim3(nb), dim3(256),
-                           0, st, src, dst, occ, bu, K, G, worig, part, linf);
-    return hipGetLastError();
+    return pgk::for_box_chunks(G, [&](const pgk::BoxGeom& g, unsigned nb) {
+        if (write_v && write_pol)
+            hipLaunchKernelGGL((pgk::sweep_lds_kernel<W, DIAG, MAXS, true, true, SRC, DST>), dim3(nb), dim3(256), 0,
+                               st, src, dst, occ, bu, K, g, worig, part, linf);
+        else if (write_v)
+            hipLaunchKernelGGL((pgk::sweep_lds_kernel<W, DIAG, MAXS, true, false, SRC, DST>), dim3(nb), dim3(256),
+                               0, st, src, dst, occ, bu, K, g, worig, part, linf);
+        else
+            hipLaunchKernelGGL((pgk::sweep_lds_kernel<W, DIAG, MAXS, false, true, SRC, DST>), dim3(nb), dim3(256),
+                               0, st, src, dst, occ, bu, K, g, worig, part, linf);
+    });
 }
 
 template <class SRC, class DST>
@@ -427,14 +427,13 @@ int pgk_window_shapes(const int (**shapes)[6]) {
 hipError_t pgk_launch_plan(const pgk::Consts& K, const pgk::BoxGeom& G, const uint8_t* occ,
                            const float* bu, int16_t* worig, uint8_t* wext, int diag,
                            hipStream_t st) {
-    const unsigned nb = (unsigned)(((G.nbox + 7) / 8) * 8);
-    if (diag)
-        hipLaunchKernelGGL((pgk::plan_kernel<true, 128>), dim3(nb), dim3(256), 0, st, occ, bu, K, G,
-                           worig, wext);
-    else
-        hipLaunchKernelGGL((pgk::plan_kernel<false, 128>), dim3(nb), dim3(256), 0, st, occ, bu, K, G,
-                           worig, wext);
-    return hipGetLastError();
+    return pgk::for_box_chunks(G, [&](const pgk::BoxGeom& g, unsigned nb) {
+        if (diag)
+            hipLaunchKernelGGL((pgk::plan_kernel<true, 128>), dim3(nb), dim3(256), 0, st, occ, bu, K, g, worig, wext);
+        else
+            hipLaunchKernelGGL((pgk::plan_kernel<false, 128>), dim3(nb), dim3(256), 0, st, occ, bu, K, g, worig,
+                               wext);
+    });
 }
 
 hipError_t pgk_launch_sweep_lds(int shape, int diag, const pgk::Consts& K, const pgk::BoxGeom& G,

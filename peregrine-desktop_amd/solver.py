@@ -213,6 +213,17 @@ class HipSolver:
                                                                  ctypes.byref(nb), ctypes.byref(amp)))
         return tuple(int(v) for v in tile), nt.value, nb.value, amp.value
 
+    def ooc_tiles(self) -> np.ndarray:
+        """Per tile of an out-of-core solver: interior origin, interior
+        extent, window origin, window extent (each x, y, heading), (n, 12)."""
+        n = self.ooc_info()[1]
+        out = np.zeros((n, 12), np.int32)
+        for k in range(n):
+            v = [np.zeros(3, np.int32) for _ in range(4)]
+            abi.check(self.lib, "pg_ooc_tile", self.lib.pg_ooc_tile(self._h, k, *(a.ctypes.data for a in v)))
+            out[k] = np.concatenate(v)
+        return out
+
     # FP64 tables ---------------------------------------------------------
     def solve_f64(self, iters: int, tol_linf: float, diagonal: bool = True):
         """FP64 table from cap and the seeds; returns (sweeps, last ||dV||_inf)."""

@@ -156,3 +156,24 @@ def test_full_size_problem2_lds_equals_global_and_oracle(pg, orc):
     want = np.minimum(c, V3[cells])
     want[np.isin(cells, prob.seeds)] = 0.0
     assert bits_equal(V4_lds[cells], want), diff_report(V4_lds[cells], want)
+
+
+def test_box_grid_in_two_launch_chunks(pg):
+    """A grid of 11.9M boxes is swept in two launches of at most 2^23
+    workgroups (a dispatch counts work-items in 32 bits): one LDS sweep of a
+    random table equals the global-gather sweep bit for bit, so every box of
+    both chunks was swept once."""
+    prob = pg.configs.problem(2, shape=(64, 64, 48, 17, 17, 21))
+    rng = np.random.default_rng(7)
+    V0 = rng.uniform(0.0, 61.0, prob.grid.ncells).astype(np.float32)
+    V0[V0 > 60.0] = 60.0
+    V0[prob.seeds] = 0.0
+    out = {}
+    for kern in ("global", "lds"):
+        with pg.HipSolver(prob.grid, prob.model, prob.params, prob.ctl, prob.occ) as s:
+            s.set_kernel(kernel_id(pg, kern))
+            s.upload(V0)
+            s.sweep(0, False)
+            out[kern] = s.download()
+    assert bits_equal(out["global"], out["lds"]), diff_report(out["lds"], out["global"])
+    assert np.count_nonzero(out["lds"] != V0) > 0.5 * V0.size
