@@ -20,6 +20,10 @@ the same problem and reports both forms' errors against it on a seeded
 FP64 values, FP64 policy) to --f64-out; --f64-sample reads such a file in
 place of the FP64 solve.
 
+--adjoint computes dL/dtheta for the 30 drivetrain parameters (default
+workload: problem 5, L = mean V over 4096 seeded starts) on the FP64 table and
+checks --fd-params of them against central differences of L.
+
 --ooc runs the out-of-core sweep instead (default workload: problem 4,
 96x96x64x17x17x21, device budget 8 GB, 16x16x8 tiles): --steps tiled sweeps
 from the seeded start, timed, each compared bit for bit with the in-HBM sweep
@@ -181,6 +185,46 @@ def accuracy(pg, prob, args, tables):
     return out
 
 
+def run_adjoint(pg, prob, args) -> dict:
+    """dL/dtheta by the adjoint against central differences of L."""
+    g = prob.grid
+    starts = pg.configs.adjoint_starts(prob, 4096)
+
+    def solve(model):
+        t = time.perf_counter()
+        with pg.HipSolver(g, model, prob.params, prob.ctl, prob.occ) as s:
+            s.fill_seed(prob.seeds)
+            its, l = s.solve_f64(50000, args.f64_tol)
+            t_solve = time.perf_counter() - t
+            t = time.perf_counter()
+            L, grad, it = s.adjoint(starts)
+            return L, grad, its, l, t_solve, time.perf_counter() - t, it
+
+    L, grad, its, l, t_solve, t_adj, it_adj = solve(prob.model)
+    print(f"# adjoint: L {L:.6f}, FP64 table {its} sweeps ({t_solve:.1f} s, linf {l:.1e}), adjoint {it_adj} "
+          f"iterations {t_adj:.2f} s", file=sys.stderr, flush=True)
+    th = prob.model.theta()
+    fd = {}
+    for j in [int(v) for v in args.fd_params.split(",") if v]:
+        h = args.fd_step * max(abs(th[j]), 1e-2)
+        tp, tm = th.copy(), th.copy()
+        tp[j] += h
+        tm[j] -= h
+        h_eff = float(np.float32(tp[j])) - float(np.float32(tm[j]))
+        Lp = solve(prob.model.with_theta(tp))[0]
+        Lm = solve(prob.model.with_theta(tm))[0]
+        d = (Lp - Lm) / h_eff
+        rel = abs(grad[j] - d) / max(abs(d), 1e-300)
+        fd[j] = {"adjoint": float(grad[j]), "fd": d, "rel": rel}
+        print(f"# theta[{j}] = {th[j]:+.5g}: adjoint {grad[j]:+.6e} fd {d:+.6e} rel {rel:.1e}", file=sys.stderr,
+              flush=True)
+    return {"metric": "adjoint dL/dtheta vs central differences", "L": L, "grad": [float(v) for v in grad],
+            "f64_sweeps": its, "f64_seconds": t_solve, "adjoint_seconds": t_adj, "adjoint_iterations": it_adj,
+            "fd_step_rel": args.fd_step, "fd": fd, "max_rel": max((v["rel"] for v in fd.values()), default=None),
+            "config": {"workload": f"problem {args.config}", "grid": list(g.n), "states": g.ncells,
+                       "controls": prob.nctl, "starts": 4096}}
+
+
 def run_ooc(pg, prob, args) -> dict:
     """Tiled sweeps from pinned host memory against in-HBM sweeps."""
     g = prob.grid
@@ -257,6 +301,9 @@ def main() -> int:
     ap.add_argument("--f64-max-sweeps", type=int, default=1500)
     ap.add_argument("--polish", type=int, default=3)
     ap.add_argument("--f64-only", action="store_true")
+    ap.add_argument("--adjoint", action="store_true")
+    ap.add_argument("--fd-params", default="0,4,8,9,13,17,19,21,25,27,28,29")
+    ap.add_argument("--fd-step", type=float, default=1e-4)
     ap.add_argument("--ooc", action="store_true")
     ap.add_argument("--ooc-budget", type=float, default=8e9)
     ap.add_argument("--ooc-tile", default="16,16,8")
@@ -299,6 +346,10 @@ def main() -> int:
           file=sys.stderr, flush=True)
     if args.ooc:
         print(json.dumps(run_ooc(pg, prob, args)), flush=True)
+        return 0
+    if args.adjoint:
+        args.f64_tol = min(args.f64_tol, 1e-12)
+        print(json.dumps(run_adjoint(pg, prob, args)), flush=True)
         return 0
     if args.f64_only:
         idx, V64, P64, info = f64_sample(pg, prob, args)
