@@ -175,6 +175,17 @@ int pg_upload_V(pg_solver* s, const float* src, int64_t off, int64_t n);
  * multiplies, hardware sin/cos, tensor-product interpolation). AUTO (the
  * default) takes LDS for an FP32 store with at most 128 controls, GLOBAL
  * otherwise; an FP16 store sweeps with FAST until pg_polish. */
+/* Sweep order. JACOBI (the default) reads the previous table and writes the
+ * next: deterministic, and for the exact kernels equal bit for bit to the
+ * CPU restatement. IN_PLACE updates one table as the reference's GPU sweep
+ * does (Gpu.jl:5-25, direction alternating with `rev`, Run.jl:1457-1458):
+ * backups may read values already lowered in the same sweep, so fewer sweeps
+ * reach a tolerance, and the result depends on the order the GPU schedules
+ * boxes in (each sweep is still monotone and reaches the same fixed point). */
+#define PG_SWEEP_JACOBI   0
+#define PG_SWEEP_IN_PLACE 1
+int pg_solver_set_sweep_mode(pg_solver* s, int32_t mode);
+
 #define PG_KERNEL_AUTO   0
 #define PG_KERNEL_GLOBAL 1
 #define PG_KERNEL_LDS    2

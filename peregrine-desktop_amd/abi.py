@@ -30,6 +30,8 @@ PG_VSTORE_F32 = 0
 PG_VSTORE_F16_TILE = 1
 PG_VSTORE_U16_TILE = 2
 PG_F64_SWEEP_PLAIN = 0
+PG_SWEEP_JACOBI = 0
+PG_SWEEP_IN_PLACE = 1
 PG_F64_SWEEP_DIAGONAL = 1
 
 
@@ -119,6 +121,7 @@ SIGNATURES = {
                                   ctypes.POINTER(_I32)]),
     "pg_upload_V": (ctypes.c_int, [_P, _P, _I64, _I64]),
     "pg_solver_set_kernel": (ctypes.c_int, [_P, _I32]),
+    "pg_solver_set_sweep_mode": (ctypes.c_int, [_P, _I32]),
     "pg_solver_kernel_info": (ctypes.c_int, [_P, ctypes.POINTER(_I32), ctypes.POINTER(ctypes.c_double)]),
     "pg_last_sweep_ms": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_float)]),
     "pg_solver_ncells": (ctypes.c_int, [_P, ctypes.POINTER(_I64)]),

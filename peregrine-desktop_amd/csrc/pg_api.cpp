@@ -604,7 +604,6 @@ int pg_fill_seed(pg_solver* s, const int64_t* seeds, int64_t nseeds) {
 
 int pg_sweep(pg_solver* s, int32_t phase, int32_t rev, double* sum_decrease, float* linf) {
     (void)phase;
-    (void)rev;
     if (!s) return fail(PG_ERR_ARG, "null solver");
     if (!s->seeded) return fail(PG_ERR_STATE, "pg_fill_seed (or pg_upload_V) must come before a sweep");
     HIPCHECK(hipSetDevice(s->ctx->device));
@@ -619,16 +618,19 @@ int pg_sweep(pg_solver* s, int32_t phase, int32_t rev, double* sum_decrease, flo
     int rc = ensure_partials(s, nb);
     if (rc) return rc;
     HIPCHECK(hipMemsetAsync(s->d_linf, 0, sizeof(unsigned), st));
-    float* Vout = s->f16_active ? s->dV[0] : s->dV[1 - s->cur];
+    // in place: one buffer, boxes walked in alternating directions
+    const bool inplace = s->sweep_mode == PG_SWEEP_IN_PLACE;
+    float* Vout = s->f16_active ? s->dV[0] : (inplace ? s->dV[s->cur] : s->dV[1 - s->cur]);
+    s->G.rev = (inplace && rev) ? 1 : 0;
     int64_t nblk = nb;
     HIPCHECK(hipEventRecord(s->ev0, st));
     if (s->f16_active) {
         HIPCHECK(pgk_launch_sweep_fast(s->shape, s->diag ? 1 : 0, s->store, s->K, s->F, s->G, s->Q,
-                                       s->d_V16, s->d_tp, s->dV[0], Vout, s->d_occ, s->d_bu, s->d_worig,
-                                       nullptr, s->d_part, s->d_linf, s->d_tstat, 1, st));
+                                       s->d_V16, s->d_tp, s->dV[0], Vout, inplace ? s->d_V16 : nullptr, s->d_occ,
+                                       s->d_bu, s->d_worig, nullptr, s->d_part, s->d_linf, s->d_tstat, 1, st));
     } else if (kern == PG_KERNEL_FAST) {
         HIPCHECK(pgk_launch_sweep_fast(s->shape, s->diag ? 1 : 0, PG_STORE_F32, s->K, s->F, s->G, s->Q,
-                                       s->dV[s->cur], nullptr, s->dV[s->cur], Vout, s->d_occ, s->d_bu,
+                                       s->dV[s->cur], nullptr, s->dV[s->cur], Vout, nullptr, s->d_occ, s->d_bu,
                                        s->d_worig, nullptr, s->d_part, s->d_linf, nullptr, 1, st));
     } else if (kern == PG_KERNEL_LDS) {
         HIPCHECK(pgk_launch_sweep_lds(s->shape, s->diag ? 1 : 0, s->K, s->G, s->dV[s->cur], Vout, s->d_occ,
@@ -643,9 +645,10 @@ int pg_sweep(pg_solver* s, int32_t phase, int32_t rev, double* sum_decrease, flo
     if (s->f16_active) {
         rc = encode_from_scratch(s, false);
         if (rc) return rc;
-    } else {
+    } else if (!inplace) {
         s->cur = 1 - s->cur;
     }
+    s->G.rev = 0;
     double hsum = 0.0;
     unsigned hl = 0;
     HIPCHECK(hipMemcpyAsync(&hsum, s->d_sum, sizeof(double), hipMemcpyDeviceToHost, st));
@@ -724,11 +727,11 @@ int pg_policy(pg_solver* s, uint8_t* pol_host) {
     }
     if (s->f16_active) {
         HIPCHECK(pgk_launch_sweep_fast(s->shape, s->diag ? 1 : 0, s->store, s->K, s->F, s->G, s->Q,
-                                       s->d_V16, s->d_tp, s->dV[0], nullptr, s->d_occ, s->d_bu, s->d_worig,
-                                       s->d_pol, nullptr, nullptr, nullptr, 0, st));
+                                       s->d_V16, s->d_tp, s->dV[0], nullptr, nullptr, s->d_occ, s->d_bu,
+                                       s->d_worig, s->d_pol, nullptr, nullptr, nullptr, 0, st));
     } else if (kern == PG_KERNEL_FAST) {
         HIPCHECK(pgk_launch_sweep_fast(s->shape, s->diag ? 1 : 0, PG_STORE_F32, s->K, s->F, s->G, s->Q,
-                                       s->dV[s->cur], nullptr, s->dV[s->cur], nullptr, s->d_occ, s->d_bu,
+                                       s->dV[s->cur], nullptr, s->dV[s->cur], nullptr, nullptr, s->d_occ, s->d_bu,
                                        s->d_worig, s->d_pol, nullptr, nullptr, nullptr, 0, st));
     } else if (kern == PG_KERNEL_LDS) {
         HIPCHECK(pgk_launch_sweep_lds(s->shape, s->diag ? 1 : 0, s->K, s->G, s->dV[s->cur], nullptr, s->d_occ,
@@ -1036,6 +1039,15 @@ int pg_release_f64(pg_solver* s) {
         s->dV64[j] = nullptr;
     }
     s->f64_valid = false;
+    return PG_OK;
+}
+
+int pg_solver_set_sweep_mode(pg_solver* s, int32_t mode) {
+    if (!s) return fail(PG_ERR_ARG, "null solver");
+    if (mode != PG_SWEEP_JACOBI && mode != PG_SWEEP_IN_PLACE) return fail(PG_ERR_ARG, "unknown sweep mode %d", mode);
+    if (s->ooc && mode != PG_SWEEP_JACOBI)
+        return fail(PG_ERR_UNSUPPORTED, "an out-of-core solver sweeps Jacobi (its tiles read the old table)");
+    s->sweep_mode = mode;
     return PG_OK;
 }
 

@@ -37,7 +37,9 @@ using Win2 = Win<4, 4, 4, 5, 5, 5>;
 // each XCD walks neighbouring boxes (whose windows overlap) through its L2.
 __device__ __forceinline__ int64_t box_of_block(const BoxGeom& G) {
     const int64_t per = (G.nbox + 7) / 8;
-    const int64_t b = G.blk0 + (int64_t)blockIdx.x;
+    int64_t b = G.blk0 + (int64_t)blockIdx.x;
+    // reversed order: the last boxes of each XCD's range first
+    if (G.rev) b = (b % 8) + 8 * (per - 1 - b / 8);
     return (b % 8) * per + b / 8;
 }
 

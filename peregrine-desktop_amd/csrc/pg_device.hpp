@@ -403,6 +403,7 @@ struct BoxGeom {
     int32_t rn[6];    // cells of the region per axis
     int32_t nbg[6];   // boxes per axis over the whole grid
     int64_t blk0;     // first workgroup of this launch (launches are chunked)
+    int32_t rev;      // walk the boxes in reverse order (in-place sweeps alternate)
 };
 
 // Block-wide sum (FP64) and max of the per-cell decreases: wave shuffles,

@@ -74,6 +74,7 @@ struct pg_solver {
     bool planned = false;
     int shape = 0;
     double coverage = 0.0;
+    int32_t sweep_mode = PG_SWEEP_JACOBI;
     // FP64 tables (allocated on first use)
     pgk::Consts64 K64{};
     std::vector<double> bu64;

@@ -35,7 +35,8 @@ hipError_t pgk_launch_sweep_tile(int shape, int diag, const pgk::Consts& K, cons
 hipError_t pgk_launch_sweep_fast(int shape, int diag, int store, const pgk::Consts& K,
                                  const pgk::FastConsts& F, const pgk::BoxGeom& G,
                                  const pgk::QuantGeom& Q, const void* Vin, const float2* tp,
-                                 const float* vself, float* Vout, const uint8_t* occ, const float* bu,
+                                 const float* vself, float* Vout, uint16_t* codes_out, const uint8_t* occ,
+                                 const float* bu,
                                  const int16_t* worig, uint8_t* pol, double* part, unsigned* linf,
                                  unsigned* tstat, int write_v, hipStream_t st);
 hipError_t pgk_launch_tile_stats(const float* V, const pgk::Consts& K, const pgk::QuantGeom& Q,

@@ -119,7 +119,7 @@ __device__ __forceinline__ float lookahead_global(const Consts& K, const float* 
 
 template <bool WRITE_V, bool WRITE_POL>
 __global__ __launch_bounds__(256) void sweep_global_kernel(
-    const float* __restrict__ Vin, float* __restrict__ Vout, const uint8_t* __restrict__ occ,
+    const float* Vin, float* Vout, const uint8_t* __restrict__ occ,
     const float* __restrict__ bu, Consts K, uint8_t* __restrict__ pol,
     double* __restrict__ part_sum, unsigned* __restrict__ linf_bits) {
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -67,9 +67,10 @@ __device__ __forceinline__ int64_t tile_of_coords(const QuantGeom& Q, const int*
 }
 
 // Reads of V in either store.
+// (no __restrict__ on the table: in place, a sweep writes what it reads)
 template <int STORE>
 struct VRead {
-    const uint16_t* __restrict__ V;
+    const uint16_t* V;
     const float2* __restrict__ tp;
     __device__ __forceinline__ float operator()(const Consts& K, const QuantGeom& Q, const int* gi,
                                                 int64_t fl) const {
@@ -79,7 +80,7 @@ struct VRead {
 
 template <>
 struct VRead<PG_STORE_F32> {
-    const float* __restrict__ V;
+    const float* V;
     __device__ __forceinline__ float operator()(const Consts&, const QuantGeom&, const int*, int64_t fl) const {
         return V[fl];
     }
@@ -102,7 +103,7 @@ struct BoxShared {
 
 template <class W, bool DIAG, int STORE, int MAXS, bool WRITE_V, bool WRITE_POL>
 __global__ __launch_bounds__(256, 4) void sweep_fast_kernel(
-    fastk::VRead<STORE> vin, const float* vself, float* Vout, const uint8_t* __restrict__ occ,
+    fastk::VRead<STORE> vin, const float* vself, float* Vout, uint16_t* codes_out, const uint8_t* __restrict__ occ,
     const float* __restrict__ bu_g, Consts K, FastConsts F, BoxGeom G, QuantGeom Q,
     const int16_t* __restrict__ worig, uint8_t* __restrict__ pol, double* __restrict__ part,
     unsigned* __restrict__ linf_bits, unsigned* __restrict__ tstat) {
@@ -285,6 +286,10 @@ __global__ __launch_bounds__(256, 4) void sweep_fast_kernel(
                     dmax = fmaxf(dmax, d);
                 }
                 Vout[idx] = out;
+                // in place: this state's code now, for the boxes swept after it
+                if constexpr (STORE != PG_STORE_F32) {
+                    if (codes_out) codes_out[idx] = Codec<STORE>::encode(out, vin.tp[tile_of_coords(Q, i)], K.cap);
+                }
                 if (out < K.cap) {
                     lmin = min(lmin, __float_as_uint(out));
                     lmax = max(lmax, __float_as_uint(out));
@@ -429,6 +434,7 @@ namespace {
 template <class W, bool DIAG, int STORE>
 hipError_t launch_fast_w(const pgk::Consts& K, const pgk::FastConsts& F, const pgk::BoxGeom& G,
                          const pgk::QuantGeom& Q, const void* Vin, const float2* tp, const float* vself, float* Vout,
+                         uint16_t* codes_out,
                          const uint8_t* occ, const float* bu, const int16_t* worig, uint8_t* pol,
                          double* part, unsigned* linf, unsigned* tstat, int write_v, hipStream_t st) {
     pgk::fastk::VRead<STORE> rd;
@@ -441,29 +447,30 @@ hipError_t launch_fast_w(const pgk::Consts& K, const pgk::FastConsts& F, const p
     return pgk::for_box_chunks(G, [&](const pgk::BoxGeom& g, unsigned nb) {
         if (write_v && pol)
             hipLaunchKernelGGL((pgk::sweep_fast_kernel<W, DIAG, STORE, 128, true, true>), dim3(nb), dim3(256), 0, st,
-                               rd, vself, Vout, occ, bu, K, F, g, Q, worig, pol, part, linf, tstat);
+                               rd, vself, Vout, codes_out, occ, bu, K, F, g, Q, worig, pol, part, linf, tstat);
         else if (write_v)
             hipLaunchKernelGGL((pgk::sweep_fast_kernel<W, DIAG, STORE, 128, true, false>), dim3(nb), dim3(256), 0, st,
-                               rd, vself, Vout, occ, bu, K, F, g, Q, worig, pol, part, linf, tstat);
+                               rd, vself, Vout, codes_out, occ, bu, K, F, g, Q, worig, pol, part, linf, tstat);
         else
             hipLaunchKernelGGL((pgk::sweep_fast_kernel<W, DIAG, STORE, 128, false, true>), dim3(nb), dim3(256), 0, st,
-                               rd, vself, Vout, occ, bu, K, F, g, Q, worig, pol, part, linf, tstat);
+                               rd, vself, Vout, codes_out, occ, bu, K, F, g, Q, worig, pol, part, linf, tstat);
     });
 }
 
 template <bool DIAG, int STORE>
 hipError_t launch_fast_shape(int shape, const pgk::Consts& K, const pgk::FastConsts& F,
                              const pgk::BoxGeom& G, const pgk::QuantGeom& Q, const void* Vin,
-                             const float2* tp, const float* vself, float* Vout, const uint8_t* occ, const float* bu,
+                             const float2* tp, const float* vself, float* Vout, uint16_t* codes_out,
+                             const uint8_t* occ, const float* bu,
                              const int16_t* worig, uint8_t* pol, double* part, unsigned* linf,
                              unsigned* tstat, int write_v, hipStream_t st) {
     using W0 = pgk::Win0;
     using W1 = pgk::Win1;
     using W2 = pgk::Win2;
     switch (shape) {
-        case 0: return launch_fast_w<W0, DIAG, STORE>(K, F, G, Q, Vin, tp, vself, Vout, occ, bu, worig, pol, part, linf, tstat, write_v, st);
-        case 1: return launch_fast_w<W1, DIAG, STORE>(K, F, G, Q, Vin, tp, vself, Vout, occ, bu, worig, pol, part, linf, tstat, write_v, st);
-        case 2: return launch_fast_w<W2, DIAG, STORE>(K, F, G, Q, Vin, tp, vself, Vout, occ, bu, worig, pol, part, linf, tstat, write_v, st);
+        case 0: return launch_fast_w<W0, DIAG, STORE>(K, F, G, Q, Vin, tp, vself, Vout, codes_out, occ, bu, worig, pol, part, linf, tstat, write_v, st);
+        case 1: return launch_fast_w<W1, DIAG, STORE>(K, F, G, Q, Vin, tp, vself, Vout, codes_out, occ, bu, worig, pol, part, linf, tstat, write_v, st);
+        case 2: return launch_fast_w<W2, DIAG, STORE>(K, F, G, Q, Vin, tp, vself, Vout, codes_out, occ, bu, worig, pol, part, linf, tstat, write_v, st);
         default: return hipErrorInvalidValue;
     }
 }
@@ -473,16 +480,16 @@ hipError_t launch_fast_shape(int shape, const pgk::Consts& K, const pgk::FastCon
 hipError_t pgk_launch_sweep_fast(int shape, int diag, int store, const pgk::Consts& K,
                                  const pgk::FastConsts& F, const pgk::BoxGeom& G,
                                  const pgk::QuantGeom& Q, const void* Vin, const float2* tp,
-                                 const float* vself, float* Vout, const uint8_t* occ, const float* bu,
-                                 const int16_t* worig, uint8_t* pol, double* part, unsigned* linf,
+                                 const float* vself, float* Vout, uint16_t* codes_out, const uint8_t* occ,
+                                 const float* bu, const int16_t* worig, uint8_t* pol, double* part, unsigned* linf,
                                  unsigned* tstat, int write_v, hipStream_t st) {
     switch (store * 2 + (diag ? 1 : 0)) {
-        case PG_STORE_F32 * 2 + 0: return launch_fast_shape<false, PG_STORE_F32>(shape, K, F, G, Q, Vin, tp, vself, Vout, occ, bu, worig, pol, part, linf, tstat, write_v, st);
-        case PG_STORE_F32 * 2 + 1: return launch_fast_shape<true, PG_STORE_F32>(shape, K, F, G, Q, Vin, tp, vself, Vout, occ, bu, worig, pol, part, linf, tstat, write_v, st);
-        case PG_STORE_F16 * 2 + 0: return launch_fast_shape<false, PG_STORE_F16>(shape, K, F, G, Q, Vin, tp, vself, Vout, occ, bu, worig, pol, part, linf, tstat, write_v, st);
-        case PG_STORE_F16 * 2 + 1: return launch_fast_shape<true, PG_STORE_F16>(shape, K, F, G, Q, Vin, tp, vself, Vout, occ, bu, worig, pol, part, linf, tstat, write_v, st);
-        case PG_STORE_U16 * 2 + 0: return launch_fast_shape<false, PG_STORE_U16>(shape, K, F, G, Q, Vin, tp, vself, Vout, occ, bu, worig, pol, part, linf, tstat, write_v, st);
-        case PG_STORE_U16 * 2 + 1: return launch_fast_shape<true, PG_STORE_U16>(shape, K, F, G, Q, Vin, tp, vself, Vout, occ, bu, worig, pol, part, linf, tstat, write_v, st);
+        case PG_STORE_F32 * 2 + 0: return launch_fast_shape<false, PG_STORE_F32>(shape, K, F, G, Q, Vin, tp, vself, Vout, codes_out, occ, bu, worig, pol, part, linf, tstat, write_v, st);
+        case PG_STORE_F32 * 2 + 1: return launch_fast_shape<true, PG_STORE_F32>(shape, K, F, G, Q, Vin, tp, vself, Vout, codes_out, occ, bu, worig, pol, part, linf, tstat, write_v, st);
+        case PG_STORE_F16 * 2 + 0: return launch_fast_shape<false, PG_STORE_F16>(shape, K, F, G, Q, Vin, tp, vself, Vout, codes_out, occ, bu, worig, pol, part, linf, tstat, write_v, st);
+        case PG_STORE_F16 * 2 + 1: return launch_fast_shape<true, PG_STORE_F16>(shape, K, F, G, Q, Vin, tp, vself, Vout, codes_out, occ, bu, worig, pol, part, linf, tstat, write_v, st);
+        case PG_STORE_U16 * 2 + 0: return launch_fast_shape<false, PG_STORE_U16>(shape, K, F, G, Q, Vin, tp, vself, Vout, codes_out, occ, bu, worig, pol, part, linf, tstat, write_v, st);
+        case PG_STORE_U16 * 2 + 1: return launch_fast_shape<true, PG_STORE_U16>(shape, K, F, G, Q, Vin, tp, vself, Vout, codes_out, occ, bu, worig, pol, part, linf, tstat, write_v, st);
         default: return hipErrorInvalidValue;
     }
 }

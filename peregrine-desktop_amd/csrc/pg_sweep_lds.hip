@@ -150,8 +150,10 @@ __global__ __launch_bounds__(256) void plan_kernel(const uint8_t* __restrict__ o
 // Where a sweep reads V and writes its results: the whole table in HBM, or
 // (out-of-core) the window of one tile and that tile's interior.
 // ---------------------------------------------------------------------------
+// (no __restrict__ on the table: an in-place sweep reads and writes one
+// buffer, other workgroups' results arriving while a window is staged)
 struct DevSrc {
-    const float* __restrict__ V;
+    const float* V;
     // a window cell to stage (may lie anywhere on the grid)
     __device__ __forceinline__ float stage(const Consts& K, const int gi[6]) const { return read(K, gi); }
     // a value the backup uses
@@ -164,7 +166,7 @@ struct DevSrc {
 };
 
 struct DevDst {
-    float* __restrict__ V;
+    float* V;
     uint8_t* __restrict__ pol;
     __device__ __forceinline__ void put(const Consts& K, const int i[6], float v, int k) const {
         int64_t fl = 0;

@@ -177,6 +177,10 @@ class HipSolver:
                                     ctypes.byref(d), ctypes.byref(l)))
         return it.value, d.value, l.value
 
+    def set_sweep_mode(self, mode: int) -> None:
+        """abi.PG_SWEEP_JACOBI (default, deterministic) or abi.PG_SWEEP_IN_PLACE."""
+        abi.check(self.lib, "pg_solver_set_sweep_mode", self.lib.pg_solver_set_sweep_mode(self._h, int(mode)))
+
     def polish(self, nsweeps: int):
         """Decode an FP16 store to FP32 and run `nsweeps` exact sweeps."""
         d = ctypes.c_double()
