@@ -13,7 +13,9 @@ sample of the same workload.
 --converge additionally times, for the plain FP32 form and for the
 precision form, a full solve from V = cap to the first sweep with
 ||dV||_inf < 1e-3 s (the precision form then runs its 3 exact FP32 polish
-sweeps); reported under "converge". --accuracy also solves the FP64 table of
+sweeps); reported under "converge". Converge runs skip the boxes whose
+inputs did not change (bit-identical results; --no-active computes every
+box); the throughput steps always compute every box. --accuracy also solves the FP64 table of
 the same problem and reports both forms' errors against it on a seeded
 1M-state subsample, with argmin-policy agreement (under "accuracy").
 --f64-only solves just the FP64 table and writes the subsample (indices,
@@ -90,19 +92,25 @@ def solve_timed(pg, prob, vstore, args, tag):
     table and policy at the end (after the polish sweeps for a 16-bit store)."""
     g = prob.grid
     s = pg.HipSolver(g, prob.model, prob.params, prob.ctl, prob.occ, vstore=vstore)
+    s.set_active(not args.no_active)
     s.fill_seed(prob.seeds)
     t = time.perf_counter()
     its, linf = 0, float("inf")
+    computed = total = 0
     for its in range(1, args.max_sweeps + 1):
         r = s.sweep(its - 1, its % 2 == 1)
         linf = r.linf
+        c, nb = s.last_sweep_boxes()
+        computed += c
+        total += nb
         if its % 10 == 0:
-            print(f"# converge {tag} sweep {its}: linf {linf:.5f}, {time.perf_counter() - t:.1f} s",
-                  file=sys.stderr, flush=True)
+            print(f"# converge {tag} sweep {its}: linf {linf:.5f}, {time.perf_counter() - t:.1f} s, "
+                  f"boxes computed {c / nb:.3f}", file=sys.stderr, flush=True)
         if linf < args.tol:
             break
     t_conv = time.perf_counter() - t
-    res = {"seconds": t_conv, "sweeps": its, "linf": linf, "tol": args.tol, "converged": bool(linf < args.tol)}
+    res = {"seconds": t_conv, "sweeps": its, "linf": linf, "tol": args.tol, "converged": bool(linf < args.tol),
+           "active_boxes": not args.no_active, "boxes_computed_frac": computed / max(total, 1)}
     if vstore != pg.abi.PG_VSTORE_F32:
         t = time.perf_counter()
         _, lp = s.polish(args.polish)
@@ -313,6 +321,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tol", type=float, default=1e-3)
     ap.add_argument("--max-sweeps", type=int, default=2000)
+    ap.add_argument("--no-active", action="store_true",
+                    help="converge runs compute every box of every sweep (no active set)")
     ap.add_argument("--kernel", choices=["auto", "global", "lds", "fast"], default="auto",
                     help="sweep kernel of the FP32 run (exact: global, lds; contracted: fast)")
     ap.add_argument("--no-fp16", action="store_true",
@@ -360,6 +370,9 @@ def main() -> int:
         return 0
     def timed(vstore, kernel):
         s = pg.HipSolver(g, prob.model, prob.params, prob.ctl, prob.occ, vstore=vstore)
+        # throughput is that of sweeps computing every box (an early sweep
+        # of a solve would otherwise skip most of the grid)
+        s.set_active(False)
         if kernel != "auto":
             s.set_kernel({"global": pg.abi.PG_KERNEL_GLOBAL, "lds": pg.abi.PG_KERNEL_LDS,
                           "fast": pg.abi.PG_KERNEL_FAST}[kernel])

@@ -200,6 +200,21 @@ int pg_solver_kernel_info(pg_solver* s, int32_t* kernel, double* window_coverage
 /* Duration of the last sweep kernel, from HIP events on the solver stream. */
 int pg_last_sweep_ms(pg_solver* s, float* ms);
 
+/* Active boxes (on by default; no counterpart in the reference, whose
+ * sweep_gpu! launches every cell each sweep, Gpu.jl:27-37). A Jacobi sweep of
+ * the LDS-staged or precision-form kernels skips a box of states when no
+ * cell that its backups can read (its read range, from a plan pass with the
+ * sweep's own arithmetic), and none of its own cells, changed in the
+ * previous sweep: min(V, TV) would leave it as it is. Tables, sums and
+ * sup-norms are bit for bit those of computing every box. The set restarts
+ * (every box computed once) after pg_fill_seed, pg_upload_V, pg_polish, a
+ * kernel change, or a sweep it does not cover (global kernel, in-place
+ * order, out-of-core tiles, omega cut into more than 32 boxes). */
+int pg_solver_set_active(pg_solver* s, int32_t on);
+
+/* Boxes the last sweep computed, and boxes of the grid. */
+int pg_last_sweep_boxes(pg_solver* s, int64_t* computed, int64_t* total);
+
 /* Number of cells of the solver's grid. */
 int pg_solver_ncells(pg_solver* s, int64_t* n);
 

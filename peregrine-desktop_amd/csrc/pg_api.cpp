@@ -109,6 +109,12 @@ void release(pg_solver* s) {
     (void)hipFree(s->d_linf);
     (void)hipFree(s->d_pol);
     (void)hipFree(s->d_worig);
+    (void)hipFree(s->d_wext);
+    (void)hipFree(s->d_forig);
+    (void)hipFree(s->d_fext);
+    (void)hipFree(s->d_act[0]);
+    (void)hipFree(s->d_act[1]);
+    (void)hipFree(s->d_nwork);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
     delete s;
@@ -241,7 +247,7 @@ void box_geometry(pg_solver* s) {
     s->G.nbox = nbox;
 }
 
-int64_t lds_blocks(const pg_solver* s) { return ((s->G.nbox + 7) / 8) * 8; }
+int64_t lds_blocks(const pg_solver* s) { return pgk::box_blocks(s->G.nbox); }
 
 }  // namespace
 
@@ -265,10 +271,10 @@ int plan(pg_solver* s, bool keep_host, std::vector<int16_t>* h_orig, std::vector
     hipStream_t st = s->ctx->stream;
     box_geometry(s);
     const int64_t nbox = s->G.nbox;
-    uint8_t* d_ext = nullptr;
     if (!s->d_worig) HIPCHECK(hipMalloc(&s->d_worig, (size_t)nbox * 6 * sizeof(int16_t)));
-    HIPCHECK(hipMalloc(&d_ext, (size_t)nbox * 6));
-    hipError_t e = pgk_launch_plan(s->K, s->G, s->d_occ, s->d_bu, s->d_worig, d_ext, s->diag ? 1 : 0, st);
+    if (!s->d_wext) HIPCHECK(hipMalloc(&s->d_wext, (size_t)nbox * 6));
+    uint8_t* d_ext = s->d_wext;
+    hipError_t e = pgk_launch_plan(s->K, nullptr, s->G, s->d_occ, s->d_bu, s->d_worig, d_ext, s->diag ? 1 : 0, st);
     std::vector<uint8_t> ext_local;
     std::vector<uint8_t>& ext = h_ext ? *h_ext : ext_local;
     ext.resize((size_t)nbox * 6);
@@ -278,7 +284,6 @@ int plan(pg_solver* s, bool keep_host, std::vector<int16_t>* h_orig, std::vector
         e = hipMemcpyAsync(h_orig->data(), s->d_worig, h_orig->size() * sizeof(int16_t), hipMemcpyDeviceToHost, st);
     }
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    (void)hipFree(d_ext);
     if (e != hipSuccess) return fail(PG_ERR_HIP, "window plan failed: %s", hipGetErrorString(e));
     const int (*shapes)[6] = nullptr;
     const int nsh = pgk_window_shapes(&shapes);
@@ -311,18 +316,51 @@ int auto_kernel(const pg_solver* s) {
 // FP32 table in dV[0] -> 16-bit mirror. A whole-table load (force) fits
 // every tile afresh; after a sweep, tiles are refit only when their values
 // left the representable range and only changed codes are written.
-int encode_from_scratch(pg_solver* s, bool force) {
+int encode_from_scratch(pg_solver* s, bool force, uint32_t* mark = nullptr) {
     hipStream_t st = s->ctx->stream;
     if (force) HIPCHECK(pgk_launch_tile_stats(s->dV[0], s->K, s->Q, s->d_tstat, st));
     else HIPCHECK(pgk_launch_seed_stats(s->d_seeds, s->nseeds, s->K, s->Q, s->d_tstat, st));
     HIPCHECK(pgk_launch_tile_refit(s->store, s->d_tstat, s->d_tp, s->d_trefit, s->Q.ntile, force ? 1 : 0, st));
-    HIPCHECK(pgk_launch_encode(s->store, s->dV[0], s->K, s->Q, s->d_tp, s->d_trefit, s->d_V16, st));
+    HIPCHECK(pgk_launch_encode(s->store, s->dV[0], s->K, s->Q, s->d_tp, s->d_trefit, s->d_V16, mark, s->G, st));
     return PG_OK;
 }
 
 // The current FP32 table on the device. With a 16-bit store, dV[0] holds
 // every state's own value in FP32 and the codes mirror it for the gathers.
 float* current_f32(pg_solver* s) { return s->f16_active ? s->dV[0] : s->dV[s->cur]; }
+
+// The active set of the coming sweep in `regime` (see pg_solver::act_for):
+// A.prev holds the previous sweep's change bits when they were made in the
+// same regime, A.next the cleared bits this sweep marks. Returns false (A
+// empty, every box computed) when the set is off or cannot be kept.
+int active_setup(pg_solver* s, int regime, bool inplace, pgk::ActiveG& A, bool& track) {
+    A = pgk::ActiveG{};
+    track = s->active_on && !inplace && regime >= 0 && s->G.nbg[5] <= 32;
+    if (!track) return PG_OK;
+    hipStream_t st = s->ctx->stream;
+    const int64_t words = s->G.nbox / s->G.nbg[5];
+    if (!s->d_act[0]) {
+        HIPCHECK(hipMalloc(&s->d_act[0], (size_t)words * sizeof(uint32_t)));
+        HIPCHECK(hipMalloc(&s->d_act[1], (size_t)words * sizeof(uint32_t)));
+        HIPCHECK(hipMalloc(&s->d_nwork, sizeof(unsigned)));
+        s->act_for = -1;
+    }
+    if (regime != 0 && !s->d_forig) {
+        // read ranges of the precision form's successors (one pass per solver)
+        HIPCHECK(hipMalloc(&s->d_forig, (size_t)s->G.nbox * 6 * sizeof(int16_t)));
+        HIPCHECK(hipMalloc(&s->d_fext, (size_t)s->G.nbox * 6));
+        HIPCHECK(pgk_launch_plan(s->K, &s->F, s->G, s->d_occ, s->d_bu, s->d_forig, s->d_fext, s->diag ? 1 : 0, st));
+    }
+    uint32_t* next = s->d_act[1 - s->act_cur];
+    HIPCHECK(hipMemsetAsync(next, 0, (size_t)words * sizeof(uint32_t), st));
+    HIPCHECK(hipMemsetAsync(s->d_nwork, 0, sizeof(unsigned), st));
+    A.prev = s->act_for == regime ? s->d_act[s->act_cur] : nullptr;
+    A.next = regime == 2 ? nullptr : next;   // the 16-bit store marks in its encode pass
+    A.rorig = regime == 0 ? s->d_worig : s->d_forig;
+    A.rext = regime == 0 ? s->d_wext : s->d_fext;
+    A.nwork = s->d_nwork;
+    return PG_OK;
+}
 
 }  // namespace
 
@@ -582,6 +620,7 @@ int pg_fill_seed(pg_solver* s, const int64_t* seeds, int64_t nseeds) {
     }
     s->h_seeds.assign(seeds, seeds + nseeds);
     s->cur = 0;
+    s->act_for = -1;
     if (s->ooc) {
         HIPCHECK(hipStreamSynchronize(st));
         int rc = pgh::ooc_fill_seed(s);
@@ -622,38 +661,55 @@ int pg_sweep(pg_solver* s, int32_t phase, int32_t rev, double* sum_decrease, flo
     const bool inplace = s->sweep_mode == PG_SWEEP_IN_PLACE;
     float* Vout = s->f16_active ? s->dV[0] : (inplace ? s->dV[s->cur] : s->dV[1 - s->cur]);
     s->G.rev = (inplace && rev) ? 1 : 0;
+    // boxes whose inputs did not change since they were last computed are skipped
+    const int regime = s->f16_active ? 2 : (kern == PG_KERNEL_FAST ? 1 : (kern == PG_KERNEL_LDS ? 0 : -1));
+    pgk::ActiveG A;
+    bool track = false;
+    rc = active_setup(s, regime, inplace, A, track);
+    if (rc) return rc;
+    uint32_t* mark = track ? s->d_act[1 - s->act_cur] : nullptr;
     int64_t nblk = nb;
     HIPCHECK(hipEventRecord(s->ev0, st));
     if (s->f16_active) {
         HIPCHECK(pgk_launch_sweep_fast(s->shape, s->diag ? 1 : 0, s->store, s->K, s->F, s->G, s->Q,
                                        s->d_V16, s->d_tp, s->dV[0], Vout, inplace ? s->d_V16 : nullptr, s->d_occ,
-                                       s->d_bu, s->d_worig, nullptr, s->d_part, s->d_linf, s->d_tstat, 1, st));
+                                       s->d_bu, s->d_worig, nullptr, s->d_part, s->d_linf, s->d_tstat, 1, A, st));
     } else if (kern == PG_KERNEL_FAST) {
         HIPCHECK(pgk_launch_sweep_fast(s->shape, s->diag ? 1 : 0, PG_STORE_F32, s->K, s->F, s->G, s->Q,
                                        s->dV[s->cur], nullptr, s->dV[s->cur], Vout, nullptr, s->d_occ, s->d_bu,
-                                       s->d_worig, nullptr, s->d_part, s->d_linf, nullptr, 1, st));
+                                       s->d_worig, nullptr, s->d_part, s->d_linf, nullptr, 1, A, st));
     } else if (kern == PG_KERNEL_LDS) {
         HIPCHECK(pgk_launch_sweep_lds(s->shape, s->diag ? 1 : 0, s->K, s->G, s->dV[s->cur], Vout, s->d_occ,
-                                      s->d_bu, s->d_worig, nullptr, s->d_part, s->d_linf, 1, st));
+                                      s->d_bu, s->d_worig, nullptr, s->d_part, s->d_linf, 1, A, st));
     } else {
         HIPCHECK(pgk_launch_sweep(s->K, s->dV[s->cur], Vout, s->d_occ, s->d_bu, nullptr, s->d_part,
                                   s->d_linf, 1, st, &nblk));
     }
     HIPCHECK(hipEventRecord(s->ev1, st));
     HIPCHECK(pgk_launch_reduce(s->d_part, nblk, s->d_sum, st));
-    HIPCHECK(pgk_launch_seed(Vout, s->d_seeds, s->nseeds, st));
+    if (mark && !s->f16_active) HIPCHECK(pgk_launch_seed_mark(Vout, s->d_seeds, s->nseeds, s->K, s->G, mark, st));
+    else HIPCHECK(pgk_launch_seed(Vout, s->d_seeds, s->nseeds, st));
     if (s->f16_active) {
-        rc = encode_from_scratch(s, false);
+        rc = encode_from_scratch(s, false, mark);
         if (rc) return rc;
     } else if (!inplace) {
         s->cur = 1 - s->cur;
     }
     s->G.rev = 0;
     double hsum = 0.0;
-    unsigned hl = 0;
+    unsigned hl = 0, hw = 0;
     HIPCHECK(hipMemcpyAsync(&hsum, s->d_sum, sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHECK(hipMemcpyAsync(&hl, s->d_linf, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    if (track) HIPCHECK(hipMemcpyAsync(&hw, s->d_nwork, sizeof(unsigned), hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
+    if (track) {
+        s->act_cur = 1 - s->act_cur;
+        s->act_for = regime;
+        s->last_work = hw;
+    } else {
+        s->act_for = -1;
+        s->last_work = -1;
+    }
     HIPCHECK(hipEventElapsedTime(&s->last_ms, s->ev0, s->ev1));
     float lf;
     std::memcpy(&lf, &hl, 4);
@@ -728,14 +784,14 @@ int pg_policy(pg_solver* s, uint8_t* pol_host) {
     if (s->f16_active) {
         HIPCHECK(pgk_launch_sweep_fast(s->shape, s->diag ? 1 : 0, s->store, s->K, s->F, s->G, s->Q,
                                        s->d_V16, s->d_tp, s->dV[0], nullptr, nullptr, s->d_occ, s->d_bu,
-                                       s->d_worig, s->d_pol, nullptr, nullptr, nullptr, 0, st));
+                                       s->d_worig, s->d_pol, nullptr, nullptr, nullptr, 0, pgk::ActiveG{}, st));
     } else if (kern == PG_KERNEL_FAST) {
         HIPCHECK(pgk_launch_sweep_fast(s->shape, s->diag ? 1 : 0, PG_STORE_F32, s->K, s->F, s->G, s->Q,
                                        s->dV[s->cur], nullptr, s->dV[s->cur], nullptr, nullptr, s->d_occ, s->d_bu,
-                                       s->d_worig, s->d_pol, nullptr, nullptr, nullptr, 0, st));
+                                       s->d_worig, s->d_pol, nullptr, nullptr, nullptr, 0, pgk::ActiveG{}, st));
     } else if (kern == PG_KERNEL_LDS) {
         HIPCHECK(pgk_launch_sweep_lds(s->shape, s->diag ? 1 : 0, s->K, s->G, s->dV[s->cur], nullptr, s->d_occ,
-                                      s->d_bu, s->d_worig, s->d_pol, nullptr, nullptr, 0, st));
+                                      s->d_bu, s->d_worig, s->d_pol, nullptr, nullptr, 0, pgk::ActiveG{}, st));
     } else {
         HIPCHECK(pgk_launch_sweep(s->K, s->dV[s->cur], nullptr, s->d_occ, s->d_bu, s->d_pol, nullptr, nullptr, 0, st, &nblk));
     }
@@ -784,6 +840,7 @@ int pg_upload_V(pg_solver* s, const float* src, int64_t off, int64_t n) {
     }
     HIPCHECK(hipSetDevice(s->ctx->device));
     hipStream_t st = s->ctx->stream;
+    s->act_for = -1;
     if (s->vstore != PG_VSTORE_F32 && (s->f16_active || !s->seeded)) {
         if (!s->seeded && (off > 0 || n < s->ncell)) {
             HIPCHECK(pgk_launch_fill(s->dV[0], s->ncell, s->p.cap, st));
@@ -1080,6 +1137,21 @@ int pg_solver_kernel_info(pg_solver* s, int32_t* kernel, double* window_coverage
 int pg_last_sweep_ms(pg_solver* s, float* ms) {
     if (!s || !ms) return fail(PG_ERR_ARG, "null argument");
     *ms = s->last_ms;
+    return PG_OK;
+}
+
+int pg_solver_set_active(pg_solver* s, int32_t on) {
+    if (!s) return fail(PG_ERR_ARG, "null solver");
+    s->active_on = on != 0;
+    s->act_for = -1;
+    return PG_OK;
+}
+
+int pg_last_sweep_boxes(pg_solver* s, int64_t* computed, int64_t* total) {
+    if (!s) return fail(PG_ERR_ARG, "null solver");
+    const int64_t nbox = s->G.nbox;
+    if (total) *total = nbox;
+    if (computed) *computed = s->last_work >= 0 ? s->last_work : nbox;
     return PG_OK;
 }
 

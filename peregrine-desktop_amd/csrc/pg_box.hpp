@@ -32,15 +32,17 @@ using Win0 = Win<4, 4, 4, 4, 4, 6>;
 using Win1 = Win<4, 4, 4, 4, 4, 4>;
 using Win2 = Win<4, 4, 4, 5, 5, 5>;
 
-// The box a workgroup owns, with XCD-contiguous numbering: consecutive
-// workgroup ids land on different XCDs, so box ranges are dealt out so that
-// each XCD walks neighbouring boxes (whose windows overlap) through its L2.
+// The box a workgroup owns. Workgroups are dealt to the 8 XCDs in turn
+// (workgroup b runs on XCD b % 8), and work per box is uneven (obstacles,
+// rejected successors, boxes an active set skips), so each XCD walks chunks
+// of kBoxChunk consecutive boxes (neighbours, whose windows overlap, share
+// its L2) taken from all over the grid: chunk q of XCD x is chunk 8q + x.
 __device__ __forceinline__ int64_t box_of_block(const BoxGeom& G) {
-    const int64_t per = (G.nbox + 7) / 8;
     int64_t b = G.blk0 + (int64_t)blockIdx.x;
-    // reversed order: the last boxes of each XCD's range first
-    if (G.rev) b = (b % 8) + 8 * (per - 1 - b / 8);
-    return (b % 8) * per + b / 8;
+    // reversed order (in-place sweeps alternate): the last boxes first
+    if (G.rev) b = box_blocks(G.nbox) - 1 - b;
+    const int64_t j = b / 8;
+    return ((j / kBoxChunk) * 8 + b % 8) * kBoxChunk + j % kBoxChunk;
 }
 
 // A dispatch counts its work-items in 32 bits, so a grid of boxes is
@@ -50,7 +52,7 @@ constexpr int64_t kMaxBoxBlocks = (int64_t)1 << 23;
 
 template <class Launch>
 inline hipError_t for_box_chunks(const BoxGeom& G, Launch launch) {
-    const int64_t nb = ((G.nbox + 7) / 8) * 8;
+    const int64_t nb = box_blocks(G.nbox);
     for (int64_t b0 = 0; b0 < nb; b0 += kMaxBoxBlocks) {
         BoxGeom g = G;
         g.blk0 = b0;
@@ -123,6 +125,78 @@ __device__ __forceinline__ bool window_cell(const int32_t n[6], const int o[6], 
         gi[a] = gv;
     }
     return on;
+}
+
+// The boxes of one axis that meet the cells [c, c + ext): `cnt` boxes from
+// `first`, counted modulo nb on the (periodic) heading axis. Off-grid cells
+// of the other axes are clamped, as the corner indices are.
+__device__ __forceinline__ void box_span(bool periodic, int c, int ext, int n, int b, int nb, int& first,
+                                         int& cnt) {
+    if (periodic) {
+        const int last = c + ext - 1;
+        if (ext >= n) { first = 0; cnt = nb; return; }
+        first = c / b;
+        if (last < n) { cnt = last / b - first + 1; return; }
+        cnt = (nb - first) + (last - n) / b + 1;
+        if (cnt >= nb) { first = 0; cnt = nb; }
+        return;
+    }
+    const int lo = c < 0 ? 0 : (c > n - 1 ? n - 1 : c);
+    int hi = c + ext - 1;
+    hi = hi > n - 1 ? n - 1 : (hi < lo ? lo : hi);
+    first = lo / b;
+    cnt = hi / b - first + 1;
+}
+
+// Nonzero in at least one thread of the block iff a box of the box's read
+// range, or the box itself, changed in the previous sweep (every thread
+// calls it and the block ORs the results).
+__device__ __forceinline__ int box_probe(const ActiveG& A, const BoxGeom& G, const int32_t n[6],
+                                         const BoxInfo& B) {
+    const int64_t gb = B.gbox;
+    int first[6], cnt[6];
+#pragma unroll
+    for (int a = 0; a < 6; ++a)
+        box_span(a == 2, A.rorig[gb * 6 + a], A.rext[gb * 6 + a], n[a], G.b[a], G.nbg[a], first[a], cnt[a]);
+    const uint32_t mask = (cnt[5] >= 32 ? ~0u : ((1u << cnt[5]) - 1u)) << first[5];
+    const int total = cnt[0] * cnt[1] * cnt[2] * cnt[3] * cnt[4];
+    int hit = 0;
+    for (int t = threadIdx.x; t < total; t += kThreads) {
+        int r = t, bi[5];
+#pragma unroll
+        for (int a = 4; a >= 0; --a) {
+            int v = first[a] + r % cnt[a];
+            r /= cnt[a];
+            if (a == 2 && v >= G.nbg[2]) v -= G.nbg[2];
+            bi[a] = v;
+        }
+        int64_t w = 0;
+#pragma unroll
+        for (int a = 0; a < 5; ++a) w = w * G.nbg[a] + bi[a];
+        hit |= (A.prev[w] & mask) != 0u;
+    }
+    if (threadIdx.x == 0) hit |= (int)((A.prev[gb / G.nbg[5]] >> (int)(gb % G.nbg[5])) & 1u);
+    return hit;
+}
+
+// Whether the block computes its box this sweep (block-uniform; every
+// thread of a live block must call it).
+__device__ __forceinline__ bool box_active(const ActiveG& A, const BoxGeom& G, const int32_t n[6],
+                                           const BoxInfo& B) {
+    if (!A.prev) return true;
+    return __syncthreads_or(box_probe(A, G, n, B)) != 0;
+}
+
+__device__ __forceinline__ void box_mark(uint32_t* bits, const BoxGeom& G, int64_t gbox) {
+    atomicOr(&bits[gbox / G.nbg[5]], 1u << (int)(gbox % G.nbg[5]));
+}
+
+// The whole-grid box number of cell i.
+__device__ __forceinline__ int64_t box_of_cell(const BoxGeom& G, const int i[6]) {
+    int64_t g = 0;
+#pragma unroll
+    for (int a = 0; a < 6; ++a) g = g * G.nbg[a] + i[a] / G.b[a];
+    return g;
 }
 
 struct OccG {

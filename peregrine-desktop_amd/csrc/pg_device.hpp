@@ -379,7 +379,7 @@ __device__ __forceinline__ void fetch_global(const Consts& K, const float* __res
     }
 }
 
-// Quantization tiles of the FP16 store: 8^6 index cells, edge tiles partial.
+// Value stores of the precision-form kernels.
 #define PG_STORE_F32 0
 #define PG_STORE_F16 1   // IEEE binary16 codes of (v - off) / scale
 #define PG_STORE_U16 2   // 16-bit fixed-point codes of (v - off) / scale
@@ -406,11 +406,36 @@ struct BoxGeom {
     int32_t rev;      // walk the boxes in reverse order (in-place sweeps alternate)
 };
 
+// Workgroups of a launch over nbox boxes (one box each): whole rounds of 8
+// chunks of kBoxChunk boxes, the tail's workgroups idle.
+constexpr int64_t kBoxChunk = 32;
+__host__ __device__ inline int64_t box_blocks(int64_t nbox) {
+    const int64_t chunks = (nbox + kBoxChunk - 1) / kBoxChunk;
+    return ((chunks + 7) / 8) * 8 * kBoxChunk;
+}
+
+// Active boxes of a Jacobi sweep. One bit per box over the whole grid's
+// boxes, numbered as BoxGeom's (omega fastest): the nbg[5] <= 32 boxes of a
+// row along omega share a word (word gbox / nbg[5], bit gbox % nbg[5]).
+// A box whose read range (every cell any of its backups can read, from a
+// plan pass with the sweep's own arithmetic) holds no box that changed in
+// the previous sweep, and which did not change itself, is skipped: its
+// backups would see the inputs of the last sweep that computed them, so
+// V' = min(V, TV) = V, and both Jacobi buffers already hold V.
+struct ActiveG {
+    const uint32_t* prev;   // boxes changed by the previous sweep; null: every box is computed
+    uint32_t* next;         // this sweep marks the boxes it changes (FP32 stores; null: not marked)
+    const int16_t* rorig;   // per box: first cell of the read range per axis (heading modulo n3)
+    const uint8_t* rext;    //          cells of the read range per axis
+    unsigned* nwork;        // counts the boxes computed (may be null)
+};
+
 // Block-wide sum (FP64) and max of the per-cell decreases: wave shuffles,
 // then one LDS slot per wave; thread 0 writes the block's partial sum and
 // folds the max into *linf_bits (non-negative floats order as their bits).
-__device__ __forceinline__ void block_reduce_sum_max(double dsum, float dmax, double* part_sum,
-                                                     unsigned* linf_bits) {
+// Returns the block's max to thread 0 (0 to the others).
+__device__ __forceinline__ float block_reduce_sum_max(double dsum, float dmax, double* part_sum,
+                                                      unsigned* linf_bits) {
     __shared__ double ssum[16];
     __shared__ float smax[16];
 #pragma unroll
@@ -428,7 +453,9 @@ __device__ __forceinline__ void block_reduce_sum_max(double dsum, float dmax, do
         for (int i = 1; i < nw; ++i) { s += ssum[i]; m = fmaxf(m, smax[i]); }
         part_sum[blockIdx.x] = s;
         if (m > 0.0f) atomicMax(linf_bits, __float_as_uint(m));
+        return m;
     }
+    return 0.0f;
 }
 
 }  // namespace pgk

@@ -75,6 +75,18 @@ struct pg_solver {
     int shape = 0;
     double coverage = 0.0;
     int32_t sweep_mode = PG_SWEEP_JACOBI;
+    // active boxes of Jacobi sweeps: the change bits of the last sweep in
+    // d_act[act_cur], valid for the sweep regime act_for (0 exact FP32,
+    // 1 precision-form FP32, 2 16-bit store; -1: none, every box computed)
+    bool active_on = true;
+    uint32_t* d_act[2] = {nullptr, nullptr};
+    int act_cur = 0;
+    int act_for = -1;
+    uint8_t* d_wext = nullptr;      // exact read ranges: cells per axis from d_worig
+    int16_t* d_forig = nullptr;     // precision-form read ranges
+    uint8_t* d_fext = nullptr;
+    unsigned* d_nwork = nullptr;
+    int64_t last_work = -1;         // boxes the last sweep computed (-1: every box)
     // FP64 tables (allocated on first use)
     pgk::Consts64 K64{};
     std::vector<double> bu64;

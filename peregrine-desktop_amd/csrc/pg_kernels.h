@@ -11,16 +11,21 @@ hipError_t pgk_launch_sweep(const pgk::Consts& K, const float* Vin, float* Vout,
 hipError_t pgk_launch_reduce(const double* part, int64_t n, double* out, hipStream_t st);
 hipError_t pgk_launch_fill(float* V, int64_t n, float v, hipStream_t st);
 hipError_t pgk_launch_seed(float* V, const int64_t* seeds, int64_t n, hipStream_t st);
+// The same, marking in `mark` the box of every seed whose value it changes.
+hipError_t pgk_launch_seed_mark(float* V, const int64_t* seeds, int64_t n, const pgk::Consts& K,
+                                const pgk::BoxGeom& G, uint32_t* mark, hipStream_t st);
 
 // LDS-staged path (pg_sweep_lds.hip).
 int pgk_window_shapes(const int (**shapes)[6]);
-hipError_t pgk_launch_plan(const pgk::Consts& K, const pgk::BoxGeom& G, const uint8_t* occ,
-                           const float* bu, int16_t* worig, uint8_t* wext, int diag,
+// Per-box corner ranges (origin, cells per axis): with F null those of the
+// exact successor, else those of the precision form's.
+hipError_t pgk_launch_plan(const pgk::Consts& K, const pgk::FastConsts* F, const pgk::BoxGeom& G,
+                           const uint8_t* occ, const float* bu, int16_t* worig, uint8_t* wext, int diag,
                            hipStream_t st);
 hipError_t pgk_launch_sweep_lds(int shape, int diag, const pgk::Consts& K, const pgk::BoxGeom& G,
                                 const float* Vin, float* Vout, const uint8_t* occ, const float* bu,
                                 const int16_t* worig, uint8_t* pol, double* part, unsigned* linf,
-                                int write_v, hipStream_t st);
+                                int write_v, const pgk::ActiveG& A, hipStream_t st);
 // One tile of an out-of-core sweep: G is the tile's interior region; the
 // window (cells wo[a] .. wo[a] + wl[a] - 1 on x, y and heading modulo n3,
 // every velocity cell) holds the old V; results go to the interior buffer
@@ -38,7 +43,7 @@ hipError_t pgk_launch_sweep_fast(int shape, int diag, int store, const pgk::Cons
                                  const float* vself, float* Vout, uint16_t* codes_out, const uint8_t* occ,
                                  const float* bu,
                                  const int16_t* worig, uint8_t* pol, double* part, unsigned* linf,
-                                 unsigned* tstat, int write_v, hipStream_t st);
+                                 unsigned* tstat, int write_v, const pgk::ActiveG& A, hipStream_t st);
 hipError_t pgk_launch_tile_stats(const float* V, const pgk::Consts& K, const pgk::QuantGeom& Q,
                                  unsigned* tstat, hipStream_t st);
 hipError_t pgk_launch_seed_stats(const int64_t* seeds, int64_t nseeds, const pgk::Consts& K,
@@ -46,9 +51,11 @@ hipError_t pgk_launch_seed_stats(const int64_t* seeds, int64_t nseeds, const pgk
 // Refit tiles whose reached values left their range (every tile with force).
 hipError_t pgk_launch_tile_refit(int store, unsigned* tstat, float2* tp, uint8_t* trefit, int64_t ntile,
                                  int force, hipStream_t st);
-// FP32 table -> 16-bit mirror (changed codes and refit tiles only).
+// FP32 table -> 16-bit mirror (changed codes and refit tiles only); with
+// `mark`, the boxes whose decoded values change are marked there.
 hipError_t pgk_launch_encode(int store, const float* V32, const pgk::Consts& K, const pgk::QuantGeom& Q,
-                             const float2* tp, const uint8_t* trefit, uint16_t* codes, hipStream_t st);
+                             const float2* tp, const uint8_t* trefit, uint16_t* codes, uint32_t* mark,
+                             const pgk::BoxGeom& G, hipStream_t st);
 // Codes of cells [off, off + n) -> V32[0, n).
 hipError_t pgk_launch_decode(int store, const uint16_t* codes, const pgk::Consts& K, const pgk::QuantGeom& Q,
                              const float2* tp, int64_t off, int64_t n, float* V32, hipStream_t st);

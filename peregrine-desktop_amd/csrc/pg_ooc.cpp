@@ -198,7 +198,7 @@ int ooc_create(pg_solver* s, int64_t budget_bytes, const int32_t tile_req[3]) {
         for (const OocTile& t : oc.tiles) {
             win = std::max(win, window_cells(t, oc.colv));
             in = std::max(in, interior_cells(t, oc.colv));
-            blocks = std::max(blocks, ceil_div(t.G.nbox, 8) * 8);
+            blocks = std::max(blocks, pgk::box_blocks(t.G.nbox));
         }
         return fixed + 2 * (win + in) * (int64_t)sizeof(float) + blocks * (int64_t)sizeof(double) +
                (int64_t)oc.tiles.size() * (int64_t)sizeof(double);
@@ -267,7 +267,7 @@ int ooc_create(pg_solver* s, int64_t budget_bytes, const int32_t tile_req[3]) {
     chk(hipMalloc(&o->d_miss, sizeof(unsigned)));
     chk(hipMalloc(&o->d_tsum, o->tiles.size() * sizeof(double)));
     int64_t blocks = 0;
-    for (const OocTile& t : o->tiles) blocks = std::max(blocks, ceil_div(t.G.nbox, 8) * 8);
+    for (const OocTile& t : o->tiles) blocks = std::max(blocks, pgk::box_blocks(t.G.nbox));
     if (e != hipSuccess)
         return fail(e == hipErrorOutOfMemory ? PG_ERR_NOMEM : PG_ERR_HIP, "out-of-core buffers: %s",
                     hipGetErrorString(e));
@@ -324,7 +324,7 @@ int ooc_sweep(pg_solver* s, double* sum_decrease, float* linf) {
         if (k >= 2) HIPCHECK(hipStreamWaitEvent(st, o.ev_d2h[j], 0));
         HIPCHECK(pgk_launch_sweep_tile(s->shape, s->diag ? 1 : 0, s->K, t.G, o.dwin[j], t.wo, t.wl, o.dout[j],
                                        nullptr, s->d_occ, s->d_bu, s->d_worig, s->d_part, s->d_linf, o.d_miss, st));
-        HIPCHECK(pgk_launch_reduce(s->d_part, ceil_div(t.G.nbox, 8) * 8, o.d_tsum + k, st));
+        HIPCHECK(pgk_launch_reduce(s->d_part, pgk::box_blocks(t.G.nbox), o.d_tsum + k, st));
         HIPCHECK(hipEventRecord(o.ev_k[j], st));
         // tile k+1's window, into the buffer tile k-1 has finished with
         if (k + 1 < nt) {

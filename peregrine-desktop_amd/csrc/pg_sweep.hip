@@ -7,7 +7,7 @@
 // Jacobi: every backup reads Vin and writes min(Vin, TVin) to Vout, which
 // makes the result independent of scheduling and equal bit for bit to the
 // CPU restatement.
-#include "pg_device.hpp"
+#include "pg_box.hpp"
 #include "pg_kernels.h"
 
 namespace pgk {
@@ -191,6 +191,26 @@ __global__ void seed_kernel(float* __restrict__ V, const int64_t* __restrict__ s
     if (i < n) V[seeds[i]] = 0.0f;
 }
 
+// The seed reset of an FP32 table under an active set: a seed whose value
+// it changes marks its box changed for the next sweep.
+__global__ void seed_mark_kernel(float* __restrict__ V, const int64_t* __restrict__ seeds, int64_t n, Consts K,
+                                 BoxGeom G, uint32_t* __restrict__ mark) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const int64_t c = seeds[j];
+    const float v = V[c];
+    V[c] = 0.0f;
+    if (__float_as_uint(v) != 0u) {
+        int64_t r = c;
+        int i[6];
+        for (int a = 5; a >= 0; --a) {
+            i[a] = (int)(r % K.n[a]);
+            r /= K.n[a];
+        }
+        box_mark(mark, G, box_of_cell(G, i));
+    }
+}
+
 }  // namespace pgk
 
 // ---------------------------------------------------------------------------
@@ -229,5 +249,13 @@ hipError_t pgk_launch_seed(float* V, const int64_t* seeds, int64_t n, hipStream_
     if (n <= 0) return hipSuccess;
     const int64_t nb = (n + 255) / 256;
     hipLaunchKernelGGL(pgk::seed_kernel, dim3((unsigned)nb), dim3(256), 0, st, V, seeds, n);
+    return hipGetLastError();
+}
+
+hipError_t pgk_launch_seed_mark(float* V, const int64_t* seeds, int64_t n, const pgk::Consts& K,
+                                const pgk::BoxGeom& G, uint32_t* mark, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    const int64_t nb = (n + 255) / 256;
+    hipLaunchKernelGGL(pgk::seed_mark_kernel, dim3((unsigned)nb), dim3(256), 0, st, V, seeds, n, K, G, mark);
     return hipGetLastError();
 }

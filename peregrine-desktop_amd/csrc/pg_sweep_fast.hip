@@ -106,7 +106,7 @@ __global__ __launch_bounds__(256, 4) void sweep_fast_kernel(
     fastk::VRead<STORE> vin, const float* vself, float* Vout, uint16_t* codes_out, const uint8_t* __restrict__ occ,
     const float* __restrict__ bu_g, Consts K, FastConsts F, BoxGeom G, QuantGeom Q,
     const int16_t* __restrict__ worig, uint8_t* __restrict__ pol, double* __restrict__ part,
-    unsigned* __restrict__ linf_bits, unsigned* __restrict__ tstat) {
+    unsigned* __restrict__ linf_bits, unsigned* __restrict__ tstat, ActiveG A) {
     using namespace fastk;
     // the 16-bit stores solve each backup's diagonal (see below); with the
     // FP32 store the kernel is the contracted twin of the exact backup
@@ -114,12 +114,14 @@ __global__ __launch_bounds__(256, 4) void sweep_fast_kernel(
     __shared__ float win[W::SIZE];
     __shared__ BoxShared<MAXS> sm;
     const int64_t box = box_of_block(G);
-    const bool live = box < G.nbox;
     BoxInfo B;
     int o[6];
     if (threadIdx.x == 0) { sm.bmin = 0x7f800000u; sm.bmax = 0u; }
+    if (box < G.nbox) box_info(G, box, B);
+    // live: the box is computed this sweep; a skipped box keeps its values
+    const bool live = box < G.nbox && box_active(A, G, K.n, B);
     if (live) {
-        box_info(G, box, B);
+        if (A.nwork && threadIdx.x == 0) atomicAdd(A.nwork, 1u);
 #pragma unroll
         for (int a = 0; a < 6; ++a) o[a] = worig[B.gbox * 6 + a];
         for (int j = threadIdx.x; j < W::CELLS; j += kThreads) {
@@ -297,14 +299,29 @@ __global__ __launch_bounds__(256, 4) void sweep_fast_kernel(
             }
             if (WRITE_POL) pol[idx] = (uint8_t)(key & 0xffffffffu);
         }
+    } else if (STORE != PG_STORE_F32 && WRITE_V && box < G.nbox) {
+        // skipped: its unchanged values still count in its tile's statistics
+        for (int s = threadIdx.x; s < B.ns; s += kThreads) {
+            int i[6];
+            state_coords(B, s, i);
+            int64_t idx = 0;
+#pragma unroll
+            for (int a = 0; a < 6; ++a) idx += (int64_t)i[a] * K.stride[a];
+            const float out = vself[idx];
+            if (out < K.cap) {
+                lmin = min(lmin, __float_as_uint(out));
+                lmax = max(lmax, __float_as_uint(out));
+            }
+        }
     }
     if (WRITE_V) {
         if (STORE != PG_STORE_F32) {
             atomicMin(&sm.bmin, lmin);
             atomicMax(&sm.bmax, lmax);
         }
-        block_reduce_sum_max(dsum, dmax, part + G.blk0, linf_bits);
-        if (STORE != PG_STORE_F32 && live && threadIdx.x == 0 && sm.bmin <= sm.bmax) {
+        const float m = block_reduce_sum_max(dsum, dmax, part + G.blk0, linf_bits);
+        if (A.next && m > 0.0f) box_mark(A.next, G, B.gbox);
+        if (STORE != PG_STORE_F32 && box < G.nbox && threadIdx.x == 0 && sm.bmin <= sm.bmax) {
             int64_t t = 0;
 #pragma unroll
             for (int a = 0; a < 6; ++a) t = t * Q.nt[a] + (B.s0[a] >> kQuantShift);
@@ -392,22 +409,49 @@ __global__ __launch_bounds__(256) void tile_refit_kernel(unsigned* __restrict__ 
 
 // FP32 table -> codes. In a refit tile every cell is encoded afresh; in
 // any other tile only a cell whose code changed is written.
+// With `mark`, the box of every cell whose decoded value changes (a new code,
+// or a refit tile) is marked changed for the next sweep's active set; the
+// lanes of a wave that hit one word of bits fold them into one atomic.
 template <int STORE>
 __global__ __launch_bounds__(256) void encode_kernel(const float* __restrict__ V32, Consts K,
                                                      QuantGeom Q, const float2* __restrict__ tp,
                                                      const uint8_t* __restrict__ trefit,
-                                                     uint16_t* __restrict__ codes) {
+                                                     uint16_t* __restrict__ codes, uint32_t* __restrict__ mark,
+                                                     BoxGeom G) {
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= K.ncell) return;
-    int64_t r = idx;
-    int i[6];
-    for (int a = 5; a >= 0; --a) {
-        i[a] = (int)(r % K.n[a]);
-        r /= K.n[a];
+    bool changed = false;
+    int64_t word = 0;
+    uint32_t bit = 0;
+    if (idx < K.ncell) {
+        int64_t r = idx;
+        int i[6];
+        for (int a = 5; a >= 0; --a) {
+            i[a] = (int)(r % K.n[a]);
+            r /= K.n[a];
+        }
+        const int64_t t = fastk::tile_of_coords(Q, i);
+        const uint16_t qn = fastk::Codec<STORE>::encode(V32[idx], tp[t], K.cap);
+        if (trefit[t] || qn != codes[idx]) {
+            codes[idx] = qn;
+            changed = true;
+            const int64_t gb = box_of_cell(G, i);
+            word = gb / G.nbg[5];
+            bit = 1u << (int)(gb % G.nbg[5]);
+        }
     }
-    const int64_t t = fastk::tile_of_coords(Q, i);
-    const uint16_t qn = fastk::Codec<STORE>::encode(V32[idx], tp[t], K.cap);
-    if (trefit[t] || qn != codes[idx]) codes[idx] = qn;
+    if (!mark) return;
+    unsigned long long pending = __ballot(changed);
+    const int lane = threadIdx.x & 63;
+    while (pending) {
+        const int leader = __ffsll((long long)pending) - 1;
+        const int64_t lw = __shfl(word, leader, 64);
+        const bool same = changed && word == lw;
+        uint32_t b = same ? bit : 0u;
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) b |= __shfl_xor(b, off, 64);
+        if (lane == leader) atomicOr(&mark[lw], b);
+        pending &= ~__ballot(same);
+    }
 }
 
 // Codes -> FP32 table.
@@ -436,7 +480,7 @@ hipError_t launch_fast_w(const pgk::Consts& K, const pgk::FastConsts& F, const p
                          const pgk::QuantGeom& Q, const void* Vin, const float2* tp, const float* vself, float* Vout,
                          uint16_t* codes_out,
                          const uint8_t* occ, const float* bu, const int16_t* worig, uint8_t* pol,
-                         double* part, unsigned* linf, unsigned* tstat, int write_v, hipStream_t st) {
+                         double* part, unsigned* linf, unsigned* tstat, int write_v, const pgk::ActiveG& A, hipStream_t st) {
     pgk::fastk::VRead<STORE> rd;
     if constexpr (STORE == PG_STORE_F32) {
         rd.V = (const float*)Vin;
@@ -447,13 +491,13 @@ hipError_t launch_fast_w(const pgk::Consts& K, const pgk::FastConsts& F, const p
     return pgk::for_box_chunks(G, [&](const pgk::BoxGeom& g, unsigned nb) {
         if (write_v && pol)
             hipLaunchKernelGGL((pgk::sweep_fast_kernel<W, DIAG, STORE, 128, true, true>), dim3(nb), dim3(256), 0, st,
-                               rd, vself, Vout, codes_out, occ, bu, K, F, g, Q, worig, pol, part, linf, tstat);
+                               rd, vself, Vout, codes_out, occ, bu, K, F, g, Q, worig, pol, part, linf, tstat, A);
         else if (write_v)
             hipLaunchKernelGGL((pgk::sweep_fast_kernel<W, DIAG, STORE, 128, true, false>), dim3(nb), dim3(256), 0, st,
-                               rd, vself, Vout, codes_out, occ, bu, K, F, g, Q, worig, pol, part, linf, tstat);
+                               rd, vself, Vout, codes_out, occ, bu, K, F, g, Q, worig, pol, part, linf, tstat, A);
         else
             hipLaunchKernelGGL((pgk::sweep_fast_kernel<W, DIAG, STORE, 128, false, true>), dim3(nb), dim3(256), 0, st,
-                               rd, vself, Vout, codes_out, occ, bu, K, F, g, Q, worig, pol, part, linf, tstat);
+                               rd, vself, Vout, codes_out, occ, bu, K, F, g, Q, worig, pol, part, linf, tstat, A);
     });
 }
 
@@ -463,14 +507,14 @@ hipError_t launch_fast_shape(int shape, const pgk::Consts& K, const pgk::FastCon
                              const float2* tp, const float* vself, float* Vout, uint16_t* codes_out,
                              const uint8_t* occ, const float* bu,
                              const int16_t* worig, uint8_t* pol, double* part, unsigned* linf,
-                             unsigned* tstat, int write_v, hipStream_t st) {
+                             unsigned* tstat, int write_v, const pgk::ActiveG& A, hipStream_t st) {
     using W0 = pgk::Win0;
     using W1 = pgk::Win1;
     using W2 = pgk::Win2;
     switch (shape) {
-        case 0: return launch_fast_w<W0, DIAG, STORE>(K, F, G, Q, Vin, tp, vself, Vout, codes_out, occ, bu, worig, pol, part, linf, tstat, write_v, st);
-        case 1: return launch_fast_w<W1, DIAG, STORE>(K, F, G, Q, Vin, tp, vself, Vout, codes_out, occ, bu, worig, pol, part, linf, tstat, write_v, st);
-        case 2: return launch_fast_w<W2, DIAG, STORE>(K, F, G, Q, Vin, tp, vself, Vout, codes_out, occ, bu, worig, pol, part, linf, tstat, write_v, st);
+        case 0: return launch_fast_w<W0, DIAG, STORE>(K, F, G, Q, Vin, tp, vself, Vout, codes_out, occ, bu, worig, pol, part, linf, tstat, write_v, A, st);
+        case 1: return launch_fast_w<W1, DIAG, STORE>(K, F, G, Q, Vin, tp, vself, Vout, codes_out, occ, bu, worig, pol, part, linf, tstat, write_v, A, st);
+        case 2: return launch_fast_w<W2, DIAG, STORE>(K, F, G, Q, Vin, tp, vself, Vout, codes_out, occ, bu, worig, pol, part, linf, tstat, write_v, A, st);
         default: return hipErrorInvalidValue;
     }
 }
@@ -482,14 +526,14 @@ hipError_t pgk_launch_sweep_fast(int shape, int diag, int store, const pgk::Cons
                                  const pgk::QuantGeom& Q, const void* Vin, const float2* tp,
                                  const float* vself, float* Vout, uint16_t* codes_out, const uint8_t* occ,
                                  const float* bu, const int16_t* worig, uint8_t* pol, double* part, unsigned* linf,
-                                 unsigned* tstat, int write_v, hipStream_t st) {
+                                 unsigned* tstat, int write_v, const pgk::ActiveG& A, hipStream_t st) {
     switch (store * 2 + (diag ? 1 : 0)) {
-        case PG_STORE_F32 * 2 + 0: return launch_fast_shape<false, PG_STORE_F32>(shape, K, F, G, Q, Vin, tp, vself, Vout, codes_out, occ, bu, worig, pol, part, linf, tstat, write_v, st);
-        case PG_STORE_F32 * 2 + 1: return launch_fast_shape<true, PG_STORE_F32>(shape, K, F, G, Q, Vin, tp, vself, Vout, codes_out, occ, bu, worig, pol, part, linf, tstat, write_v, st);
-        case PG_STORE_F16 * 2 + 0: return launch_fast_shape<false, PG_STORE_F16>(shape, K, F, G, Q, Vin, tp, vself, Vout, codes_out, occ, bu, worig, pol, part, linf, tstat, write_v, st);
-        case PG_STORE_F16 * 2 + 1: return launch_fast_shape<true, PG_STORE_F16>(shape, K, F, G, Q, Vin, tp, vself, Vout, codes_out, occ, bu, worig, pol, part, linf, tstat, write_v, st);
-        case PG_STORE_U16 * 2 + 0: return launch_fast_shape<false, PG_STORE_U16>(shape, K, F, G, Q, Vin, tp, vself, Vout, codes_out, occ, bu, worig, pol, part, linf, tstat, write_v, st);
-        case PG_STORE_U16 * 2 + 1: return launch_fast_shape<true, PG_STORE_U16>(shape, K, F, G, Q, Vin, tp, vself, Vout, codes_out, occ, bu, worig, pol, part, linf, tstat, write_v, st);
+        case PG_STORE_F32 * 2 + 0: return launch_fast_shape<false, PG_STORE_F32>(shape, K, F, G, Q, Vin, tp, vself, Vout, codes_out, occ, bu, worig, pol, part, linf, tstat, write_v, A, st);
+        case PG_STORE_F32 * 2 + 1: return launch_fast_shape<true, PG_STORE_F32>(shape, K, F, G, Q, Vin, tp, vself, Vout, codes_out, occ, bu, worig, pol, part, linf, tstat, write_v, A, st);
+        case PG_STORE_F16 * 2 + 0: return launch_fast_shape<false, PG_STORE_F16>(shape, K, F, G, Q, Vin, tp, vself, Vout, codes_out, occ, bu, worig, pol, part, linf, tstat, write_v, A, st);
+        case PG_STORE_F16 * 2 + 1: return launch_fast_shape<true, PG_STORE_F16>(shape, K, F, G, Q, Vin, tp, vself, Vout, codes_out, occ, bu, worig, pol, part, linf, tstat, write_v, A, st);
+        case PG_STORE_U16 * 2 + 0: return launch_fast_shape<false, PG_STORE_U16>(shape, K, F, G, Q, Vin, tp, vself, Vout, codes_out, occ, bu, worig, pol, part, linf, tstat, write_v, A, st);
+        case PG_STORE_U16 * 2 + 1: return launch_fast_shape<true, PG_STORE_U16>(shape, K, F, G, Q, Vin, tp, vself, Vout, codes_out, occ, bu, worig, pol, part, linf, tstat, write_v, A, st);
         default: return hipErrorInvalidValue;
     }
 }
@@ -519,14 +563,15 @@ hipError_t pgk_launch_tile_refit(int store, unsigned* tstat, float2* tp, uint8_t
 }
 
 hipError_t pgk_launch_encode(int store, const float* V32, const pgk::Consts& K, const pgk::QuantGeom& Q,
-                             const float2* tp, const uint8_t* trefit, uint16_t* codes, hipStream_t st) {
+                             const float2* tp, const uint8_t* trefit, uint16_t* codes, uint32_t* mark,
+                             const pgk::BoxGeom& G, hipStream_t st) {
     const int64_t nb = (K.ncell + 255) / 256;
     if (store == PG_STORE_U16)
         hipLaunchKernelGGL(pgk::encode_kernel<PG_STORE_U16>, dim3((unsigned)nb), dim3(256), 0, st, V32, K, Q, tp,
-                           trefit, codes);
+                           trefit, codes, mark, G);
     else
         hipLaunchKernelGGL(pgk::encode_kernel<PG_STORE_F16>, dim3((unsigned)nb), dim3(256), 0, st, V32, K, Q, tp,
-                           trefit, codes);
+                           trefit, codes, mark, G);
     return hipGetLastError();
 }
 

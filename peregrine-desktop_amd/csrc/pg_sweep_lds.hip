@@ -76,11 +76,13 @@ __device__ __forceinline__ int successor(const Consts& K, const OccG& og, const 
 // ---------------------------------------------------------------------------
 // Plan: per box, the smallest and largest base corner index per axis over
 // every (state, control) pair whose successor is interpolated. Heading is
-// measured relative to the box's first heading, in (-n3/2, n3/2].
+// measured relative to the box's first heading, in (-n3/2, n3/2]. FAST
+// plans with the precision form's successor (its read ranges; the window
+// origins always come from the exact successor).
 // ---------------------------------------------------------------------------
-template <bool DIAG, int MAXS>
+template <bool DIAG, bool FAST, int MAXS>
 __global__ __launch_bounds__(256) void plan_kernel(const uint8_t* __restrict__ occ,
-                                                   const float* __restrict__ bu_g, Consts K,
+                                                   const float* __restrict__ bu_g, Consts K, FastConsts F,
                                                    BoxGeom G, int16_t* __restrict__ worig,
                                                    uint8_t* __restrict__ wext) {
     const int64_t box = box_of_block(G);
@@ -105,7 +107,12 @@ __global__ __launch_bounds__(256) void plan_kernel(const uint8_t* __restrict__ o
             CellStart c;
             load_cell_start(sm, s, c);
             float f[6];
-            if (successor<DIAG>(K, og, c, sm.bu[3 * k], sm.bu[3 * k + 1], sm.bu[3 * k + 2], f)) {
+            bool interp;
+            if constexpr (FAST)
+                interp = successor_fast<DIAG>(K, F, og, c, sm.bu[3 * k], sm.bu[3 * k + 1], sm.bu[3 * k + 2], f);
+            else
+                interp = successor<DIAG>(K, og, c, sm.bu[3 * k], sm.bu[3 * k + 1], sm.bu[3 * k + 2], f) != 0;
+            if (interp) {
 #pragma unroll
                 for (int a = 0; a < 6; ++a) {
                     int i = (int)__builtin_floorf(f[a]);
@@ -255,15 +262,17 @@ template <class W, bool DIAG, int MAXS, bool WRITE_V, bool WRITE_POL, class SRC,
 __global__ __launch_bounds__(256, 4) void sweep_lds_kernel(
     SRC src, DST dst, const uint8_t* __restrict__ occ,
     const float* __restrict__ bu_g, Consts K, BoxGeom G, const int16_t* __restrict__ worig,
-    double* __restrict__ part, unsigned* __restrict__ linf_bits) {
+    double* __restrict__ part, unsigned* __restrict__ linf_bits, ActiveG A) {
     __shared__ float win[W::SIZE];
     __shared__ BoxShared<MAXS> sm;
     const int64_t box = box_of_block(G);
-    const bool live = box < G.nbox;
     BoxInfo B;
     int o[6];
+    if (box < G.nbox) box_info(G, box, B);
+    // live: the box is computed this sweep
+    const bool live = box < G.nbox && box_active(A, G, K.n, B);
     if (live) {
-        box_info(G, box, B);
+        if (A.nwork && threadIdx.x == 0) atomicAdd(A.nwork, 1u);
 #pragma unroll
         for (int a = 0; a < 6; ++a) o[a] = worig[B.gbox * 6 + a];
         // stage the window; cells off the grid hold cap (read only with zero weight)
@@ -368,7 +377,10 @@ __global__ __launch_bounds__(256, 4) void sweep_lds_kernel(
             dst.put(K, i, out, bk);
         }
     }
-    if (WRITE_V) block_reduce_sum_max(dsum, dmax, part + G.blk0, linf_bits);
+    if (WRITE_V) {
+        const float m = block_reduce_sum_max(dsum, dmax, part + G.blk0, linf_bits);
+        if (A.next && m > 0.0f) box_mark(A.next, G, B.gbox);
+    }
 }
 
 }  // namespace pgk
@@ -382,31 +394,31 @@ namespace {
 template <class W, bool DIAG, int MAXS, class SRC, class DST>
 hipError_t launch_w(const pgk::Consts& K, const pgk::BoxGeom& G, const SRC& src, const DST& dst,
                     const uint8_t* occ, const float* bu, const int16_t* worig, double* part, unsigned* linf,
-                    int write_v, int write_pol, hipStream_t st) {
+                    int write_v, int write_pol, const pgk::ActiveG& A, hipStream_t st) {
     return pgk::for_box_chunks(G, [&](const pgk::BoxGeom& g, unsigned nb) {
         if (write_v && write_pol)
             hipLaunchKernelGGL((pgk::sweep_lds_kernel<W, DIAG, MAXS, true, true, SRC, DST>), dim3(nb), dim3(256), 0,
-                               st, src, dst, occ, bu, K, g, worig, part, linf);
+                               st, src, dst, occ, bu, K, g, worig, part, linf, A);
         else if (write_v)
             hipLaunchKernelGGL((pgk::sweep_lds_kernel<W, DIAG, MAXS, true, false, SRC, DST>), dim3(nb), dim3(256),
-                               0, st, src, dst, occ, bu, K, g, worig, part, linf);
+                               0, st, src, dst, occ, bu, K, g, worig, part, linf, A);
         else
             hipLaunchKernelGGL((pgk::sweep_lds_kernel<W, DIAG, MAXS, false, true, SRC, DST>), dim3(nb), dim3(256),
-                               0, st, src, dst, occ, bu, K, g, worig, part, linf);
+                               0, st, src, dst, occ, bu, K, g, worig, part, linf, A);
     });
 }
 
 template <class SRC, class DST>
 hipError_t launch_shape(int shape, int diag, const pgk::Consts& K, const pgk::BoxGeom& G, const SRC& src,
                         const DST& dst, const uint8_t* occ, const float* bu, const int16_t* worig, double* part,
-                        unsigned* linf, int write_v, int write_pol, hipStream_t st) {
+                        unsigned* linf, int write_v, int write_pol, const pgk::ActiveG& A, hipStream_t st) {
     switch (shape * 2 + (diag ? 1 : 0)) {
-        case 0: return launch_w<pgk::Win0, false, 128>(K, G, src, dst, occ, bu, worig, part, linf, write_v, write_pol, st);
-        case 1: return launch_w<pgk::Win0, true, 128>(K, G, src, dst, occ, bu, worig, part, linf, write_v, write_pol, st);
-        case 2: return launch_w<pgk::Win1, false, 128>(K, G, src, dst, occ, bu, worig, part, linf, write_v, write_pol, st);
-        case 3: return launch_w<pgk::Win1, true, 128>(K, G, src, dst, occ, bu, worig, part, linf, write_v, write_pol, st);
-        case 4: return launch_w<pgk::Win2, false, 128>(K, G, src, dst, occ, bu, worig, part, linf, write_v, write_pol, st);
-        case 5: return launch_w<pgk::Win2, true, 128>(K, G, src, dst, occ, bu, worig, part, linf, write_v, write_pol, st);
+        case 0: return launch_w<pgk::Win0, false, 128>(K, G, src, dst, occ, bu, worig, part, linf, write_v, write_pol, A, st);
+        case 1: return launch_w<pgk::Win0, true, 128>(K, G, src, dst, occ, bu, worig, part, linf, write_v, write_pol, A, st);
+        case 2: return launch_w<pgk::Win1, false, 128>(K, G, src, dst, occ, bu, worig, part, linf, write_v, write_pol, A, st);
+        case 3: return launch_w<pgk::Win1, true, 128>(K, G, src, dst, occ, bu, worig, part, linf, write_v, write_pol, A, st);
+        case 4: return launch_w<pgk::Win2, false, 128>(K, G, src, dst, occ, bu, worig, part, linf, write_v, write_pol, A, st);
+        case 5: return launch_w<pgk::Win2, true, 128>(K, G, src, dst, occ, bu, worig, part, linf, write_v, write_pol, A, st);
         default: return hipErrorInvalidValue;
     }
 }
@@ -426,25 +438,28 @@ int pgk_window_shapes(const int (**shapes)[6]) {
     return (int)(sizeof kShapes / sizeof kShapes[0]);
 }
 
-hipError_t pgk_launch_plan(const pgk::Consts& K, const pgk::BoxGeom& G, const uint8_t* occ,
-                           const float* bu, int16_t* worig, uint8_t* wext, int diag,
+hipError_t pgk_launch_plan(const pgk::Consts& K, const pgk::FastConsts* F, const pgk::BoxGeom& G,
+                           const uint8_t* occ, const float* bu, int16_t* worig, uint8_t* wext, int diag,
                            hipStream_t st) {
+    const pgk::FastConsts f = F ? *F : pgk::FastConsts{};
     return pgk::for_box_chunks(G, [&](const pgk::BoxGeom& g, unsigned nb) {
-        if (diag)
-            hipLaunchKernelGGL((pgk::plan_kernel<true, 128>), dim3(nb), dim3(256), 0, st, occ, bu, K, g, worig, wext);
-        else
-            hipLaunchKernelGGL((pgk::plan_kernel<false, 128>), dim3(nb), dim3(256), 0, st, occ, bu, K, g, worig,
-                               wext);
+        const dim3 grid(nb), blk(256);
+        switch ((F ? 2 : 0) + (diag ? 1 : 0)) {
+            case 0: hipLaunchKernelGGL((pgk::plan_kernel<false, false, 128>), grid, blk, 0, st, occ, bu, K, f, g, worig, wext); break;
+            case 1: hipLaunchKernelGGL((pgk::plan_kernel<true, false, 128>), grid, blk, 0, st, occ, bu, K, f, g, worig, wext); break;
+            case 2: hipLaunchKernelGGL((pgk::plan_kernel<false, true, 128>), grid, blk, 0, st, occ, bu, K, f, g, worig, wext); break;
+            default: hipLaunchKernelGGL((pgk::plan_kernel<true, true, 128>), grid, blk, 0, st, occ, bu, K, f, g, worig, wext); break;
+        }
     });
 }
 
 hipError_t pgk_launch_sweep_lds(int shape, int diag, const pgk::Consts& K, const pgk::BoxGeom& G,
                                 const float* Vin, float* Vout, const uint8_t* occ, const float* bu,
                                 const int16_t* worig, uint8_t* pol, double* part, unsigned* linf,
-                                int write_v, hipStream_t st) {
+                                int write_v, const pgk::ActiveG& A, hipStream_t st) {
     const pgk::DevSrc src{Vin};
     const pgk::DevDst dst{write_v ? Vout : nullptr, pol};
-    return launch_shape(shape, diag, K, G, src, dst, occ, bu, worig, part, linf, write_v, pol != nullptr, st);
+    return launch_shape(shape, diag, K, G, src, dst, occ, bu, worig, part, linf, write_v, pol != nullptr, A, st);
 }
 
 hipError_t pgk_launch_sweep_tile(int shape, int diag, const pgk::Consts& K, const pgk::BoxGeom& G,
@@ -476,5 +491,5 @@ hipError_t pgk_launch_sweep_tile(int shape, int diag, const pgk::Consts& K, cons
     dst.os[1] = dst.os[2] * G.rn[2];
     dst.os[0] = dst.os[1] * G.rn[1];
     return launch_shape(shape, diag, K, G, src, dst, occ, bu, worig, part, linf, out != nullptr, pol != nullptr,
-                        st);
+                        pgk::ActiveG{}, st);
 }

@@ -181,6 +181,19 @@ class HipSolver:
         """abi.PG_SWEEP_JACOBI (default, deterministic) or abi.PG_SWEEP_IN_PLACE."""
         abi.check(self.lib, "pg_solver_set_sweep_mode", self.lib.pg_solver_set_sweep_mode(self._h, int(mode)))
 
+    def set_active(self, on: bool) -> None:
+        """Skip the boxes whose inputs did not change (default on; results
+        are bit for bit those of computing every box)."""
+        abi.check(self.lib, "pg_solver_set_active", self.lib.pg_solver_set_active(self._h, int(bool(on))))
+
+    def last_sweep_boxes(self):
+        """(boxes the last sweep computed, boxes of the grid)."""
+        c = ctypes.c_int64()
+        t = ctypes.c_int64()
+        abi.check(self.lib, "pg_last_sweep_boxes",
+                  self.lib.pg_last_sweep_boxes(self._h, ctypes.byref(c), ctypes.byref(t)))
+        return c.value, t.value
+
     def polish(self, nsweeps: int):
         """Decode an FP16 store to FP32 and run `nsweeps` exact sweeps."""
         d = ctypes.c_double()
