@@ -112,6 +112,7 @@ void release(pg_solver* s) {
     (void)hipFree(s->d_wext);
     (void)hipFree(s->d_forig);
     (void)hipFree(s->d_fext);
+    (void)hipFree(s->d_ffree);
     (void)hipFree(s->d_act[0]);
     (void)hipFree(s->d_act[1]);
     (void)hipFree(s->d_nwork);
@@ -274,7 +275,8 @@ int plan(pg_solver* s, bool keep_host, std::vector<int16_t>* h_orig, std::vector
     if (!s->d_worig) HIPCHECK(hipMalloc(&s->d_worig, (size_t)nbox * 6 * sizeof(int16_t)));
     if (!s->d_wext) HIPCHECK(hipMalloc(&s->d_wext, (size_t)nbox * 6));
     uint8_t* d_ext = s->d_wext;
-    hipError_t e = pgk_launch_plan(s->K, nullptr, s->G, s->d_occ, s->d_bu, s->d_worig, d_ext, s->diag ? 1 : 0, st);
+    hipError_t e = pgk_launch_plan(s->K, nullptr, s->G, s->d_occ, s->d_bu, s->d_worig, d_ext, nullptr,
+                                   s->diag ? 1 : 0, st);
     std::vector<uint8_t> ext_local;
     std::vector<uint8_t>& ext = h_ext ? *h_ext : ext_local;
     ext.resize((size_t)nbox * 6);
@@ -329,6 +331,19 @@ int encode_from_scratch(pg_solver* s, bool force, uint32_t* mark = nullptr) {
 // every state's own value in FP32 and the codes mirror it for the gathers.
 float* current_f32(pg_solver* s) { return s->f16_active ? s->dV[0] : s->dV[s->cur]; }
 
+// The precision form's plan (once per solver, after the exact one): its
+// read ranges and the boxes whose probes it skips.
+int plan_fast(pg_solver* s) {
+    if (s->d_ffree) return PG_OK;
+    hipStream_t st = s->ctx->stream;
+    HIPCHECK(hipMalloc(&s->d_forig, (size_t)s->G.nbox * 6 * sizeof(int16_t)));
+    HIPCHECK(hipMalloc(&s->d_fext, (size_t)s->G.nbox * 6));
+    HIPCHECK(hipMalloc(&s->d_ffree, (size_t)s->G.nbox));
+    HIPCHECK(pgk_launch_plan(s->K, &s->F, s->G, s->d_occ, s->d_bu, s->d_forig, s->d_fext, s->d_ffree,
+                             s->diag ? 1 : 0, st));
+    return PG_OK;
+}
+
 // The active set of the coming sweep in `regime` (see pg_solver::act_for):
 // A.prev holds the previous sweep's change bits when they were made in the
 // same regime, A.next the cleared bits this sweep marks. Returns false (A
@@ -345,11 +360,9 @@ int active_setup(pg_solver* s, int regime, bool inplace, pgk::ActiveG& A, bool& 
         HIPCHECK(hipMalloc(&s->d_nwork, sizeof(unsigned)));
         s->act_for = -1;
     }
-    if (regime != 0 && !s->d_forig) {
-        // read ranges of the precision form's successors (one pass per solver)
-        HIPCHECK(hipMalloc(&s->d_forig, (size_t)s->G.nbox * 6 * sizeof(int16_t)));
-        HIPCHECK(hipMalloc(&s->d_fext, (size_t)s->G.nbox * 6));
-        HIPCHECK(pgk_launch_plan(s->K, &s->F, s->G, s->d_occ, s->d_bu, s->d_forig, s->d_fext, s->diag ? 1 : 0, st));
+    if (regime != 0) {
+        const int rc = plan_fast(s);
+        if (rc) return rc;
     }
     uint32_t* next = s->d_act[1 - s->act_cur];
     HIPCHECK(hipMemsetAsync(next, 0, (size_t)words * sizeof(uint32_t), st));
@@ -651,6 +664,7 @@ int pg_sweep(pg_solver* s, int32_t phase, int32_t rev, double* sum_decrease, flo
     const int kern = s->f16_active ? PG_KERNEL_FAST : s->kernel;
     if (kern != PG_KERNEL_GLOBAL) {
         int rc = plan(s);
+        if (rc == PG_OK && kern == PG_KERNEL_FAST) rc = plan_fast(s);
         if (rc) return rc;
     }
     const int64_t nb = (kern == PG_KERNEL_GLOBAL) ? (s->ncell + 255) / 256 : lds_blocks(s);
@@ -673,11 +687,12 @@ int pg_sweep(pg_solver* s, int32_t phase, int32_t rev, double* sum_decrease, flo
     if (s->f16_active) {
         HIPCHECK(pgk_launch_sweep_fast(s->shape, s->diag ? 1 : 0, s->store, s->K, s->F, s->G, s->Q,
                                        s->d_V16, s->d_tp, s->dV[0], Vout, inplace ? s->d_V16 : nullptr, s->d_occ,
-                                       s->d_bu, s->d_worig, nullptr, s->d_part, s->d_linf, s->d_tstat, 1, A, st));
+                                       s->d_bu, s->d_worig, s->d_ffree, nullptr, s->d_part, s->d_linf, s->d_tstat, 1,
+                                       A, st));
     } else if (kern == PG_KERNEL_FAST) {
         HIPCHECK(pgk_launch_sweep_fast(s->shape, s->diag ? 1 : 0, PG_STORE_F32, s->K, s->F, s->G, s->Q,
                                        s->dV[s->cur], nullptr, s->dV[s->cur], Vout, nullptr, s->d_occ, s->d_bu,
-                                       s->d_worig, nullptr, s->d_part, s->d_linf, nullptr, 1, A, st));
+                                       s->d_worig, s->d_ffree, nullptr, s->d_part, s->d_linf, nullptr, 1, A, st));
     } else if (kern == PG_KERNEL_LDS) {
         HIPCHECK(pgk_launch_sweep_lds(s->shape, s->diag ? 1 : 0, s->K, s->G, s->dV[s->cur], Vout, s->d_occ,
                                       s->d_bu, s->d_worig, nullptr, s->d_part, s->d_linf, 1, A, st));
@@ -779,16 +794,19 @@ int pg_policy(pg_solver* s, uint8_t* pol_host) {
     const int kern = s->f16_active ? PG_KERNEL_FAST : s->kernel;
     if (kern != PG_KERNEL_GLOBAL) {
         int rc = plan(s);
+        if (rc == PG_OK && kern == PG_KERNEL_FAST) rc = plan_fast(s);
         if (rc) return rc;
     }
     if (s->f16_active) {
         HIPCHECK(pgk_launch_sweep_fast(s->shape, s->diag ? 1 : 0, s->store, s->K, s->F, s->G, s->Q,
                                        s->d_V16, s->d_tp, s->dV[0], nullptr, nullptr, s->d_occ, s->d_bu,
-                                       s->d_worig, s->d_pol, nullptr, nullptr, nullptr, 0, pgk::ActiveG{}, st));
+                                       s->d_worig, s->d_ffree, s->d_pol, nullptr, nullptr, nullptr, 0, pgk::ActiveG{},
+                                       st));
     } else if (kern == PG_KERNEL_FAST) {
         HIPCHECK(pgk_launch_sweep_fast(s->shape, s->diag ? 1 : 0, PG_STORE_F32, s->K, s->F, s->G, s->Q,
                                        s->dV[s->cur], nullptr, s->dV[s->cur], nullptr, nullptr, s->d_occ, s->d_bu,
-                                       s->d_worig, s->d_pol, nullptr, nullptr, nullptr, 0, pgk::ActiveG{}, st));
+                                       s->d_worig, s->d_ffree, s->d_pol, nullptr, nullptr, nullptr, 0, pgk::ActiveG{},
+                                       st));
     } else if (kern == PG_KERNEL_LDS) {
         HIPCHECK(pgk_launch_sweep_lds(s->shape, s->diag ? 1 : 0, s->K, s->G, s->dV[s->cur], nullptr, s->d_occ,
                                       s->d_bu, s->d_worig, s->d_pol, nullptr, nullptr, 0, pgk::ActiveG{}, st));

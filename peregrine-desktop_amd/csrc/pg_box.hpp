@@ -132,6 +132,7 @@ __device__ __forceinline__ bool window_cell(const int32_t n[6], const int o[6], 
 // of the other axes are clamped, as the corner indices are.
 __device__ __forceinline__ void box_span(bool periodic, int c, int ext, int n, int b, int nb, int& first,
                                          int& cnt) {
+    if (ext >= 255) { first = 0; cnt = nb; return; }   // a saturated extent: the whole axis
     if (periodic) {
         const int last = c + ext - 1;
         if (ext >= n) { first = 0; cnt = nb; return; }

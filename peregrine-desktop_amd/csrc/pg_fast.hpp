@@ -54,17 +54,48 @@ __device__ __forceinline__ void accel_fast(const Consts& K, const FastConsts& F,
     }
 }
 
-// Forward Euler, all derivatives from the old state (PeregrineSolver.jl:208-221).
-// Divergence is caught by the caller's finiteness test on the successor.
+// The state-only part of the integration. With its B*u_sat term set apart,
+// the first Euler substep depends on the state alone: positions and heading
+// after it (with sin/cos of that heading) and the velocities less hs*B*u_sat.
+struct FastStart {
+    float x0, y0, h0;          // the grid state
+    float x1, y1, h1, sh1, ch1;
+    float vx1, vy1, w1;
+};
+
 template <bool DIAG>
-__device__ __forceinline__ void euler_fast(const Consts& K, const FastConsts& F, const CellStart& c,
+__device__ __forceinline__ void fast_start(const Consts& K, const FastConsts& F, const CellStart& c,
+                                           FastStart& S) {
+    const float hs = K.hs;
+    float ax, ay, al;
+    accel_fast<DIAG>(K, F, 0.0f, 0.0f, 0.0f, c.vx, c.vy, c.w, ax, ay, al);
+    S.x0 = c.x;
+    S.y0 = c.y;
+    S.h0 = c.h;
+    S.x1 = c.x + (c.ch * c.vx - c.sh * c.vy) * hs;
+    S.y1 = c.y + (c.sh * c.vx + c.ch * c.vy) * hs;
+    S.h1 = c.h + c.w * hs;
+    S.vx1 = c.vx + (ax + c.w * c.vy) * hs;
+    S.vy1 = c.vy + (ay - c.w * c.vx) * hs;
+    S.w1 = c.w + al * hs;
+    sincos_fast(S.h1, S.sh1, S.ch1);
+}
+
+// Forward Euler, all derivatives from the old state (PeregrineSolver.jl:208-221),
+// from the state-only first substep. Divergence is caught by the caller's
+// finiteness test on the successor.
+template <bool DIAG>
+__device__ __forceinline__ void euler_fast(const Consts& K, const FastConsts& F, const FastStart& S,
                                            float bux, float buy, float buw, float& x, float& y,
                                            float& h, float& vx, float& vy, float& w) {
     const float hs = K.hs;
-    x = c.x; y = c.y; h = c.h; vx = c.vx; vy = c.vy; w = c.w;
-    float sh = c.sh, ch = c.ch;
-    for (int k = 0; k < K.nsub; ++k) {
-        if (k > 0) sincos_fast(h, sh, ch);
+    x = S.x1; y = S.y1; h = S.h1;
+    vx = S.vx1 + bux * hs;
+    vy = S.vy1 + buy * hs;
+    w = S.w1 + buw * hs;
+    float sh = S.sh1, ch = S.ch1;
+    for (int k = 1; k < K.nsub; ++k) {
+        if (k > 1) sincos_fast(h, sh, ch);
         float ax, ay, al;
         accel_fast<DIAG>(K, F, bux, buy, buw, vx, vy, w, ax, ay, al);
         const float dvx = ax + w * vy;
@@ -102,16 +133,21 @@ __device__ __forceinline__ bool probes_fast(const Consts& K, const FastConsts& F
     return true;
 }
 
+// The successor's fractional grid coordinates; false if rejected. With
+// probe false the swept probes are skipped: a box whose probes can only
+// reach free cells (the plan's flag) keeps just the final bounds test, which
+// holds the probes' own (the swept points lie between the in-grid start and
+// the successor).
 template <bool DIAG, class OccRead>
 __device__ __forceinline__ bool successor_fast(const Consts& K, const FastConsts& F,
-                                               const OccRead& og, const CellStart& c, float bux,
-                                               float buy, float buw, float f[6]) {
+                                               const OccRead& og, const FastStart& S, float bux,
+                                               float buy, float buw, bool probe, float f[6]) {
     float nx, ny, nh, nvx, nvy, nw;
-    euler_fast<DIAG>(K, F, c, bux, buy, buw, nx, ny, nh, nvx, nvy, nw);
-    const float dx = nx - c.x, dy = ny - c.y, dh = nh - c.h;
+    euler_fast<DIAG>(K, F, S, bux, buy, buw, nx, ny, nh, nvx, nvy, nw);
+    const float dx = nx - S.x0, dy = ny - S.y0, dh = nh - S.h0;
     if (!(finite_f(dx) & finite_f(dy) & finite_f(dh) & finite_f(nvx) & finite_f(nvy) & finite_f(nw)))
         return false;
-    if (!probes_fast(K, F, og, c.x, c.y, c.h, dx, dy, dh)) return false;
+    if (probe && !probes_fast(K, F, og, S.x0, S.y0, S.h0, dx, dy, dh)) return false;
     float snh, cnh;
     sincos_fast(nh, snh, cnh);
     const float nvfx = cnh * nvx - snh * nvy;
@@ -126,6 +162,20 @@ __device__ __forceinline__ bool successor_fast(const Consts& K, const FastConsts
 #pragma unroll
     for (int k = 0; k < 6; ++k) in = in & (f[k] >= 0.0f) & (f[k] <= K.lim[k]);
     return in;
+}
+
+// The x, y, heading grid coordinates of the successor's position (heading
+// unwrapped), for every finite successor whatever the tests decide; false
+// if not finite. The plan bounds the cells a box's probes can touch by them.
+template <bool DIAG>
+__device__ __forceinline__ bool successor_reach_fast(const Consts& K, const FastConsts& F, const FastStart& S,
+                                                     float bux, float buy, float buw, float g[3]) {
+    float nx, ny, nh, nvx, nvy, nw;
+    euler_fast<DIAG>(K, F, S, bux, buy, buw, nx, ny, nh, nvx, nvy, nw);
+    g[0] = (nx - K.lo[0]) * F.rstep[0];
+    g[1] = (ny - K.lo[1]) * F.rstep[1];
+    g[2] = (nh - K.lo[2]) * F.rstep[2];
+    return finite_f(g[0]) & finite_f(g[1]) & finite_f(g[2]) & finite_f(nvx) & finite_f(nvy) & finite_f(nw);
 }
 
 __device__ __forceinline__ float lerp_fast(float a, float b, float t) { return a + t * (b - a); }

@@ -85,6 +85,7 @@ struct pg_solver {
     uint8_t* d_wext = nullptr;      // exact read ranges: cells per axis from d_worig
     int16_t* d_forig = nullptr;     // precision-form read ranges
     uint8_t* d_fext = nullptr;
+    uint8_t* d_ffree = nullptr;     // precision form: boxes whose probes only meet free cells
     unsigned* d_nwork = nullptr;
     int64_t last_work = -1;         // boxes the last sweep computed (-1: every box)
     // FP64 tables (allocated on first use)

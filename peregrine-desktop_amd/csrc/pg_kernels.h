@@ -18,10 +18,11 @@ hipError_t pgk_launch_seed_mark(float* V, const int64_t* seeds, int64_t n, const
 // LDS-staged path (pg_sweep_lds.hip).
 int pgk_window_shapes(const int (**shapes)[6]);
 // Per-box corner ranges (origin, cells per axis): with F null those of the
-// exact successor, else those of the precision form's.
+// exact successor, else those of the precision form's, with pfree[box] set
+// where the box's probes can only touch free cells (then skipped).
 hipError_t pgk_launch_plan(const pgk::Consts& K, const pgk::FastConsts* F, const pgk::BoxGeom& G,
-                           const uint8_t* occ, const float* bu, int16_t* worig, uint8_t* wext, int diag,
-                           hipStream_t st);
+                           const uint8_t* occ, const float* bu, int16_t* worig, uint8_t* wext, uint8_t* pfree,
+                           int diag, hipStream_t st);
 hipError_t pgk_launch_sweep_lds(int shape, int diag, const pgk::Consts& K, const pgk::BoxGeom& G,
                                 const float* Vin, float* Vout, const uint8_t* occ, const float* bu,
                                 const int16_t* worig, uint8_t* pol, double* part, unsigned* linf,
@@ -36,14 +37,15 @@ hipError_t pgk_launch_sweep_tile(int shape, int diag, const pgk::Consts& K, cons
                                  uint8_t* pol, const uint8_t* occ, const float* bu, const int16_t* worig,
                                  double* part, unsigned* linf, unsigned* miss, hipStream_t st);
 
-// Precision form (pg_sweep_fast.hip).
+// Precision form (pg_sweep_fast.hip). pfree: the fast plan's per-box flags
+// (probes skipped where set).
 hipError_t pgk_launch_sweep_fast(int shape, int diag, int store, const pgk::Consts& K,
                                  const pgk::FastConsts& F, const pgk::BoxGeom& G,
                                  const pgk::QuantGeom& Q, const void* Vin, const float2* tp,
                                  const float* vself, float* Vout, uint16_t* codes_out, const uint8_t* occ,
-                                 const float* bu,
-                                 const int16_t* worig, uint8_t* pol, double* part, unsigned* linf,
-                                 unsigned* tstat, int write_v, const pgk::ActiveG& A, hipStream_t st);
+                                 const float* bu, const int16_t* worig, const uint8_t* pfree, uint8_t* pol,
+                                 double* part, unsigned* linf, unsigned* tstat, int write_v, const pgk::ActiveG& A,
+                                 hipStream_t st);
 hipError_t pgk_launch_tile_stats(const float* V, const pgk::Consts& K, const pgk::QuantGeom& Q,
                                  unsigned* tstat, hipStream_t st);
 hipError_t pgk_launch_seed_stats(const int64_t* seeds, int64_t nseeds, const pgk::Consts& K,

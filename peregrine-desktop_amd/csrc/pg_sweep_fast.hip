@@ -88,25 +88,25 @@ struct VRead<PG_STORE_F32> {
 
 template <int MAXS>
 struct BoxShared {
-    float x[MAXS], y[MAXS], h[MAXS], vx[MAXS], vy[MAXS], w[MAXS];
-    float sh[MAXS], ch[MAXS];
-    float vold[MAXS];          // the state's value before the sweep (FP32)
-    float vmir[MAXS];          // the same value as the gathers see it
-    uint16_t loc[MAXS];        // its coordinates inside the box, 3 bits per axis
-    unsigned long long best[MAXS];
-    uint8_t blocked[MAXS];
+    unsigned long long best[MAXS];   // per state: (value bits, control) minimum
     float bu[3 * kMaxCtl];
     unsigned bmin, bmax;
 };
 
 }  // namespace fastk
 
+// One workgroup per box. The tps = 256 / ns threads of a state share its
+// controls (k = k0, k0 + tps, ...): each thread holds the state's
+// control-independent start in registers, keeps its running minimum of
+// (value, control) in registers (strict <, so the lowest control wins a
+// tie) and folds it into the state's LDS minimum once. Threads 0..ns-1
+// then write the states.
 template <class W, bool DIAG, int STORE, int MAXS, bool WRITE_V, bool WRITE_POL>
 __global__ __launch_bounds__(256, 4) void sweep_fast_kernel(
     fastk::VRead<STORE> vin, const float* vself, float* Vout, uint16_t* codes_out, const uint8_t* __restrict__ occ,
     const float* __restrict__ bu_g, Consts K, FastConsts F, BoxGeom G, QuantGeom Q,
-    const int16_t* __restrict__ worig, uint8_t* __restrict__ pol, double* __restrict__ part,
-    unsigned* __restrict__ linf_bits, unsigned* __restrict__ tstat, ActiveG A) {
+    const int16_t* __restrict__ worig, const uint8_t* __restrict__ pfree, uint8_t* __restrict__ pol,
+    double* __restrict__ part, unsigned* __restrict__ linf_bits, unsigned* __restrict__ tstat, ActiveG A) {
     using namespace fastk;
     // the 16-bit stores solve each backup's diagonal (see below); with the
     // FP32 store the kernel is the contracted twin of the exact backup
@@ -139,57 +139,60 @@ __global__ __launch_bounds__(256, 4) void sweep_fast_kernel(
             for (int a = 0; a < 6; ++a) lo += l[a] * W::S[a];
             win[lo] = v;
         }
-        for (int s = threadIdx.x; s < B.ns; s += kThreads) {
+        for (int j = threadIdx.x; j < B.ns; j += kThreads) {
             int i[6];
-            state_coords(B, s, i);
-            CellStart c;
-            cell_start(K, i[0], i[1], i[2], i[3], i[4], i[5], c);
-            sm.x[s] = c.x; sm.y[s] = c.y; sm.h[s] = c.h; sm.vx[s] = c.vx; sm.vy[s] = c.vy; sm.w[s] = c.w;
-            sm.sh[s] = c.sh; sm.ch[s] = c.ch;
-            {
-                int64_t fl = 0;
-                unsigned pk = 0;
-#pragma unroll
-                for (int a = 0; a < 6; ++a) {
-                    fl += (int64_t)i[a] * K.stride[a];
-                    pk |= (unsigned)(i[a] - B.s0[a]) << (3 * a);
-                }
-                sm.vold[s] = vself[fl];
-                sm.vmir[s] = STORE == PG_STORE_F32 ? sm.vold[s] : vin(K, Q, i, fl);
-                sm.loc[s] = (uint16_t)pk;
-            }
+            state_coords(B, j, i);
             const bool blk = occ[((int64_t)i[0] * K.n[1] + i[1]) * K.n[2] + i[2]] != 0;
-            sm.blocked[s] = blk ? 1 : 0;
-            sm.best[s] = blk ? ((unsigned long long)__float_as_uint(K.cap) << 32)
+            // blocked cells keep (cap, coast) (PeregrineSolver.jl:1226-1228)
+            sm.best[j] = blk ? ((unsigned long long)__float_as_uint(K.cap) << 32)
                              : ((unsigned long long)0x7f800000u << 32);
         }
         for (int j = threadIdx.x; j < 3 * K.nctl; j += kThreads) sm.bu[j] = bu_g[j];
     }
     __syncthreads();
     if (live) {
-        const OccG og{occ, K.n[1], K.n[2]};
         const int ns = B.ns;
-        const int total = ns * K.nctl;
-        int s = threadIdx.x % ns, k = threadIdx.x / ns;
-        const int ds = kThreads % ns, dk = kThreads / ns;
-        for (int p = threadIdx.x; p < total; p += kThreads) {
-            if (!sm.blocked[s]) {
+        const int tps = kThreads / ns;
+        const int s = threadIdx.x % ns, k0 = threadIdx.x / ns;
+        int i[6];
+        state_coords(B, s, i);
+        const bool blk = occ[((int64_t)i[0] * K.n[1] + i[1]) * K.n[2] + i[2]] != 0;
+        if (k0 < tps && !blk) {
+            const OccG og{occ, K.n[1], K.n[2]};
+            const bool probe = pfree[B.gbox] == 0;
+            FastStart S;
+            {
                 CellStart c;
-                c.x = sm.x[s]; c.y = sm.y[s]; c.h = sm.h[s]; c.vx = sm.vx[s]; c.vy = sm.vy[s];
-                c.w = sm.w[s]; c.sh = sm.sh[s]; c.ch = sm.ch[s];
+                cell_start(K, i[0], i[1], i[2], i[3], i[4], i[5], c);
+                fast_start<DIAG>(K, F, c, S);
+            }
+            unsigned pk = 0;
+            float vmir = 0.0f;
+            if (SELF) {
+                int64_t fl = 0;
+#pragma unroll
+                for (int a = 0; a < 6; ++a) {
+                    fl += (int64_t)i[a] * K.stride[a];
+                    pk |= (unsigned)(i[a] - B.s0[a]) << (3 * a);
+                }
+                vmir = vin(K, Q, i, fl);
+            }
+            float bv = __uint_as_float(0x7f800000u);
+            int bk = 0;
+            for (int k = k0; k < K.nctl; k += tps) {
                 float f[6];
                 float val = K.cap;
-                if (successor_fast<DIAG>(K, F, og, c, sm.bu[3 * k], sm.bu[3 * k + 1], sm.bu[3 * k + 2], f)) {
+                if (successor_fast<DIAG>(K, F, og, S, sm.bu[3 * k], sm.bu[3 * k + 1], sm.bu[3 * k + 2], probe, f)) {
                     float t[6];
-                    int i[6];
+                    int si[6];
                     int l[6];
                     bool inwin = true;
 #pragma unroll
                     for (int a = 0; a < 6; ++a) {
                         const float b = __builtin_floorf(f[a]);
                         t[a] = f[a] - b;
-                        i[a] = (int)b;
-                        int d = i[a] - o[a];
+                        si[a] = (int)b;
+                        int d = si[a] - o[a];
                         if (a == 2) {
                             d = d < 0 ? d + K.n[2] : d;
                             d = d >= K.n[2] ? d - K.n[2] : d;
@@ -200,11 +203,10 @@ __global__ __launch_bounds__(256, 4) void sweep_fast_kernel(
                     // Weight of the state's own cell in the successor's stencil.
                     float wself = 0.0f;
                     if (SELF) {
-                        const unsigned pk = sm.loc[s];
                         wself = 1.0f;
 #pragma unroll
                         for (int a = 0; a < 6; ++a) {
-                            int d = B.s0[a] + (int)((pk >> (3 * a)) & 7u) - i[a];
+                            int d = B.s0[a] + (int)((pk >> (3 * a)) & 7u) - si[a];
                             if (a == 2) d = d < 0 ? d + K.n[2] : d;
                             wself *= d == 0 ? 1.0f - t[a] : (d == 1 ? t[a] : 0.0f);
                         }
@@ -230,7 +232,7 @@ __global__ __launch_bounds__(256, 4) void sweep_fast_kernel(
                                 int64_t fl = 0;
 #pragma unroll
                                 for (int a = 0; a < 6; ++a) {
-                                    int j = i[a] + ((d >> a) & 1);
+                                    int j = si[a] + ((d >> a) & 1);
                                     if (a == 2) {
                                         j = j >= K.n[2] ? j - K.n[2] : j;
                                         j = j >= K.n[2] ? j - K.n[2] : j;
@@ -249,7 +251,7 @@ __global__ __launch_bounds__(256, 4) void sweep_fast_kernel(
                         // policy system with its diagonal): same fixed point
                         // as tau + min(acc, cap), without the 1/(1-w)-times
                         // slower approach through the cell's own value.
-                        const float rest = acc - wself * sm.vmir[s];
+                        const float rest = acc - wself * vmir;
                         const float cc = (K.tau + rest) * __builtin_amdgcn_rcpf(1.0f - wself);
                         val = cc < K.cap ? cc : K.cap;
                     } else {
@@ -258,11 +260,12 @@ __global__ __launch_bounds__(256, 4) void sweep_fast_kernel(
                         val = cc < K.cap ? cc : K.cap;
                     }
                 }
-                const unsigned long long key = ((unsigned long long)__float_as_uint(val) << 32) | (unsigned)k;
-                atomicMin(&sm.best[s], key);
+                if (val < bv) {
+                    bv = val;
+                    bk = k;
+                }
             }
-            s += ds; k += dk;
-            if (s >= ns) { s -= ns; ++k; }
+            atomicMin(&sm.best[s], ((unsigned long long)__float_as_uint(bv) << 32) | (unsigned)bk);
         }
     }
     __syncthreads();
@@ -279,7 +282,7 @@ __global__ __launch_bounds__(256, 4) void sweep_fast_kernel(
             const unsigned long long key = sm.best[s];
             const float best = __uint_as_float((unsigned)(key >> 32));
             if (WRITE_V) {
-                const float old = sm.vold[s];
+                const float old = vself[idx];
                 float out = old;
                 if (best < old) {
                     out = best;
@@ -479,7 +482,7 @@ template <class W, bool DIAG, int STORE>
 hipError_t launch_fast_w(const pgk::Consts& K, const pgk::FastConsts& F, const pgk::BoxGeom& G,
                          const pgk::QuantGeom& Q, const void* Vin, const float2* tp, const float* vself, float* Vout,
                          uint16_t* codes_out,
-                         const uint8_t* occ, const float* bu, const int16_t* worig, uint8_t* pol,
+                         const uint8_t* occ, const float* bu, const int16_t* worig, const uint8_t* pfree, uint8_t* pol,
                          double* part, unsigned* linf, unsigned* tstat, int write_v, const pgk::ActiveG& A, hipStream_t st) {
     pgk::fastk::VRead<STORE> rd;
     if constexpr (STORE == PG_STORE_F32) {
@@ -491,13 +494,13 @@ hipError_t launch_fast_w(const pgk::Consts& K, const pgk::FastConsts& F, const p
     return pgk::for_box_chunks(G, [&](const pgk::BoxGeom& g, unsigned nb) {
         if (write_v && pol)
             hipLaunchKernelGGL((pgk::sweep_fast_kernel<W, DIAG, STORE, 128, true, true>), dim3(nb), dim3(256), 0, st,
-                               rd, vself, Vout, codes_out, occ, bu, K, F, g, Q, worig, pol, part, linf, tstat, A);
+                               rd, vself, Vout, codes_out, occ, bu, K, F, g, Q, worig, pfree, pol, part, linf, tstat, A);
         else if (write_v)
             hipLaunchKernelGGL((pgk::sweep_fast_kernel<W, DIAG, STORE, 128, true, false>), dim3(nb), dim3(256), 0, st,
-                               rd, vself, Vout, codes_out, occ, bu, K, F, g, Q, worig, pol, part, linf, tstat, A);
+                               rd, vself, Vout, codes_out, occ, bu, K, F, g, Q, worig, pfree, pol, part, linf, tstat, A);
         else
             hipLaunchKernelGGL((pgk::sweep_fast_kernel<W, DIAG, STORE, 128, false, true>), dim3(nb), dim3(256), 0, st,
-                               rd, vself, Vout, codes_out, occ, bu, K, F, g, Q, worig, pol, part, linf, tstat, A);
+                               rd, vself, Vout, codes_out, occ, bu, K, F, g, Q, worig, pfree, pol, part, linf, tstat, A);
     });
 }
 
@@ -506,15 +509,15 @@ hipError_t launch_fast_shape(int shape, const pgk::Consts& K, const pgk::FastCon
                              const pgk::BoxGeom& G, const pgk::QuantGeom& Q, const void* Vin,
                              const float2* tp, const float* vself, float* Vout, uint16_t* codes_out,
                              const uint8_t* occ, const float* bu,
-                             const int16_t* worig, uint8_t* pol, double* part, unsigned* linf,
+                             const int16_t* worig, const uint8_t* pfree, uint8_t* pol, double* part, unsigned* linf,
                              unsigned* tstat, int write_v, const pgk::ActiveG& A, hipStream_t st) {
     using W0 = pgk::Win0;
     using W1 = pgk::Win1;
     using W2 = pgk::Win2;
     switch (shape) {
-        case 0: return launch_fast_w<W0, DIAG, STORE>(K, F, G, Q, Vin, tp, vself, Vout, codes_out, occ, bu, worig, pol, part, linf, tstat, write_v, A, st);
-        case 1: return launch_fast_w<W1, DIAG, STORE>(K, F, G, Q, Vin, tp, vself, Vout, codes_out, occ, bu, worig, pol, part, linf, tstat, write_v, A, st);
-        case 2: return launch_fast_w<W2, DIAG, STORE>(K, F, G, Q, Vin, tp, vself, Vout, codes_out, occ, bu, worig, pol, part, linf, tstat, write_v, A, st);
+        case 0: return launch_fast_w<W0, DIAG, STORE>(K, F, G, Q, Vin, tp, vself, Vout, codes_out, occ, bu, worig, pfree, pol, part, linf, tstat, write_v, A, st);
+        case 1: return launch_fast_w<W1, DIAG, STORE>(K, F, G, Q, Vin, tp, vself, Vout, codes_out, occ, bu, worig, pfree, pol, part, linf, tstat, write_v, A, st);
+        case 2: return launch_fast_w<W2, DIAG, STORE>(K, F, G, Q, Vin, tp, vself, Vout, codes_out, occ, bu, worig, pfree, pol, part, linf, tstat, write_v, A, st);
         default: return hipErrorInvalidValue;
     }
 }
@@ -525,15 +528,15 @@ hipError_t pgk_launch_sweep_fast(int shape, int diag, int store, const pgk::Cons
                                  const pgk::FastConsts& F, const pgk::BoxGeom& G,
                                  const pgk::QuantGeom& Q, const void* Vin, const float2* tp,
                                  const float* vself, float* Vout, uint16_t* codes_out, const uint8_t* occ,
-                                 const float* bu, const int16_t* worig, uint8_t* pol, double* part, unsigned* linf,
+                                 const float* bu, const int16_t* worig, const uint8_t* pfree, uint8_t* pol, double* part, unsigned* linf,
                                  unsigned* tstat, int write_v, const pgk::ActiveG& A, hipStream_t st) {
     switch (store * 2 + (diag ? 1 : 0)) {
-        case PG_STORE_F32 * 2 + 0: return launch_fast_shape<false, PG_STORE_F32>(shape, K, F, G, Q, Vin, tp, vself, Vout, codes_out, occ, bu, worig, pol, part, linf, tstat, write_v, A, st);
-        case PG_STORE_F32 * 2 + 1: return launch_fast_shape<true, PG_STORE_F32>(shape, K, F, G, Q, Vin, tp, vself, Vout, codes_out, occ, bu, worig, pol, part, linf, tstat, write_v, A, st);
-        case PG_STORE_F16 * 2 + 0: return launch_fast_shape<false, PG_STORE_F16>(shape, K, F, G, Q, Vin, tp, vself, Vout, codes_out, occ, bu, worig, pol, part, linf, tstat, write_v, A, st);
-        case PG_STORE_F16 * 2 + 1: return launch_fast_shape<true, PG_STORE_F16>(shape, K, F, G, Q, Vin, tp, vself, Vout, codes_out, occ, bu, worig, pol, part, linf, tstat, write_v, A, st);
-        case PG_STORE_U16 * 2 + 0: return launch_fast_shape<false, PG_STORE_U16>(shape, K, F, G, Q, Vin, tp, vself, Vout, codes_out, occ, bu, worig, pol, part, linf, tstat, write_v, A, st);
-        case PG_STORE_U16 * 2 + 1: return launch_fast_shape<true, PG_STORE_U16>(shape, K, F, G, Q, Vin, tp, vself, Vout, codes_out, occ, bu, worig, pol, part, linf, tstat, write_v, A, st);
+        case PG_STORE_F32 * 2 + 0: return launch_fast_shape<false, PG_STORE_F32>(shape, K, F, G, Q, Vin, tp, vself, Vout, codes_out, occ, bu, worig, pfree, pol, part, linf, tstat, write_v, A, st);
+        case PG_STORE_F32 * 2 + 1: return launch_fast_shape<true, PG_STORE_F32>(shape, K, F, G, Q, Vin, tp, vself, Vout, codes_out, occ, bu, worig, pfree, pol, part, linf, tstat, write_v, A, st);
+        case PG_STORE_F16 * 2 + 0: return launch_fast_shape<false, PG_STORE_F16>(shape, K, F, G, Q, Vin, tp, vself, Vout, codes_out, occ, bu, worig, pfree, pol, part, linf, tstat, write_v, A, st);
+        case PG_STORE_F16 * 2 + 1: return launch_fast_shape<true, PG_STORE_F16>(shape, K, F, G, Q, Vin, tp, vself, Vout, codes_out, occ, bu, worig, pfree, pol, part, linf, tstat, write_v, A, st);
+        case PG_STORE_U16 * 2 + 0: return launch_fast_shape<false, PG_STORE_U16>(shape, K, F, G, Q, Vin, tp, vself, Vout, codes_out, occ, bu, worig, pfree, pol, part, linf, tstat, write_v, A, st);
+        case PG_STORE_U16 * 2 + 1: return launch_fast_shape<true, PG_STORE_U16>(shape, K, F, G, Q, Vin, tp, vself, Vout, codes_out, occ, bu, worig, pfree, pol, part, linf, tstat, write_v, A, st);
         default: return hipErrorInvalidValue;
     }
 }

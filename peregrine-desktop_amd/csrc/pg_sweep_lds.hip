@@ -78,13 +78,16 @@ __device__ __forceinline__ int successor(const Consts& K, const OccG& og, const 
 // every (state, control) pair whose successor is interpolated. Heading is
 // measured relative to the box's first heading, in (-n3/2, n3/2]. FAST
 // plans with the precision form's successor (its read ranges; the window
-// origins always come from the exact successor).
+// origins always come from the exact successor) and first flags, in pfree,
+// the boxes whose swept probes can only touch free cells: the (x, y,
+// heading) cells between the box's states and any finite successor hold no
+// obstacle. Such a box's sweeps, and its read ranges here, skip the probes.
 // ---------------------------------------------------------------------------
 template <bool DIAG, bool FAST, int MAXS>
 __global__ __launch_bounds__(256) void plan_kernel(const uint8_t* __restrict__ occ,
                                                    const float* __restrict__ bu_g, Consts K, FastConsts F,
                                                    BoxGeom G, int16_t* __restrict__ worig,
-                                                   uint8_t* __restrict__ wext) {
+                                                   uint8_t* __restrict__ wext, uint8_t* __restrict__ pfree) {
     const int64_t box = box_of_block(G);
     if (box >= G.nbox) return;
     __shared__ BoxShared<MAXS> sm;
@@ -99,6 +102,61 @@ __global__ __launch_bounds__(256) void plan_kernel(const uint8_t* __restrict__ o
     const int total = ns * K.nctl;
     int s = threadIdx.x % ns, k = threadIdx.x / ns;
     const int ds = kThreads % ns, dk = kThreads / ns;
+    bool probe = true;
+    if constexpr (FAST) {
+        // cells (x, y, heading relative to the box) the probes can reach
+        int rmn[3] = {B.s0[0], B.s0[1], 0}, rmx[3] = {B.s0[0] + B.vb[0] - 1, B.s0[1] + B.vb[1] - 1, B.vb[2] - 1};
+        for (int p = threadIdx.x; p < total; p += kThreads) {
+            if (!sm.blocked[s]) {
+                CellStart c;
+                load_cell_start(sm, s, c);
+                FastStart S;
+                fast_start<DIAG>(K, F, c, S);
+                float g[3];
+                if (successor_reach_fast<DIAG>(K, F, S, sm.bu[3 * k], sm.bu[3 * k + 1], sm.bu[3 * k + 2], g)) {
+                    // heading unwrapped, relative to the box: the swept
+                    // headings lie between the start and this
+                    const float lo[3] = {g[0], g[1], g[2] - (float)B.s0[2]};
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        const float fl = __builtin_floorf(lo[a]), ce = __builtin_ceilf(lo[a]);
+                        // far outside the grid: the bounds tests reject it
+                        const int il = fl < -1.0e6f ? -1000000 : (int)fl;
+                        const int ic = ce > 1.0e6f ? 1000000 : (int)ce;
+                        rmn[a] = il < rmn[a] ? il : rmn[a];
+                        rmx[a] = ic > rmx[a] ? ic : rmx[a];
+                    }
+                }
+            }
+            s += ds; k += dk;
+            if (s >= ns) { s -= ns; ++k; }
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            atomicMin(&mn[a], rmn[a]);
+            atomicMax(&mx[a], rmx[a]);
+        }
+        __syncthreads();
+        int x0 = mn[0] < 0 ? 0 : mn[0], x1 = mx[0] > K.n[0] - 1 ? K.n[0] - 1 : mx[0];
+        int y0 = mn[1] < 0 ? 0 : mn[1], y1 = mx[1] > K.n[1] - 1 ? K.n[1] - 1 : mx[1];
+        int h0 = mn[2], h1 = mx[2];
+        if (h1 - h0 + 1 >= K.n[2]) { h0 = 0; h1 = K.n[2] - 1; }
+        const int nx = x1 >= x0 ? x1 - x0 + 1 : 0, ny = y1 >= y0 ? y1 - y0 + 1 : 0, nh = h1 - h0 + 1;
+        int hit = 0;
+        for (int j = threadIdx.x; j < nx * ny * nh; j += kThreads) {
+            const int ih = j % nh, iy = (j / nh) % ny, ix = j / (nh * ny);
+            int ch = B.s0[2] + h0 + ih;
+            ch %= K.n[2];
+            if (ch < 0) ch += K.n[2];
+            hit |= og(x0 + ix, y0 + iy, ch) ? 1 : 0;
+        }
+        probe = __syncthreads_or(hit) != 0;
+        if (threadIdx.x < 6) { mn[threadIdx.x] = 1 << 30; mx[threadIdx.x] = -(1 << 30); }
+        if (threadIdx.x == 0) pfree[B.gbox] = probe ? 0 : 1;
+        __syncthreads();
+        s = threadIdx.x % ns;
+        k = threadIdx.x / ns;
+    }
     int lmn[6], lmx[6];
 #pragma unroll
     for (int a = 0; a < 6; ++a) { lmn[a] = 1 << 30; lmx[a] = -(1 << 30); }
@@ -108,10 +166,14 @@ __global__ __launch_bounds__(256) void plan_kernel(const uint8_t* __restrict__ o
             load_cell_start(sm, s, c);
             float f[6];
             bool interp;
-            if constexpr (FAST)
-                interp = successor_fast<DIAG>(K, F, og, c, sm.bu[3 * k], sm.bu[3 * k + 1], sm.bu[3 * k + 2], f);
-            else
+            if constexpr (FAST) {
+                FastStart S;
+                fast_start<DIAG>(K, F, c, S);
+                interp = successor_fast<DIAG>(K, F, og, S, sm.bu[3 * k], sm.bu[3 * k + 1], sm.bu[3 * k + 2], probe,
+                                              f);
+            } else {
                 interp = successor<DIAG>(K, og, c, sm.bu[3 * k], sm.bu[3 * k + 1], sm.bu[3 * k + 2], f) != 0;
+            }
             if (interp) {
 #pragma unroll
                 for (int a = 0; a < 6; ++a) {
@@ -439,16 +501,16 @@ int pgk_window_shapes(const int (**shapes)[6]) {
 }
 
 hipError_t pgk_launch_plan(const pgk::Consts& K, const pgk::FastConsts* F, const pgk::BoxGeom& G,
-                           const uint8_t* occ, const float* bu, int16_t* worig, uint8_t* wext, int diag,
-                           hipStream_t st) {
+                           const uint8_t* occ, const float* bu, int16_t* worig, uint8_t* wext, uint8_t* pfree,
+                           int diag, hipStream_t st) {
     const pgk::FastConsts f = F ? *F : pgk::FastConsts{};
     return pgk::for_box_chunks(G, [&](const pgk::BoxGeom& g, unsigned nb) {
         const dim3 grid(nb), blk(256);
         switch ((F ? 2 : 0) + (diag ? 1 : 0)) {
-            case 0: hipLaunchKernelGGL((pgk::plan_kernel<false, false, 128>), grid, blk, 0, st, occ, bu, K, f, g, worig, wext); break;
-            case 1: hipLaunchKernelGGL((pgk::plan_kernel<true, false, 128>), grid, blk, 0, st, occ, bu, K, f, g, worig, wext); break;
-            case 2: hipLaunchKernelGGL((pgk::plan_kernel<false, true, 128>), grid, blk, 0, st, occ, bu, K, f, g, worig, wext); break;
-            default: hipLaunchKernelGGL((pgk::plan_kernel<true, true, 128>), grid, blk, 0, st, occ, bu, K, f, g, worig, wext); break;
+            case 0: hipLaunchKernelGGL((pgk::plan_kernel<false, false, 128>), grid, blk, 0, st, occ, bu, K, f, g, worig, wext, pfree); break;
+            case 1: hipLaunchKernelGGL((pgk::plan_kernel<true, false, 128>), grid, blk, 0, st, occ, bu, K, f, g, worig, wext, pfree); break;
+            case 2: hipLaunchKernelGGL((pgk::plan_kernel<false, true, 128>), grid, blk, 0, st, occ, bu, K, f, g, worig, wext, pfree); break;
+            default: hipLaunchKernelGGL((pgk::plan_kernel<true, true, 128>), grid, blk, 0, st, occ, bu, K, f, g, worig, wext, pfree); break;
         }
     });
 }
