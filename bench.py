@@ -185,7 +185,11 @@ def accuracy(pg, prob, args, tables):
         e = np.abs(a - b)[both]
         q = np.quantile(e, [0.5, 0.99, 0.999])
         live = both & ~blocked
+        sd = (a - b)[both]
+        jm = int(np.argmax(e))
         out[tag] = {"max": float(e.max()), "p50": float(q[0]), "p99": float(q[1]), "p999": float(q[2]),
+                    "max_cell": int(idx[both][jm]), "max_signed": float(sd[jm]),
+                    "above_1e-3": int(np.count_nonzero(e > 1e-3)), "above_5e-3": int(np.count_nonzero(e > 5e-3)),
                     "mean_signed": float(np.mean((a - b)[both])), "n": int(both.sum()),
                     "reach_mismatch": float(np.mean((a < cap) != (b < cap))),
                     "policy_agreement": float(np.mean(P[idx][live] == P64[live]))}
