@@ -78,13 +78,15 @@ typedef struct pg_params {
     int32_t ncoarse;
     int32_t rounds;
     float   delta0;
-    int32_t rk2;
+    int32_t rk2;      /* explicit midpoint substeps (step_state_rk2); exact FP32
+                         kernels only: the 16-bit stores, the FP64 tables and the
+                         adjoint integrate with forward Euler and refuse it */
     int32_t vclamp;
     int32_t simplex;
 } pg_params;
 
 /* How V is held on the device. The 16-bit stores hold one code per cell and
- * an FP32 (offset, scale) per 8^6-cell tile, value = offset + scale * code;
+ * an FP32 (offset, scale) per 4^6-cell tile, value = offset + scale * code;
  * unreached cells (value cap) hold an out-of-band code. */
 #define PG_VSTORE_F32       0   /* plain FP32, Jacobi double buffer             */
 #define PG_VSTORE_F16_TILE  1   /* IEEE binary16 codes, per-tile offset/scale    */

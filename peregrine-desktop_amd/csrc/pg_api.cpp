@@ -137,7 +137,6 @@ int check_params(const pg_params* p, int32_t nctl) {
     if (p->rounds > 0) return fail(PG_ERR_UNSUPPORTED, "refine_rounds > 0 (pattern search) is not built");
     if (p->ncoarse > 0 && p->ncoarse < nctl - 1)
         return fail(PG_ERR_UNSUPPORTED, "control_scan < nctl-1 (rotating slice) is not built");
-    if (p->rk2) return fail(PG_ERR_UNSUPPORTED, "rk2 integration is not built");
     if (p->simplex) return fail(PG_ERR_UNSUPPORTED, "simplex (Kuhn) interpolation is not built");
     if (p->nearest) return fail(PG_ERR_UNSUPPORTED, "nearest-cell lookup is not built");
     if (p->vclamp) return fail(PG_ERR_UNSUPPORTED, "velocity_clamp is not built");
@@ -185,6 +184,7 @@ void build_consts(pg_solver* s) {
     K.nchk = s->p.checks;
     for (int sc = 1; sc <= K.nchk; ++sc) K.probe_a[sc - 1] = (float)sc / (float)K.nchk;
     K.nctl = s->nctl;
+    K.rk2 = s->p.rk2 ? 1 : 0;
     int64_t nt = 1;
     for (int k = 0; k < 6; ++k) {
         s->Q.nt[k] = (g.n[k] + (1 << pgk::kQuantShift) - 1) >> pgk::kQuantShift;
@@ -465,6 +465,8 @@ int create_solver(pg_ctx* ctx, const pg_grid* g, const pg_model* m, const pg_par
     if (rc) return rc;
     if (vstore != PG_VSTORE_F32 && vstore != PG_VSTORE_F16_TILE && vstore != PG_VSTORE_U16_TILE)
         return fail(PG_ERR_ARG, "unknown vstore %d", vstore);
+    if (vstore != PG_VSTORE_F32 && p->rk2)
+        return fail(PG_ERR_UNSUPPORTED, "the 16-bit stores (precision form) integrate with forward Euler only");
     if (vstore != PG_VSTORE_F32 && nctl > kLdsMaxCtl)
         return fail(PG_ERR_UNSUPPORTED, "the 16-bit stores sweep at most %d controls", kLdsMaxCtl);
     for (int k = 0; k < 3; ++k)
@@ -532,7 +534,7 @@ int create_solver(pg_ctx* ctx, const pg_grid* g, const pg_model* m, const pg_par
     }
     chk(hipMalloc(&s->d_occ, (size_t)nocc));
     chk(hipMalloc(&s->d_bu, bu.size() * sizeof(float)));
-    chk(hipMalloc(&s->d_sum, sizeof(double)));
+    chk(hipMalloc(&s->d_sum, (1 + kReduceBlocks) * sizeof(double)));   // the sum, then reduction scratch
     chk(hipMalloc(&s->d_linf, sizeof(unsigned)));
     if (e != hipSuccess) {
         release(s);
@@ -701,7 +703,7 @@ int pg_sweep(pg_solver* s, int32_t phase, int32_t rev, double* sum_decrease, flo
                                   s->d_linf, 1, st, &nblk));
     }
     HIPCHECK(hipEventRecord(s->ev1, st));
-    HIPCHECK(pgk_launch_reduce(s->d_part, nblk, s->d_sum, st));
+    HIPCHECK(pgk_launch_reduce(s->d_part, nblk, s->d_sum, s->d_sum + 1, st));
     if (mark && !s->f16_active) HIPCHECK(pgk_launch_seed_mark(Vout, s->d_seeds, s->nseeds, s->K, s->G, mark, st));
     else HIPCHECK(pgk_launch_seed(Vout, s->d_seeds, s->nseeds, st));
     if (s->f16_active) {
@@ -759,7 +761,6 @@ int pg_polish(pg_solver* s, int32_t nsweeps, double* last_sum, float* last_linf)
     if (!s->seeded) return fail(PG_ERR_STATE, "no value table yet");
     if (nsweeps < 0) return fail(PG_ERR_ARG, "nsweeps must be >= 0");
     HIPCHECK(hipSetDevice(s->ctx->device));
-    hipStream_t st = s->ctx->stream;
     if (s->f16_active) {
         if (!s->dV[1]) {
             hipError_t e = hipMalloc(&s->dV[1], (size_t)s->ncell * sizeof(float));
@@ -884,6 +885,7 @@ namespace {
 
 int f64_setup(pg_solver* s) {
     if (s->ooc) return fail(PG_ERR_UNSUPPORTED, "the FP64 tables are not built for an out-of-core solver");
+    if (s->p.rk2) return fail(PG_ERR_UNSUPPORTED, "the FP64 tables and the adjoint integrate with forward Euler only");
     if (s->nctl > kLdsMaxCtl)
         return fail(PG_ERR_UNSUPPORTED, "the FP64 sweeps hold at most %d controls", kLdsMaxCtl);
     int rc = plan(s);
@@ -915,7 +917,7 @@ int f64_pass(pg_solver* s, int mode, const double* Vin, double* Vout, uint8_t* p
     HIPCHECK(hipMemsetAsync(s->d_linf, 0, sizeof(unsigned), st));
     HIPCHECK(pgk_launch_sweep_f64(mode, s->shape, s->K64, s->G, Vin, Vout, s->d_occ, s->d_bu64, s->d_worig,
                                   s->d_seedmask, pol, s->d_part, s->d_linf, st));
-    HIPCHECK(pgk_launch_reduce(s->d_part, lds_blocks(s), s->d_sum, st));
+    HIPCHECK(pgk_launch_reduce(s->d_part, lds_blocks(s), s->d_sum, s->d_sum + 1, st));
     unsigned hl = 0;
     double hs = 0.0;
     HIPCHECK(hipMemcpyAsync(&hs, s->d_sum, sizeof(double), hipMemcpyDeviceToHost, st));
@@ -1136,6 +1138,8 @@ int pg_solver_set_kernel(pg_solver* s, int32_t kernel) {
         return fail(PG_ERR_ARG, "unknown kernel %d", kernel);
     if (kernel != PG_KERNEL_GLOBAL && s->nctl > kLdsMaxCtl)
         return fail(PG_ERR_UNSUPPORTED, "the LDS-staged sweeps hold at most %d controls", kLdsMaxCtl);
+    if (kernel == PG_KERNEL_FAST && s->p.rk2)
+        return fail(PG_ERR_UNSUPPORTED, "the precision-form kernel integrates with forward Euler only");
     if (s->f16_active && kernel != PG_KERNEL_FAST)
         return fail(PG_ERR_STATE, "a 16-bit store sweeps with the fast kernel until pg_polish");
     if (s->ooc && kernel != PG_KERNEL_LDS)

@@ -36,6 +36,7 @@ struct Consts {
     int32_t nsub;
     int32_t nchk;
     int32_t nctl;
+    int32_t rk2;       // explicit midpoint substeps instead of forward Euler
     float probe_a[24]; // Float32(sc) / Float32(nchk), IEEE
 };
 
@@ -299,6 +300,103 @@ __device__ __forceinline__ bool euler_from(const Consts& K, const CellStart& c, 
         vy = vy + dvy * hs;
         w = w + al * hs;
     }
+    return ok;
+}
+
+// Left-fold sum x + y + h + vx + vy + w of a state is finite (diverged,
+// PeregrineSolver.jl:191-192, negated).
+__device__ __forceinline__ bool state_finite(const float s[6]) {
+    float sum = s[0] + s[1];
+    sum = sum + s[2];
+    sum = sum + s[3];
+    sum = sum + s[4];
+    sum = sum + s[5];
+    return finite_f(sum);
+}
+
+// state_deriv (PeregrineSolver.jl:226-236) of state s under B*u_sat.
+template <bool DIAG>
+__device__ __forceinline__ void state_deriv(const Consts& K, float bux, float buy, float buw, const float s[6],
+                                            float d[6]) {
+    const float vx = s[3], vy = s[4], w = s[5];
+    const float w2 = DIAG ? 0.0f : w * w;
+    const float sx = clamp1(qdiv(vx, K.reps[0]));
+    const float sy = clamp1(qdiv(vy, K.reps[1]));
+    const float sw = clamp1(qdiv(w, K.reps[2]));
+    const float dx = __builtin_fabsf(vx) * vx;
+    const float dy = __builtin_fabsf(vy) * vy;
+    const float dw = __builtin_fabsf(w) * w;
+    float ax, ay, al;
+    accel_rows<DIAG>(K, bux, buy, buw, vx, vy, w, w2, sx, sy, sw, dx, dy, dw, ax, ay, al);
+    float sh, ch;
+    sincos_ref(s[2], sh, ch);
+    d[0] = ch * vx - sh * vy;
+    d[1] = sh * vx + ch * vy;
+    d[2] = w;
+    d[3] = ax + w * vy;
+    d[4] = ay - w * vx;
+    d[5] = al;
+}
+
+// Explicit midpoint from a cell start (step_state_rk2, PeregrineSolver.jl:259-279):
+// per substep d1 = f(s), mid = s + d1*hs/2, d2 = f(mid), s = s + d2*hs, with
+// the divergence test on s and on mid. The first substep's f(s) is the
+// cell's own terms (a grid state is finite).
+template <bool DIAG>
+__device__ __forceinline__ bool rk2_from(const Consts& K, const CellStart& c, float bux, float buy, float buw,
+                                         float& x, float& y, float& h, float& vx, float& vy, float& w) {
+    const float hs = K.hs;
+    const float hh = 0.5f * hs;
+    float s[6] = {c.x, c.y, c.h, c.vx, c.vy, c.w};
+    bool ok = true;
+    for (int k = 0; k < K.nsub; ++k) {
+        float d[6];
+        if (k == 0) {
+            float ax, ay, al;
+            accel_rows<DIAG>(K, bux, buy, buw, c.vx, c.vy, c.w, c.w2, c.sx, c.sy, c.sw, c.dx, c.dy, c.dw, ax, ay,
+                             al);
+            d[0] = c.ch * c.vx - c.sh * c.vy;
+            d[1] = c.sh * c.vx + c.ch * c.vy;
+            d[2] = c.w;
+            d[3] = ax + c.w * c.vy;
+            d[4] = ay - c.w * c.vx;
+            d[5] = al;
+        } else {
+            ok = ok && state_finite(s);
+            state_deriv<DIAG>(K, bux, buy, buw, s, d);
+        }
+        float mid[6];
+#pragma unroll
+        for (int j = 0; j < 6; ++j) mid[j] = s[j] + d[j] * hh;
+        ok = ok && state_finite(mid);
+        state_deriv<DIAG>(K, bux, buy, buw, mid, d);
+#pragma unroll
+        for (int j = 0; j < 6; ++j) s[j] = s[j] + d[j] * hs;
+    }
+    x = s[0]; y = s[1]; h = s[2]; vx = s[3]; vy = s[4]; w = s[5];
+    return ok;
+}
+
+// The same from an arbitrary start state (the global-gather kernel).
+__device__ __forceinline__ bool rk2_state(const Consts& K, float bux, float buy, float buw, float& x, float& y,
+                                          float& h, float& vx, float& vy, float& w) {
+    const float hs = K.hs;
+    const float hh = 0.5f * hs;
+    float s[6] = {x, y, h, vx, vy, w};
+    bool ok = true;
+    for (int k = 0; k < K.nsub; ++k) {
+        float d[6];
+        ok = ok && state_finite(s);
+        state_deriv<false>(K, bux, buy, buw, s, d);
+        float mid[6];
+#pragma unroll
+        for (int j = 0; j < 6; ++j) mid[j] = s[j] + d[j] * hh;
+        ok = ok && state_finite(mid);
+        state_deriv<false>(K, bux, buy, buw, mid, d);
+#pragma unroll
+        for (int j = 0; j < 6; ++j) s[j] = s[j] + d[j] * hs;
+    }
+    x = s[0]; y = s[1]; h = s[2]; vx = s[3]; vy = s[4]; w = s[5];
     return ok;
 }
 

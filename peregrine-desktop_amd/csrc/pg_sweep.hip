@@ -102,7 +102,8 @@ __device__ __forceinline__ float lookahead_global(const Consts& K, const float* 
                                                   float h, float vx, float vy, float w,
                                                   float bux, float buy, float buw) {
     float nx = x, ny = y, nh = h, nvx = vx, nvy = vy, nw = w;
-    const bool ok = euler(K, bux, buy, buw, nx, ny, nh, nvx, nvy, nw);
+    const bool ok = K.rk2 ? rk2_state(K, bux, buy, buw, nx, ny, nh, nvx, nvy, nw)
+                          : euler(K, bux, buy, buw, nx, ny, nh, nvx, nvy, nw);
     const float dx = nx - x, dy = ny - y, dh = nh - h;
     if (!(ok & finite_f(dx) & finite_f(dy) & finite_f(dh) & finite_f(nvx) & finite_f(nvy) &
           finite_f(nw)))
@@ -166,19 +167,23 @@ __global__ __launch_bounds__(256) void sweep_global_kernel(
     if (WRITE_V) block_reduce_sum_max((double)dec, dec, part_sum, linf_bits);
 }
 
-// Deterministic sum of the per-block partials (fixed order per launch shape).
+// Deterministic sum of the per-block partials (a fixed order for each n):
+// block b sums the b-th of gridDim.x contiguous slices into out[b].
 __global__ __launch_bounds__(256) void reduce_partials_kernel(const double* __restrict__ part,
                                                               int64_t n, double* __restrict__ out) {
     __shared__ double s[256];
+    const int64_t per = (n + gridDim.x - 1) / gridDim.x;
+    const int64_t i0 = (int64_t)blockIdx.x * per;
+    const int64_t i1 = i0 + per < n ? i0 + per : n;
     double acc = 0.0;
-    for (int64_t i = threadIdx.x; i < n; i += 256) acc += part[i];
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) acc += part[i];
     s[threadIdx.x] = acc;
     __syncthreads();
     for (int off = 128; off >= 1; off >>= 1) {
         if ((int)threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[0] = s[0];
+    if (threadIdx.x == 0) out[blockIdx.x] = s[0];
 }
 
 __global__ void fill_kernel(float* __restrict__ V, int64_t n, float v) {
@@ -234,8 +239,13 @@ hipError_t pgk_launch_sweep(const pgk::Consts& K, const float* Vin, float* Vout,
     return hipGetLastError();
 }
 
-hipError_t pgk_launch_reduce(const double* part, int64_t n, double* out, hipStream_t st) {
-    hipLaunchKernelGGL(pgk::reduce_partials_kernel, dim3(1), dim3(256), 0, st, part, n, out);
+hipError_t pgk_launch_reduce(const double* part, int64_t n, double* out, double* mid, hipStream_t st) {
+    if (mid && n > 64 * 256) {
+        hipLaunchKernelGGL(pgk::reduce_partials_kernel, dim3(kReduceBlocks), dim3(256), 0, st, part, n, mid);
+        hipLaunchKernelGGL(pgk::reduce_partials_kernel, dim3(1), dim3(256), 0, st, mid, (int64_t)kReduceBlocks, out);
+    } else {
+        hipLaunchKernelGGL(pgk::reduce_partials_kernel, dim3(1), dim3(256), 0, st, part, n, out);
+    }
     return hipGetLastError();
 }
 

@@ -54,13 +54,15 @@ __device__ __forceinline__ void setup_box(const Consts& K, const BoxInfo& B,
     for (int j = threadIdx.x; j < 3 * K.nctl; j += kThreads) sm.bu[j] = bu_g[j];
 }
 
-// One lookahead up to the successor's fractional coordinates. Returns 0 if
+// One lookahead up to the successor's fractional coordinates, integrating
+// with explicit midpoint substeps (RK2) or forward Euler. Returns 0 if
 // rejected (value cap), 1 if the successor must be interpolated.
-template <bool DIAG>
-__device__ __forceinline__ int successor(const Consts& K, const OccG& og, const CellStart& c,
-                                         float bux, float buy, float buw, float f[6]) {
+template <bool DIAG, bool RK2>
+__device__ __forceinline__ int successor_t(const Consts& K, const OccG& og, const CellStart& c,
+                                           float bux, float buy, float buw, float f[6]) {
     float nx, ny, nh, nvx, nvy, nw;
-    const bool ok = euler_from<DIAG>(K, c, bux, buy, buw, nx, ny, nh, nvx, nvy, nw);
+    const bool ok = RK2 ? rk2_from<DIAG>(K, c, bux, buy, buw, nx, ny, nh, nvx, nvy, nw)
+                        : euler_from<DIAG>(K, c, bux, buy, buw, nx, ny, nh, nvx, nvy, nw);
     const float dx = nx - c.x, dy = ny - c.y, dh = nh - c.h;
     if (!(ok & finite_f(dx) & finite_f(dy) & finite_f(dh) & finite_f(nvx) & finite_f(nvy) &
           finite_f(nw)))
@@ -71,6 +73,14 @@ __device__ __forceinline__ int successor(const Consts& K, const OccG& og, const 
     const float nvfx = cnh * nvx - snh * nvy;
     const float nvfy = snh * nvx + cnh * nvy;
     return interp_coords(K, nx, ny, nh, nvfx, nvfy, nw, f) ? 1 : 0;
+}
+
+// The same with the integrator chosen by K.rk2 at run time (the plan pass).
+template <bool DIAG>
+__device__ __forceinline__ int successor(const Consts& K, const OccG& og, const CellStart& c,
+                                         float bux, float buy, float buw, float f[6]) {
+    return K.rk2 ? successor_t<DIAG, true>(K, og, c, bux, buy, buw, f)
+                 : successor_t<DIAG, false>(K, og, c, bux, buy, buw, f);
 }
 
 // ---------------------------------------------------------------------------
@@ -320,7 +330,7 @@ __device__ __forceinline__ void fetch_src(const Consts& K, const SRC& src, const
 // ---------------------------------------------------------------------------
 // Sweep
 // ---------------------------------------------------------------------------
-template <class W, bool DIAG, int MAXS, bool WRITE_V, bool WRITE_POL, class SRC, class DST>
+template <class W, bool DIAG, bool RK2, int MAXS, bool WRITE_V, bool WRITE_POL, class SRC, class DST>
 __global__ __launch_bounds__(256, 4) void sweep_lds_kernel(
     SRC src, DST dst, const uint8_t* __restrict__ occ,
     const float* __restrict__ bu_g, Consts K, BoxGeom G, const int16_t* __restrict__ worig,
@@ -362,7 +372,7 @@ __global__ __launch_bounds__(256, 4) void sweep_lds_kernel(
                 load_cell_start(sm, s, c);
                 float f[6];
                 float val = K.cap;
-                if (successor<DIAG>(K, og, c, sm.bu[3 * k], sm.bu[3 * k + 1], sm.bu[3 * k + 2], f)) {
+                if (successor_t<DIAG, RK2>(K, og, c, sm.bu[3 * k], sm.bu[3 * k + 1], sm.bu[3 * k + 2], f)) {
                     float t[6], u[6];
                     int i[6];
 #pragma unroll
@@ -453,19 +463,19 @@ __global__ __launch_bounds__(256, 4) void sweep_lds_kernel(
 
 namespace {
 
-template <class W, bool DIAG, int MAXS, class SRC, class DST>
+template <class W, bool DIAG, bool RK2, int MAXS, class SRC, class DST>
 hipError_t launch_w(const pgk::Consts& K, const pgk::BoxGeom& G, const SRC& src, const DST& dst,
                     const uint8_t* occ, const float* bu, const int16_t* worig, double* part, unsigned* linf,
                     int write_v, int write_pol, const pgk::ActiveG& A, hipStream_t st) {
     return pgk::for_box_chunks(G, [&](const pgk::BoxGeom& g, unsigned nb) {
         if (write_v && write_pol)
-            hipLaunchKernelGGL((pgk::sweep_lds_kernel<W, DIAG, MAXS, true, true, SRC, DST>), dim3(nb), dim3(256), 0,
+            hipLaunchKernelGGL((pgk::sweep_lds_kernel<W, DIAG, RK2, MAXS, true, true, SRC, DST>), dim3(nb), dim3(256), 0,
                                st, src, dst, occ, bu, K, g, worig, part, linf, A);
         else if (write_v)
-            hipLaunchKernelGGL((pgk::sweep_lds_kernel<W, DIAG, MAXS, true, false, SRC, DST>), dim3(nb), dim3(256),
+            hipLaunchKernelGGL((pgk::sweep_lds_kernel<W, DIAG, RK2, MAXS, true, false, SRC, DST>), dim3(nb), dim3(256),
                                0, st, src, dst, occ, bu, K, g, worig, part, linf, A);
         else
-            hipLaunchKernelGGL((pgk::sweep_lds_kernel<W, DIAG, MAXS, false, true, SRC, DST>), dim3(nb), dim3(256),
+            hipLaunchKernelGGL((pgk::sweep_lds_kernel<W, DIAG, RK2, MAXS, false, true, SRC, DST>), dim3(nb), dim3(256),
                                0, st, src, dst, occ, bu, K, g, worig, part, linf, A);
     });
 }
@@ -474,13 +484,19 @@ template <class SRC, class DST>
 hipError_t launch_shape(int shape, int diag, const pgk::Consts& K, const pgk::BoxGeom& G, const SRC& src,
                         const DST& dst, const uint8_t* occ, const float* bu, const int16_t* worig, double* part,
                         unsigned* linf, int write_v, int write_pol, const pgk::ActiveG& A, hipStream_t st) {
-    switch (shape * 2 + (diag ? 1 : 0)) {
-        case 0: return launch_w<pgk::Win0, false, 128>(K, G, src, dst, occ, bu, worig, part, linf, write_v, write_pol, A, st);
-        case 1: return launch_w<pgk::Win0, true, 128>(K, G, src, dst, occ, bu, worig, part, linf, write_v, write_pol, A, st);
-        case 2: return launch_w<pgk::Win1, false, 128>(K, G, src, dst, occ, bu, worig, part, linf, write_v, write_pol, A, st);
-        case 3: return launch_w<pgk::Win1, true, 128>(K, G, src, dst, occ, bu, worig, part, linf, write_v, write_pol, A, st);
-        case 4: return launch_w<pgk::Win2, false, 128>(K, G, src, dst, occ, bu, worig, part, linf, write_v, write_pol, A, st);
-        case 5: return launch_w<pgk::Win2, true, 128>(K, G, src, dst, occ, bu, worig, part, linf, write_v, write_pol, A, st);
+    switch ((shape * 2 + (diag ? 1 : 0)) * 2 + (K.rk2 ? 1 : 0)) {
+        case 0: return launch_w<pgk::Win0, false, false, 128>(K, G, src, dst, occ, bu, worig, part, linf, write_v, write_pol, A, st);
+        case 1: return launch_w<pgk::Win0, false, true, 128>(K, G, src, dst, occ, bu, worig, part, linf, write_v, write_pol, A, st);
+        case 2: return launch_w<pgk::Win0, true, false, 128>(K, G, src, dst, occ, bu, worig, part, linf, write_v, write_pol, A, st);
+        case 3: return launch_w<pgk::Win0, true, true, 128>(K, G, src, dst, occ, bu, worig, part, linf, write_v, write_pol, A, st);
+        case 4: return launch_w<pgk::Win1, false, false, 128>(K, G, src, dst, occ, bu, worig, part, linf, write_v, write_pol, A, st);
+        case 5: return launch_w<pgk::Win1, false, true, 128>(K, G, src, dst, occ, bu, worig, part, linf, write_v, write_pol, A, st);
+        case 6: return launch_w<pgk::Win1, true, false, 128>(K, G, src, dst, occ, bu, worig, part, linf, write_v, write_pol, A, st);
+        case 7: return launch_w<pgk::Win1, true, true, 128>(K, G, src, dst, occ, bu, worig, part, linf, write_v, write_pol, A, st);
+        case 8: return launch_w<pgk::Win2, false, false, 128>(K, G, src, dst, occ, bu, worig, part, linf, write_v, write_pol, A, st);
+        case 9: return launch_w<pgk::Win2, false, true, 128>(K, G, src, dst, occ, bu, worig, part, linf, write_v, write_pol, A, st);
+        case 10: return launch_w<pgk::Win2, true, false, 128>(K, G, src, dst, occ, bu, worig, part, linf, write_v, write_pol, A, st);
+        case 11: return launch_w<pgk::Win2, true, true, 128>(K, G, src, dst, occ, bu, worig, part, linf, write_v, write_pol, A, st);
         default: return hipErrorInvalidValue;
     }
 }
