@@ -149,6 +149,22 @@ int pg_sweep(pg_solver* s, int32_t phase, int32_t rev, double* sum_decrease,
 int pg_solve(pg_solver* s, int32_t max_iters, double tol_sum, float tol_linf,
              int32_t* iters, double* last_sum, float* last_linf);
 
+/* The escape pass over a solved table (in place of `sweep_escape_gpu!` and
+ * `solve_escape!`, solver/src/Gpu.jl:39-80, solver/src/Run.jl:1503-1530).
+ * V keeps its sign convention (PeregrineSolver.jl:756-856): 0 <= v < cap is a
+ * real route (the boundary, never written), v < 0 an escape taking -v
+ * seconds, v >= cap no way out yet. A sweep backs up every other cell, blocked
+ * ones included (they may move through obstacle space, free cells may not
+ * enter it), reading corners as the escape time they hold (routes as 0,
+ * obstacle corners as cap for a free cell), and stores -c where the backup c
+ * beats the cell's escape time. Jacobi, global-gather kernel, bit-exact with
+ * the CPU restatement. Returns the summed and the largest improvement.
+ * pg_solve_escape stops as pg_solve does. A 16-bit store must be polished
+ * first; not built out of core. */
+int pg_escape_sweep(pg_solver* s, int32_t phase, int32_t rev, double* sum_improve, float* linf);
+int pg_solve_escape(pg_solver* s, int32_t max_iters, double tol_sum, float tol_linf, int32_t* iters,
+                    double* last_sum, float* last_linf);
+
 /* Finish a precision-form solve: decode the FP16 store into an FP32 Jacobi
  * pair (allocated here) and run nsweeps exact FP32 sweeps on it; afterwards
  * the solver behaves as an FP32 solver. With an FP32 store, runs nsweeps

@@ -125,6 +125,9 @@ SIGNATURES = {
     "pg_solver_kernel_info": (ctypes.c_int, [_P, ctypes.POINTER(_I32), ctypes.POINTER(ctypes.c_double)]),
     "pg_last_sweep_ms": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_float)]),
     "pg_solver_set_active": (ctypes.c_int, [_P, _I32]),
+    "pg_escape_sweep": (ctypes.c_int, [_P, _I32, _I32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_float)]),
+    "pg_solve_escape": (ctypes.c_int, [_P, _I32, ctypes.c_double, ctypes.c_float, ctypes.POINTER(_I32),
+                                       ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_float)]),
     "pg_last_sweep_boxes": (ctypes.c_int, [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
     "pg_solver_ncells": (ctypes.c_int, [_P, ctypes.POINTER(_I64)]),
 }

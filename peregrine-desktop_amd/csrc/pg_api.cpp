@@ -755,6 +755,67 @@ int pg_solve(pg_solver* s, int32_t max_iters, double tol_sum, float tol_linf, in
     return PG_OK;
 }
 
+int pg_escape_sweep(pg_solver* s, int32_t phase, int32_t rev, double* sum_improve, float* linf) {
+    (void)phase;
+    (void)rev;
+    if (!s) return fail(PG_ERR_ARG, "null solver");
+    if (s->ooc) return fail(PG_ERR_UNSUPPORTED, "the escape pass is not built for an out-of-core solver");
+    if (!s->seeded) return fail(PG_ERR_STATE, "no value table yet");
+    if (s->f16_active) return fail(PG_ERR_STATE, "a 16-bit store must be polished (pg_polish) before the escape pass");
+    HIPCHECK(hipSetDevice(s->ctx->device));
+    hipStream_t st = s->ctx->stream;
+    const int64_t nb = (s->ncell + 255) / 256;
+    int rc = ensure_partials(s, nb);
+    if (rc) return rc;
+    if (!s->dV[1]) {
+        hipError_t e = hipMalloc(&s->dV[1], (size_t)s->ncell * sizeof(float));
+        if (e != hipSuccess)
+            return fail(e == hipErrorOutOfMemory ? PG_ERR_NOMEM : PG_ERR_HIP, "escape buffer allocation failed: %s",
+                        hipGetErrorString(e));
+    }
+    HIPCHECK(hipMemsetAsync(s->d_linf, 0, sizeof(unsigned), st));
+    int64_t nblk = nb;
+    HIPCHECK(hipEventRecord(s->ev0, st));
+    HIPCHECK(pgk_launch_escape(s->K, s->dV[s->cur], s->dV[1 - s->cur], s->d_occ, s->d_bu, nullptr, s->d_part,
+                               s->d_linf, st, &nblk));
+    HIPCHECK(hipEventRecord(s->ev1, st));
+    HIPCHECK(pgk_launch_reduce(s->d_part, nblk, s->d_sum, s->d_sum + 1, st));
+    s->cur = 1 - s->cur;
+    s->act_for = -1;
+    s->last_work = -1;
+    double hsum = 0.0;
+    unsigned hl = 0;
+    HIPCHECK(hipMemcpyAsync(&hsum, s->d_sum, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(&hl, s->d_linf, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    HIPCHECK(hipEventElapsedTime(&s->last_ms, s->ev0, s->ev1));
+    float lf;
+    std::memcpy(&lf, &hl, 4);
+    if (sum_improve) *sum_improve = hsum;
+    if (linf) *linf = lf;
+    return PG_OK;
+}
+
+int pg_solve_escape(pg_solver* s, int32_t max_iters, double tol_sum, float tol_linf, int32_t* iters,
+                    double* last_sum, float* last_linf) {
+    if (!s) return fail(PG_ERR_ARG, "null solver");
+    int32_t it = 0;
+    double d = INFINITY;
+    float l = INFINITY;
+    for (it = 1; it <= max_iters; ++it) {
+        int rc = pg_escape_sweep(s, it - 1, it & 1, &d, &l);
+        if (rc) return rc;
+        const bool quiet_sum = tol_sum >= 0.0 && d <= tol_sum;
+        const bool quiet_linf = tol_linf > 0.0f && l < tol_linf;
+        if (quiet_sum || quiet_linf) break;
+    }
+    if (it > max_iters) it = max_iters;
+    if (iters) *iters = it;
+    if (last_sum) *last_sum = d;
+    if (last_linf) *last_linf = l;
+    return PG_OK;
+}
+
 int pg_polish(pg_solver* s, int32_t nsweeps, double* last_sum, float* last_linf) {
     if (!s) return fail(PG_ERR_ARG, "null solver");
     if (s->ooc) return fail(PG_ERR_UNSUPPORTED, "an out-of-core solver has no 16-bit store to polish");

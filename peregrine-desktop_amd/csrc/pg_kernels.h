@@ -11,6 +11,11 @@ hipError_t pgk_launch_sweep(const pgk::Consts& K, const float* Vin, float* Vout,
 // Sum of part[0, n) into *out; with mid (room for kReduceBlocks doubles)
 // in two levels.
 constexpr int kReduceBlocks = 1024;
+// One Jacobi escape sweep (gpu_escape_kernel!, Gpu.jl:51-67) of the whole
+// grid, global-gather kernel.
+hipError_t pgk_launch_escape(const pgk::Consts& K, const float* Vin, float* Vout, const uint8_t* occ,
+                             const float* bu, uint8_t* pol, double* part, unsigned* linf_bits, hipStream_t st,
+                             int64_t* nblocks);
 hipError_t pgk_launch_reduce(const double* part, int64_t n, double* out, double* mid, hipStream_t st);
 hipError_t pgk_launch_fill(float* V, int64_t n, float v, hipStream_t st);
 hipError_t pgk_launch_seed(float* V, const int64_t* seeds, int64_t n, hipStream_t st);

@@ -1,5 +1,6 @@
 // Bellman sweep kernels for gfx950 (plain scheme: fixed horizon, full
-// control scan, forward Euler, fixed swept probes, 64-corner multilinear).
+// control scan, forward Euler or explicit midpoint, fixed swept probes,
+// 64-corner multilinear), and the escape pass over the same backup.
 //
 // One thread per cell; the control table is scanned in order with a strict
 // `<`, coast first, so ties resolve to the lowest index exactly as
@@ -18,7 +19,14 @@ namespace pgk {
 // the reference's order d = 0..63 (bit 0 = x ... bit 5 = w); a zero weight
 // adds +0 to a non-negative running sum, so the reference's skip of those
 // corners needs no branch here.
+//
+// ESC reads V as the escape pass does (EscapeView, PeregrineSolver.jl:
+// 836-842): a corner whose (x, y, h) cell is an obstacle reads cap unless
+// `ignore` (the updating cell is itself blocked); otherwise a negative value
+// reads as its escape time, a real route (0 <= v < cap) as 0, the rest as cap.
+template <bool ESC>
 __device__ __forceinline__ float interp_global(const Consts& K, const float* __restrict__ V,
+                                               const uint8_t* __restrict__ occ, bool ignore,
                                                float x, float y, float h, float vx, float vy,
                                                float w) {
     float f[6];
@@ -34,6 +42,7 @@ __device__ __forceinline__ float interp_global(const Consts& K, const float* __r
     if (!in) return K.cap;
     float t[6], u[6];
     int64_t o0[6], o1[6];
+    int jc[3][2];   // corner cells of x, y, h (the escape reads' occupancy)
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
         const float b = __builtin_floorf(f[k]);
@@ -51,6 +60,7 @@ __device__ __forceinline__ float interp_global(const Consts& K, const float* __r
         }
         o0[k] = (int64_t)j0 * K.stride[k];
         o1[k] = (int64_t)j1 * K.stride[k];
+        if (k < 3) { jc[k][0] = j0; jc[k][1] = j1; }
     }
     // weights: p[d] for the first five axes, built level by level so that
     // every weight is ((((w1*w2)*w3)*w4)*w5)*w6 in the reference's order
@@ -79,6 +89,18 @@ __device__ __forceinline__ float interp_global(const Consts& K, const float* __r
     float vals[64];
 #pragma unroll
     for (int d = 0; d < 64; ++d) vals[d] = V[a5[d & 31] + ((d & 32) ? o1[5] : o0[5])];
+    if (ESC) {
+        bool wall[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+            wall[c] = !ignore && occ[((int64_t)jc[0][c & 1] * K.n[1] + jc[1][(c >> 1) & 1]) * K.n[2] +
+                                     jc[2][(c >> 2) & 1]] != 0;
+#pragma unroll
+        for (int d = 0; d < 64; ++d) {
+            const float v = vals[d];
+            vals[d] = wall[d & 7] ? K.cap : (v < 0.0f ? -v : (v < K.cap ? 0.0f : K.cap));
+        }
+    }
     float acc = 0.0f;
 #pragma unroll
     for (int d = 0; d < 64; ++d) {
@@ -88,15 +110,20 @@ __device__ __forceinline__ float interp_global(const Consts& K, const float* __r
     return (acc < K.cap) ? acc : K.cap;
 }
 
+// Occupancy as the probes read it; `ignore` waives it (the escape pass's
+// OccView, PeregrineSolver.jl:864-874: a blocked cell may move through
+// obstacle space).
 struct OccGlobal {
     const uint8_t* __restrict__ occ;
     int32_t n2, n3;
+    bool ignore;
     __device__ __forceinline__ bool operator()(int ci, int cj, int ck) const {
-        return occ[((int64_t)ci * n2 + cj) * n3 + ck] != 0;
+        return !ignore && occ[((int64_t)ci * n2 + cj) * n3 + ck] != 0;
     }
 };
 
 // lookahead (PeregrineSolver.jl:1100-1153) for one control.
+template <bool ESC>
 __device__ __forceinline__ float lookahead_global(const Consts& K, const float* __restrict__ V,
                                                   const OccGlobal& occ, float x, float y,
                                                   float h, float vx, float vy, float w,
@@ -113,12 +140,17 @@ __device__ __forceinline__ float lookahead_global(const Consts& K, const float* 
     sincos_ref(nh, snh, cnh);
     const float nvfx = cnh * nvx - snh * nvy;
     const float nvfy = snh * nvx + cnh * nvy;
-    const float nv = interp_global(K, V, nx, ny, nh, nvfx, nvfy, nw);
+    const float nv = interp_global<ESC>(K, V, occ.occ, occ.ignore, nx, ny, nh, nvfx, nvfy, nw);
     const float c = K.tau + nv;
     return (c < K.cap) ? c : K.cap;
 }
 
-template <bool WRITE_V, bool WRITE_POL>
+// One thread per cell. ESC: the escape pass (gpu_escape_kernel!, Gpu.jl:51-67)
+// in Jacobi form: a cell with a real route (0 <= v < cap) is the boundary and
+// keeps its value; any other cell, blocked ones included (through obstacle
+// space, PeregrineSolver.jl:864-874), takes -c when its backup c beats the
+// escape time it holds (-v if v < 0, else cap).
+template <bool WRITE_V, bool WRITE_POL, bool ESC>
 __global__ __launch_bounds__(256) void sweep_global_kernel(
     const float* Vin, float* Vout, const uint8_t* __restrict__ occ,
     const float* __restrict__ bu, Consts K, uint8_t* __restrict__ pol,
@@ -135,9 +167,11 @@ __global__ __launch_bounds__(256) void sweep_global_kernel(
         }
         i[0] = (int32_t)r;
         const bool blocked = occ[((int64_t)i[0] * K.n[1] + i[1]) * K.n[2] + i[2]] != 0;
+        const float old = Vin[idx];
+        const bool terminal = old >= 0.0f && old < K.cap;
         float best = K.cap;
         int bk = 0;
-        if (!blocked) {
+        if (ESC ? !terminal : !blocked) {
             const float x = K.lo[0] + (float)i[0] * K.step[0];
             const float y = K.lo[1] + (float)i[1] * K.step[1];
             const float h = K.lo[2] + (float)i[2] * K.step[2];
@@ -148,18 +182,23 @@ __global__ __launch_bounds__(256) void sweep_global_kernel(
             sincos_ref(h, sh0, ch0);
             const float vx = ch0 * vfx + sh0 * vfy;
             const float vy = -sh0 * vfx + ch0 * vfy;
-            const OccGlobal og{occ, K.n[1], K.n[2]};
-            best = lookahead_global(K, Vin, og, x, y, h, vx, vy, w, bu[0], bu[1], bu[2]);
+            const OccGlobal og{occ, K.n[1], K.n[2], ESC && blocked};
+            best = lookahead_global<ESC>(K, Vin, og, x, y, h, vx, vy, w, bu[0], bu[1], bu[2]);
             for (int k = 1; k < K.nctl; ++k) {
-                const float c = lookahead_global(K, Vin, og, x, y, h, vx, vy, w, bu[3 * k],
-                                                 bu[3 * k + 1], bu[3 * k + 2]);
+                const float c = lookahead_global<ESC>(K, Vin, og, x, y, h, vx, vy, w, bu[3 * k],
+                                                      bu[3 * k + 1], bu[3 * k + 2]);
                 if (c < best) { best = c; bk = k; }
             }
         }
-        const float old = Vin[idx];
         if (WRITE_V) {
             float out = old;
-            if (best < old) { out = best; dec = old - best; }
+            if (ESC) {
+                const float e_old = old < 0.0f ? -old : K.cap;
+                if (!terminal && best < e_old) { out = -best; dec = e_old - best; }
+            } else if (best < old) {
+                out = best;
+                dec = old - best;
+            }
             Vout[idx] = out;
         }
         if (WRITE_POL) pol[idx] = (uint8_t)bk;
@@ -228,14 +267,28 @@ hipError_t pgk_launch_sweep(const pgk::Consts& K, const float* Vin, float* Vout,
     const int64_t nb = (K.ncell + 255) / 256;
     *nblocks = nb;
     if (write_v && pol)
-        hipLaunchKernelGGL((pgk::sweep_global_kernel<true, true>), dim3((unsigned)nb), dim3(256),
+        hipLaunchKernelGGL((pgk::sweep_global_kernel<true, true, false>), dim3((unsigned)nb), dim3(256),
                            0, st, Vin, Vout, occ, bu, K, pol, part, linf_bits);
     else if (write_v)
-        hipLaunchKernelGGL((pgk::sweep_global_kernel<true, false>), dim3((unsigned)nb), dim3(256),
+        hipLaunchKernelGGL((pgk::sweep_global_kernel<true, false, false>), dim3((unsigned)nb), dim3(256),
                            0, st, Vin, Vout, occ, bu, K, pol, part, linf_bits);
     else
-        hipLaunchKernelGGL((pgk::sweep_global_kernel<false, true>), dim3((unsigned)nb), dim3(256),
+        hipLaunchKernelGGL((pgk::sweep_global_kernel<false, true, false>), dim3((unsigned)nb), dim3(256),
                            0, st, Vin, Vout, occ, bu, K, pol, part, linf_bits);
+    return hipGetLastError();
+}
+
+hipError_t pgk_launch_escape(const pgk::Consts& K, const float* Vin, float* Vout, const uint8_t* occ,
+                             const float* bu, uint8_t* pol, double* part, unsigned* linf_bits, hipStream_t st,
+                             int64_t* nblocks) {
+    const int64_t nb = (K.ncell + 255) / 256;
+    *nblocks = nb;
+    if (pol)
+        hipLaunchKernelGGL((pgk::sweep_global_kernel<true, true, true>), dim3((unsigned)nb), dim3(256), 0, st,
+                           Vin, Vout, occ, bu, K, pol, part, linf_bits);
+    else
+        hipLaunchKernelGGL((pgk::sweep_global_kernel<true, false, true>), dim3((unsigned)nb), dim3(256), 0, st,
+                           Vin, Vout, occ, bu, K, pol, part, linf_bits);
     return hipGetLastError();
 }
 

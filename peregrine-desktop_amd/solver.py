@@ -177,6 +177,27 @@ class HipSolver:
                                     ctypes.byref(d), ctypes.byref(l)))
         return it.value, d.value, l.value
 
+    def escape_sweep(self, phase: int = 0, rev: bool = False) -> SweepResult:
+        """One escape sweep (sweep_escape_gpu!): returns the summed and the
+        largest improvement of the escape times."""
+        d = ctypes.c_double()
+        l = ctypes.c_float()
+        abi.check(self.lib, "pg_escape_sweep",
+                  self.lib.pg_escape_sweep(self._h, int(phase), int(bool(rev)), ctypes.byref(d), ctypes.byref(l)))
+        ms = ctypes.c_float()
+        self.lib.pg_last_sweep_ms(self._h, ctypes.byref(ms))
+        return SweepResult(d.value, l.value, ms.value)
+
+    def solve_escape(self, iters: int, tol: float = -1.0, tol_linf: float = 0.0):
+        """solve_escape!: escape sweeps until quiet (sum <= tol, or max < tol_linf)."""
+        it = ctypes.c_int32()
+        d = ctypes.c_double()
+        l = ctypes.c_float()
+        abi.check(self.lib, "pg_solve_escape",
+                  self.lib.pg_solve_escape(self._h, int(iters), float(tol), float(tol_linf), ctypes.byref(it),
+                                           ctypes.byref(d), ctypes.byref(l)))
+        return it.value, d.value, l.value
+
     def set_sweep_mode(self, mode: int) -> None:
         """abi.PG_SWEEP_JACOBI (default, deterministic) or abi.PG_SWEEP_IN_PLACE."""
         abi.check(self.lib, "pg_solver_set_sweep_mode", self.lib.pg_solver_set_sweep_mode(self._h, int(mode)))
