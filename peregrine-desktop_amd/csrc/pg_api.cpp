@@ -135,12 +135,7 @@ int check_params(const pg_params* p, int32_t nctl) {
     if (p->cfl > 0.0f) return fail(PG_ERR_UNSUPPORTED, "cfl > 0 (per-cell horizon) is not built");
     if (p->ntau > 1) return fail(PG_ERR_UNSUPPORTED, "tau_levels > 1 (horizon ladder) is not built");
     if (p->rounds > 0) return fail(PG_ERR_UNSUPPORTED, "refine_rounds > 0 (pattern search) is not built");
-    if (p->ncoarse > 0 && p->ncoarse < nctl - 1)
-        return fail(PG_ERR_UNSUPPORTED, "control_scan < nctl-1 (rotating slice) is not built");
     if (p->simplex) return fail(PG_ERR_UNSUPPORTED, "simplex (Kuhn) interpolation is not built");
-    if (p->nearest) return fail(PG_ERR_UNSUPPORTED, "nearest-cell lookup is not built");
-    if (p->vclamp) return fail(PG_ERR_UNSUPPORTED, "velocity_clamp is not built");
-    if (p->adaptive_checks) return fail(PG_ERR_UNSUPPORTED, "adaptive_checks is not built");
     if (p->checks < 1 || p->checks > 24) return fail(PG_ERR_ARG, "sweep_checks must be in [1, 24], got %d", p->checks);
     if (!(p->dt > 0.0f) || !std::isfinite(p->dt)) return fail(PG_ERR_ARG, "dt must be positive");
     if (!(p->cap > 0.0f) || !std::isfinite(p->cap)) return fail(PG_ERR_ARG, "value_cap must be positive");
@@ -185,6 +180,14 @@ void build_consts(pg_solver* s) {
     for (int sc = 1; sc <= K.nchk; ++sc) K.probe_a[sc - 1] = (float)sc / (float)K.nchk;
     K.nctl = s->nctl;
     K.rk2 = s->p.rk2 ? 1 : 0;
+    K.nearest = s->p.nearest ? 1 : 0;
+    K.vclamp = s->p.vclamp ? 1 : 0;
+    K.adaptive = s->p.adaptive_checks ? 1 : 0;
+    K.half_xy = 0.5f * (g.step[0] < g.step[1] ? g.step[0] : g.step[1]);
+    K.half_h = 0.5f * g.step[2];
+    K.nnc = s->nctl - 1 > 1 ? s->nctl - 1 : 1;
+    K.span = s->p.ncoarse <= 0 ? K.nnc : (s->p.ncoarse < K.nnc ? s->p.ncoarse : K.nnc);
+    K.off = 0;
     int64_t nt = 1;
     for (int k = 0; k < 6; ++k) {
         s->Q.nt[k] = (g.n[k] + (1 << pgk::kQuantShift) - 1) >> pgk::kQuantShift;
@@ -331,6 +334,33 @@ int encode_from_scratch(pg_solver* s, bool force, uint32_t* mark = nullptr) {
 // every state's own value in FP32 and the codes mirror it for the gathers.
 float* current_f32(pg_solver* s) { return s->f16_active ? s->dV[0] : s->dV[s->cur]; }
 
+// Scheme options that only the exact FP32 kernels carry (the precision form,
+// the FP64 tables and the adjoint integrate the plain default): a message
+// naming the first one set, or null.
+const char* exact_only_option(const pg_solver* s) {
+    if (s->p.rk2) return "rk2";
+    if (s->p.nearest) return "nearest";
+    if (s->p.vclamp) return "velocity_clamp";
+    if (s->p.adaptive_checks) return "adaptive_checks";
+    if (s->K.span < s->K.nnc) return "control_scan (rotating slice)";
+    return nullptr;
+}
+
+// The scan offset of a sweep's rotating slice (PeregrineSolver.jl:1263-1275):
+// (phase * span) mod nnc, 0 for the whole lattice.
+int32_t slice_offset(const pg_solver* s, int32_t phase) {
+    if (s->p.ncoarse <= 0) return 0;
+    const int64_t nnc = s->K.nnc;
+    return (int32_t)((((int64_t)phase * s->K.span) % nnc + nnc) % nnc);
+}
+
+// Consecutive quiet sweeps that end a solve: a whole turn of a rotating slice.
+int32_t quiet_needed(const pg_solver* s) {
+    if (s->p.ncoarse <= 0) return 1;
+    const int32_t n = (s->nctl - 1 + s->p.ncoarse - 1) / s->p.ncoarse;
+    return n > 1 ? n : 1;
+}
+
 // The precision form's plan (once per solver, after the exact one): its
 // read ranges and the boxes whose probes it skips.
 int plan_fast(pg_solver* s) {
@@ -350,7 +380,9 @@ int plan_fast(pg_solver* s) {
 // empty, every box computed) when the set is off or cannot be kept.
 int active_setup(pg_solver* s, int regime, bool inplace, pgk::ActiveG& A, bool& track) {
     A = pgk::ActiveG{};
-    track = s->active_on && !inplace && regime >= 0 && s->G.nbg[5] <= 32;
+    // a rotating slice backs up with other controls each sweep: unchanged
+    // inputs do not imply an unchanged result
+    track = s->active_on && !inplace && regime >= 0 && s->G.nbg[5] <= 32 && s->K.span == s->K.nnc;
     if (!track) return PG_OK;
     hipStream_t st = s->ctx->stream;
     const int64_t words = s->G.nbox / s->G.nbg[5];
@@ -465,8 +497,11 @@ int create_solver(pg_ctx* ctx, const pg_grid* g, const pg_model* m, const pg_par
     if (rc) return rc;
     if (vstore != PG_VSTORE_F32 && vstore != PG_VSTORE_F16_TILE && vstore != PG_VSTORE_U16_TILE)
         return fail(PG_ERR_ARG, "unknown vstore %d", vstore);
-    if (vstore != PG_VSTORE_F32 && p->rk2)
-        return fail(PG_ERR_UNSUPPORTED, "the 16-bit stores (precision form) integrate with forward Euler only");
+    if (vstore != PG_VSTORE_F32 &&
+        (p->rk2 || p->nearest || p->vclamp || p->adaptive_checks || (p->ncoarse > 0 && p->ncoarse < nctl - 1)))
+        return fail(PG_ERR_UNSUPPORTED,
+                    "the 16-bit stores (precision form) run the plain default scheme only (no rk2, nearest, "
+                    "velocity_clamp, adaptive_checks or control_scan slice)");
     if (vstore != PG_VSTORE_F32 && nctl > kLdsMaxCtl)
         return fail(PG_ERR_UNSUPPORTED, "the 16-bit stores sweep at most %d controls", kLdsMaxCtl);
     for (int k = 0; k < 3; ++k)
@@ -657,10 +692,10 @@ int pg_fill_seed(pg_solver* s, const int64_t* seeds, int64_t nseeds) {
 }
 
 int pg_sweep(pg_solver* s, int32_t phase, int32_t rev, double* sum_decrease, float* linf) {
-    (void)phase;
     if (!s) return fail(PG_ERR_ARG, "null solver");
     if (!s->seeded) return fail(PG_ERR_STATE, "pg_fill_seed (or pg_upload_V) must come before a sweep");
     HIPCHECK(hipSetDevice(s->ctx->device));
+    s->K.off = slice_offset(s, phase);
     if (s->ooc) return pgh::ooc_sweep(s, sum_decrease, linf);
     hipStream_t st = s->ctx->stream;
     const int kern = s->f16_active ? PG_KERNEL_FAST : s->kernel;
@@ -741,12 +776,17 @@ int pg_solve(pg_solver* s, int32_t max_iters, double tol_sum, float tol_linf, in
     int32_t it = 0;
     double d = INFINITY;
     float l = INFINITY;
+    // with a rotating slice, one quiet sweep only proves that its slice
+    // helps nobody: a whole turn of the lattice must be quiet (Run.jl:1449-1453)
+    const int32_t need = quiet_needed(s);
+    int32_t quiet = 0;
     for (it = 1; it <= max_iters; ++it) {
         int rc = pg_sweep(s, it - 1, it & 1, &d, &l);
         if (rc) return rc;
         const bool quiet_sum = tol_sum >= 0.0 && d <= tol_sum;
         const bool quiet_linf = tol_linf > 0.0f && l < tol_linf;
-        if (quiet_sum || quiet_linf) break;
+        quiet = (quiet_sum || quiet_linf) ? quiet + 1 : 0;
+        if (quiet >= need) break;
     }
     if (it > max_iters) it = max_iters;
     if (iters) *iters = it;
@@ -756,13 +796,13 @@ int pg_solve(pg_solver* s, int32_t max_iters, double tol_sum, float tol_linf, in
 }
 
 int pg_escape_sweep(pg_solver* s, int32_t phase, int32_t rev, double* sum_improve, float* linf) {
-    (void)phase;
     (void)rev;
     if (!s) return fail(PG_ERR_ARG, "null solver");
     if (s->ooc) return fail(PG_ERR_UNSUPPORTED, "the escape pass is not built for an out-of-core solver");
     if (!s->seeded) return fail(PG_ERR_STATE, "no value table yet");
     if (s->f16_active) return fail(PG_ERR_STATE, "a 16-bit store must be polished (pg_polish) before the escape pass");
     HIPCHECK(hipSetDevice(s->ctx->device));
+    s->K.off = slice_offset(s, phase);
     hipStream_t st = s->ctx->stream;
     const int64_t nb = (s->ncell + 255) / 256;
     int rc = ensure_partials(s, nb);
@@ -802,12 +842,15 @@ int pg_solve_escape(pg_solver* s, int32_t max_iters, double tol_sum, float tol_l
     int32_t it = 0;
     double d = INFINITY;
     float l = INFINITY;
+    const int32_t need = quiet_needed(s);
+    int32_t quiet = 0;
     for (it = 1; it <= max_iters; ++it) {
         int rc = pg_escape_sweep(s, it - 1, it & 1, &d, &l);
         if (rc) return rc;
         const bool quiet_sum = tol_sum >= 0.0 && d <= tol_sum;
         const bool quiet_linf = tol_linf > 0.0f && l < tol_linf;
-        if (quiet_sum || quiet_linf) break;
+        quiet = (quiet_sum || quiet_linf) ? quiet + 1 : 0;
+        if (quiet >= need) break;
     }
     if (it > max_iters) it = max_iters;
     if (iters) *iters = it;
@@ -853,6 +896,15 @@ int pg_policy(pg_solver* s, uint8_t* pol_host) {
     hipStream_t st = s->ctx->stream;
     if (!s->d_pol) HIPCHECK(hipMalloc(&s->d_pol, (size_t)s->ncell));
     int64_t nblk = 0;
+    // the argmin over the whole lattice, whatever slice the last sweep used
+    const int32_t span = s->K.span, off = s->K.off;
+    s->K.span = s->K.nnc;
+    s->K.off = 0;
+    struct Restore {
+        pg_solver* s;
+        int32_t span, off;
+        ~Restore() { s->K.span = span; s->K.off = off; }
+    } restore{s, span, off};
     const int kern = s->f16_active ? PG_KERNEL_FAST : s->kernel;
     if (kern != PG_KERNEL_GLOBAL) {
         int rc = plan(s);
@@ -946,7 +998,9 @@ namespace {
 
 int f64_setup(pg_solver* s) {
     if (s->ooc) return fail(PG_ERR_UNSUPPORTED, "the FP64 tables are not built for an out-of-core solver");
-    if (s->p.rk2) return fail(PG_ERR_UNSUPPORTED, "the FP64 tables and the adjoint integrate with forward Euler only");
+    if (const char* opt = exact_only_option(s))
+        return fail(PG_ERR_UNSUPPORTED, "the FP64 tables and the adjoint run the plain default scheme only (%s set)",
+                    opt);
     if (s->nctl > kLdsMaxCtl)
         return fail(PG_ERR_UNSUPPORTED, "the FP64 sweeps hold at most %d controls", kLdsMaxCtl);
     int rc = plan(s);
@@ -1199,8 +1253,9 @@ int pg_solver_set_kernel(pg_solver* s, int32_t kernel) {
         return fail(PG_ERR_ARG, "unknown kernel %d", kernel);
     if (kernel != PG_KERNEL_GLOBAL && s->nctl > kLdsMaxCtl)
         return fail(PG_ERR_UNSUPPORTED, "the LDS-staged sweeps hold at most %d controls", kLdsMaxCtl);
-    if (kernel == PG_KERNEL_FAST && s->p.rk2)
-        return fail(PG_ERR_UNSUPPORTED, "the precision-form kernel integrates with forward Euler only");
+    if (kernel == PG_KERNEL_FAST && exact_only_option(s))
+        return fail(PG_ERR_UNSUPPORTED, "the precision-form kernel runs the plain default scheme only (%s set)",
+                    exact_only_option(s));
     if (s->f16_active && kernel != PG_KERNEL_FAST)
         return fail(PG_ERR_STATE, "a 16-bit store sweeps with the fast kernel until pg_polish");
     if (s->ooc && kernel != PG_KERNEL_LDS)

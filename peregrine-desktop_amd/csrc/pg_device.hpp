@@ -37,8 +37,24 @@ struct Consts {
     int32_t nchk;
     int32_t nctl;
     int32_t rk2;       // explicit midpoint substeps instead of forward Euler
+    int32_t nearest;   // read the nearest node instead of the 64-corner interpolant
+    int32_t vclamp;    // clamp velocity lookups into the grid instead of rejecting them
+    int32_t adaptive;  // probe count from the step length (adaptive_checks)
+    float half_xy;     // 0.5 * min(step x, step y), IEEE (adaptive probe count)
+    float half_h;      // 0.5 * step h, IEEE
+    // the rotating slice of the control lattice (ncoarse > 0): controls
+    // ((off + j) mod nnc) + 1 for j < span after coast; off from the sweep's phase
+    int32_t nnc, span, off;
     float probe_a[24]; // Float32(sc) / Float32(nchk), IEEE
 };
+
+// Control of scan position r (0 = coast, then the slice in scan order).
+__device__ __forceinline__ int slice_control(const Consts& K, int r) {
+    if (r == 0) return 0;
+    int k = K.off + r - 1;
+    k = k >= K.nnc ? k - K.nnc : k;
+    return k + 1;
+}
 
 __device__ __forceinline__ float qdiv(float a, double rb) {
     return (float)((double)a * rb);
@@ -157,9 +173,21 @@ __device__ __forceinline__ float wrap_h(const Consts& K, float f) {
 template <class OccRead>
 __device__ __forceinline__ bool probes_free(const Consts& K, const OccRead& occ, float x,
                                             float y, float h, float dx, float dy, float dh) {
+    int nchk = K.nchk;
+    if (K.adaptive) {
+        // enough probes that none is more than half a cell from the next
+        // (PeregrineSolver.jl:1127-1133)
+        const float cells_pos = sqrtf(dx * dx + dy * dy) / K.half_xy;
+        const float cells_rot = __builtin_fabsf(dh) / K.half_h;
+        float cells = cells_pos > cells_rot ? cells_pos : cells_rot;
+        cells = cells < 1.0e4f ? cells : 1.0e4f;
+        const int need = (int)cells + 1;
+        const int cap24 = need < 24 ? need : 24;
+        nchk = K.nchk > cap24 ? K.nchk : cap24;
+    }
     bool ok = true;
-    for (int sc = 0; sc < K.nchk; ++sc) {
-        const float a = K.probe_a[sc];
+    for (int sc = 0; sc < nchk; ++sc) {
+        const float a = K.adaptive ? (float)(sc + 1) / (float)nchk : K.probe_a[sc];
         const float px = x + dx * a;
         const float py = y + dy * a;
         const float ph = h + dh * a;
@@ -412,7 +440,15 @@ __device__ __forceinline__ bool interp_coords(const Consts& K, float x, float y,
     f[5] = qdiv(w - K.lo[5], K.rstep[5]);
     bool in = true;
 #pragma unroll
-    for (int k = 0; k < 6; ++k) in = in & (f[k] >= 0.0f) & (f[k] <= K.lim[k]);
+    for (int k = 0; k < 6; ++k) {
+        if (k >= 3 && K.vclamp) {
+            // velocity_clamp: a finite velocity off the grid reads the edge
+            in = in & finite_f(f[k]);
+            f[k] = f[k] > K.lim[k] ? K.lim[k] : (f[k] < 0.0f ? 0.0f : f[k]);
+        } else {
+            in = in & (f[k] >= 0.0f) & (f[k] <= K.lim[k]);
+        }
+    }
     return in;
 }
 

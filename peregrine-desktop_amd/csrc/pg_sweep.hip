@@ -38,7 +38,14 @@ __device__ __forceinline__ float interp_global(const Consts& K, const float* __r
     f[5] = qdiv(w - K.lo[5], K.rstep[5]);
     bool in = true;
 #pragma unroll
-    for (int k = 0; k < 6; ++k) in = in & (f[k] >= 0.0f) & (f[k] <= K.lim[k]);
+    for (int k = 0; k < 6; ++k) {
+        if (k >= 3 && K.vclamp) {
+            in = in & finite_f(f[k]);
+            f[k] = f[k] > K.lim[k] ? K.lim[k] : (f[k] < 0.0f ? 0.0f : f[k]);
+        } else {
+            in = in & (f[k] >= 0.0f) & (f[k] <= K.lim[k]);
+        }
+    }
     if (!in) return K.cap;
     float t[6], u[6];
     int64_t o0[6], o1[6];
@@ -61,6 +68,25 @@ __device__ __forceinline__ float interp_global(const Consts& K, const float* __r
         o0[k] = (int64_t)j0 * K.stride[k];
         o1[k] = (int64_t)j1 * K.stride[k];
         if (k < 3) { jc[k][0] = j0; jc[k][1] = j1; }
+    }
+    if (K.nearest) {
+        // the node nearest the successor (PeregrineSolver.jl:424-428): per
+        // axis the lower or upper corner, as rint(f) falls
+        int64_t a = 0;
+        int c = 0;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const bool up = __builtin_rintf(f[k]) != __builtin_floorf(f[k]);
+            a += up ? o1[k] : o0[k];
+            if (k < 3) c |= (up ? 1 : 0) << k;
+        }
+        float v = V[a];
+        if (ESC) {
+            const bool wall = !ignore && occ[((int64_t)jc[0][c & 1] * K.n[1] + jc[1][(c >> 1) & 1]) * K.n[2] +
+                                             jc[2][(c >> 2) & 1]] != 0;
+            v = wall ? K.cap : (v < 0.0f ? -v : (v < K.cap ? 0.0f : K.cap));
+        }
+        return v;
     }
     // weights: p[d] for the first five axes, built level by level so that
     // every weight is ((((w1*w2)*w3)*w4)*w5)*w6 in the reference's order
@@ -184,7 +210,10 @@ __global__ __launch_bounds__(256) void sweep_global_kernel(
             const float vy = -sh0 * vfx + ch0 * vfy;
             const OccGlobal og{occ, K.n[1], K.n[2], ESC && blocked};
             best = lookahead_global<ESC>(K, Vin, og, x, y, h, vx, vy, w, bu[0], bu[1], bu[2]);
-            for (int k = 1; k < K.nctl; ++k) {
+            // coast, then the controls in scan order (the whole lattice, or
+            // this sweep's rotating slice)
+            for (int r = 1; r <= K.span; ++r) {
+                const int k = slice_control(K, r);
                 const float c = lookahead_global<ESC>(K, Vin, og, x, y, h, vx, vy, w, bu[3 * k],
                                                       bu[3 * k + 1], bu[3 * k + 2]);
                 if (c < best) { best = c; bk = k; }

@@ -327,6 +327,30 @@ __device__ __forceinline__ void fetch_src(const Consts& K, const SRC& src, const
     }
 }
 
+// The value at the grid node nearest the successor (nearest_cell lookups,
+// PeregrineSolver.jl:424-428): per axis the lower or the upper corner of the
+// successor's cell as rint(f) falls, from the window or through src.
+template <class SRC, class W>
+__device__ __forceinline__ float nearest_value(const Consts& K, const SRC& src, const float* win, W,
+                                               const int l[6], const int i[6], const float f[6], bool inwin) {
+    int c[6];
+    int off = 0;
+#pragma unroll
+    for (int a = 0; a < 6; ++a) {
+        const int up = __builtin_rintf(f[a]) != __builtin_floorf(f[a]) ? 1 : 0;
+        off += (l[a] + up) * W::S[a];
+        int j = i[a] + up;
+        if (a == 2) {
+            j = j >= K.n[2] ? j - K.n[2] : j;
+            j = j >= K.n[2] ? j - K.n[2] : j;
+        } else {
+            j = j < K.n[a] - 1 ? j : K.n[a] - 1;
+        }
+        c[a] = j;
+    }
+    return inwin ? win[off] : src.read(K, c);
+}
+
 // ---------------------------------------------------------------------------
 // Sweep
 // ---------------------------------------------------------------------------
@@ -363,15 +387,19 @@ __global__ __launch_bounds__(256, 4) void sweep_lds_kernel(
     if (live) {
         const OccG og{occ, K.n[1], K.n[2]};
         const int ns = B.ns;
-        const int total = ns * K.nctl;
-        int s = threadIdx.x % ns, k = threadIdx.x / ns;
-        const int ds = kThreads % ns, dk = kThreads / ns;
+        // pairs (state, scan position r): coast, then the lattice (or this
+        // sweep's slice of it) in scan order; the minimum key keeps the
+        // earliest position among equal values, as the strict `<` scan does
+        const int total = ns * (K.span + 1);
+        int s = threadIdx.x % ns, r = threadIdx.x / ns;
+        const int ds = kThreads % ns, dr = kThreads / ns;
         for (int p = threadIdx.x; p < total; p += kThreads) {
             if (!sm.blocked[s]) {
                 CellStart c;
                 load_cell_start(sm, s, c);
                 float f[6];
                 float val = K.cap;
+                const int k = slice_control(K, r);
                 if (successor_t<DIAG, RK2>(K, og, c, sm.bu[3 * k], sm.bu[3 * k + 1], sm.bu[3 * k + 2], f)) {
                     float t[6], u[6];
                     int i[6];
@@ -382,8 +410,6 @@ __global__ __launch_bounds__(256, 4) void sweep_lds_kernel(
                         u[a] = 1.0f - t[a];
                         i[a] = (int)b;
                     }
-                    Weights Wt;
-                    corner_weights(t, u, Wt);
                     int l[6];
                     bool inwin = true;
 #pragma unroll
@@ -396,6 +422,12 @@ __global__ __launch_bounds__(256, 4) void sweep_lds_kernel(
                         l[a] = d;
                         inwin = inwin & ((unsigned)d <= (unsigned)(W::E[a] - 2));
                     }
+                    float acc;
+                    if (K.nearest) {
+                        acc = nearest_value(K, src, win, W{}, l, i, f, inwin);
+                    } else {
+                    Weights Wt;
+                    corner_weights(t, u, Wt);
                     float vals[64];
                     if (inwin) {
                         int base = 0;
@@ -412,17 +444,18 @@ __global__ __launch_bounds__(256, 4) void sweep_lds_kernel(
                     } else {
                         fetch_src(K, src, i, vals);
                     }
-                    const float acc = corner_sum(Wt, vals);
+                    acc = corner_sum(Wt, vals);
+                    }
                     const float nv = (acc < K.cap) ? acc : K.cap;
                     const float cc = K.tau + nv;
                     val = (cc < K.cap) ? cc : K.cap;
                 }
                 const unsigned long long key =
-                    ((unsigned long long)__float_as_uint(val) << 32) | (unsigned)k;
+                    ((unsigned long long)__float_as_uint(val) << 32) | (unsigned)r;
                 atomicMin(&sm.best[s], key);
             }
-            s += ds; k += dk;
-            if (s >= ns) { s -= ns; ++k; }
+            s += ds; r += dr;
+            if (s >= ns) { s -= ns; ++r; }
         }
     }
     __syncthreads();
@@ -434,7 +467,7 @@ __global__ __launch_bounds__(256, 4) void sweep_lds_kernel(
             state_coords(B, s, i);
             const unsigned long long key = sm.best[s];
             const float best = __uint_as_float((unsigned)(key >> 32));
-            const int bk = (int)(key & 0xffffffffu);
+            const int bk = slice_control(K, (int)(key & 0xffffffffu));
             float out = 0.0f;
             if (WRITE_V) {
                 const float old = src.read(K, i);
