@@ -132,10 +132,10 @@ int check_grid(const pg_grid* g) {
 }
 
 int check_params(const pg_params* p, int32_t nctl) {
-    if (p->cfl > 0.0f) return fail(PG_ERR_UNSUPPORTED, "cfl > 0 (per-cell horizon) is not built");
+    if (p->cfl > 0.0f && !(p->tau_min > 0.0f && p->tau_min <= p->tau_max))
+        return fail(PG_ERR_ARG, "cfl > 0 needs 0 < tau_min <= tau_max");
     if (p->ntau > 1) return fail(PG_ERR_UNSUPPORTED, "tau_levels > 1 (horizon ladder) is not built");
     if (p->rounds > 0) return fail(PG_ERR_UNSUPPORTED, "refine_rounds > 0 (pattern search) is not built");
-    if (p->simplex) return fail(PG_ERR_UNSUPPORTED, "simplex (Kuhn) interpolation is not built");
     if (p->checks < 1 || p->checks > 24) return fail(PG_ERR_ARG, "sweep_checks must be in [1, 24], got %d", p->checks);
     if (!(p->dt > 0.0f) || !std::isfinite(p->dt)) return fail(PG_ERR_ARG, "dt must be positive");
     if (!(p->cap > 0.0f) || !std::isfinite(p->cap)) return fail(PG_ERR_ARG, "value_cap must be positive");
@@ -180,7 +180,34 @@ void build_consts(pg_solver* s) {
     for (int sc = 1; sc <= K.nchk; ++sc) K.probe_a[sc - 1] = (float)sc / (float)K.nchk;
     K.nctl = s->nctl;
     K.rk2 = s->p.rk2 ? 1 : 0;
-    K.nearest = s->p.nearest ? 1 : 0;
+    // Kuhn lookups take precedence over nearest-node ones (value_at,
+    // PeregrineSolver.jl:1141-1146)
+    K.simplex = s->p.simplex ? 1 : 0;
+    K.nearest = (s->p.nearest && !s->p.simplex) ? 1 : 0;
+    // per-cell horizon: the bound from tau_max and the acceleration rows,
+    // folded here in float as cfl_tau does (PeregrineSolver.jl:954-976)
+    K.cfl = s->p.cfl > 0.0f ? 1 : 0;
+    {
+        const float* B = s->m.B;
+        auto amax3 = [](float a, float b, float c) {
+            const float ab = a > b ? a : b;
+            return ab > c ? ab : c;
+        };
+        const float ax = amax3(std::fabs(B[0]), std::fabs(B[1]), std::fabs(B[2]));
+        const float ay = amax3(std::fabs(B[3]), std::fabs(B[4]), std::fabs(B[5]));
+        const float aw = amax3(std::fabs(B[6]), std::fabs(B[7]), std::fabs(B[8]));
+        float t = s->p.tau_max;
+        const float cfl = s->p.cfl;
+        if (ax > 1.0e-6f) { const float c = cfl * g.step[3] / ax; t = t < c ? t : c; }
+        if (ay > 1.0e-6f) { const float c = cfl * g.step[4] / ay; t = t < c ? t : c; }
+        if (aw > 1.0e-6f) { const float c = cfl * g.step[5] / aw; t = t < c ? t : c; }
+        K.cfl_tb = t;
+        K.cfl_xy = cfl * (g.step[0] < g.step[1] ? g.step[0] : g.step[1]);
+        K.cfl_h = cfl * g.step[2];
+        K.tau_min = s->p.tau_min;
+        K.tau_max = s->p.tau_max;
+        K.hmax = s->p.hmax;
+    }
     K.vclamp = s->p.vclamp ? 1 : 0;
     K.adaptive = s->p.adaptive_checks ? 1 : 0;
     K.half_xy = 0.5f * (g.step[0] < g.step[1] ? g.step[0] : g.step[1]);
@@ -342,6 +369,8 @@ const char* exact_only_option(const pg_solver* s) {
     if (s->p.nearest) return "nearest";
     if (s->p.vclamp) return "velocity_clamp";
     if (s->p.adaptive_checks) return "adaptive_checks";
+    if (s->p.simplex) return "simplex";
+    if (s->p.cfl > 0.0f) return "cfl";
     if (s->K.span < s->K.nnc) return "control_scan (rotating slice)";
     return nullptr;
 }
@@ -498,10 +527,11 @@ int create_solver(pg_ctx* ctx, const pg_grid* g, const pg_model* m, const pg_par
     if (vstore != PG_VSTORE_F32 && vstore != PG_VSTORE_F16_TILE && vstore != PG_VSTORE_U16_TILE)
         return fail(PG_ERR_ARG, "unknown vstore %d", vstore);
     if (vstore != PG_VSTORE_F32 &&
-        (p->rk2 || p->nearest || p->vclamp || p->adaptive_checks || (p->ncoarse > 0 && p->ncoarse < nctl - 1)))
+        (p->rk2 || p->nearest || p->vclamp || p->adaptive_checks || p->simplex || p->cfl > 0.0f ||
+         (p->ncoarse > 0 && p->ncoarse < nctl - 1)))
         return fail(PG_ERR_UNSUPPORTED,
                     "the 16-bit stores (precision form) run the plain default scheme only (no rk2, nearest, "
-                    "velocity_clamp, adaptive_checks or control_scan slice)");
+                    "velocity_clamp, adaptive_checks, simplex, cfl or control_scan slice)");
     if (vstore != PG_VSTORE_F32 && nctl > kLdsMaxCtl)
         return fail(PG_ERR_UNSUPPORTED, "the 16-bit stores sweep at most %d controls", kLdsMaxCtl);
     for (int k = 0; k < 3; ++k)

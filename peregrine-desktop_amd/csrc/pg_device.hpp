@@ -45,6 +45,12 @@ struct Consts {
     // the rotating slice of the control lattice (ncoarse > 0): controls
     // ((off + j) mod nnc) + 1 for j < span after coast; off from the sweep's phase
     int32_t nnc, span, off;
+    int32_t simplex;   // Kuhn (7-vertex simplex) lookups instead of 64-corner multilinear
+    // per-cell horizon (cfl > 0, PeregrineSolver.jl:954-989): tau from the
+    // cell's speed and spin, bounded by cfl_tb (tau_max and the acceleration
+    // terms, host-folded); substeps from tau / hmax
+    int32_t cfl;
+    float cfl_tb, cfl_xy, cfl_h, tau_min, tau_max, hmax;
     float probe_a[24]; // Float32(sc) / Float32(nchk), IEEE
 };
 
@@ -108,12 +114,11 @@ __device__ __forceinline__ bool finite_f(float v) {
 // Returns false if the integration diverged (the reference returns a NaN
 // state, PeregrineSolver.jl:191-192,213).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ bool euler(const Consts& K, float bux, float buy, float buw,
+__device__ __forceinline__ bool euler(const Consts& K, float hs, int nsub, float bux, float buy, float buw,
                                       float& x, float& y, float& h, float& vx, float& vy,
                                       float& w) {
     bool ok = true;
-    const float hs = K.hs;
-    for (int k = 0; k < K.nsub; ++k) {
+    for (int k = 0; k < nsub; ++k) {
         float sum = x + y;
         sum = sum + h;
         sum = sum + vx;
@@ -253,7 +258,37 @@ struct CellStart {
     float x, y, h, vx, vy, w;
     float sh, ch;
     float sx, sy, sw, dx, dy, dw, w2;
+    float tau, hs;     // the cell's horizon and substep
+    int32_t nsub;
 };
+
+// cfl_tau (PeregrineSolver.jl:954-980) of a cell with field velocity (vfx,
+// vfy) and spin w: the host-folded bound, then the speed and spin terms,
+// clamped to [tau_min, tau_max].
+__device__ __forceinline__ float cfl_tau_dev(const Consts& K, float vfx, float vfy, float w) {
+    float t = K.cfl_tb;
+    const float sp = sqrtf(vfx * vfx + vfy * vfy);
+    if (sp > 1.0e-3f) {
+        const float c = K.cfl_xy / sp;
+        t = t < c ? t : c;
+    }
+    const float aw = __builtin_fabsf(w);
+    if (aw > 1.0e-4f) {
+        const float c = K.cfl_h / aw;
+        t = t < c ? t : c;
+    }
+    return t > K.tau_max ? K.tau_max : (t < K.tau_min ? K.tau_min : t);
+}
+
+// substeps_for (PeregrineSolver.jl:985-989) of a horizon.
+__device__ __forceinline__ int substeps_dev(const Consts& K, float tau) {
+    if (K.hmax <= 0.0f) return K.nsub;
+    float q = tau / K.hmax;
+    q = q < 1.0e4f ? q : 1.0e4f;
+    int k = (int)q + 1;
+    k = k < 64 ? k : 64;
+    return k > 1 ? k : 1;
+}
 
 __device__ __forceinline__ void cell_start(const Consts& K, int i0, int i1, int i2, int i3, int i4,
                                            int i5, CellStart& c) {
@@ -273,6 +308,54 @@ __device__ __forceinline__ void cell_start(const Consts& K, int i0, int i1, int 
     c.dx = __builtin_fabsf(c.vx) * c.vx;
     c.dy = __builtin_fabsf(c.vy) * c.vy;
     c.dw = __builtin_fabsf(c.w) * c.w;
+    if (K.cfl) {
+        c.tau = cfl_tau_dev(K, vfx, vfy, c.w);
+        c.nsub = substeps_dev(K, c.tau);
+        c.hs = c.tau / (float)c.nsub;
+    } else {
+        c.tau = K.tau;
+        c.nsub = K.nsub;
+        c.hs = K.hs;
+    }
+}
+
+// Kuhn / Freudenthal simplex interpolation (PeregrineSolver.jl:571-665): the
+// fractional parts sorted, largest first, by the reference's fixed network of
+// 12 comparators (:475-489); the simplex's 7 vertices are walked from the
+// cell's base corner adding one axis at a time, weights (1 - s1), s1 - s2,
+// ..., s6, summed in that order. corner(mask) reads the vertex whose axes in
+// mask (bit a = axis a) take their upper corner.
+__device__ __forceinline__ void kuhn_cmpx(float& ta, int& ka, float& tb, int& kb) {
+    if (!(ta >= tb)) {
+        const float t = ta; ta = tb; tb = t;
+        const int k = ka; ka = kb; kb = k;
+    }
+}
+
+template <class Corner>
+__device__ __forceinline__ float kuhn_interp(const float f[6], const Corner& corner) {
+    float s[6];
+    int a[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        s[k] = f[k] - __builtin_floorf(f[k]);
+        a[k] = k;
+    }
+    kuhn_cmpx(s[0], a[0], s[5], a[5]); kuhn_cmpx(s[1], a[1], s[3], a[3]); kuhn_cmpx(s[2], a[2], s[4], a[4]);
+    kuhn_cmpx(s[1], a[1], s[2], a[2]); kuhn_cmpx(s[3], a[3], s[4], a[4]); kuhn_cmpx(s[0], a[0], s[3], a[3]);
+    kuhn_cmpx(s[2], a[2], s[5], a[5]); kuhn_cmpx(s[0], a[0], s[1], a[1]); kuhn_cmpx(s[2], a[2], s[3], a[3]);
+    kuhn_cmpx(s[4], a[4], s[5], a[5]); kuhn_cmpx(s[1], a[1], s[2], a[2]); kuhn_cmpx(s[3], a[3], s[4], a[4]);
+    const float v0 = corner(0);
+    float acc = (1.0f - s[0]) * v0;
+    int mask = 0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        mask |= 1 << a[k];
+        const float v = corner(mask);
+        const float wk = (k < 5) ? (s[k] - s[k + 1]) : s[5];
+        acc = acc + wk * v;
+    }
+    return acc;
 }
 
 // Forward Euler from a cell start: substep 1 reuses the cell's state-only
@@ -282,7 +365,7 @@ template <bool DIAG>
 __device__ __forceinline__ bool euler_from(const Consts& K, const CellStart& c, float bux, float buy,
                                            float buw, float& x, float& y, float& h, float& vx,
                                            float& vy, float& w) {
-    const float hs = K.hs;
+    const float hs = c.hs;
     bool ok = true;
     {
         float ax, ay, al;
@@ -299,7 +382,7 @@ __device__ __forceinline__ bool euler_from(const Consts& K, const CellStart& c, 
         vy = c.vy + dvy * hs;
         w = c.w + al * hs;
     }
-    for (int k = 1; k < K.nsub; ++k) {
+    for (int k = 1; k < c.nsub; ++k) {
         float sum = x + y;
         sum = sum + h;
         sum = sum + vx;
@@ -373,11 +456,11 @@ __device__ __forceinline__ void state_deriv(const Consts& K, float bux, float bu
 template <bool DIAG>
 __device__ __forceinline__ bool rk2_from(const Consts& K, const CellStart& c, float bux, float buy, float buw,
                                          float& x, float& y, float& h, float& vx, float& vy, float& w) {
-    const float hs = K.hs;
+    const float hs = c.hs;
     const float hh = 0.5f * hs;
     float s[6] = {c.x, c.y, c.h, c.vx, c.vy, c.w};
     bool ok = true;
-    for (int k = 0; k < K.nsub; ++k) {
+    for (int k = 0; k < c.nsub; ++k) {
         float d[6];
         if (k == 0) {
             float ax, ay, al;
@@ -406,13 +489,12 @@ __device__ __forceinline__ bool rk2_from(const Consts& K, const CellStart& c, fl
 }
 
 // The same from an arbitrary start state (the global-gather kernel).
-__device__ __forceinline__ bool rk2_state(const Consts& K, float bux, float buy, float buw, float& x, float& y,
-                                          float& h, float& vx, float& vy, float& w) {
-    const float hs = K.hs;
+__device__ __forceinline__ bool rk2_state(const Consts& K, float hs, int nsub, float bux, float buy, float buw,
+                                          float& x, float& y, float& h, float& vx, float& vy, float& w) {
     const float hh = 0.5f * hs;
     float s[6] = {x, y, h, vx, vy, w};
     bool ok = true;
-    for (int k = 0; k < K.nsub; ++k) {
+    for (int k = 0; k < nsub; ++k) {
         float d[6];
         ok = ok && state_finite(s);
         state_deriv<false>(K, bux, buy, buw, s, d);

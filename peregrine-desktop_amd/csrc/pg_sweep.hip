@@ -88,6 +88,26 @@ __device__ __forceinline__ float interp_global(const Consts& K, const float* __r
         }
         return v;
     }
+    if (K.simplex) {
+        const float acc = kuhn_interp(f, [&](int mask) {
+            int64_t a = 0;
+            int c = 0;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                const bool up = (mask >> k) & 1;
+                a += up ? o1[k] : o0[k];
+                if (k < 3) c |= (up ? 1 : 0) << k;
+            }
+            float v = V[a];
+            if (ESC) {
+                const bool wall = !ignore && occ[((int64_t)jc[0][c & 1] * K.n[1] + jc[1][(c >> 1) & 1]) * K.n[2] +
+                                                 jc[2][(c >> 2) & 1]] != 0;
+                v = wall ? K.cap : (v < 0.0f ? -v : (v < K.cap ? 0.0f : K.cap));
+            }
+            return v;
+        });
+        return (acc < K.cap) ? acc : K.cap;
+    }
     // weights: p[d] for the first five axes, built level by level so that
     // every weight is ((((w1*w2)*w3)*w4)*w5)*w6 in the reference's order
     float p2[4], p3[8], p4[16], p5[32];
@@ -151,12 +171,13 @@ struct OccGlobal {
 // lookahead (PeregrineSolver.jl:1100-1153) for one control.
 template <bool ESC>
 __device__ __forceinline__ float lookahead_global(const Consts& K, const float* __restrict__ V,
-                                                  const OccGlobal& occ, float x, float y,
+                                                  const OccGlobal& occ, float tau, float hs, int nsub,
+                                                  float x, float y,
                                                   float h, float vx, float vy, float w,
                                                   float bux, float buy, float buw) {
     float nx = x, ny = y, nh = h, nvx = vx, nvy = vy, nw = w;
-    const bool ok = K.rk2 ? rk2_state(K, bux, buy, buw, nx, ny, nh, nvx, nvy, nw)
-                          : euler(K, bux, buy, buw, nx, ny, nh, nvx, nvy, nw);
+    const bool ok = K.rk2 ? rk2_state(K, hs, nsub, bux, buy, buw, nx, ny, nh, nvx, nvy, nw)
+                          : euler(K, hs, nsub, bux, buy, buw, nx, ny, nh, nvx, nvy, nw);
     const float dx = nx - x, dy = ny - y, dh = nh - h;
     if (!(ok & finite_f(dx) & finite_f(dy) & finite_f(dh) & finite_f(nvx) & finite_f(nvy) &
           finite_f(nw)))
@@ -167,7 +188,7 @@ __device__ __forceinline__ float lookahead_global(const Consts& K, const float* 
     const float nvfx = cnh * nvx - snh * nvy;
     const float nvfy = snh * nvx + cnh * nvy;
     const float nv = interp_global<ESC>(K, V, occ.occ, occ.ignore, nx, ny, nh, nvfx, nvfy, nw);
-    const float c = K.tau + nv;
+    const float c = tau + nv;
     return (c < K.cap) ? c : K.cap;
 }
 
@@ -209,12 +230,20 @@ __global__ __launch_bounds__(256) void sweep_global_kernel(
             const float vx = ch0 * vfx + sh0 * vfy;
             const float vy = -sh0 * vfx + ch0 * vfy;
             const OccGlobal og{occ, K.n[1], K.n[2], ESC && blocked};
-            best = lookahead_global<ESC>(K, Vin, og, x, y, h, vx, vy, w, bu[0], bu[1], bu[2]);
+            // the cell's horizon: dt, or from its speed and spin (cfl > 0)
+            float tau = K.tau, hs = K.hs;
+            int nsub = K.nsub;
+            if (K.cfl) {
+                tau = cfl_tau_dev(K, vfx, vfy, w);
+                nsub = substeps_dev(K, tau);
+                hs = tau / (float)nsub;
+            }
+            best = lookahead_global<ESC>(K, Vin, og, tau, hs, nsub, x, y, h, vx, vy, w, bu[0], bu[1], bu[2]);
             // coast, then the controls in scan order (the whole lattice, or
             // this sweep's rotating slice)
             for (int r = 1; r <= K.span; ++r) {
                 const int k = slice_control(K, r);
-                const float c = lookahead_global<ESC>(K, Vin, og, x, y, h, vx, vy, w, bu[3 * k],
+                const float c = lookahead_global<ESC>(K, Vin, og, tau, hs, nsub, x, y, h, vx, vy, w, bu[3 * k],
                                                       bu[3 * k + 1], bu[3 * k + 2]);
                 if (c < best) { best = c; bk = k; }
             }

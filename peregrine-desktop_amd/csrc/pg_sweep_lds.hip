@@ -21,16 +21,23 @@ template <int MAXS>
 struct BoxShared {
     float x[MAXS], y[MAXS], h[MAXS], vx[MAXS], vy[MAXS], w[MAXS];
     float sh[MAXS], ch[MAXS], sx[MAXS], sy[MAXS], sw[MAXS], dx[MAXS], dy[MAXS], dw[MAXS], w2[MAXS];
+    float tau[MAXS], hs[MAXS];   // per-cell horizon (cfl > 0)
     unsigned long long best[MAXS];
     uint8_t blocked[MAXS];
+    uint8_t nsub[MAXS];
     float bu[3 * kMaxCtlLds];
 };
 
 template <int MAXS>
-__device__ __forceinline__ void load_cell_start(const BoxShared<MAXS>& sm, int s, CellStart& c) {
+__device__ __forceinline__ void load_cell_start(const Consts& K, const BoxShared<MAXS>& sm, int s, CellStart& c) {
     c.x = sm.x[s]; c.y = sm.y[s]; c.h = sm.h[s]; c.vx = sm.vx[s]; c.vy = sm.vy[s]; c.w = sm.w[s];
     c.sh = sm.sh[s]; c.ch = sm.ch[s]; c.sx = sm.sx[s]; c.sy = sm.sy[s]; c.sw = sm.sw[s];
     c.dx = sm.dx[s]; c.dy = sm.dy[s]; c.dw = sm.dw[s]; c.w2 = sm.w2[s];
+    if (K.cfl) {
+        c.tau = sm.tau[s]; c.hs = sm.hs[s]; c.nsub = sm.nsub[s];
+    } else {
+        c.tau = K.tau; c.hs = K.hs; c.nsub = K.nsub;
+    }
 }
 
 template <int MAXS>
@@ -45,6 +52,7 @@ __device__ __forceinline__ void setup_box(const Consts& K, const BoxInfo& B,
         sm.x[s] = c.x; sm.y[s] = c.y; sm.h[s] = c.h; sm.vx[s] = c.vx; sm.vy[s] = c.vy; sm.w[s] = c.w;
         sm.sh[s] = c.sh; sm.ch[s] = c.ch; sm.sx[s] = c.sx; sm.sy[s] = c.sy; sm.sw[s] = c.sw;
         sm.dx[s] = c.dx; sm.dy[s] = c.dy; sm.dw[s] = c.dw; sm.w2[s] = c.w2;
+        sm.tau[s] = c.tau; sm.hs[s] = c.hs; sm.nsub[s] = (uint8_t)c.nsub;
         const bool blk = occ[((int64_t)i[0] * K.n[1] + i[1]) * K.n[2] + i[2]] != 0;
         sm.blocked[s] = blk ? 1 : 0;
         // blocked cells keep (cap, coast) (PeregrineSolver.jl:1226-1228)
@@ -119,7 +127,7 @@ __global__ __launch_bounds__(256) void plan_kernel(const uint8_t* __restrict__ o
         for (int p = threadIdx.x; p < total; p += kThreads) {
             if (!sm.blocked[s]) {
                 CellStart c;
-                load_cell_start(sm, s, c);
+                load_cell_start(K, sm, s, c);
                 FastStart S;
                 fast_start<DIAG>(K, F, c, S);
                 float g[3];
@@ -173,7 +181,7 @@ __global__ __launch_bounds__(256) void plan_kernel(const uint8_t* __restrict__ o
     for (int p = threadIdx.x; p < total; p += kThreads) {
         if (!sm.blocked[s]) {
             CellStart c;
-            load_cell_start(sm, s, c);
+            load_cell_start(K, sm, s, c);
             float f[6];
             bool interp;
             if constexpr (FAST) {
@@ -351,6 +359,37 @@ __device__ __forceinline__ float nearest_value(const Consts& K, const SRC& src, 
     return inwin ? win[off] : src.read(K, c);
 }
 
+// The Kuhn simplex interpolant of the successor (simplex lookups): its 7
+// vertices from the window or through src.
+template <class SRC, class W>
+__device__ __forceinline__ float kuhn_value(const Consts& K, const SRC& src, const float* win, W, const int l[6],
+                                            const int i[6], const float f[6], bool inwin) {
+    int base = 0;
+#pragma unroll
+    for (int a = 0; a < 6; ++a) base += l[a] * W::S[a];
+    return kuhn_interp(f, [&](int mask) {
+        if (inwin) {
+            int off = base;
+#pragma unroll
+            for (int a = 0; a < 6; ++a) off += ((mask >> a) & 1) * W::S[a];
+            return win[off];
+        }
+        int c[6];
+#pragma unroll
+        for (int a = 0; a < 6; ++a) {
+            int j = i[a] + ((mask >> a) & 1);
+            if (a == 2) {
+                j = j >= K.n[2] ? j - K.n[2] : j;
+                j = j >= K.n[2] ? j - K.n[2] : j;
+            } else {
+                j = j < K.n[a] - 1 ? j : K.n[a] - 1;
+            }
+            c[a] = j;
+        }
+        return src.read(K, c);
+    });
+}
+
 // ---------------------------------------------------------------------------
 // Sweep
 // ---------------------------------------------------------------------------
@@ -396,7 +435,7 @@ __global__ __launch_bounds__(256, 4) void sweep_lds_kernel(
         for (int p = threadIdx.x; p < total; p += kThreads) {
             if (!sm.blocked[s]) {
                 CellStart c;
-                load_cell_start(sm, s, c);
+                load_cell_start(K, sm, s, c);
                 float f[6];
                 float val = K.cap;
                 const int k = slice_control(K, r);
@@ -425,6 +464,8 @@ __global__ __launch_bounds__(256, 4) void sweep_lds_kernel(
                     float acc;
                     if (K.nearest) {
                         acc = nearest_value(K, src, win, W{}, l, i, f, inwin);
+                    } else if (K.simplex) {
+                        acc = kuhn_value(K, src, win, W{}, l, i, f, inwin);
                     } else {
                     Weights Wt;
                     corner_weights(t, u, Wt);
@@ -447,7 +488,7 @@ __global__ __launch_bounds__(256, 4) void sweep_lds_kernel(
                     acc = corner_sum(Wt, vals);
                     }
                     const float nv = (acc < K.cap) ? acc : K.cap;
-                    const float cc = K.tau + nv;
+                    const float cc = c.tau + nv;
                     val = (cc < K.cap) ? cc : K.cap;
                 }
                 const unsigned long long key =

@@ -1,7 +1,7 @@
-"""The plain scheme's options on the device against the CPU oracle:
-nearest-node lookups, velocity_clamp, adaptive probe counts and the
-rotating control slice (control_scan), alone and together, in both exact
-kernels. The bar is the parity bar: every V value equal, sweep by sweep,
+"""The scheme's options on the device against the CPU oracle: nearest-node
+lookups, velocity_clamp, adaptive probe counts, the rotating control slice
+(control_scan), Kuhn simplex lookups and the per-cell CFL horizon, alone and
+together, in both exact kernels. The bar is the parity bar: every V value equal, sweep by sweep,
 each sweep at its own phase (the slice rotates with it).
 """
 import dataclasses
@@ -20,6 +20,9 @@ OPTIONS = {
     "adaptive": dict(adaptive_checks=True),
     "slice": dict(ncoarse=24),
     "all": dict(nearest=True, vclamp=True, adaptive_checks=True, ncoarse=24, rk2=True),
+    "simplex": dict(simplex=True),
+    "cfl": dict(cfl=0.5),
+    "production": dict(simplex=True, cfl=0.5, rk2=True, adaptive_checks=True, ncoarse=24, vclamp=True),
 }
 
 

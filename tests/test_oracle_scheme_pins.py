@@ -74,3 +74,30 @@ def test_rotating_slice_covers_the_lattice_in_a_turn(pg, orc):
         got = [part.cell_update(V, int(idx), phase=ph)[0] for ph in range(6)]
         assert min(got) == want
         assert all(v >= want for v in got)
+
+
+def test_cfl_horizon_takes_the_tightest_term(pg, orc):
+    """cfl_tau: tau_max, cfl * step / |B| per velocity axis, cfl * min xy step
+    / speed and cfl * heading step / |w|, the smallest, clamped to
+    [tau_min, tau_max]. On a grid with 10 cm x/y steps, 50 cm/s velocity
+    steps and a 0.5 rad heading step, B = diag(1000, 1000, 100), cfl 0.5:
+    the acceleration terms give 0.025 s, a 400 cm/s speed 0.0125 s, a 10 rad/s
+    spin 0.5 * 2pi/13 / 10 = 0.0242 s."""
+    L = orc.lib()
+    g = pg.Grid6.build((11, 11, 13, 9, 9, 9), (0, 0, -math.pi, -200, -200, -20), (100, 100, math.pi, 200, 200, 20))
+    assert g.step[0] == pytest.approx(10) and g.step[3] == pytest.approx(50)
+    m = pg.Model((1000, 0, 0, 0, 1000, 0, 0, 0, 100), (0.0,) * 9, (0, 0, 0), (0.0,) * 9, (0.0,) * 9, (0, 0, 0),
+                 (5, 5, 0.15), 0.0).c_struct()
+    p = dataclasses.replace(pg.Params.plain(), cfl=0.5, tau_min=0.004, tau_max=0.5)
+    pc = p.c()
+    gc = g.c()
+    tau = lambda vx, vy, w: L.orc_cfl_tau(ctypes.byref(gc), ctypes.byref(m), vx, vy, w, ctypes.byref(pc))
+    h_step = float(np.float32(g.step[2]))
+    assert tau(0.0, 0.0, 0.0) == pytest.approx(0.025, rel=1e-6)             # 0.5 * 50 / 1000
+    assert tau(400.0, 0.0, 0.0) == pytest.approx(0.0125, rel=1e-6)          # 0.5 * 10 / 400
+    assert tau(0.0, 0.0, 10.0) == pytest.approx(0.5 * h_step / 10.0, rel=1e-6)
+    assert tau(0.0, 0.0, 100.0) == pytest.approx(0.004)                   # below tau_min
+    slow = dataclasses.replace(p, tau_max=0.01).c()
+    assert L.orc_cfl_tau(ctypes.byref(gc), ctypes.byref(m), 0.0, 0.0, 0.0, ctypes.byref(slow)) == pytest.approx(0.01)
+    floor = dataclasses.replace(p, tau_min=0.02).c()
+    assert L.orc_cfl_tau(ctypes.byref(gc), ctypes.byref(m), 400.0, 0.0, 0.0, ctypes.byref(floor)) == pytest.approx(0.02)
