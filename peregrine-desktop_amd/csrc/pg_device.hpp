@@ -175,11 +175,14 @@ __device__ __forceinline__ float wrap_h(const Consts& K, float f) {
 // Returns true if every probe is on the field and in free space. A wrapped
 // heading outside [0, n3] (only reachable from a non-finite or astronomically
 // large heading, where the reference raises) is rejected.
-template <class OccRead>
+// GEN = false compiles the default scheme only (fixed probe count); the
+// kernels pick it when no option is set.
+template <class OccRead, bool GEN = true>
 __device__ __forceinline__ bool probes_free(const Consts& K, const OccRead& occ, float x,
                                             float y, float h, float dx, float dy, float dh) {
     int nchk = K.nchk;
-    if (K.adaptive) {
+    const bool adaptive = GEN && K.adaptive;
+    if (adaptive) {
         // enough probes that none is more than half a cell from the next
         // (PeregrineSolver.jl:1127-1133)
         const float cells_pos = sqrtf(dx * dx + dy * dy) / K.half_xy;
@@ -192,7 +195,7 @@ __device__ __forceinline__ bool probes_free(const Consts& K, const OccRead& occ,
     }
     bool ok = true;
     for (int sc = 0; sc < nchk; ++sc) {
-        const float a = K.adaptive ? (float)(sc + 1) / (float)nchk : K.probe_a[sc];
+        const float a = adaptive ? (float)(sc + 1) / (float)nchk : K.probe_a[sc];
         const float px = x + dx * a;
         const float py = y + dy * a;
         const float ph = h + dh * a;
@@ -512,6 +515,7 @@ __device__ __forceinline__ bool rk2_state(const Consts& K, float hs, int nsub, f
 
 // Fractional coordinates of a successor, with the range tests of interp
 // (PeregrineSolver.jl:401-428). Returns false when the lookup rejects.
+template <bool GEN = true>
 __device__ __forceinline__ bool interp_coords(const Consts& K, float x, float y, float h, float vx,
                                               float vy, float w, float f[6]) {
     f[0] = qdiv(x - K.lo[0], K.rstep[0]);
@@ -523,7 +527,7 @@ __device__ __forceinline__ bool interp_coords(const Consts& K, float x, float y,
     bool in = true;
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
-        if (k >= 3 && K.vclamp) {
+        if (k >= 3 && GEN && K.vclamp) {
             // velocity_clamp: a finite velocity off the grid reads the edge
             in = in & finite_f(f[k]);
             f[k] = f[k] > K.lim[k] ? K.lim[k] : (f[k] < 0.0f ? 0.0f : f[k]);

@@ -28,12 +28,12 @@ struct BoxShared {
     float bu[3 * kMaxCtlLds];
 };
 
-template <int MAXS>
+template <int MAXS, bool GEN = true>
 __device__ __forceinline__ void load_cell_start(const Consts& K, const BoxShared<MAXS>& sm, int s, CellStart& c) {
     c.x = sm.x[s]; c.y = sm.y[s]; c.h = sm.h[s]; c.vx = sm.vx[s]; c.vy = sm.vy[s]; c.w = sm.w[s];
     c.sh = sm.sh[s]; c.ch = sm.ch[s]; c.sx = sm.sx[s]; c.sy = sm.sy[s]; c.sw = sm.sw[s];
     c.dx = sm.dx[s]; c.dy = sm.dy[s]; c.dw = sm.dw[s]; c.w2 = sm.w2[s];
-    if (K.cfl) {
+    if (GEN && K.cfl) {
         c.tau = sm.tau[s]; c.hs = sm.hs[s]; c.nsub = sm.nsub[s];
     } else {
         c.tau = K.tau; c.hs = K.hs; c.nsub = K.nsub;
@@ -62,33 +62,33 @@ __device__ __forceinline__ void setup_box(const Consts& K, const BoxInfo& B,
     for (int j = threadIdx.x; j < 3 * K.nctl; j += kThreads) sm.bu[j] = bu_g[j];
 }
 
-// One lookahead up to the successor's fractional coordinates, integrating
-// with explicit midpoint substeps (RK2) or forward Euler. Returns 0 if
-// rejected (value cap), 1 if the successor must be interpolated.
-template <bool DIAG, bool RK2>
+// One lookahead up to the successor's fractional coordinates. GEN: every
+// scheme option as K sets it (explicit midpoint substeps, adaptive probes,
+// velocity clamp); otherwise the default scheme, compiled without them.
+// Returns 0 if rejected (value cap), 1 if the successor must be interpolated.
+template <bool DIAG, bool GEN>
 __device__ __forceinline__ int successor_t(const Consts& K, const OccG& og, const CellStart& c,
                                            float bux, float buy, float buw, float f[6]) {
     float nx, ny, nh, nvx, nvy, nw;
-    const bool ok = RK2 ? rk2_from<DIAG>(K, c, bux, buy, buw, nx, ny, nh, nvx, nvy, nw)
-                        : euler_from<DIAG>(K, c, bux, buy, buw, nx, ny, nh, nvx, nvy, nw);
+    const bool ok = (GEN && K.rk2) ? rk2_from<DIAG>(K, c, bux, buy, buw, nx, ny, nh, nvx, nvy, nw)
+                                   : euler_from<DIAG>(K, c, bux, buy, buw, nx, ny, nh, nvx, nvy, nw);
     const float dx = nx - c.x, dy = ny - c.y, dh = nh - c.h;
     if (!(ok & finite_f(dx) & finite_f(dy) & finite_f(dh) & finite_f(nvx) & finite_f(nvy) &
           finite_f(nw)))
         return 0;
-    if (!probes_free(K, og, c.x, c.y, c.h, dx, dy, dh)) return 0;
+    if (!probes_free<OccG, GEN>(K, og, c.x, c.y, c.h, dx, dy, dh)) return 0;
     float snh, cnh;
     sincos_ref(nh, snh, cnh);
     const float nvfx = cnh * nvx - snh * nvy;
     const float nvfy = snh * nvx + cnh * nvy;
-    return interp_coords(K, nx, ny, nh, nvfx, nvfy, nw, f) ? 1 : 0;
+    return interp_coords<GEN>(K, nx, ny, nh, nvfx, nvfy, nw, f) ? 1 : 0;
 }
 
-// The same with the integrator chosen by K.rk2 at run time (the plan pass).
+// The same with every option as K sets it (the plan pass).
 template <bool DIAG>
 __device__ __forceinline__ int successor(const Consts& K, const OccG& og, const CellStart& c,
                                          float bux, float buy, float buw, float f[6]) {
-    return K.rk2 ? successor_t<DIAG, true>(K, og, c, bux, buy, buw, f)
-                 : successor_t<DIAG, false>(K, og, c, bux, buy, buw, f);
+    return successor_t<DIAG, true>(K, og, c, bux, buy, buw, f);
 }
 
 // ---------------------------------------------------------------------------
@@ -393,7 +393,7 @@ __device__ __forceinline__ float kuhn_value(const Consts& K, const SRC& src, con
 // ---------------------------------------------------------------------------
 // Sweep
 // ---------------------------------------------------------------------------
-template <class W, bool DIAG, bool RK2, int MAXS, bool WRITE_V, bool WRITE_POL, class SRC, class DST>
+template <class W, bool DIAG, bool GEN, int MAXS, bool WRITE_V, bool WRITE_POL, class SRC, class DST>
 __global__ __launch_bounds__(256, 4) void sweep_lds_kernel(
     SRC src, DST dst, const uint8_t* __restrict__ occ,
     const float* __restrict__ bu_g, Consts K, BoxGeom G, const int16_t* __restrict__ worig,
@@ -435,11 +435,11 @@ __global__ __launch_bounds__(256, 4) void sweep_lds_kernel(
         for (int p = threadIdx.x; p < total; p += kThreads) {
             if (!sm.blocked[s]) {
                 CellStart c;
-                load_cell_start(K, sm, s, c);
+                load_cell_start<MAXS, GEN>(K, sm, s, c);
                 float f[6];
                 float val = K.cap;
                 const int k = slice_control(K, r);
-                if (successor_t<DIAG, RK2>(K, og, c, sm.bu[3 * k], sm.bu[3 * k + 1], sm.bu[3 * k + 2], f)) {
+                if (successor_t<DIAG, GEN>(K, og, c, sm.bu[3 * k], sm.bu[3 * k + 1], sm.bu[3 * k + 2], f)) {
                     float t[6], u[6];
                     int i[6];
 #pragma unroll
@@ -462,9 +462,9 @@ __global__ __launch_bounds__(256, 4) void sweep_lds_kernel(
                         inwin = inwin & ((unsigned)d <= (unsigned)(W::E[a] - 2));
                     }
                     float acc;
-                    if (K.nearest) {
+                    if (GEN && K.nearest) {
                         acc = nearest_value(K, src, win, W{}, l, i, f, inwin);
-                    } else if (K.simplex) {
+                    } else if (GEN && K.simplex) {
                         acc = kuhn_value(K, src, win, W{}, l, i, f, inwin);
                     } else {
                     Weights Wt;
@@ -537,19 +537,19 @@ __global__ __launch_bounds__(256, 4) void sweep_lds_kernel(
 
 namespace {
 
-template <class W, bool DIAG, bool RK2, int MAXS, class SRC, class DST>
+template <class W, bool DIAG, bool GEN, int MAXS, class SRC, class DST>
 hipError_t launch_w(const pgk::Consts& K, const pgk::BoxGeom& G, const SRC& src, const DST& dst,
                     const uint8_t* occ, const float* bu, const int16_t* worig, double* part, unsigned* linf,
                     int write_v, int write_pol, const pgk::ActiveG& A, hipStream_t st) {
     return pgk::for_box_chunks(G, [&](const pgk::BoxGeom& g, unsigned nb) {
         if (write_v && write_pol)
-            hipLaunchKernelGGL((pgk::sweep_lds_kernel<W, DIAG, RK2, MAXS, true, true, SRC, DST>), dim3(nb), dim3(256), 0,
+            hipLaunchKernelGGL((pgk::sweep_lds_kernel<W, DIAG, GEN, MAXS, true, true, SRC, DST>), dim3(nb), dim3(256), 0,
                                st, src, dst, occ, bu, K, g, worig, part, linf, A);
         else if (write_v)
-            hipLaunchKernelGGL((pgk::sweep_lds_kernel<W, DIAG, RK2, MAXS, true, false, SRC, DST>), dim3(nb), dim3(256),
+            hipLaunchKernelGGL((pgk::sweep_lds_kernel<W, DIAG, GEN, MAXS, true, false, SRC, DST>), dim3(nb), dim3(256),
                                0, st, src, dst, occ, bu, K, g, worig, part, linf, A);
         else
-            hipLaunchKernelGGL((pgk::sweep_lds_kernel<W, DIAG, RK2, MAXS, false, true, SRC, DST>), dim3(nb), dim3(256),
+            hipLaunchKernelGGL((pgk::sweep_lds_kernel<W, DIAG, GEN, MAXS, false, true, SRC, DST>), dim3(nb), dim3(256),
                                0, st, src, dst, occ, bu, K, g, worig, part, linf, A);
     });
 }
@@ -558,7 +558,9 @@ template <class SRC, class DST>
 hipError_t launch_shape(int shape, int diag, const pgk::Consts& K, const pgk::BoxGeom& G, const SRC& src,
                         const DST& dst, const uint8_t* occ, const float* bu, const int16_t* worig, double* part,
                         unsigned* linf, int write_v, int write_pol, const pgk::ActiveG& A, hipStream_t st) {
-    switch ((shape * 2 + (diag ? 1 : 0)) * 2 + (K.rk2 ? 1 : 0)) {
+    // the general instantiation when any scheme option is set
+    const bool gen = K.rk2 || K.nearest || K.simplex || K.cfl || K.adaptive || K.vclamp;
+    switch ((shape * 2 + (diag ? 1 : 0)) * 2 + (gen ? 1 : 0)) {
         case 0: return launch_w<pgk::Win0, false, false, 128>(K, G, src, dst, occ, bu, worig, part, linf, write_v, write_pol, A, st);
         case 1: return launch_w<pgk::Win0, false, true, 128>(K, G, src, dst, occ, bu, worig, part, linf, write_v, write_pol, A, st);
         case 2: return launch_w<pgk::Win0, true, false, 128>(K, G, src, dst, occ, bu, worig, part, linf, write_v, write_pol, A, st);
