@@ -1,0 +1,219 @@
+"""The solve.jl job interface on the host: loaders, grid construction,
+target checks, MODEL.JSON and the honing PID, and the CLI's modes, output
+lines and exit codes (solver/solve.jl, solver/src/Run.jl). Each pin states
+what the quantity means and checks that, not the formula that computes it.
+"""
+import json
+import math
+import os
+import subprocess
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, load_package
+
+GOLD = Path(__file__).resolve().parent / "golden" / "cli"
+SOLVE = ROOT / "peregrine-desktop_amd" / "solve.py"
+REF_EXAMPLES = Path("/root/reference/solver/examples")
+
+
+@pytest.fixture(scope="module")
+def run():
+    load_package()
+    from peregrine_desktop_amd import run
+    return run
+
+
+def config(tmp_path, **kw):
+    cfg = {"regression": str(GOLD / "regression.toml"), "field": str(GOLD / "field.json"),
+           "targets": str(GOLD / "targets.json"), "out_dir": str(tmp_path / "out"),
+           "grid": {"vmax": 120.0, "wmax": 4.0,
+                    "resolution": {"xy_cm": 20.0, "heading_deg": 45.0, "v_cm_s": 60.0, "w_rad_s": 2.0}},
+           "tau_levels": 1, "refine_rounds": 0, "warm_start": False}
+    cfg.update(kw)
+    path = tmp_path / "cfg.json"
+    path.write_text(json.dumps(cfg))
+    return cfg, path
+
+
+def cli(*args):
+    return subprocess.run([sys.executable, str(SOLVE), *map(str, args)], capture_output=True, text=True,
+                          timeout=120)
+
+
+def test_feasible_bounds_square_robot(run):
+    """A 20 cm square robot, 5 cm wall clearance, 8 heading bins of 3 angles:
+    the loosest bin is the axis-aligned one, whose widest angle is 22.5 deg
+    off axis, where the square reaches 10*(cos + sin) past its centre."""
+    b = run.feasible_bounds((0.0, 0.0, 200.0, 160.0), np.array([[-10, 10, 10, -10], [-10, -10, 10, 10]], float),
+                            5.0, 8, 3)
+    reach = 10 * (math.cos(math.radians(22.5)) + math.sin(math.radians(22.5)))
+    assert b["xlo"] == pytest.approx(5 + reach) and b["xhi"] == pytest.approx(200 - 5 - reach)
+    assert b["ylo"] == pytest.approx(5 + reach) and b["yhi"] == pytest.approx(160 - 5 - reach)
+    # a point robot: the field inset by the clearance alone
+    p = run.feasible_bounds((0.0, 0.0, 200.0, 160.0), None, 5.0, 8, 3)
+    assert (p["xlo"], p["xhi"], p["ylo"], p["yhi"]) == pytest.approx((5, 195, 5, 155))
+    with pytest.raises(run.SolveError):
+        run.feasible_bounds((0.0, 0.0, 20.0, 20.0), np.array([[-10, 10, 10, -10], [-10, -10, 10, 10]], float),
+                            5.0, 8, 3)
+
+
+def test_build_grid_covers_the_feasible_box(run):
+    """Sample counts cover the span at no more than the requested cell; the
+    surplus is split evenly; velocity axes are odd with a node at zero."""
+    gi = run.GridInputs((0.0, 0.0, 200.0, 160.0), None, 5.0, 5.0, 3)
+    g = run.build_grid({"vmax": 100.0, "wmax": 3.0,
+                        "resolution": {"xy_cm": 7.0, "heading_deg": 20.0, "v_cm_s": 30.0, "w_rad_s": 1.0}}, gi)
+    assert g.n[2] == 18
+    for k, span in ((0, 190.0), (1, 150.0)):
+        assert g.step[k] <= 7.0 + 1e-5 and (g.n[k] - 2) * 7.0 < span
+        lo, hi = float(g.lo[k]), float(g.lo[k]) + float(g.step[k]) * (g.n[k] - 1)
+        assert (lo + hi) / 2 == pytest.approx((5 + (200 if k == 0 else 160) - 5) / 2, abs=1e-3)
+    assert g.n[3] == g.n[4] == 9 and g.n[5] == 7
+    for k in (3, 4, 5):
+        assert np.count_nonzero(g.axisvalue(k, np.arange(g.n[k])) == 0) == 1
+    pinned = run.build_grid({"n": [11, 9, 8, 5, 5, 3], "bounds": [0, 0, 100, 80]}, gi)
+    assert pinned.n == (11, 9, 8, 5, 5, 3) and float(pinned.step[0]) == pytest.approx(10.0)
+
+
+def test_check_targets_names_what_blocks(run, pg):
+    g = run.build_grid({"n": [21, 17, 8, 5, 5, 3], "bounds": [0, 0, 200, 160]},
+                       run.GridInputs((0.0, 0.0, 200.0, 160.0), None, 0.0, 0.0, 3))
+    block = np.array([[80.0, 120.0, 120.0, 80.0], [40.0, 40.0, 120.0, 120.0]])
+    occ = pg.build_occupancy(g, [block], 0.0, None, 3, 0.0, bounds=(0.0, 0.0, 200.0, 160.0))
+    ttol = tuple(np.float32(s) * np.float32(0.5) for s in g.step)
+    states = [(np.float32(30), np.float32(30), np.float32(0), 0, 0, 0),
+              (np.float32(100), np.float32(80), np.float32(0), 0, 0, 0),
+              (np.float32(100), np.float32(80), np.float32(0), np.float32(900), 0, 0)]
+    out = run.check_targets(g, occ, ["free", "inside", "fast"], states, ttol, (0.0, 0.0, 200.0, 160.0), [block],
+                            None, 0.0, 0.0)
+    assert out[0]["ok"] and out[0]["free_seed_cells"] == out[0]["seed_cells"]
+    assert not out[1]["ok"] and any("obstacle 1" in w for w in out[1]["blocked_by"])
+    assert out[2]["off_axes"] == ["vx"]
+
+
+def closed_loop(h, B, A, S, eps):
+    """The honing loop on the linearised plant dp/dt = v,
+    dv/dt = Beff*u - Lambda*v, u = -Kp*p + Ki*z - Kd*v, dz/dt = -p, as a
+    9x9 state matrix over (p, v, z)."""
+    tg = h["traction_gain"]["value"]
+    Beff = tg * np.asarray(B)
+    Lam = -(np.asarray(A) + np.asarray(S) / np.asarray(eps)[None, :])
+    Kp, Ki, Kd = (np.asarray(h[k]) for k in ("Kp", "Ki", "Kd"))
+    Z, I = np.zeros((3, 3)), np.eye(3)
+    return np.block([[Z, I, Z], [-Beff @ Kp, -Beff @ Kd - Lam, Beff @ Ki], [-I, Z, Z]])
+
+
+def test_honing_places_the_poles(run):
+    """Per axis the closed loop is (s + third)(s + omega)^2: the eigenvalues
+    of the full coupled 9x9 loop are exactly those, kd is never negative,
+    Kp times the fine band stays inside the budget with omega the largest
+    such (or the loop-rate cap), and the integral clamp holds the integral
+    term to its share of the budget."""
+    m, _ = run.load_model(str(GOLD / "regression.toml"))
+    for cfg in ({}, {"honing_loop_hz": 5.0}, {"honing_budget_frac": 0.3, "honing_bandwidth_scale": 0.7}):
+        h = run.honing_gains(m, cfg)
+        B = np.array(m.B).reshape(3, 3)
+        A = np.array(m.A).reshape(3, 3)
+        S = np.array(m.S).reshape(3, 3)
+        ev = np.sort_complex(np.linalg.eigvals(closed_loop(h, B, A, S, m.eps)))
+        w = h["omega"]
+        want = np.sort_complex(np.array([-t for t in h["poles"]["third"]] + [-w] * 6, complex))
+        assert np.allclose(ev, want, atol=2e-3 * max(1.0, w)), (ev, want)
+        assert min(h["poles"]["kd_diag"]) >= -1e-12
+        fine = np.array([h["fine_band"]["cm"], h["fine_band"]["cm"], h["fine_band"]["rad"]])
+        budget = h["config"]["budget"]
+        norm = lambda K: np.abs(np.asarray(K) @ fine).sum()
+        assert norm(h["Kp"]) <= budget * (1 + 1e-9)
+        if h["omega_limits"]["bound_by"] == "saturation" and cfg.get("honing_bandwidth_scale", 1.0) == 1.0:
+            assert norm(h["Kp"]) == pytest.approx(budget, rel=1e-6)
+        else:
+            assert w == pytest.approx(cfg.get("honing_bandwidth_scale", 1.0)
+                                      * min(h["omega_limits"]["saturation"], h["omega_limits"]["loop_rate"]))
+        ilim = np.array(h["integral_limit"]["value"])
+        worst = np.abs(np.asarray(h["Ki"])).sum(axis=0) @ ilim
+        assert worst <= h["config"]["integral_share"] * budget * (1 + 1e-9)
+    with pytest.raises(run.SolveError):
+        run.honing_gains(m, {"honing_budget": 1.5})
+
+
+def test_model_json_blocks(run, tmp_path):
+    m, _ = run.load_model(str(GOLD / "regression.toml"))
+    mj = run.model_json(m, str(GOLD / "regression.toml"), False)
+    assert mj["schema_version"] == 2
+    for name, cols in (("A_s", 6), ("A_u", 3), ("A_ss", 6), ("A_uu", 3), ("A_sgn", 6), ("A_absv", 6)):
+        assert len(mj[name]) == 3 and all(len(r) == cols for r in mj[name])
+    for name in ("A_s", "A_ss", "A_sgn", "A_absv"):
+        assert all(v == 0 for r in mj[name] for v in r[:3])
+    # the model evaluated through the card's blocks equals the solver's body acceleration
+    s = np.array([0, 0, 0, 37.0, -12.0, 1.3])
+    u = np.array([0.2, -0.1, 0.05])
+    csign = np.clip(s / np.maximum(np.array(mj["csign"]["coulomb_eps"]), 1e-30), -1, 1)
+    csign[:3] = 0
+    a = (np.array(mj["A_s"]) @ s + np.array(mj["A_u"]) @ u + np.array(mj["A_ss"]) @ (s * s)
+         + np.array(mj["A_sgn"]) @ csign + np.array(mj["A_absv"]) @ (np.abs(s) * s) + np.array(mj["k"]))
+    B, A, S, D = (np.array(getattr(m, k)).reshape(3, 3) for k in "BASD")
+    v = s[3:]
+    want = B @ u + A @ v + np.array(m.q) * v[2] ** 2 + S @ np.clip(v / np.array(m.eps), -1, 1) \
+        + D @ (np.abs(v) * v) + np.array(m.c)
+    assert np.allclose(a, want)
+    assert set(mj["nonzero_blocks"]) == {"A_s", "A_u", "A_ss", "A_sgn", "A_absv", "k"}
+    assert run.model_json(m.__class__(*(m.B, m.A, m.q, m.S, m.D, (0, 0, 0), m.eps, m.knee)),
+                          str(GOLD / "regression.toml"), True)["nonzero_blocks"][-1] != "k"
+
+
+def test_load_model_refuses_a_missing_knee(run, tmp_path):
+    bad = tmp_path / "fit.toml"
+    bad.write_text((GOLD / "regression.toml").read_text().replace("traction_knee = 0.8", ""))
+    with pytest.raises(run.SolveError, match="traction_knee"):
+        run.load_model(str(bad))
+    bad.write_text("[units]\n")
+    with pytest.raises(run.SolveError, match="mecanum_basis"):
+        run.load_model(str(bad))
+
+
+@pytest.mark.skipif(not REF_EXAMPLES.exists(), reason="reference examples absent")
+def test_loads_the_reference_example_files(run):
+    """The reference's own example field and targets files parse."""
+    bounds, polys, robot, _ = run.load_field(str(REF_EXAMPLES / "field.json"))
+    assert bounds == (0.0, 0.0, 366.0, 366.0) and len(polys) == 3 and robot.shape == (2, 4)
+    names, states, _ = run.load_targets(str(REF_EXAMPLES / "targets.json"))
+    assert names == ["score_left", "score_right", "park_center"] and len(states[0]) == 6
+
+
+def test_cli_modes_and_exit_codes(tmp_path):
+    cfg, path = config(tmp_path)
+    r = cli()
+    assert r.returncode == 2 and "usage" in r.stderr
+    assert cli(tmp_path / "missing.json").returncode == 2
+    assert cli(path, "bogus").returncode == 2
+    r = cli(path, "honing")
+    assert r.returncode == 0
+    h = json.loads(r.stdout.split("HONING ", 1)[1])
+    assert h["frame"] == "robot body" and len(h["Kp"]) == 3
+    # honing into a run directory whose MODEL.JSON came from another fit
+    run_dir = tmp_path / "run"
+    run_dir.mkdir()
+    (run_dir / "MODEL.JSON").write_text(json.dumps({"regression_sha256": "0" * 64}))
+    r = cli(path, "honing", run_dir)
+    assert r.returncode == 1 and r.stderr.startswith("ERROR ") and "different regression" in r.stderr
+    r = cli(path, "plan")
+    assert r.returncode == 0
+    p = json.loads(r.stdout.split("PLAN ", 1)[1])
+    assert p["cells"] == math.prod(p["n"]) and p["targets_ok"] and p["unsupported_options"] == []
+
+
+def test_solve_refuses_unbuilt_scheme_options(tmp_path):
+    """The reference defaults (5-rung horizon ladder, pattern search, warm
+    start) are refused with the setting that avoids each, before any work."""
+    cfg, path = config(tmp_path)
+    for k in ("tau_levels", "refine_rounds", "warm_start"):
+        del cfg[k]
+    path.write_text(json.dumps(cfg))
+    r = cli(path, "solve")
+    assert r.returncode == 1
+    assert "tau_levels: 1" in r.stderr and "refine_rounds: 0" in r.stderr and "warm_start: false" in r.stderr
+    assert not os.path.exists(cfg["out_dir"])
