@@ -76,7 +76,8 @@ typedef struct pg_params {
     float   tau_max;
     float   hmax;
     int32_t ncoarse;
-    int32_t rounds;
+    int32_t rounds;   /* pattern-search rounds; with ntau > 1 (horizon ladder) runs on
+                         the global-gather in-HBM FP32 kernel only */
     float   delta0;
     int32_t rk2;      /* explicit midpoint substeps (step_state_rk2); exact FP32
                          kernels only: the 16-bit stores, the FP64 tables and the
@@ -232,6 +233,19 @@ int pg_solver_set_active(pg_solver* s, int32_t on);
 
 /* Boxes the last sweep computed, and boxes of the grid. */
 int pg_last_sweep_boxes(pg_solver* s, int64_t* computed, int64_t* total);
+
+/* The warm-start policy store (replaces `pol = warm ? CUDA.zeros(Int8,
+ * 3 * cells) : nothing`, Run.jl:1782, its reset in solve_value!,
+ * Run.jl:1439, and pol_get / pol_set!, PeregrineSolver.jl:728-744): one
+ * Int8 command per cell and axis, x 1/127, zeroed by pg_fill_seed. Each
+ * sweep evaluates the cell's stored command after coast and writes the
+ * winning command back when the cell's value improves. With the store on,
+ * and with refine_rounds > 0 or tau_levels > 1, sweeps run on the
+ * global-gather kernel (in-HBM FP32 solvers only). on = 0 frees it. */
+int pg_solver_set_warm_start(pg_solver* s, int32_t on);
+
+/* Copy the warm-start store (3 * ncells bytes) to the host. */
+int pg_download_warm_start(pg_solver* s, int8_t* dst);
 
 /* Number of cells of the solver's grid. */
 int pg_solver_ncells(pg_solver* s, int64_t* n);

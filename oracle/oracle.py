@@ -40,6 +40,8 @@ SIG = {
     "orc_backup_cells_f64": (ctypes.c_int, [_P, _P, _P, _P, _P, _I32, _P, _P, _I64, _P, _P, _I32]),
     "orc_backup_cells_f32": (ctypes.c_int, [_P, _P, _P, _P, _P, _I32, _P, _P, _I64, _P, _I32]),
     "orc_sweep_f32": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _I32, _P, _I32, _I32, _I32, _I32, _P, _P, _P, _I32]),
+    "orc_sweep_warm_f32": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _I32, _P, _I32, _I32, _I32, _I32, _P, _P, _P]),
+    "orc_tanh_f32": (_F, [_F]),
     "orc_sweep_f64": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _I32, _P, _I32, _I32, _I32, _I32, _P, _P, _P, _I32]),
     "orc_target_cells": (_I64, [_P, _P, _P, _P, _I64]),
     "orc_check_quotient": (_I64, [_F, _I32]),
@@ -95,6 +97,20 @@ class Problem:
         rc = f(_ptr(Vin), _ptr(Vout), _ptr(self.occ), ctypes.byref(self.gc), ctypes.byref(self.mc),
                _ptr(self.ctl), self.nctl, ctypes.byref(self.pc), mode, rev, phase, nthreads,
                _ptr(pol), ctypes.byref(d), ctypes.byref(l), esc)
+        if rc != 0:
+            raise RuntimeError(f"oracle sweep failed ({rc})")
+        return d.value, l.value
+
+    def sweep_warm(self, Vin: np.ndarray, Vout: np.ndarray, wpol: np.ndarray, phase: int = 0, nthreads: int = 0):
+        """A Jacobi float sweep reading and updating the warm-start store
+        wpol (int8, 3 per cell, interleaved) in place."""
+        assert wpol.dtype == np.int8 and wpol.size == 3 * self.ncell and wpol.flags.c_contiguous
+        L = lib()
+        d = ctypes.c_double()
+        l = ctypes.c_double()
+        rc = L.orc_sweep_warm_f32(_ptr(Vin), _ptr(Vout), _ptr(self.occ), ctypes.byref(self.gc), ctypes.byref(self.mc),
+                                  _ptr(self.ctl), self.nctl, ctypes.byref(self.pc), 0, 0, phase, nthreads,
+                                  _ptr(wpol), ctypes.byref(d), ctypes.byref(l))
         if rc != 0:
             raise RuntimeError(f"oracle sweep failed ({rc})")
         return d.value, l.value

@@ -17,14 +17,14 @@ using namespace orc;
 namespace {
 
 bool params_supported(const pg_params* p) {
-    return p->rounds <= 0 && p->ntau <= 1;
+    return p->rounds >= 0 && p->ntau >= 1 && p->ntau <= 64;
 }
 
 template <class T>
 int sweep_impl(const T* Vin, T* Vout, const uint8_t* occ, const pg_grid* g, const pg_model* m,
                const float* ctl, int32_t nctl, const pg_params* p, int32_t mode, int32_t rev,
                int32_t phase, int32_t nthreads, uint8_t* pol, double* sum_out, double* linf_out,
-               int32_t esc) {
+               int32_t esc, int8_t* wpol = nullptr) {
     if (!params_supported(p) || nctl < 2) return -3;
     const int64_t n = ncells(*g);
     double sum = 0.0, linf = 0.0;
@@ -52,11 +52,14 @@ int sweep_impl(const T* Vin, T* Vout, const uint8_t* occ, const pg_grid* g, cons
                 continue;
             }
             int32_t k = 0;
-            T c = cell_update<T>(i, Vin, occ, *g, *m, ctl, nctl, *p, phase, false, &k);
+            T b[3];
+            T c = cell_update<T>(i, Vin, occ, *g, *m, ctl, nctl, *p, phase, false, &k, wpol, b);
             if (c < old) {
                 Vout[i] = c;
                 sum += (double)(old - c);
                 linf = std::max(linf, (double)(old - c));
+                if (wpol)
+                    for (int a = 0; a < 3; ++a) wpol[3 * i + a] = pol_quant((float)b[a]);
             } else {
                 Vout[i] = old;
             }
@@ -81,12 +84,15 @@ int sweep_impl(const T* Vin, T* Vout, const uint8_t* occ, const pg_grid* g, cons
                 }
                 return;
             }
-            T c = cell_update<T>(idx, V, occ, *g, *m, ctl, nctl, *p, phase, false, &k);
+            T b[3];
+            T c = cell_update<T>(idx, V, occ, *g, *m, ctl, nctl, *p, phase, false, &k, wpol, b);
             if (c < old) {
                 V[idx] = c;
                 s += (double)(old - c);
                 l = std::max(l, (double)(old - c));
                 if (pol) pol[idx] = (uint8_t)k;
+                if (wpol)
+                    for (int a = 0; a < 3; ++a) wpol[3 * idx + a] = pol_quant((float)b[a]);
             }
         };
         if (mode == 1) {
@@ -321,6 +327,20 @@ int orc_sweep_f32(const float* Vin, float* Vout, const uint8_t* occ, const pg_gr
     return sweep_impl<float>(Vin, Vout, occ, g, m, ctl, nctl, p, mode, rev, phase, nthreads, pol,
                              sum, linf, esc);
 }
+
+// A sweep with the warm-start policy store (Int8, 3 per cell, interleaved):
+// each cell reads its entry and, when its value improves, writes the
+// winning command back (gpu_sweep_kernel!, Gpu.jl:5-25).
+int orc_sweep_warm_f32(const float* Vin, float* Vout, const uint8_t* occ, const pg_grid* g,
+                       const pg_model* m, const float* ctl, int32_t nctl, const pg_params* p,
+                       int32_t mode, int32_t rev, int32_t phase, int32_t nthreads, int8_t* wpol,
+                       double* sum, double* linf) {
+    return sweep_impl<float>(Vin, Vout, occ, g, m, ctl, nctl, p, mode, rev, phase, nthreads, nullptr,
+                             sum, linf, 0, wpol);
+}
+
+// The oracle's float tanh of a non-negative argument (the traction gain's).
+float orc_tanh_f32(float r) { return tanh_t(r); }
 
 int orc_sweep_f64(const double* Vin, double* Vout, const uint8_t* occ, const pg_grid* g,
                   const pg_model* m, const float* ctl, int32_t nctl, const pg_params* p,

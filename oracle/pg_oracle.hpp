@@ -145,13 +145,45 @@ inline int64_t flatten(const pg_grid& g, int32_t i1, int32_t i2, int32_t i3, int
 // Drivetrain model (PeregrineSolver.jl:100-171)
 // ---------------------------------------------------------------------------
 
-// traction_gain (:126-136); tanh from the C library.
+// tanh of a non-negative argument. The reference's CPU and GPU builds take
+// tanhf from different libraries (Julia's, CUDA's), which round
+// differently, so its float bits are implementation-defined. The oracle
+// uses the correctly rounded value: tanh computed in double from exp
+// (argument reduction t = k ln2 + r with ln2 in two parts, Taylor series
+// of exp(r); expm1 of small arguments from its series), tanh(x) =
+// expm1(2x) / (expm1(2x) + 2), rounded to float. Pinned against the C
+// library's double tanh, rounded to float, over every float in [1e-6, 30].
+inline double tanh_series(double x) {
+    static const double inv_fact[18] = {1.0, 1.0, 1.0 / 2.0, 1.0 / 6.0, 1.0 / 24.0, 1.0 / 120.0, 1.0 / 720.0,
+                                        1.0 / 5040.0, 1.0 / 40320.0, 1.0 / 362880.0, 1.0 / 3628800.0,
+                                        1.0 / 39916800.0, 1.0 / 479001600.0, 1.0 / 6227020800.0,
+                                        1.0 / 87178291200.0, 1.0 / 1307674368000.0, 1.0 / 20922789888000.0,
+                                        1.0 / 355687428096000.0};
+    const double t = 2.0 * std::min(x, 20.0);
+    double em;
+    if (t < 0.5) {
+        double p = inv_fact[17];
+        for (int n = 16; n >= 1; --n) p = inv_fact[n] + t * p;
+        em = t * p;
+    } else {
+        const double k = std::rint(t * 1.4426950408889634);
+        const double r = (t - k * 6.93147180369123816490e-01) - k * 1.90821492927058770002e-10;
+        double p = inv_fact[14];
+        for (int n = 13; n >= 0; --n) p = inv_fact[n] + r * p;
+        em = std::ldexp(p, (int)k) - 1.0;
+    }
+    return em / (em + 2.0);
+}
+inline float tanh_t(float r) { return (float)tanh_series((double)r); }
+inline double tanh_t(double r) { return std::tanh(r); }
+
+// traction_gain (:126-136)
 template <class T> inline T traction_gain(T u1, T u2, T u3, float knee) {
     if (std::isnan(knee) || knee <= 0.0f) return (T)1;
     T m = std::sqrt(u1 * u1 + u2 * u2 + u3 * u3);
     T r = m / (T)knee;
     if (r < (T)1.0e-6f) return (T)1;
-    return std::tanh(r) / r;
+    return tanh_t(r) / r;
 }
 
 template <class T> inline void saturate(const pg_model& m, T u1r, T u2r, T u3r, T& u1, T& u2, T& u3) {
@@ -503,22 +535,47 @@ inline T lookahead(const VR& V, const uint8_t* occ, bool occ_ignore, const pg_gr
 }
 
 // ---------------------------------------------------------------------------
-// cell_update, plain scheme and the full-scan lattice (:1205-1344)
+// cell_update (:1205-1344)
 // ---------------------------------------------------------------------------
 //
-// Returns the backed-up value and the index of the winning control (0 =
-// coast). The warm start (`pol`), pattern search and step-length ladder are
-// the reference's production scheme; this restatement covers the scan with
-// ncoarse >= 0 and rejects rounds > 0 / ntau > 1 at the API.
+// proj_oct (:1165-1170): scale a command onto the octahedron surface
+// |u1| + |u2| + |u3| = 1; the zero command stays zero.
+template <class T> inline void proj_oct(T& u1, T& u2, T& u3) {
+    T s = std::fabs(u1) + std::fabs(u2) + std::fabs(u3);
+    if (s < (T)1.0e-6f) { u1 = u2 = u3 = (T)0; return; }
+    T r = (T)1 / s;
+    u1 = u1 * r; u2 = u2 * r; u3 = u3 * r;
+}
+
+// The warm-start policy store, Int8 interleaved per cell (pol_get / pol_set!,
+// :728-744): a command component v is kept as round(clamp(v, -1, 1) * 127),
+// ties to even, and read back as Float32(q) * Float32(1/127).
+inline float pol_read(int8_t q) { return (float)q * (1.0f / 127.0f); }
+inline int8_t pol_quant(float v) {
+    float c = v < -1.0f ? -1.0f : (v > 1.0f ? 1.0f : v);
+    return (int8_t)(int32_t)std::nearbyint(c * 127.0f);
+}
+
+// Returns the backed-up value, the index of the winning lattice control in
+// *best_k (0 = coast; -1 when the warm-start or a pattern-search command
+// won) and the winning command in bout[3] when bout is given. Each sweep
+// evaluates coast, the cell's warm-start command (wpol, when given and
+// non-zero), the lattice (whole, or this phase's rotating slice), `rounds`
+// of pattern search around the best so far (six axis moves of +-delta
+// projected onto the octahedron, delta halving each round), then the
+// horizon ladder (ntau > 1) and the dt rung (cfl > 0) with the best
+// command. Strict < throughout: the earlier candidate wins ties.
 template <class T>
 inline T cell_update(int64_t idx, const T* V, const uint8_t* occ, const pg_grid& g,
                      const pg_model& m, const float* ctl, int32_t nctl, const pg_params& p,
-                     int32_t phase, bool esc, int32_t* best_k) {
+                     int32_t phase, bool esc, int32_t* best_k, const int8_t* wpol = nullptr,
+                     T* bout = nullptr) {
     int32_t i[6];
     unflatten(g, idx, i);
     bool blocked = occ[((int64_t)i[0] * g.n[1] + i[1]) * g.n[2] + i[2]] != 0;
     const T cap = (T)p.cap;
     *best_k = 0;
+    if (bout) bout[0] = bout[1] = bout[2] = (T)0;
     if (!esc && blocked) return cap;
     T x = axisvalue<T>(g, 0, i[0]), y = axisvalue<T>(g, 1, i[1]);
     T h = axisvalue<T>(g, 2, i[2]), vfx = axisvalue<T>(g, 3, i[3]);
@@ -531,24 +588,66 @@ inline T cell_update(int64_t idx, const T* V, const uint8_t* occ, const pg_grid&
     int32_t ns0 = substeps_for<T>(tau0, p);
 
     int64_t blk = (int64_t)g.n[3] * g.n[4] * g.n[5];
-    auto run = [&](T u1, T u2, T u3) -> T {
+    auto run = [&](T u1, T u2, T u3, T tau, int32_t ns) -> T {
         if (esc) {
             EscV<T> ev{V, occ, blk, cap, blocked};
-            return lookahead<T>(ev, occ, blocked, g, m, x, y, h, vx, vy, w, u1, u2, u3, tau0, ns0, p);
+            return lookahead<T>(ev, occ, blocked, g, m, x, y, h, vx, vy, w, u1, u2, u3, tau, ns, p);
         }
         PlainV<T> pv{V};
-        return lookahead<T>(pv, occ, false, g, m, x, y, h, vx, vy, w, u1, u2, u3, tau0, ns0, p);
+        return lookahead<T>(pv, occ, false, g, m, x, y, h, vx, vy, w, u1, u2, u3, tau, ns, p);
     };
 
-    T best = run((T)0, (T)0, (T)0);
+    T best = run((T)0, (T)0, (T)0, tau0, ns0);
+    T b1 = (T)0, b2 = (T)0, b3 = (T)0;
+    if (wpol) {
+        const int8_t* q = wpol + 3 * idx;
+        T q1 = (T)pol_read(q[0]), q2 = (T)pol_read(q[1]), q3 = (T)pol_read(q[2]);
+        if (std::fabs(q1) + std::fabs(q2) + std::fabs(q3) > (T)1.0e-6f) {
+            T c = run(q1, q2, q3, tau0, ns0);
+            if (c < best) { best = c; b1 = q1; b2 = q2; b3 = q3; *best_k = -1; }
+        }
+    }
     int32_t nnc = jmax<int32_t>(nctl - 1, 1);
     int32_t span = p.ncoarse <= 0 ? nnc : jmin<int32_t>(p.ncoarse, nnc);
     int32_t off = p.ncoarse <= 0 ? 0 : (int32_t)(((int64_t)phase * span) % nnc + nnc) % nnc;
     for (int32_t j = 0; j < span; ++j) {
         int32_t k = (int32_t)(((int64_t)off + j) % nnc) + 1;   // 0-based, skips coast
-        T c = run((T)ctl[k], (T)ctl[k + nctl], (T)ctl[k + 2 * nctl]);
-        if (c < best) { best = c; *best_k = k; }
+        T u1 = (T)ctl[k], u2 = (T)ctl[k + nctl], u3 = (T)ctl[k + 2 * nctl];
+        T c = run(u1, u2, u3, tau0, ns0);
+        if (c < best) { best = c; *best_k = k; b1 = u1; b2 = u2; b3 = u3; }
     }
+    T delta = (T)p.delta0;
+    for (int32_t rd = 0; rd < p.rounds; ++rd) {
+        for (int32_t d = 1; d <= 6; ++d) {
+            T sg = (d % 2 == 0) ? -delta : delta;
+            T e1 = d <= 2 ? sg : (T)0;
+            T e2 = (d == 3 || d == 4) ? sg : (T)0;
+            T e3 = d >= 5 ? sg : (T)0;
+            T c1 = b1 + e1, c2 = b2 + e2, c3 = b3 + e3;
+            proj_oct<T>(c1, c2, c3);
+            if (std::fabs(c1) + std::fabs(c2) + std::fabs(c3) < (T)1.0e-6f) continue;
+            T c = run(c1, c2, c3, tau0, ns0);
+            if (c < best) { best = c; b1 = c1; b2 = c2; b3 = c3; *best_k = -1; }
+        }
+        delta = delta * (T)0.5f;
+    }
+    if (p.ntau > 1) {
+        T tau = tau0 * (T)0.5f;
+        for (int32_t k = 1; k <= p.ntau; ++k) {
+            if (k != 2) {   // rung 2 is tau0 at ratio 2, already evaluated
+                T t = jmin(jmax(tau, (T)p.tau_min), (T)p.tau_max);
+                T c = run(b1, b2, b3, t, substeps_for<T>(t, p));
+                if (c < best) best = c;
+            }
+            tau = tau * (T)p.tau_ratio;
+        }
+        if (p.cfl > 0.0f) {
+            T t = jmin(jmax((T)p.dt, (T)p.tau_min), (T)p.tau_max);
+            T c = run(b1, b2, b3, t, substeps_for<T>(t, p));
+            if (c < best) best = c;
+        }
+    }
+    if (bout) { bout[0] = b1; bout[1] = b2; bout[2] = b3; }
     return best;
 }
 

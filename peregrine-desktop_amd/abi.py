@@ -129,6 +129,8 @@ SIGNATURES = {
     "pg_solve_escape": (ctypes.c_int, [_P, _I32, ctypes.c_double, ctypes.c_float, ctypes.POINTER(_I32),
                                        ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_float)]),
     "pg_last_sweep_boxes": (ctypes.c_int, [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
+    "pg_solver_set_warm_start": (ctypes.c_int, [_P, _I32]),
+    "pg_download_warm_start": (ctypes.c_int, [_P, _P]),
     "pg_solver_ncells": (ctypes.c_int, [_P, ctypes.POINTER(_I64)]),
 }
 

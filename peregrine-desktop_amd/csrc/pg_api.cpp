@@ -14,6 +14,7 @@
 #include <string>
 #include <vector>
 #include "pg_internal.hpp"
+#include "pg_tanh.hpp"
 
 namespace pgh {
 
@@ -43,13 +44,9 @@ using pgh::grid_cells;
 
 namespace {
 
-// traction_gain (PeregrineSolver.jl:126-136)
+// traction_gain (PeregrineSolver.jl:126-136), the device's own definition
 float traction_gain(float u1, float u2, float u3, float knee) {
-    if (std::isnan(knee) || knee <= 0.0f) return 1.0f;
-    float m = std::sqrt(u1 * u1 + u2 * u2 + u3 * u3);
-    float r = m / knee;
-    if (r < 1.0e-6f) return 1.0f;
-    return std::tanh(r) / r;
+    return pgt::traction_gain_f(u1, u2, u3, knee);
 }
 
 // substeps_for (PeregrineSolver.jl:985-989)
@@ -103,6 +100,8 @@ void release(pg_solver* s) {
     (void)hipFree(s->d_trefit);
     (void)hipFree(s->d_occ);
     (void)hipFree(s->d_bu);
+    (void)hipFree(s->d_uraw);
+    (void)hipFree(s->d_wpol);
     (void)hipFree(s->d_seeds);
     (void)hipFree(s->d_part);
     (void)hipFree(s->d_sum);
@@ -134,8 +133,13 @@ int check_grid(const pg_grid* g) {
 int check_params(const pg_params* p, int32_t nctl) {
     if (p->cfl > 0.0f && !(p->tau_min > 0.0f && p->tau_min <= p->tau_max))
         return fail(PG_ERR_ARG, "cfl > 0 needs 0 < tau_min <= tau_max");
-    if (p->ntau > 1) return fail(PG_ERR_UNSUPPORTED, "tau_levels > 1 (horizon ladder) is not built");
-    if (p->rounds > 0) return fail(PG_ERR_UNSUPPORTED, "refine_rounds > 0 (pattern search) is not built");
+    if (p->ntau < 1 || p->ntau > 16) return fail(PG_ERR_ARG, "tau_levels must be in [1, 16], got %d", p->ntau);
+    if (p->rounds < 0 || p->rounds > 16) return fail(PG_ERR_ARG, "refine_rounds must be in [0, 16], got %d", p->rounds);
+    if (p->ntau > 1 && !(p->tau_ratio > 0.0f && std::isfinite(p->tau_ratio)))
+        return fail(PG_ERR_ARG, "tau_ratio must be positive");
+    if (p->ntau > 1 && !(p->tau_min > 0.0f && p->tau_min <= p->tau_max))
+        return fail(PG_ERR_ARG, "tau_levels > 1 needs 0 < tau_min <= tau_max");
+    if (p->rounds > 0 && !std::isfinite(p->delta0)) return fail(PG_ERR_ARG, "refine_delta must be finite");
     if (p->checks < 1 || p->checks > 24) return fail(PG_ERR_ARG, "sweep_checks must be in [1, 24], got %d", p->checks);
     if (!(p->dt > 0.0f) || !std::isfinite(p->dt)) return fail(PG_ERR_ARG, "dt must be positive");
     if (!(p->cap > 0.0f) || !std::isfinite(p->cap)) return fail(PG_ERR_ARG, "value_cap must be positive");
@@ -208,6 +212,12 @@ void build_consts(pg_solver* s) {
         K.tau_max = s->p.tau_max;
         K.hmax = s->p.hmax;
     }
+    std::memcpy(K.B, s->m.B, sizeof K.B);
+    K.knee = s->m.knee;
+    K.rounds = s->p.rounds > 0 ? s->p.rounds : 0;
+    K.ntau = s->p.ntau > 1 ? s->p.ntau : 1;
+    K.delta0 = s->p.delta0;
+    K.tau_ratio = s->p.tau_ratio;
     K.vclamp = s->p.vclamp ? 1 : 0;
     K.adaptive = s->p.adaptive_checks ? 1 : 0;
     K.half_xy = 0.5f * (g.step[0] < g.step[1] ? g.step[0] : g.step[1]);
@@ -341,8 +351,10 @@ using pgh::plan;
 
 namespace {
 
+const char* global_only_option(const pg_solver* s);
+
 int auto_kernel(const pg_solver* s) {
-    return (s->nctl <= kLdsMaxCtl) ? PG_KERNEL_LDS : PG_KERNEL_GLOBAL;
+    return (s->nctl <= kLdsMaxCtl && !global_only_option(s)) ? PG_KERNEL_LDS : PG_KERNEL_GLOBAL;
 }
 
 // FP32 table in dV[0] -> 16-bit mirror. A whole-table load (force) fits
@@ -372,6 +384,19 @@ const char* exact_only_option(const pg_solver* s) {
     if (s->p.simplex) return "simplex";
     if (s->p.cfl > 0.0f) return "cfl";
     if (s->K.span < s->K.nnc) return "control_scan (rotating slice)";
+    if (s->p.rounds > 0) return "refine_rounds";
+    if (s->p.ntau > 1) return "tau_levels";
+    if (s->d_wpol) return "warm_start";
+    return nullptr;
+}
+
+// Options only the global-gather kernel implements: the pattern search, the
+// horizon ladder and the warm start evaluate commands and horizons that are
+// not known before the sweep, so no LDS window can be planned for them.
+const char* global_only_option(const pg_solver* s) {
+    if (s->p.rounds > 0) return "refine_rounds";
+    if (s->p.ntau > 1) return "tau_levels";
+    if (s->d_wpol) return "warm_start";
     return nullptr;
 }
 
@@ -528,10 +553,11 @@ int create_solver(pg_ctx* ctx, const pg_grid* g, const pg_model* m, const pg_par
         return fail(PG_ERR_ARG, "unknown vstore %d", vstore);
     if (vstore != PG_VSTORE_F32 &&
         (p->rk2 || p->nearest || p->vclamp || p->adaptive_checks || p->simplex || p->cfl > 0.0f ||
-         (p->ncoarse > 0 && p->ncoarse < nctl - 1)))
+         (p->ncoarse > 0 && p->ncoarse < nctl - 1) || p->rounds > 0 || p->ntau > 1))
         return fail(PG_ERR_UNSUPPORTED,
                     "the 16-bit stores (precision form) run the plain default scheme only (no rk2, nearest, "
-                    "velocity_clamp, adaptive_checks, simplex, cfl or control_scan slice)");
+                    "velocity_clamp, adaptive_checks, simplex, cfl, control_scan slice, refine_rounds or "
+                    "tau_levels)");
     if (vstore != PG_VSTORE_F32 && nctl > kLdsMaxCtl)
         return fail(PG_ERR_UNSUPPORTED, "the 16-bit stores sweep at most %d controls", kLdsMaxCtl);
     for (int k = 0; k < 3; ++k)
@@ -599,6 +625,7 @@ int create_solver(pg_ctx* ctx, const pg_grid* g, const pg_model* m, const pg_par
     }
     chk(hipMalloc(&s->d_occ, (size_t)nocc));
     chk(hipMalloc(&s->d_bu, bu.size() * sizeof(float)));
+    chk(hipMalloc(&s->d_uraw, 3 * (size_t)nctl * sizeof(float)));
     chk(hipMalloc(&s->d_sum, (1 + kReduceBlocks) * sizeof(double)));   // the sum, then reduction scratch
     chk(hipMalloc(&s->d_linf, sizeof(unsigned)));
     if (e != hipSuccess) {
@@ -611,6 +638,12 @@ int create_solver(pg_ctx* ctx, const pg_grid* g, const pg_model* m, const pg_par
     for (int64_t i = 0; i < nocc; ++i) occ01[i] = occ[i] ? 1 : 0;
     chk(hipMemcpy(s->d_occ, occ01.data(), (size_t)nocc, hipMemcpyHostToDevice));
     chk(hipMemcpy(s->d_bu, bu.data(), bu.size() * sizeof(float), hipMemcpyHostToDevice));
+    {
+        std::vector<float> ur(3 * (size_t)nctl);
+        for (int k = 0; k < nctl; ++k)
+            for (int a = 0; a < 3; ++a) ur[3 * k + a] = ctl_soa[k + a * nctl];
+        chk(hipMemcpy(s->d_uraw, ur.data(), ur.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
     chk(hipEventCreate(&s->ev0));
     chk(hipEventCreate(&s->ev1));
     if (vstore != PG_VSTORE_F32) {
@@ -643,6 +676,9 @@ int pg_solver_create_ooc(pg_ctx* ctx, const pg_grid* g, const pg_model* m, const
                          const int32_t tile[3], pg_solver** out) {
     if (nctl > kLdsMaxCtl)
         return fail(PG_ERR_UNSUPPORTED, "the out-of-core sweep holds at most %d controls", kLdsMaxCtl);
+    if (p && (p->rounds > 0 || p->ntau > 1))
+        return fail(PG_ERR_UNSUPPORTED, "the out-of-core sweep does not run refine_rounds or tau_levels (%s set)",
+                    p->rounds > 0 ? "refine_rounds" : "tau_levels");
     if (!tile) return fail(PG_ERR_ARG, "null tile");
     int rc = create_solver(ctx, g, m, p, ctl_soa, nctl, occ, PG_VSTORE_F32, true, out);
     if (rc) return rc;
@@ -710,6 +746,7 @@ int pg_fill_seed(pg_solver* s, const int64_t* seeds, int64_t nseeds) {
     }
     HIPCHECK(pgk_launch_fill(s->dV[0], s->ncell, s->p.cap, st));
     HIPCHECK(pgk_launch_seed(s->dV[0], s->d_seeds, s->nseeds, st));
+    if (s->d_wpol) HIPCHECK(hipMemsetAsync(s->d_wpol, 0, 3 * (size_t)s->ncell, st));
     if (s->vstore != PG_VSTORE_F32) {
         s->f16_active = true;
         s->kernel = PG_KERNEL_FAST;
@@ -764,8 +801,8 @@ int pg_sweep(pg_solver* s, int32_t phase, int32_t rev, double* sum_decrease, flo
         HIPCHECK(pgk_launch_sweep_lds(s->shape, s->diag ? 1 : 0, s->K, s->G, s->dV[s->cur], Vout, s->d_occ,
                                       s->d_bu, s->d_worig, nullptr, s->d_part, s->d_linf, 1, A, st));
     } else {
-        HIPCHECK(pgk_launch_sweep(s->K, s->dV[s->cur], Vout, s->d_occ, s->d_bu, nullptr, s->d_part,
-                                  s->d_linf, 1, st, &nblk));
+        HIPCHECK(pgk_launch_sweep(s->K, s->dV[s->cur], Vout, s->d_occ, s->d_bu, s->d_uraw, nullptr, s->d_wpol,
+                                  s->d_part, s->d_linf, 1, st, &nblk));
     }
     HIPCHECK(hipEventRecord(s->ev1, st));
     HIPCHECK(pgk_launch_reduce(s->d_part, nblk, s->d_sum, s->d_sum + 1, st));
@@ -846,7 +883,7 @@ int pg_escape_sweep(pg_solver* s, int32_t phase, int32_t rev, double* sum_improv
     HIPCHECK(hipMemsetAsync(s->d_linf, 0, sizeof(unsigned), st));
     int64_t nblk = nb;
     HIPCHECK(hipEventRecord(s->ev0, st));
-    HIPCHECK(pgk_launch_escape(s->K, s->dV[s->cur], s->dV[1 - s->cur], s->d_occ, s->d_bu, nullptr, s->d_part,
+    HIPCHECK(pgk_launch_escape(s->K, s->dV[s->cur], s->dV[1 - s->cur], s->d_occ, s->d_bu, s->d_uraw, nullptr, s->d_part,
                                s->d_linf, st, &nblk));
     HIPCHECK(hipEventRecord(s->ev1, st));
     HIPCHECK(pgk_launch_reduce(s->d_part, nblk, s->d_sum, s->d_sum + 1, st));
@@ -927,14 +964,17 @@ int pg_policy(pg_solver* s, uint8_t* pol_host) {
     if (!s->d_pol) HIPCHECK(hipMalloc(&s->d_pol, (size_t)s->ncell));
     int64_t nblk = 0;
     // the argmin over the whole lattice, whatever slice the last sweep used
-    const int32_t span = s->K.span, off = s->K.off;
+    // (no pattern search or ladder: the policy is a lattice index)
+    const int32_t span = s->K.span, off = s->K.off, rounds = s->K.rounds, ntau = s->K.ntau;
     s->K.span = s->K.nnc;
     s->K.off = 0;
+    s->K.rounds = 0;
+    s->K.ntau = 1;
     struct Restore {
         pg_solver* s;
-        int32_t span, off;
-        ~Restore() { s->K.span = span; s->K.off = off; }
-    } restore{s, span, off};
+        int32_t span, off, rounds, ntau;
+        ~Restore() { s->K.span = span; s->K.off = off; s->K.rounds = rounds; s->K.ntau = ntau; }
+    } restore{s, span, off, rounds, ntau};
     const int kern = s->f16_active ? PG_KERNEL_FAST : s->kernel;
     if (kern != PG_KERNEL_GLOBAL) {
         int rc = plan(s);
@@ -955,7 +995,8 @@ int pg_policy(pg_solver* s, uint8_t* pol_host) {
         HIPCHECK(pgk_launch_sweep_lds(s->shape, s->diag ? 1 : 0, s->K, s->G, s->dV[s->cur], nullptr, s->d_occ,
                                       s->d_bu, s->d_worig, s->d_pol, nullptr, nullptr, 0, pgk::ActiveG{}, st));
     } else {
-        HIPCHECK(pgk_launch_sweep(s->K, s->dV[s->cur], nullptr, s->d_occ, s->d_bu, s->d_pol, nullptr, nullptr, 0, st, &nblk));
+        HIPCHECK(pgk_launch_sweep(s->K, s->dV[s->cur], nullptr, s->d_occ, s->d_bu, s->d_uraw, s->d_pol, nullptr,
+                                  nullptr, nullptr, 0, st, &nblk));
     }
     HIPCHECK(hipMemcpyAsync(pol_host, s->d_pol, (size_t)s->ncell, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
@@ -1283,6 +1324,8 @@ int pg_solver_set_kernel(pg_solver* s, int32_t kernel) {
         return fail(PG_ERR_ARG, "unknown kernel %d", kernel);
     if (kernel != PG_KERNEL_GLOBAL && s->nctl > kLdsMaxCtl)
         return fail(PG_ERR_UNSUPPORTED, "the LDS-staged sweeps hold at most %d controls", kLdsMaxCtl);
+    if (kernel != PG_KERNEL_GLOBAL && global_only_option(s))
+        return fail(PG_ERR_UNSUPPORTED, "%s runs on the global-gather kernel only", global_only_option(s));
     if (kernel == PG_KERNEL_FAST && exact_only_option(s))
         return fail(PG_ERR_UNSUPPORTED, "the precision-form kernel runs the plain default scheme only (%s set)",
                     exact_only_option(s));
@@ -1312,6 +1355,40 @@ int pg_solver_set_active(pg_solver* s, int32_t on) {
     if (!s) return fail(PG_ERR_ARG, "null solver");
     s->active_on = on != 0;
     s->act_for = -1;
+    return PG_OK;
+}
+
+int pg_solver_set_warm_start(pg_solver* s, int32_t on) {
+    if (!s) return fail(PG_ERR_ARG, "null solver");
+    if (on && (s->ooc || s->vstore != PG_VSTORE_F32))
+        return fail(PG_ERR_UNSUPPORTED, "the warm start runs on an in-HBM FP32 solver only");
+    HIPCHECK(hipSetDevice(s->ctx->device));
+    if (!on) {
+        if (s->d_wpol) HIPCHECK(hipFree(s->d_wpol));
+        s->d_wpol = nullptr;
+        s->kernel = auto_kernel(s);
+        return PG_OK;
+    }
+    if (!s->d_wpol) {
+        hipError_t e = hipMalloc(&s->d_wpol, 3 * (size_t)s->ncell);
+        if (e != hipSuccess) {
+            s->d_wpol = nullptr;
+            return fail(e == hipErrorOutOfMemory ? PG_ERR_NOMEM : PG_ERR_HIP, "warm-start store allocation failed: %s",
+                        hipGetErrorString(e));
+        }
+        HIPCHECK(hipMemsetAsync(s->d_wpol, 0, 3 * (size_t)s->ncell, s->ctx->stream));
+        HIPCHECK(hipStreamSynchronize(s->ctx->stream));
+    }
+    s->kernel = PG_KERNEL_GLOBAL;
+    return PG_OK;
+}
+
+int pg_download_warm_start(pg_solver* s, int8_t* dst) {
+    if (!s || !dst) return fail(PG_ERR_ARG, "null argument");
+    if (!s->d_wpol) return fail(PG_ERR_STATE, "the warm start is off");
+    HIPCHECK(hipSetDevice(s->ctx->device));
+    HIPCHECK(hipMemcpyAsync(dst, s->d_wpol, 3 * (size_t)s->ncell, hipMemcpyDeviceToHost, s->ctx->stream));
+    HIPCHECK(hipStreamSynchronize(s->ctx->stream));
     return PG_OK;
 }
 

@@ -52,6 +52,13 @@ struct Consts {
     int32_t cfl;
     float cfl_tb, cfl_xy, cfl_h, tau_min, tau_max, hmax;
     float probe_a[24]; // Float32(sc) / Float32(nchk), IEEE
+    // commands evaluated on the device (warm start, pattern search, ladder;
+    // PeregrineSolver.jl:1240-1343): B and the traction knee for B * u_sat,
+    // the search rounds and first radius, the ladder's rungs and ratio
+    float B[9];
+    float knee;
+    int32_t rounds, ntau;
+    float delta0, tau_ratio;
 };
 
 // Control of scan position r (0 = coast, then the slice in scan order).

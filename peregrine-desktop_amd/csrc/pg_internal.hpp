@@ -55,6 +55,8 @@ struct pg_solver {
     pgk::QuantGeom Q{};
     uint8_t* d_occ = nullptr;
     float* d_bu = nullptr;
+    float* d_uraw = nullptr;     // raw lattice commands, 3 per control
+    int8_t* d_wpol = nullptr;    // warm-start store, Int8 x 3 per cell (null: off)
     int64_t* d_seeds = nullptr;
     int64_t nseeds = 0;
     std::vector<int64_t> h_seeds;

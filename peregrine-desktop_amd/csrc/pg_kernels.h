@@ -5,17 +5,20 @@
 #include "pg_fast.hpp"
 #include "pg_f64.hpp"
 
+// One Jacobi sweep of the whole grid, global-gather kernel. uraw: the raw
+// lattice commands (3 per control); wpol: the warm-start store (Int8, 3 per
+// cell) or null.
 hipError_t pgk_launch_sweep(const pgk::Consts& K, const float* Vin, float* Vout,
-                            const uint8_t* occ, const float* bu, uint8_t* pol, double* part,
-                            unsigned* linf_bits, int write_v, hipStream_t st, int64_t* nblocks);
+                            const uint8_t* occ, const float* bu, const float* uraw, uint8_t* pol, int8_t* wpol,
+                            double* part, unsigned* linf_bits, int write_v, hipStream_t st, int64_t* nblocks);
 // Sum of part[0, n) into *out; with mid (room for kReduceBlocks doubles)
 // in two levels.
 constexpr int kReduceBlocks = 1024;
 // One Jacobi escape sweep (gpu_escape_kernel!, Gpu.jl:51-67) of the whole
 // grid, global-gather kernel.
 hipError_t pgk_launch_escape(const pgk::Consts& K, const float* Vin, float* Vout, const uint8_t* occ,
-                             const float* bu, uint8_t* pol, double* part, unsigned* linf_bits, hipStream_t st,
-                             int64_t* nblocks);
+                             const float* bu, const float* uraw, uint8_t* pol, double* part, unsigned* linf_bits,
+                             hipStream_t st, int64_t* nblocks);
 hipError_t pgk_launch_reduce(const double* part, int64_t n, double* out, double* mid, hipStream_t st);
 hipError_t pgk_launch_fill(float* V, int64_t n, float v, hipStream_t st);
 hipError_t pgk_launch_seed(float* V, const int64_t* seeds, int64_t n, hipStream_t st);

@@ -9,6 +9,7 @@
 // makes the result independent of scheduling and equal bit for bit to the
 // CPU restatement.
 #include "pg_box.hpp"
+#include "pg_tanh.hpp"
 #include "pg_kernels.h"
 
 namespace pgk {
@@ -192,16 +193,59 @@ __device__ __forceinline__ float lookahead_global(const Consts& K, const float* 
     return (c < K.cap) ? c : K.cap;
 }
 
+// B * u_sat of a raw command (body_accel_sat's first three terms per row,
+// PeregrineSolver.jl:139-162): the host precomputes these for the lattice,
+// the device for the warm-start and pattern-search commands, with the same
+// arithmetic.
+__device__ __forceinline__ void bu_of(const Consts& K, float u1r, float u2r, float u3r, float& bx, float& by,
+                                      float& bw) {
+    const float gn = pgt::traction_gain_f(u1r, u2r, u3r, K.knee);
+    const float u1 = u1r * gn, u2 = u2r * gn, u3 = u3r * gn;
+    float a[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        float t = K.B[3 * r] * u1;
+        t = t + K.B[3 * r + 1] * u2;
+        t = t + K.B[3 * r + 2] * u3;
+        a[r] = t;
+    }
+    bx = a[0]; by = a[1]; bw = a[2];
+}
+
+// proj_oct (PeregrineSolver.jl:1165-1170): onto |u1| + |u2| + |u3| = 1.
+__device__ __forceinline__ void proj_oct(float& u1, float& u2, float& u3) {
+    const float s = __builtin_fabsf(u1) + __builtin_fabsf(u2) + __builtin_fabsf(u3);
+    if (s < 1.0e-6f) { u1 = u2 = u3 = 0.0f; return; }
+    const float r = 1.0f / s;
+    u1 = u1 * r; u2 = u2 * r; u3 = u3 * r;
+}
+
+// Warm-start store entry (pol_set!, PeregrineSolver.jl:736-744).
+__device__ __forceinline__ int8_t pol_quant(float v) {
+    const float c = v < -1.0f ? -1.0f : (v > 1.0f ? 1.0f : v);
+    return (int8_t)(int32_t)__builtin_rintf(c * 127.0f);
+}
+
 // One thread per cell. ESC: the escape pass (gpu_escape_kernel!, Gpu.jl:51-67)
 // in Jacobi form: a cell with a real route (0 <= v < cap) is the boundary and
 // keeps its value; any other cell, blocked ones included (through obstacle
 // space, PeregrineSolver.jl:864-874), takes -c when its backup c beats the
 // escape time it holds (-v if v < 0, else cap).
+//
+// The candidates, in the order cell_update takes them (PeregrineSolver.jl:
+// 1240-1343), the earlier winning ties: coast; the cell's warm-start command
+// (wpol, Int8 x 1/127, when non-zero); the lattice (whole or this phase's
+// slice); K.rounds rounds of pattern search around the best command (six
+// axis moves of +-delta projected onto the octahedron, delta halving per
+// round); the horizon ladder (K.ntau > 1) and the dt rung (cfl > 0) with the
+// best command. When the value improves the winning command is written back
+// to the cell's own wpol entry (only the cell's own entry is read, so the
+// in-place update is the Jacobi one).
 template <bool WRITE_V, bool WRITE_POL, bool ESC>
 __global__ __launch_bounds__(256) void sweep_global_kernel(
     const float* Vin, float* Vout, const uint8_t* __restrict__ occ,
-    const float* __restrict__ bu, Consts K, uint8_t* __restrict__ pol,
-    double* __restrict__ part_sum, unsigned* __restrict__ linf_bits) {
+    const float* __restrict__ bu, const float* __restrict__ uraw, Consts K, uint8_t* __restrict__ pol,
+    int8_t* __restrict__ wpol, double* __restrict__ part_sum, unsigned* __restrict__ linf_bits) {
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     float dec = 0.0f;
     if (idx < K.ncell) {
@@ -218,6 +262,7 @@ __global__ __launch_bounds__(256) void sweep_global_kernel(
         const bool terminal = old >= 0.0f && old < K.cap;
         float best = K.cap;
         int bk = 0;
+        float b1 = 0.0f, b2 = 0.0f, b3 = 0.0f;
         if (ESC ? !terminal : !blocked) {
             const float x = K.lo[0] + (float)i[0] * K.step[0];
             const float y = K.lo[1] + (float)i[1] * K.step[1];
@@ -239,13 +284,67 @@ __global__ __launch_bounds__(256) void sweep_global_kernel(
                 hs = tau / (float)nsub;
             }
             best = lookahead_global<ESC>(K, Vin, og, tau, hs, nsub, x, y, h, vx, vy, w, bu[0], bu[1], bu[2]);
-            // coast, then the controls in scan order (the whole lattice, or
-            // this sweep's rotating slice)
+            if (!ESC && wpol) {
+                const float q1 = (float)wpol[3 * idx] * (1.0f / 127.0f);
+                const float q2 = (float)wpol[3 * idx + 1] * (1.0f / 127.0f);
+                const float q3 = (float)wpol[3 * idx + 2] * (1.0f / 127.0f);
+                if (__builtin_fabsf(q1) + __builtin_fabsf(q2) + __builtin_fabsf(q3) > 1.0e-6f) {
+                    float bx, by, bw;
+                    bu_of(K, q1, q2, q3, bx, by, bw);
+                    const float c = lookahead_global<ESC>(K, Vin, og, tau, hs, nsub, x, y, h, vx, vy, w, bx, by, bw);
+                    if (c < best) { best = c; b1 = q1; b2 = q2; b3 = q3; bk = -1; }
+                }
+            }
+            // the controls in scan order (the whole lattice, or this
+            // sweep's rotating slice)
             for (int r = 1; r <= K.span; ++r) {
                 const int k = slice_control(K, r);
                 const float c = lookahead_global<ESC>(K, Vin, og, tau, hs, nsub, x, y, h, vx, vy, w, bu[3 * k],
                                                       bu[3 * k + 1], bu[3 * k + 2]);
-                if (c < best) { best = c; bk = k; }
+                if (c < best) {
+                    best = c; bk = k;
+                    b1 = uraw[3 * k]; b2 = uraw[3 * k + 1]; b3 = uraw[3 * k + 2];
+                }
+            }
+            float delta = K.delta0;
+            for (int rd = 0; rd < K.rounds; ++rd) {
+                for (int d = 1; d <= 6; ++d) {
+                    const float sg = (d % 2 == 0) ? -delta : delta;
+                    float c1 = b1 + (d <= 2 ? sg : 0.0f);
+                    float c2 = b2 + ((d == 3 || d == 4) ? sg : 0.0f);
+                    float c3 = b3 + (d >= 5 ? sg : 0.0f);
+                    proj_oct(c1, c2, c3);
+                    if (__builtin_fabsf(c1) + __builtin_fabsf(c2) + __builtin_fabsf(c3) < 1.0e-6f) continue;
+                    float bx, by, bw;
+                    bu_of(K, c1, c2, c3, bx, by, bw);
+                    const float c = lookahead_global<ESC>(K, Vin, og, tau, hs, nsub, x, y, h, vx, vy, w, bx, by, bw);
+                    if (c < best) { best = c; b1 = c1; b2 = c2; b3 = c3; bk = -1; }
+                }
+                delta = delta * 0.5f;
+            }
+            if (K.ntau > 1) {
+                float bx, by, bw;
+                bu_of(K, b1, b2, b3, bx, by, bw);
+                float tl = tau * 0.5f;
+                for (int k = 1; k <= K.ntau; ++k) {
+                    if (k != 2) {
+                        float t = tl > K.tau_min ? tl : K.tau_min;
+                        t = t < K.tau_max ? t : K.tau_max;
+                        const int ns = substeps_dev(K, t);
+                        const float c = lookahead_global<ESC>(K, Vin, og, t, t / (float)ns, ns, x, y, h, vx, vy, w,
+                                                              bx, by, bw);
+                        if (c < best) best = c;
+                    }
+                    tl = tl * K.tau_ratio;
+                }
+                if (K.cfl) {
+                    float t = K.tau > K.tau_min ? K.tau : K.tau_min;
+                    t = t < K.tau_max ? t : K.tau_max;
+                    const int ns = substeps_dev(K, t);
+                    const float c = lookahead_global<ESC>(K, Vin, og, t, t / (float)ns, ns, x, y, h, vx, vy, w,
+                                                          bx, by, bw);
+                    if (c < best) best = c;
+                }
             }
         }
         if (WRITE_V) {
@@ -256,6 +355,11 @@ __global__ __launch_bounds__(256) void sweep_global_kernel(
             } else if (best < old) {
                 out = best;
                 dec = old - best;
+                if (wpol) {
+                    wpol[3 * idx] = pol_quant(b1);
+                    wpol[3 * idx + 1] = pol_quant(b2);
+                    wpol[3 * idx + 2] = pol_quant(b3);
+                }
             }
             Vout[idx] = out;
         }
@@ -320,33 +424,33 @@ __global__ void seed_mark_kernel(float* __restrict__ V, const int64_t* __restric
 // ---------------------------------------------------------------------------
 
 hipError_t pgk_launch_sweep(const pgk::Consts& K, const float* Vin, float* Vout,
-                            const uint8_t* occ, const float* bu, uint8_t* pol, double* part,
-                            unsigned* linf_bits, int write_v, hipStream_t st, int64_t* nblocks) {
+                            const uint8_t* occ, const float* bu, const float* uraw, uint8_t* pol, int8_t* wpol,
+                            double* part, unsigned* linf_bits, int write_v, hipStream_t st, int64_t* nblocks) {
     const int64_t nb = (K.ncell + 255) / 256;
     *nblocks = nb;
     if (write_v && pol)
         hipLaunchKernelGGL((pgk::sweep_global_kernel<true, true, false>), dim3((unsigned)nb), dim3(256),
-                           0, st, Vin, Vout, occ, bu, K, pol, part, linf_bits);
+                           0, st, Vin, Vout, occ, bu, uraw, K, pol, wpol, part, linf_bits);
     else if (write_v)
         hipLaunchKernelGGL((pgk::sweep_global_kernel<true, false, false>), dim3((unsigned)nb), dim3(256),
-                           0, st, Vin, Vout, occ, bu, K, pol, part, linf_bits);
+                           0, st, Vin, Vout, occ, bu, uraw, K, pol, wpol, part, linf_bits);
     else
         hipLaunchKernelGGL((pgk::sweep_global_kernel<false, true, false>), dim3((unsigned)nb), dim3(256),
-                           0, st, Vin, Vout, occ, bu, K, pol, part, linf_bits);
+                           0, st, Vin, Vout, occ, bu, uraw, K, pol, wpol, part, linf_bits);
     return hipGetLastError();
 }
 
 hipError_t pgk_launch_escape(const pgk::Consts& K, const float* Vin, float* Vout, const uint8_t* occ,
-                             const float* bu, uint8_t* pol, double* part, unsigned* linf_bits, hipStream_t st,
-                             int64_t* nblocks) {
+                             const float* bu, const float* uraw, uint8_t* pol, double* part, unsigned* linf_bits,
+                             hipStream_t st, int64_t* nblocks) {
     const int64_t nb = (K.ncell + 255) / 256;
     *nblocks = nb;
     if (pol)
         hipLaunchKernelGGL((pgk::sweep_global_kernel<true, true, true>), dim3((unsigned)nb), dim3(256), 0, st,
-                           Vin, Vout, occ, bu, K, pol, part, linf_bits);
+                           Vin, Vout, occ, bu, uraw, K, pol, nullptr, part, linf_bits);
     else
         hipLaunchKernelGGL((pgk::sweep_global_kernel<true, false, true>), dim3((unsigned)nb), dim3(256), 0, st,
-                           Vin, Vout, occ, bu, K, pol, part, linf_bits);
+                           Vin, Vout, occ, bu, uraw, K, pol, nullptr, part, linf_bits);
     return hipGetLastError();
 }
 

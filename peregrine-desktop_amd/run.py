@@ -12,10 +12,10 @@ Run.jl:1434-2133) with its PROGRESS stream, MANIFEST.JSON and MODEL.JSON
 (Run.jl:1397-1432) and a `plan` report (Run.jl:699-959).
 
 The value solve runs on the device through libperegrine_hip; there is no
-CPU backend (`backend: "cpu"` is refused). Scheme options the device
-kernels do not implement (`tau_levels` > 1, `refine_rounds` > 0,
-`warm_start`) are refused with the setting that avoids them, never
-silently dropped.
+CPU backend (`backend: "cpu"` is refused). Scheme options the tiled
+(out-of-core) driver does not implement (`tau_levels` > 1,
+`refine_rounds` > 0, `warm_start`, the escape pass) are refused there with
+the setting that avoids them, never silently dropped.
 """
 from __future__ import annotations
 
@@ -344,10 +344,13 @@ def solver_params(cfg, g: Grid6) -> SolverParams:
                         ttol=ttol, tau_auto=auto)
 
 
-def unsupported_options(sp: SolverParams) -> list:
-    """Scheme settings the device kernels do not implement, each with the
-    setting that avoids it."""
+def unsupported_options(sp: SolverParams, ooc: bool) -> list:
+    """Scheme settings the chosen driver does not implement, each with the
+    setting that avoids it: the tiled driver runs neither the pattern search,
+    the horizon ladder nor the warm start."""
     out = []
+    if not ooc:
+        return out
     if sp.p.ntau > 1:
         out.append(f"tau_levels {sp.p.ntau} (set tau_levels: 1)")
     if sp.p.rounds > 0:
@@ -698,7 +701,7 @@ def plan(cfg) -> dict:
         "out_of_core": {"mode": ("incore" if fits else "ooc") if avail else None},
         "grid_balance": bal["ratio"], "dv_distance_cm": bal["dv_distance_cm"],
         "cell_cm": float(min(g.step[0], g.step[1])), "cell_cm_s": float(min(g.step[3], g.step[4])),
-        "controls": sp.nctl, "unsupported_options": unsupported_options(sp),
+        "controls": sp.nctl, "unsupported_options": unsupported_options(sp, avail and not fits),
     }
 
 
@@ -764,9 +767,6 @@ def run_solve(cfg) -> dict:
         raise SolveError("chunk_elements must be a power of two")
     sp = solver_params(cfg, g)
     p, cap = sp.p, sp.cap
-    bad = unsupported_options(sp)
-    if bad:
-        raise SolveError("this solver's device kernels do not implement " + ", ".join(bad))
     backend = str(getc(cfg, "backend", "auto"))
     if backend not in ("auto", "cuda", "hip"):
         raise SolveError(f"backend '{backend}' is not built: this solver runs the sweep on the GPU only")
@@ -805,9 +805,10 @@ def run_solve(cfg) -> dict:
                          "whole table would come out unreachable:\n" + "\n".join(lines))
     fits, _, _ = solver.gpu_fits(g)
     ooc = not fits
-    if ooc and do_escape:
-        raise SolveError("the grid does not fit the device whole and the escape pass is not built for the "
-                         "tiled driver; set escape: false or reduce the grid")
+    bad = unsupported_options(sp, ooc) + (["the escape pass (set escape: false)"] if ooc and do_escape else [])
+    if bad:
+        raise SolveError("the grid does not fit the device whole, and the tiled driver does not run "
+                         + ", ".join(bad))
     progress(phase="backend", backend="hip_tiled" if ooc else "hip")
     tables_dir = os.path.join(out_dir, "TABLES")
     os.makedirs(tables_dir, exist_ok=True)
@@ -823,6 +824,8 @@ def run_solve(cfg) -> dict:
     entries = []
     kw = dict(ooc_budget=solver.gpu_budget()) if ooc else {}
     with solver.HipSolver(g, m, p, sp.ctl, occ, **kw) as s:
+        if sp.warm:
+            s.set_warm_start(True)
         for ti0, st in enumerate(states):
             ti = ti0 + 1
             seeds = target_cells(g, st, sp.ttol)

@@ -202,6 +202,16 @@ class HipSolver:
         """abi.PG_SWEEP_JACOBI (default, deterministic) or abi.PG_SWEEP_IN_PLACE."""
         abi.check(self.lib, "pg_solver_set_sweep_mode", self.lib.pg_solver_set_sweep_mode(self._h, int(mode)))
 
+    def set_warm_start(self, on: bool) -> None:
+        """The Int8 warm-start policy store (Run.jl:1782, PeregrineSolver.jl:
+        728-744): zeroed by fill_seed, read and written by every sweep."""
+        abi.check(self.lib, "pg_solver_set_warm_start", self.lib.pg_solver_set_warm_start(self._h, int(bool(on))))
+
+    def download_warm_start(self) -> np.ndarray:
+        out = np.empty(3 * self.g.ncells, np.int8)
+        abi.check(self.lib, "pg_download_warm_start", self.lib.pg_download_warm_start(self._h, out.ctypes.data))
+        return out
+
     def set_active(self, on: bool) -> None:
         """Skip the boxes whose inputs did not change (default on; results
         are bit for bit those of computing every box)."""

@@ -32,7 +32,7 @@ def config(tmp_path, **kw):
            "targets": str(GOLD / "targets.json"), "out_dir": str(tmp_path / "out"),
            "grid": {"vmax": 120.0, "wmax": 4.0,
                     "resolution": {"xy_cm": 20.0, "heading_deg": 45.0, "v_cm_s": 60.0, "w_rad_s": 2.0}},
-           "tau_levels": 1, "refine_rounds": 0, "warm_start": False}
+           }
     cfg.update(kw)
     path = tmp_path / "cfg.json"
     path.write_text(json.dumps(cfg))
@@ -206,14 +206,9 @@ def test_cli_modes_and_exit_codes(tmp_path):
     assert p["cells"] == math.prod(p["n"]) and p["targets_ok"] and p["unsupported_options"] == []
 
 
-def test_solve_refuses_unbuilt_scheme_options(tmp_path):
-    """The reference defaults (5-rung horizon ladder, pattern search, warm
-    start) are refused with the setting that avoids each, before any work."""
-    cfg, path = config(tmp_path)
-    for k in ("tau_levels", "refine_rounds", "warm_start"):
-        del cfg[k]
-    path.write_text(json.dumps(cfg))
+def test_solve_refuses_a_cpu_backend(tmp_path):
+    """There is no CPU path: backend "cpu" is an error before any work."""
+    cfg, path = config(tmp_path, backend="cpu")
     r = cli(path, "solve")
-    assert r.returncode == 1
-    assert "tau_levels: 1" in r.stderr and "refine_rounds: 0" in r.stderr and "warm_start: false" in r.stderr
+    assert r.returncode == 1 and r.stderr.startswith("ERROR ") and "backend 'cpu'" in r.stderr
     assert not os.path.exists(cfg["out_dir"])

@@ -1,8 +1,9 @@
 """The solve.jl job interface end to end on the device: `solve` mode on a
-small field with a square robot and one obstacle, the reference's
-production scheme options that the device kernels implement (Kuhn simplex
-lookups, RK2, per-cell CFL horizon, adaptive probes, a rotating control
-slice of 8 over the level-4 octahedron lattice) and the escape pass.
+small field with a square robot and one obstacle, under the reference's
+default scheme (Kuhn simplex lookups, RK2, per-cell CFL horizon with a
+5-rung ladder, adaptive probes, a rotating slice of 8 over the level-4
+octahedron lattice, 2 rounds of pattern search, the Int8 warm start) and
+the escape pass.
 
 Parity: every chunk file equals the oracle's table, solved with the same
 sweep count and encoded by the card writer, byte for byte. Contract: the
@@ -54,9 +55,13 @@ def oracle_tables(pg, orc, cfg, run):
         op = oracle_problem(orc, Problem(0, g, m, sp.p, sp.ctl, [], occ, seeds))
         V = np.full(g.ncells, cap, np.float32)
         V[seeds] = 0
+        wpol = np.zeros(3 * g.ncells, np.int8)
         for it in range(ITERS):
             W = np.empty_like(V)
-            op.sweep(V, W, phase=it, nthreads=nthreads())
+            if sp.warm:
+                op.sweep_warm(V, W, wpol, phase=it, nthreads=nthreads())
+            else:
+                op.sweep(V, W, phase=it, nthreads=nthreads())
             W[seeds] = 0
             V = W
         for it in range(ESC_ITERS):
