@@ -300,6 +300,44 @@ def run_ooc(pg, prob, args) -> dict:
             "bit_identical_to_in_hbm": (all(equal) if equal else None)}
 
 
+def run_reference_scheme(pg, prob, args) -> dict:
+    """Sweeps of the reference's default solve.jl scheme on this problem's
+    grid, model and obstacles: level-4 octahedron lattice (67 commands) with a
+    rotating slice of 8, 2 pattern-search rounds, the Int8 warm start, Kuhn
+    lookups, RK2, a per-cell CFL horizon (cfl 2) with a 5-rung ladder and
+    adaptive probes (Run.jl:203-294 defaults)."""
+    from dataclasses import replace
+    g = prob.grid
+    p = replace(prob.params, adaptive_checks=True, ntau=5, tau_ratio=2.0, cfl=2.0, tau_min=0.004, tau_max=0.5,
+                ncoarse=8, rounds=2, delta0=0.35, rk2=True, vclamp=False, simplex=True)
+    ctl = pg.control_set(4)
+    s = pg.HipSolver(g, prob.model, p, ctl, prob.occ)
+    s.set_warm_start(True)
+    s.fill_seed(prob.seeds)
+    for i in range(args.warmup):
+        r = s.sweep(i, i % 2 == 1)
+        print(f"# reference-scheme warmup sweep {i}: {r.kernel_ms:.1f} ms kernel", file=sys.stderr, flush=True)
+    steps = args.steps if args.steps is not None else 5
+    kms = []
+    t = time.perf_counter()
+    for i in range(steps):
+        r = s.sweep(args.warmup + i, (args.warmup + i) % 2 == 1)
+        kms.append(r.kernel_ms)
+        print(f"# reference-scheme sweep {i}: {r.kernel_ms:.1f} ms kernel, linf {r.linf:.4f}", file=sys.stderr,
+              flush=True)
+    wall = time.perf_counter() - t
+    kern, _ = s.kernel_info()
+    s.close()
+    return {"metric": "cell-updates/s, the reference's default solve.jl scheme", "value": g.ncells * steps / wall,
+            "unit": "cell-updates/s", "steps": steps, "warmup": args.warmup, "ms_per_step": wall / steps * 1e3,
+            "kernel_ms": float(np.mean(kms)), "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+            "kernel": {1: "global-gather", 2: "LDS-staged"}.get(kern, str(kern)),
+            "config": {"workload": f"problem {args.config} grid, model and obstacles, reference default scheme",
+                       "grid": list(g.n), "states": g.ncells, "controls": int(ctl.shape[0]), "control_scan": 8,
+                       "refine_rounds": 2, "tau_levels": 5, "cfl": 2.0, "warm_start": True, "simplex": True,
+                       "rk2": True, "adaptive_checks": True}}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -317,6 +355,8 @@ def main() -> int:
     ap.add_argument("--fd-params", default="0,4,8,9,13,17,19,21,25,27,28,29")
     ap.add_argument("--fd-step", type=float, default=1e-4)
     ap.add_argument("--ooc", action="store_true")
+    ap.add_argument("--reference-scheme", action="store_true",
+                    help="sweeps of the reference's default solve.jl scheme (global-gather kernel)")
     ap.add_argument("--ooc-budget", type=float, default=8e9)
     ap.add_argument("--ooc-tile", default="16,16,8")
     ap.add_argument("--no-ooc-compare", action="store_true")
@@ -360,6 +400,9 @@ def main() -> int:
           file=sys.stderr, flush=True)
     if args.ooc:
         print(json.dumps(run_ooc(pg, prob, args)), flush=True)
+        return 0
+    if args.reference_scheme:
+        print(json.dumps(run_reference_scheme(pg, prob, args)), flush=True)
         return 0
     if args.adjoint:
         args.f64_tol = min(args.f64_tol, 1e-12)

@@ -233,15 +233,16 @@ __device__ __forceinline__ int8_t pol_quant(float v) {
 // escape time it holds (-v if v < 0, else cap).
 //
 // The candidates, in the order cell_update takes them (PeregrineSolver.jl:
-// 1240-1343), the earlier winning ties: coast; the cell's warm-start command
-// (wpol, Int8 x 1/127, when non-zero); the lattice (whole or this phase's
-// slice); K.rounds rounds of pattern search around the best command (six
-// axis moves of +-delta projected onto the octahedron, delta halving per
-// round); the horizon ladder (K.ntau > 1) and the dt rung (cfl > 0) with the
-// best command. When the value improves the winning command is written back
-// to the cell's own wpol entry (only the cell's own entry is read, so the
-// in-place update is the Jacobi one).
-template <bool WRITE_V, bool WRITE_POL, bool ESC>
+// 1240-1343), the earlier winning ties: coast; with PROD, the cell's
+// warm-start command (wpol, Int8 x 1/127, when non-zero); the lattice (whole
+// or this phase's slice); with PROD, K.rounds rounds of pattern search around
+// the best command (six axis moves of +-delta projected onto the octahedron,
+// delta halving per round), the horizon ladder (K.ntau > 1) and the dt rung
+// (cfl > 0) with the best command. One loop walks the candidate slots so
+// that the lookahead is inlined once. When the value improves the winning
+// command is written back to the cell's own wpol entry (only the cell's own
+// entry is read, so the in-place update is the Jacobi one).
+template <bool WRITE_V, bool WRITE_POL, bool ESC, bool PROD>
 __global__ __launch_bounds__(256) void sweep_global_kernel(
     const float* Vin, float* Vout, const uint8_t* __restrict__ occ,
     const float* __restrict__ bu, const float* __restrict__ uraw, Consts K, uint8_t* __restrict__ pol,
@@ -283,67 +284,64 @@ __global__ __launch_bounds__(256) void sweep_global_kernel(
                 nsub = substeps_dev(K, tau);
                 hs = tau / (float)nsub;
             }
-            best = lookahead_global<ESC>(K, Vin, og, tau, hs, nsub, x, y, h, vx, vy, w, bu[0], bu[1], bu[2]);
-            if (!ESC && wpol) {
-                const float q1 = (float)wpol[3 * idx] * (1.0f / 127.0f);
-                const float q2 = (float)wpol[3 * idx + 1] * (1.0f / 127.0f);
-                const float q3 = (float)wpol[3 * idx + 2] * (1.0f / 127.0f);
-                if (__builtin_fabsf(q1) + __builtin_fabsf(q2) + __builtin_fabsf(q3) > 1.0e-6f) {
-                    float bx, by, bw;
-                    bu_of(K, q1, q2, q3, bx, by, bw);
-                    const float c = lookahead_global<ESC>(K, Vin, og, tau, hs, nsub, x, y, h, vx, vy, w, bx, by, bw);
-                    if (c < best) { best = c; b1 = q1; b2 = q2; b3 = q3; bk = -1; }
-                }
-            }
-            // the controls in scan order (the whole lattice, or this
-            // sweep's rotating slice)
-            for (int r = 1; r <= K.span; ++r) {
-                const int k = slice_control(K, r);
-                const float c = lookahead_global<ESC>(K, Vin, og, tau, hs, nsub, x, y, h, vx, vy, w, bu[3 * k],
-                                                      bu[3 * k + 1], bu[3 * k + 2]);
-                if (c < best) {
-                    best = c; bk = k;
-                    b1 = uraw[3 * k]; b2 = uraw[3 * k + 1]; b3 = uraw[3 * k + 2];
-                }
-            }
-            float delta = K.delta0;
-            for (int rd = 0; rd < K.rounds; ++rd) {
-                for (int d = 1; d <= 6; ++d) {
+            // slots: 0 coast, 1 warm start, 2 .. 1+span lattice, then 6 per
+            // search round, then ntau ladder rungs and the dt rung
+            const int s_lat = 2, s_pat = 2 + K.span;
+            const int s_lad = PROD ? s_pat + 6 * K.rounds : s_pat;
+            const int s_end = PROD ? (K.ntau > 1 ? s_lad + K.ntau + 1 : s_lad) : s_pat;
+            float tl = tau * 0.5f;        // the ladder's running horizon
+            float lx = 0.0f, ly = 0.0f, lw = 0.0f;   // B * u_sat of the ladder's command
+            for (int slot = 0; slot < s_end; ++slot) {
+                float c1 = 0.0f, c2 = 0.0f, c3 = 0.0f, bx, by, bw, t = tau, ths = hs;
+                int tn = nsub, kk = -1;
+                bool ladder = false;
+                if (slot == 0) {
+                    bx = bu[0]; by = bu[1]; bw = bu[2];
+                    kk = 0;
+                } else if (slot < s_lat) {
+                    if (!PROD || ESC || !wpol) continue;
+                    c1 = (float)wpol[3 * idx] * (1.0f / 127.0f);
+                    c2 = (float)wpol[3 * idx + 1] * (1.0f / 127.0f);
+                    c3 = (float)wpol[3 * idx + 2] * (1.0f / 127.0f);
+                    if (!(__builtin_fabsf(c1) + __builtin_fabsf(c2) + __builtin_fabsf(c3) > 1.0e-6f)) continue;
+                    bu_of(K, c1, c2, c3, bx, by, bw);
+                } else if (slot < s_pat) {
+                    kk = slice_control(K, slot - 1);
+                    bx = bu[3 * kk]; by = bu[3 * kk + 1]; bw = bu[3 * kk + 2];
+                    if (PROD) { c1 = uraw[3 * kk]; c2 = uraw[3 * kk + 1]; c3 = uraw[3 * kk + 2]; }
+                } else if (slot < s_lad) {
+                    const int j = slot - s_pat, d = j % 6 + 1;
+                    // delta0 halved once per finished round (exact: a power of two)
+                    const float delta = __builtin_ldexpf(K.delta0, -(j / 6));
                     const float sg = (d % 2 == 0) ? -delta : delta;
-                    float c1 = b1 + (d <= 2 ? sg : 0.0f);
-                    float c2 = b2 + ((d == 3 || d == 4) ? sg : 0.0f);
-                    float c3 = b3 + (d >= 5 ? sg : 0.0f);
+                    c1 = b1 + (d <= 2 ? sg : 0.0f);
+                    c2 = b2 + ((d == 3 || d == 4) ? sg : 0.0f);
+                    c3 = b3 + (d >= 5 ? sg : 0.0f);
                     proj_oct(c1, c2, c3);
                     if (__builtin_fabsf(c1) + __builtin_fabsf(c2) + __builtin_fabsf(c3) < 1.0e-6f) continue;
-                    float bx, by, bw;
                     bu_of(K, c1, c2, c3, bx, by, bw);
-                    const float c = lookahead_global<ESC>(K, Vin, og, tau, hs, nsub, x, y, h, vx, vy, w, bx, by, bw);
-                    if (c < best) { best = c; b1 = c1; b2 = c2; b3 = c3; bk = -1; }
-                }
-                delta = delta * 0.5f;
-            }
-            if (K.ntau > 1) {
-                float bx, by, bw;
-                bu_of(K, b1, b2, b3, bx, by, bw);
-                float tl = tau * 0.5f;
-                for (int k = 1; k <= K.ntau; ++k) {
-                    if (k != 2) {
-                        float t = tl > K.tau_min ? tl : K.tau_min;
-                        t = t < K.tau_max ? t : K.tau_max;
-                        const int ns = substeps_dev(K, t);
-                        const float c = lookahead_global<ESC>(K, Vin, og, t, t / (float)ns, ns, x, y, h, vx, vy, w,
-                                                              bx, by, bw);
-                        if (c < best) best = c;
+                } else {
+                    const int k = slot - s_lad + 1;   // rung k, or ntau + 1 for the dt rung
+                    if (k == 1) bu_of(K, b1, b2, b3, lx, ly, lw);
+                    if (k <= K.ntau) {
+                        t = tl;
+                        tl = tl * K.tau_ratio;
+                        if (k == 2) continue;
+                    } else {
+                        if (!K.cfl) continue;
+                        t = K.tau;
                     }
-                    tl = tl * K.tau_ratio;
-                }
-                if (K.cfl) {
-                    float t = K.tau > K.tau_min ? K.tau : K.tau_min;
+                    t = t > K.tau_min ? t : K.tau_min;
                     t = t < K.tau_max ? t : K.tau_max;
-                    const int ns = substeps_dev(K, t);
-                    const float c = lookahead_global<ESC>(K, Vin, og, t, t / (float)ns, ns, x, y, h, vx, vy, w,
-                                                          bx, by, bw);
-                    if (c < best) best = c;
+                    tn = substeps_dev(K, t);
+                    ths = t / (float)tn;
+                    bx = lx; by = ly; bw = lw;
+                    ladder = true;
+                }
+                const float c = lookahead_global<ESC>(K, Vin, og, t, ths, tn, x, y, h, vx, vy, w, bx, by, bw);
+                if (c < best) {
+                    best = c;
+                    if (!ladder) { bk = kk; b1 = c1; b2 = c2; b3 = c3; }
                 }
             }
         }
@@ -355,7 +353,7 @@ __global__ __launch_bounds__(256) void sweep_global_kernel(
             } else if (best < old) {
                 out = best;
                 dec = old - best;
-                if (wpol) {
+                if (PROD && wpol) {
                     wpol[3 * idx] = pol_quant(b1);
                     wpol[3 * idx + 1] = pol_quant(b2);
                     wpol[3 * idx + 2] = pol_quant(b3);
@@ -423,20 +421,35 @@ __global__ void seed_mark_kernel(float* __restrict__ V, const int64_t* __restric
 // launch wrappers (host side)
 // ---------------------------------------------------------------------------
 
+// Whether a sweep needs the production candidates (warm start, pattern
+// search, horizon ladder).
+static bool prod_scheme(const pgk::Consts& K, const int8_t* wpol) {
+    return wpol != nullptr || K.rounds > 0 || K.ntau > 1;
+}
+
+template <bool WV, bool WP, bool ESC>
+static void launch_global(const pgk::Consts& K, int64_t nb, hipStream_t st, const float* Vin, float* Vout,
+                          const uint8_t* occ, const float* bu, const float* uraw, uint8_t* pol, int8_t* wpol,
+                          double* part, unsigned* linf_bits) {
+    if (prod_scheme(K, wpol))
+        hipLaunchKernelGGL((pgk::sweep_global_kernel<WV, WP, ESC, true>), dim3((unsigned)nb), dim3(256), 0, st, Vin,
+                           Vout, occ, bu, uraw, K, pol, wpol, part, linf_bits);
+    else
+        hipLaunchKernelGGL((pgk::sweep_global_kernel<WV, WP, ESC, false>), dim3((unsigned)nb), dim3(256), 0, st, Vin,
+                           Vout, occ, bu, uraw, K, pol, wpol, part, linf_bits);
+}
+
 hipError_t pgk_launch_sweep(const pgk::Consts& K, const float* Vin, float* Vout,
                             const uint8_t* occ, const float* bu, const float* uraw, uint8_t* pol, int8_t* wpol,
                             double* part, unsigned* linf_bits, int write_v, hipStream_t st, int64_t* nblocks) {
     const int64_t nb = (K.ncell + 255) / 256;
     *nblocks = nb;
     if (write_v && pol)
-        hipLaunchKernelGGL((pgk::sweep_global_kernel<true, true, false>), dim3((unsigned)nb), dim3(256),
-                           0, st, Vin, Vout, occ, bu, uraw, K, pol, wpol, part, linf_bits);
+        launch_global<true, true, false>(K, nb, st, Vin, Vout, occ, bu, uraw, pol, wpol, part, linf_bits);
     else if (write_v)
-        hipLaunchKernelGGL((pgk::sweep_global_kernel<true, false, false>), dim3((unsigned)nb), dim3(256),
-                           0, st, Vin, Vout, occ, bu, uraw, K, pol, wpol, part, linf_bits);
+        launch_global<true, false, false>(K, nb, st, Vin, Vout, occ, bu, uraw, pol, wpol, part, linf_bits);
     else
-        hipLaunchKernelGGL((pgk::sweep_global_kernel<false, true, false>), dim3((unsigned)nb), dim3(256),
-                           0, st, Vin, Vout, occ, bu, uraw, K, pol, wpol, part, linf_bits);
+        launch_global<false, true, false>(K, nb, st, Vin, Vout, occ, bu, uraw, pol, wpol, part, linf_bits);
     return hipGetLastError();
 }
 
@@ -446,11 +459,9 @@ hipError_t pgk_launch_escape(const pgk::Consts& K, const float* Vin, float* Vout
     const int64_t nb = (K.ncell + 255) / 256;
     *nblocks = nb;
     if (pol)
-        hipLaunchKernelGGL((pgk::sweep_global_kernel<true, true, true>), dim3((unsigned)nb), dim3(256), 0, st,
-                           Vin, Vout, occ, bu, uraw, K, pol, nullptr, part, linf_bits);
+        launch_global<true, true, true>(K, nb, st, Vin, Vout, occ, bu, uraw, pol, nullptr, part, linf_bits);
     else
-        hipLaunchKernelGGL((pgk::sweep_global_kernel<true, false, true>), dim3((unsigned)nb), dim3(256), 0, st,
-                           Vin, Vout, occ, bu, uraw, K, pol, nullptr, part, linf_bits);
+        launch_global<true, false, true>(K, nb, st, Vin, Vout, occ, bu, uraw, pol, nullptr, part, linf_bits);
     return hipGetLastError();
 }
 
