@@ -650,11 +650,45 @@ def _device_fit(g: Grid6):
         return False, False, g.ncells * 4 / 2 ** 30 * 2, 0.0
 
 
+# Seconds per lookahead on one MI355X, measured: the global-gather kernel
+# running the default scheme (1.521 s for 564,854,784 states of at most 27
+# lookaheads each) and the LDS-staged kernel running the plain scheme (1.532 s
+# for the same states x 125 lookaheads). The escape pass costs its share of a
+# value sweep (0.059 s against 0.107 s on a 35.8M-state default job).
+LOOKAHEAD_S = {"global": 1.521 / (564854784 * 27), "lds": 1.532 / (564854784 * 125)}
+ESCAPE_SHARE = 0.059 / 0.107
+
+
+def lookaheads_per_state(p: Params, nctl: int, warm: bool) -> int:
+    """The most lookaheads one backup evaluates: coast, the warm start, the
+    lattice or its slice, the search moves, the ladder rungs and dt rung."""
+    nnc = max(nctl - 1, 1)
+    span = nnc if p.ncoarse <= 0 else min(p.ncoarse, nnc)
+    n = 1 + int(warm) + span + 6 * max(p.rounds, 0)
+    if p.ntau > 1:
+        n += p.ntau - 1 + (1 if p.cfl > 0 else 0)
+    return n
+
+
+def runtime_estimate(g: Grid6, sp: SolverParams, esc_iters: int, ntargets: int) -> dict:
+    """An upper bound on the run's time from the sweep budget and the
+    measured per-lookahead cost of the kernel the scheme runs on (the
+    iteration budget; the tolerance usually stops sooner)."""
+    p = sp.p
+    kernel = "global" if (p.rounds > 0 or p.ntau > 1 or sp.warm) else "lds"
+    unit_s = g.ncells * lookaheads_per_state(p, sp.nctl, sp.warm) * LOOKAHEAD_S[kernel]
+    units = sp.iters + esc_iters
+    per_target = sp.iters * unit_s + esc_iters * unit_s * ESCAPE_SHARE
+    return {"unit": "sweep", "unit_s": unit_s, "units": units, "escape_sweeps": esc_iters,
+            "total_s": per_target * ntargets, "kernel": kernel,
+            "lookaheads_per_state": lookaheads_per_state(p, sp.nctl, sp.warm)}
+
+
 def plan(cfg) -> dict:
     """Size and feasibility report: grid, bounds, resolution, occupancy,
-    target checks, bytes on the card and whether the table fits the device.
-    The Advise resolution/horizon recommendations and runtime estimate are
-    not part of this report."""
+    target checks, bytes on the card, whether the table fits the device, the
+    horizon in force and a run-time bound from measured kernel costs. The
+    reference's resolution and tiling advice (Advise.jl) is not part of it."""
     bounds, polys, robot, gcfg, gi, g = _setup(cfg)
     m, _, _ = _model_for(cfg)
     names, states, _ = load_targets(str(cfg["targets"]))
@@ -702,6 +736,15 @@ def plan(cfg) -> dict:
         "grid_balance": bal["ratio"], "dv_distance_cm": bal["dv_distance_cm"],
         "cell_cm": float(min(g.step[0], g.step[1])), "cell_cm_s": float(min(g.step[3], g.step[4])),
         "controls": sp.nctl, "unsupported_options": unsupported_options(sp, avail and not fits),
+        "recommend": {
+            "tau_applied": float(sp.p.tau_max), "tau_applied_cost": value_cost(sp.p.tau_max, TAU_REF),
+            "tau_auto": sp.tau_auto, "tau_reference": TAU_REF,
+            "tau_pinned_waste": (not sp.tau_auto) and bool(fits) and sp.p.tau_max < TAU_REF,
+            "tau_reason": "the grid is held whole in HBM, so the full reference horizon costs no halo"
+                          if fits else None,
+        },
+        "runtime": runtime_estimate(g, sp, int(getc(cfg, "escape_iterations", 60))
+                                    if bool(getc(cfg, "escape", True)) else 0, len(names)),
     }
 
 

@@ -204,6 +204,12 @@ def test_cli_modes_and_exit_codes(tmp_path):
     assert r.returncode == 0
     p = json.loads(r.stdout.split("PLAN ", 1)[1])
     assert p["cells"] == math.prod(p["n"]) and p["targets_ok"] and p["unsupported_options"] == []
+    # the default scheme: coast, warm start, a slice of 8, 2 x 6 search moves,
+    # 4 ladder rungs (rung 2 is the cell's own horizon) and the dt rung
+    rt = p["runtime"]
+    assert rt["lookaheads_per_state"] == 1 + 1 + 8 + 12 + 4 + 1 and rt["kernel"] == "global"
+    assert rt["units"] == 400 + 60 and rt["total_s"] > 0
+    assert p["recommend"]["tau_applied"] == 0.5 and p["recommend"]["tau_auto"]
 
 
 def test_solve_refuses_a_cpu_backend(tmp_path):
