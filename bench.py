@@ -242,7 +242,9 @@ def run_ooc(pg, prob, args) -> dict:
     g = prob.grid
     tile = tuple(int(v) for v in args.ooc_tile.split(","))
     t = time.perf_counter()
-    s = pg.HipSolver(g, prob.model, prob.params, prob.ctl, prob.occ, ooc_budget=int(args.ooc_budget), ooc_tile=tile)
+    vstore = pg.abi.PG_VSTORE_U16_TILE if args.ooc_store == "u16" else pg.abi.PG_VSTORE_F32
+    s = pg.HipSolver(g, prob.model, prob.params, prob.ctl, prob.occ, vstore=vstore, ooc_budget=int(args.ooc_budget),
+                     ooc_tile=tile)
     info = s.ooc_info()
     print(f"# ooc: tiles {info[0]} x {info[1]}, {info[2] / 1e9:.2f} GB on the device, amplification {info[3]:.2f}, "
           f"set up in {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
@@ -266,8 +268,10 @@ def run_ooc(pg, prob, args) -> dict:
     note("seeded")
     ref = None
     if not args.no_ooc_compare:
-        ref = pg.HipSolver(g, prob.model, prob.params, prob.ctl, prob.occ)
-        ref.set_kernel(pg.abi.PG_KERNEL_LDS)
+        ref = pg.HipSolver(g, prob.model, prob.params, prob.ctl, prob.occ, vstore=vstore)
+        if vstore == pg.abi.PG_VSTORE_F32:
+            ref.set_kernel(pg.abi.PG_KERNEL_LDS)
+        ref.set_active(False)
         ref.fill_seed(prob.seeds)
         note("in-HBM reference seeded")
     steps = args.steps if args.steps is not None else 3
@@ -292,7 +296,7 @@ def run_ooc(pg, prob, args) -> dict:
     s.close()
     if ref is not None:
         ref.close()
-    return {"metric": "out-of-core tiled sweep", "value": g.ncells / (float(np.mean(ms)) / 1e3),
+    return {"metric": "out-of-core tiled sweep", "store": args.ooc_store, "value": g.ncells / (float(np.mean(ms)) / 1e3),
             "unit": "state-updates/s", "ms_per_step": float(np.mean(ms)), "steps": steps,
             "config": {"workload": f"problem {args.config}", "grid": list(g.n), "states": g.ncells,
                        "controls": prob.nctl, "budget_bytes": int(args.ooc_budget), "tile": list(info[0]),
@@ -359,6 +363,8 @@ def main() -> int:
                     help="sweeps of the reference's default solve.jl scheme (global-gather kernel)")
     ap.add_argument("--ooc-budget", type=float, default=8e9)
     ap.add_argument("--ooc-tile", default="16,16,8")
+    ap.add_argument("--ooc-store", choices=["f32", "u16"], default="f32",
+                    help="the tiled sweep's table: FP32, or the precision form's 16-bit codes")
     ap.add_argument("--no-ooc-compare", action="store_true")
     ap.add_argument("--f64-out", default=None)
     ap.add_argument("--f64-sample", default=None)

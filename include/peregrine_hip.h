@@ -264,11 +264,18 @@ int pg_solver_ncells(pg_solver* s, int64_t* n);
  * {0, 0, 0} for the largest x-y tile with the whole heading axis whose
  * buffers fit budget_bytes of device memory. PG_ERR_NOMEM if they do not.
  * The solver then takes pg_fill_seed, pg_sweep, pg_solve, pg_download_V,
- * pg_upload_V, pg_last_sweep_ms as usual (at most 128 controls; no 16-bit
- * store, policy or FP64 tables). */
+ * pg_upload_V, pg_last_sweep_ms as usual (at most 128 controls; no policy,
+ * escape pass or FP64 tables).
+ *
+ * vstore = PG_VSTORE_U16_TILE runs the precision form out of core: the host also holds the Jacobi pair of 16-bit codes, windows
+ * stream in as codes (half the bytes) with the interior's own FP32 values,
+ * and each tile's quantization tiles are refit and its interior encoded
+ * before values and codes stream out; tiles are cut on multiples of 4 cells.
+ * Every sweep equals the in-HBM precision-form sweep bit for bit; pg_polish
+ * then continues with FP32 tiles. PG_VSTORE_F16_TILE is refused out of core. */
 int pg_solver_create_ooc(pg_ctx* ctx, const pg_grid* g, const pg_model* m, const pg_params* p,
-                         const float* ctl_soa, int32_t nctl, const uint8_t* occ, int64_t budget_bytes,
-                         const int32_t tile[3], pg_solver** out);
+                         const float* ctl_soa, int32_t nctl, const uint8_t* occ, int32_t vstore,
+                         int64_t budget_bytes, const int32_t tile[3], pg_solver** out);
 /* The tile interior used, the tile count, the device bytes the tiled sweep
  * holds and the cells loaded per cell updated. */
 int pg_ooc_info(pg_solver* s, int32_t tile[3], int64_t* ntiles, int64_t* device_bytes, double* amplification);

@@ -104,7 +104,7 @@ SIGNATURES = {
     "pg_download_V": (ctypes.c_int, [_P, _P, _I64, _I64]),
     "pg_download_V16": (ctypes.c_int, [_P, _P, _I64, _I64]),
     "pg_solver_create_ooc": (ctypes.c_int, [_P, ctypes.POINTER(PgGrid), ctypes.POINTER(PgModel),
-                                            ctypes.POINTER(PgParams), _P, _I32, _P, _I64, _P,
+                                            ctypes.POINTER(PgParams), _P, _I32, _P, _I32, _I64, _P,
                                             ctypes.POINTER(_P)]),
     "pg_ooc_tile": (ctypes.c_int, [_P, _I64, _P, _P, _P, _P]),
     "pg_ooc_info": (ctypes.c_int, [_P, _P, ctypes.POINTER(_I64), ctypes.POINTER(_I64),

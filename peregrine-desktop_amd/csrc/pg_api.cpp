@@ -618,7 +618,7 @@ int create_solver(pg_ctx* ctx, const pg_grid* g, const pg_model* m, const pg_par
     if (vstore == PG_VSTORE_F32) {
         if (!host_tables) chk(hipMalloc(&s->dV[1], (size_t)s->ncell * sizeof(float)));
     } else {
-        chk(hipMalloc(&s->d_V16, (size_t)s->ncell * sizeof(uint16_t)));
+        if (!host_tables) chk(hipMalloc(&s->d_V16, (size_t)s->ncell * sizeof(uint16_t)));
         chk(hipMalloc(&s->d_tp, (size_t)s->Q.ntile * sizeof(float2)));
         chk(hipMalloc(&s->d_tstat, (size_t)s->Q.ntile * 2 * sizeof(unsigned)));
         chk(hipMalloc(&s->d_trefit, (size_t)s->Q.ntile));
@@ -672,15 +672,17 @@ int pg_solver_create(pg_ctx* ctx, const pg_grid* g, const pg_model* m, const pg_
 }
 
 int pg_solver_create_ooc(pg_ctx* ctx, const pg_grid* g, const pg_model* m, const pg_params* p,
-                         const float* ctl_soa, int32_t nctl, const uint8_t* occ, int64_t budget_bytes,
+                         const float* ctl_soa, int32_t nctl, const uint8_t* occ, int32_t vstore, int64_t budget_bytes,
                          const int32_t tile[3], pg_solver** out) {
     if (nctl > kLdsMaxCtl)
         return fail(PG_ERR_UNSUPPORTED, "the out-of-core sweep holds at most %d controls", kLdsMaxCtl);
     if (p && (p->rounds > 0 || p->ntau > 1))
         return fail(PG_ERR_UNSUPPORTED, "the out-of-core sweep does not run refine_rounds or tau_levels (%s set)",
                     p->rounds > 0 ? "refine_rounds" : "tau_levels");
+    if (vstore == PG_VSTORE_F16_TILE)
+        return fail(PG_ERR_UNSUPPORTED, "the binary16 store is not built out of core (PG_VSTORE_U16_TILE is)");
     if (!tile) return fail(PG_ERR_ARG, "null tile");
-    int rc = create_solver(ctx, g, m, p, ctl_soa, nctl, occ, PG_VSTORE_F32, true, out);
+    int rc = create_solver(ctx, g, m, p, ctl_soa, nctl, occ, vstore, true, out);
     if (rc) return rc;
     rc = pgh::ooc_create(*out, budget_bytes, tile);
     if (rc) {
@@ -928,11 +930,12 @@ int pg_solve_escape(pg_solver* s, int32_t max_iters, double tol_sum, float tol_l
 
 int pg_polish(pg_solver* s, int32_t nsweeps, double* last_sum, float* last_linf) {
     if (!s) return fail(PG_ERR_ARG, "null solver");
-    if (s->ooc) return fail(PG_ERR_UNSUPPORTED, "an out-of-core solver has no 16-bit store to polish");
     if (!s->seeded) return fail(PG_ERR_STATE, "no value table yet");
     if (nsweeps < 0) return fail(PG_ERR_ARG, "nsweeps must be >= 0");
     HIPCHECK(hipSetDevice(s->ctx->device));
-    if (s->f16_active) {
+    if (s->ooc) {
+        if (s->f16_active) pgh::ooc_polish_start(s);
+    } else if (s->f16_active) {
         if (!s->dV[1]) {
             hipError_t e = hipMalloc(&s->dV[1], (size_t)s->ncell * sizeof(float));
             if (e != hipSuccess)
@@ -1023,12 +1026,24 @@ int pg_download_V16(pg_solver* s, float* dst, int64_t off, int64_t n) {
     HIPCHECK(hipSetDevice(s->ctx->device));
     hipStream_t st = s->ctx->stream;
     float* tmp = nullptr;
+    uint16_t* codes = nullptr;
     hipError_t e = hipMalloc(&tmp, (size_t)n * sizeof(float));
-    if (e != hipSuccess) return fail(PG_ERR_NOMEM, "decode buffer allocation failed: %s", hipGetErrorString(e));
-    e = pgk_launch_decode(s->store, s->d_V16, s->K, s->Q, s->d_tp, off, n, tmp, st);
+    if (e == hipSuccess && s->ooc) e = hipMalloc(&codes, (size_t)n * sizeof(uint16_t));
+    if (e != hipSuccess) {
+        (void)hipFree(tmp);
+        return fail(PG_ERR_NOMEM, "decode buffer allocation failed: %s", hipGetErrorString(e));
+    }
+    // out of core the codes are on the host: cells [off, off + n) go to a
+    // device buffer addressed from cell off
+    if (s->ooc)
+        e = hipMemcpyAsync(codes, pgh::ooc_host_codes(s) + off, (size_t)n * sizeof(uint16_t), hipMemcpyHostToDevice,
+                           st);
+    if (e == hipSuccess)
+        e = pgk_launch_decode(s->store, s->ooc ? codes - off : s->d_V16, s->K, s->Q, s->d_tp, off, n, tmp, st);
     if (e == hipSuccess) e = hipMemcpyAsync(dst, tmp, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     (void)hipFree(tmp);
+    (void)hipFree(codes);
     if (e != hipSuccess) return fail(PG_ERR_HIP, "decode failed: %s", hipGetErrorString(e));
     return PG_OK;
 }

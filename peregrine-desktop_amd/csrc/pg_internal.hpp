@@ -121,6 +121,8 @@ int ooc_fill_seed(pg_solver* s);
 int ooc_sweep(pg_solver* s, double* sum_decrease, float* linf);
 int ooc_download(pg_solver* s, float* dst, int64_t off, int64_t n);
 int ooc_upload(pg_solver* s, const float* src, int64_t off, int64_t n);
+int ooc_polish_start(pg_solver* s);
+const uint16_t* ooc_host_codes(const pg_solver* s);
 int ooc_info(pg_solver* s, int32_t tile[3], int64_t* ntiles, int64_t* device_bytes, double* amplification);
 int ooc_tile(pg_solver* s, int64_t k, int32_t t0[3], int32_t tn[3], int32_t wo[3], int32_t wl[3]);
 

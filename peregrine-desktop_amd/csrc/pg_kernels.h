@@ -69,6 +69,28 @@ hipError_t pgk_launch_tile_refit(int store, unsigned* tstat, float2* tp, uint8_t
 hipError_t pgk_launch_encode(int store, const float* V32, const pgk::Consts& K, const pgk::QuantGeom& Q,
                              const float2* tp, const uint8_t* trefit, uint16_t* codes, uint32_t* mark,
                              const pgk::BoxGeom& G, hipStream_t st);
+// Out of core, 16-bit stores: one tile's sweep reading its window of codes
+// (win: x [wo0, wo0 + wl0), y [wo1, wo1 + wl1), heading [wo2, wo2 + wl2)
+// modulo n3; a read outside raises *miss) with the grid's offsets/scales tp,
+// the interior's own FP32 values from self and its results into out (both
+// interior-shaped, G's interior); tile statistics into tstat.
+hipError_t pgk_launch_sweep_fast_tile(int shape, int diag, int store, const pgk::Consts& K, const pgk::FastConsts& F,
+                                      const pgk::BoxGeom& G, const pgk::QuantGeom& Q, const uint16_t* win,
+                                      const float2* tp, const int32_t wo[3], const int32_t wl[3], unsigned* miss,
+                                      const float* self, float* out, const uint8_t* occ, const float* bu,
+                                      const int16_t* worig, const uint8_t* pfree, double* part, unsigned* linf,
+                                      unsigned* tstat, hipStream_t st);
+// An interior buffer (x, y, heading cells t0 .. t0 + tn, every velocity
+// cell): statistics of its reached values; refit of its tiles from tpo into
+// tpn; its codes under tp.
+hipError_t pgk_launch_interior_stats(const float* V, const int32_t t0[3], const int32_t tn[3], const pgk::Consts& K,
+                                     const pgk::QuantGeom& Q, unsigned* tstat, hipStream_t st);
+hipError_t pgk_launch_interior_refit(int store, const int32_t t0[3], const int32_t tn[3], const pgk::QuantGeom& Q,
+                                     unsigned* tstat, const float2* tpo, float2* tpn, uint8_t* trefit, int force,
+                                     hipStream_t st);
+hipError_t pgk_launch_interior_encode(int store, const float* V, const int32_t t0[3], const int32_t tn[3],
+                                      const pgk::Consts& K, const pgk::QuantGeom& Q, const float2* tp,
+                                      uint16_t* codes, hipStream_t st);
 // Codes of cells [off, off + n) -> V32[0, n).
 hipError_t pgk_launch_decode(int store, const uint16_t* codes, const pgk::Consts& K, const pgk::QuantGeom& Q,
                              const float2* tp, int64_t off, int64_t n, float* V32, hipStream_t st);

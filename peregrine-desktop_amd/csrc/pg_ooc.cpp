@@ -16,6 +16,14 @@
 //     and overlap change nothing: tile k's window is copied in (side stream)
 //     while tile k-1 is swept, and tile k-1's interior is copied out (a second
 //     side stream) behind it. Two window and two interior buffers alternate.
+//   * with a 16-bit store (the precision form) the host also holds the Jacobi
+//     pair of codes. A tile's window streams in as codes (half the bytes),
+//     its interior's own FP32 values beside them; after the tile's sweep its
+//     quantization tiles (inside the interior: the cut is on multiples of the
+//     tile edge) are refit into the next sweep's offset/scale array and its
+//     interior is encoded, and both the FP32 values and the codes stream out.
+//     Every sweep reads the previous sweep's codes and offsets/scales, as the
+//     in-HBM precision form does, so the two are equal bit for bit.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -36,6 +44,13 @@ struct Ooc {
     int cur = 0;
     float* dwin[2] = {nullptr, nullptr};
     float* dout[2] = {nullptr, nullptr};
+    // 16-bit stores: the codes' Jacobi pair on the host, window codes, the
+    // interior's own values and codes, the next sweep's offsets/scales
+    uint16_t* hM[2] = {nullptr, nullptr};
+    uint16_t* dwin16[2] = {nullptr, nullptr};
+    float* dself[2] = {nullptr, nullptr};
+    uint16_t* dcode[2] = {nullptr, nullptr};
+    float2* d_tpn = nullptr;
     hipStream_t h2d = nullptr, d2h = nullptr;
     hipEvent_t ev_h2d[2] = {nullptr, nullptr}, ev_k[2] = {nullptr, nullptr}, ev_d2h[2] = {nullptr, nullptr};
     unsigned* d_miss = nullptr;
@@ -49,8 +64,14 @@ namespace {
 
 int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
-// Boxes and window of every tile, from the whole grid's plan.
-void build_tiles(const pg_solver* s, Ooc& o, const std::vector<int16_t>& orig, const std::vector<uint8_t>& ext) {
+struct PlanRef {
+    const std::vector<int16_t>* orig;
+    const std::vector<uint8_t>* ext;
+};
+
+// Boxes and window of every tile: the union of its boxes' corner ranges
+// under each plan given, and with `interior` the interior itself.
+void build_tiles(const pg_solver* s, Ooc& o, const std::vector<PlanRef>& plans, bool interior) {
     const pg_grid& g = s->g;
     const pgk::BoxGeom& GG = s->G;
     const int n3 = g.n[2];
@@ -84,12 +105,13 @@ void build_tiles(const pg_solver* s, Ooc& o, const std::vector<int16_t>& orig, c
                             const int64_t head = (((int64_t)(b0 + i0) * GG.nb[1] + (b1 + i1)) * GG.nb[2] + (b2 + i2));
                             const int32_t sh = (b2 + i2) * G.b[2];   // the box's first heading
                             const int64_t per = (int64_t)GG.nb[3] * GG.nb[4] * GG.nb[5];
+                            for (const PlanRef& pr : plans)
                             for (int64_t v = 0; v < per; ++v) {
                                 const size_t bx = (size_t)(head * per + v) * 6;
                                 for (int a = 0; a < 3; ++a) {
-                                    const int e = ext[bx + a];
+                                    const int e = (*pr.ext)[bx + a];
                                     if (e >= 255) full[a] = true;
-                                    int32_t l = orig[bx + a];
+                                    int32_t l = (*pr.orig)[bx + a];
                                     if (a == 2) {
                                         // the plan measures a box's heading range from
                                         // the box's own first heading, in (-n3/2, n3/2]
@@ -103,6 +125,12 @@ void build_tiles(const pg_solver* s, Ooc& o, const std::vector<int16_t>& orig, c
                                 }
                             }
                         }
+                if (interior)
+                    for (int a = 0; a < 3; ++a) {
+                        const int32_t l = a == 2 ? 0 : t.t0[a];
+                        lo[a] = std::min(lo[a], l);
+                        hi[a] = std::max(hi[a], l + t.tn[a]);
+                    }
                 for (int a = 0; a < 2; ++a) {
                     if (full[a] || lo[a] > hi[a]) { lo[a] = 0; hi[a] = g.n[a]; }
                     lo[a] = std::max(lo[a], 0);
@@ -126,7 +154,8 @@ int64_t interior_cells(const OocTile& t, int64_t colv) { return (int64_t)t.tn[0]
 
 // Host table rows -> window buffer, x plane by x plane; a heading range that
 // wraps is two copies.
-hipError_t copy_window_in(const pg_solver* s, const Ooc& o, const OocTile& t, const float* hV, float* dw,
+template <class T>
+hipError_t copy_window_in(const pg_solver* s, const Ooc& o, const OocTile& t, const T* hV, T* dw,
                           hipStream_t st) {
     const pg_grid& g = s->g;
     const int64_t colv = o.colv;
@@ -139,10 +168,10 @@ hipError_t copy_window_in(const pg_solver* s, const Ooc& o, const OocTile& t, co
             const int32_t hl = piece == 0 ? h_first : t.wl[2] - h_first;
             const int32_t lh = piece == 0 ? 0 : h_first;
             if (hl <= 0) continue;
-            const float* src = hV + ((x * g.n[1] + t.wo[1]) * n3 + hs) * colv;
-            float* dst = dw + (((int64_t)lx * t.wl[1]) * t.wl[2] + lh) * colv;
-            hipError_t e = hipMemcpy2DAsync(dst, (size_t)t.wl[2] * colv * sizeof(float), src,
-                                            (size_t)n3 * colv * sizeof(float), (size_t)hl * colv * sizeof(float),
+            const T* src = hV + ((x * g.n[1] + t.wo[1]) * n3 + hs) * colv;
+            T* dst = dw + (((int64_t)lx * t.wl[1]) * t.wl[2] + lh) * colv;
+            hipError_t e = hipMemcpy2DAsync(dst, (size_t)t.wl[2] * colv * sizeof(T), src,
+                                            (size_t)n3 * colv * sizeof(T), (size_t)hl * colv * sizeof(T),
                                             (size_t)t.wl[1], hipMemcpyHostToDevice, st);
             if (e != hipSuccess) return e;
         }
@@ -150,22 +179,28 @@ hipError_t copy_window_in(const pg_solver* s, const Ooc& o, const OocTile& t, co
     return hipSuccess;
 }
 
-// Interior buffer -> host table, x plane by x plane.
-hipError_t copy_interior_out(const pg_solver* s, const Ooc& o, const OocTile& t, const float* dout, float* hV,
-                             hipStream_t st) {
+// Interior buffer <-> host table, x plane by x plane.
+template <class T>
+hipError_t copy_interior(const pg_solver* s, const Ooc& o, const OocTile& t, T* dbuf, T* hV, bool to_host,
+                         hipStream_t st) {
     const pg_grid& g = s->g;
     const int64_t colv = o.colv;
     const int n3 = g.n[2];
     for (int32_t lx = 0; lx < t.tn[0]; ++lx) {
         const int64_t x = t.t0[0] + lx;
-        float* dst = hV + ((x * g.n[1] + t.t0[1]) * n3 + t.t0[2]) * colv;
-        const float* src = dout + ((int64_t)lx * t.tn[1] * t.tn[2]) * colv;
-        hipError_t e = hipMemcpy2DAsync(dst, (size_t)n3 * colv * sizeof(float), src,
-                                        (size_t)t.tn[2] * colv * sizeof(float), (size_t)t.tn[2] * colv * sizeof(float),
-                                        (size_t)t.tn[1], hipMemcpyDeviceToHost, st);
+        T* h = hV + ((x * g.n[1] + t.t0[1]) * n3 + t.t0[2]) * colv;
+        T* d = dbuf + ((int64_t)lx * t.tn[1] * t.tn[2]) * colv;
+        const size_t hp = (size_t)n3 * colv * sizeof(T), dp = (size_t)t.tn[2] * colv * sizeof(T);
+        hipError_t e = to_host ? hipMemcpy2DAsync(h, hp, d, dp, dp, (size_t)t.tn[1], hipMemcpyDeviceToHost, st)
+                               : hipMemcpy2DAsync(d, dp, h, hp, dp, (size_t)t.tn[1], hipMemcpyHostToDevice, st);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
+}
+
+hipError_t copy_interior_out(const pg_solver* s, const Ooc& o, const OocTile& t, float* dout, float* hV,
+                             hipStream_t st) {
+    return copy_interior<float>(s, o, t, dout, hV, true, st);
 }
 
 int64_t fixed_device_bytes(const pg_solver* s) {
@@ -179,10 +214,27 @@ int64_t fixed_device_bytes(const pg_solver* s) {
 int ooc_create(pg_solver* s, int64_t budget_bytes, const int32_t tile_req[3]) {
     const pg_grid& g = s->g;
     if (budget_bytes <= 0) return fail(PG_ERR_ARG, "budget_bytes must be positive");
-    std::vector<int16_t> orig;
-    std::vector<uint8_t> ext;
+    const bool f16 = s->vstore != PG_VSTORE_F32;
+    std::vector<int16_t> orig, forig;
+    std::vector<uint8_t> ext, fext;
     int rc = plan(s, true, &orig, &ext);
     if (rc) return rc;
+    std::vector<PlanRef> plans{{&orig, &ext}};
+    if (f16) {
+        // the precision form's own read ranges (its contracted successors)
+        hipStream_t st = s->ctx->stream;
+        HIPCHECK(hipMalloc(&s->d_forig, (size_t)s->G.nbox * 6 * sizeof(int16_t)));
+        HIPCHECK(hipMalloc(&s->d_fext, (size_t)s->G.nbox * 6));
+        HIPCHECK(hipMalloc(&s->d_ffree, (size_t)s->G.nbox));
+        HIPCHECK(pgk_launch_plan(s->K, &s->F, s->G, s->d_occ, s->d_bu, s->d_forig, s->d_fext, s->d_ffree,
+                                 s->diag ? 1 : 0, st));
+        forig.resize((size_t)s->G.nbox * 6);
+        fext.resize((size_t)s->G.nbox * 6);
+        HIPCHECK(hipMemcpyAsync(forig.data(), s->d_forig, forig.size() * sizeof(int16_t), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipMemcpyAsync(fext.data(), s->d_fext, fext.size(), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        plans.push_back({&forig, &fext});
+    }
     // the tiles need partial sums for the largest tile only
     (void)hipFree(s->d_part);
     s->d_part = nullptr;
@@ -200,18 +252,22 @@ int ooc_create(pg_solver* s, int64_t budget_bytes, const int32_t tile_req[3]) {
             in = std::max(in, interior_cells(t, oc.colv));
             blocks = std::max(blocks, pgk::box_blocks(t.G.nbox));
         }
-        return fixed + 2 * (win + in) * (int64_t)sizeof(float) + blocks * (int64_t)sizeof(double) +
+        int64_t b16 = 0;
+        if (f16)   // codes in and out, own values in, the next offsets/scales
+            b16 = 2 * (win * 2 + in * (4 + 2)) + s->Q.ntile * (int64_t)sizeof(float2);
+        return fixed + 2 * (win + in) * (int64_t)sizeof(float) + b16 + blocks * (int64_t)sizeof(double) +
                (int64_t)oc.tiles.size() * (int64_t)sizeof(double);
     };
     auto even = [&](int32_t v, int a) {
-        const int32_t b = s->G.b[a];
+        // whole boxes, and with codes whole quantization tiles
+        const int32_t b = f16 ? std::max(s->G.b[a], (int32_t)1 << pgk::kQuantShift) : s->G.b[a];
         v = std::max(v, b);
         v = (v + b - 1) / b * b;
         return std::min(v, g.n[a]);
     };
     if (tile_req[0] > 0 && tile_req[1] > 0 && tile_req[2] > 0) {
         for (int a = 0; a < 3; ++a) o->tile[a] = even(tile_req[a], a);
-        build_tiles(s, *o, orig, ext);
+        build_tiles(s, *o, plans, f16);
         o->device_bytes = need(*o);
         if (o->device_bytes > budget_bytes)
             return fail(PG_ERR_NOMEM, "tiles %dx%dx%d need %lld device bytes, over the budget of %lld", o->tile[0],
@@ -229,7 +285,7 @@ int ooc_create(pg_solver* s, int64_t budget_bytes, const int32_t tile_req[3]) {
             trial.tile[0] = even((int32_t)std::lround(f * g.n[0]), 0);
             trial.tile[1] = even((int32_t)std::lround(f * g.n[1]), 1);
             trial.tile[2] = g.n[2];
-            build_tiles(s, trial, orig, ext);
+            build_tiles(s, trial, plans, f16);
             const int64_t b = need(trial);
             if (b <= budget_bytes) {
                 any = true;
@@ -242,7 +298,7 @@ int ooc_create(pg_solver* s, int64_t budget_bytes, const int32_t tile_req[3]) {
             }
         }
         if (!any) return fail(PG_ERR_NOMEM, "no tile fits a device budget of %lld bytes", (long long)budget_bytes);
-        build_tiles(s, *o, orig, ext);
+        build_tiles(s, *o, plans, f16);
         o->device_bytes = need(*o);
     }
     int64_t win = 0, in = 0, loaded = 0;
@@ -261,6 +317,15 @@ int ooc_create(pg_solver* s, int64_t budget_bytes, const int32_t tile_req[3]) {
         chk(hipEventCreateWithFlags(&o->ev_h2d[j], hipEventDisableTiming));
         chk(hipEventCreateWithFlags(&o->ev_k[j], hipEventDisableTiming));
         chk(hipEventCreateWithFlags(&o->ev_d2h[j], hipEventDisableTiming));
+    }
+    if (f16) {
+        for (int j = 0; j < 2; ++j) {
+            chk(hipHostMalloc((void**)&o->hM[j], (size_t)s->ncell * sizeof(uint16_t), hipHostMallocDefault));
+            chk(hipMalloc(&o->dwin16[j], (size_t)win * sizeof(uint16_t)));
+            chk(hipMalloc(&o->dself[j], (size_t)in * sizeof(float)));
+            chk(hipMalloc(&o->dcode[j], (size_t)in * sizeof(uint16_t)));
+        }
+        chk(hipMalloc(&o->d_tpn, (size_t)s->Q.ntile * sizeof(float2)));
     }
     chk(hipStreamCreate(&o->h2d));
     chk(hipStreamCreate(&o->d2h));
@@ -283,6 +348,10 @@ void ooc_release(pg_solver* s) {
         (void)hipHostFree(o->hV[j]);
         (void)hipFree(o->dwin[j]);
         (void)hipFree(o->dout[j]);
+        (void)hipHostFree(o->hM[j]);
+        (void)hipFree(o->dwin16[j]);
+        (void)hipFree(o->dself[j]);
+        (void)hipFree(o->dcode[j]);
         if (o->ev_h2d[j]) (void)hipEventDestroy(o->ev_h2d[j]);
         if (o->ev_k[j]) (void)hipEventDestroy(o->ev_k[j]);
         if (o->ev_d2h[j]) (void)hipEventDestroy(o->ev_d2h[j]);
@@ -291,19 +360,117 @@ void ooc_release(pg_solver* s) {
     if (o->d2h) (void)hipStreamDestroy(o->d2h);
     (void)hipFree(o->d_miss);
     (void)hipFree(o->d_tsum);
+    (void)hipFree(o->d_tpn);
     delete o;
     s->ooc = nullptr;
 }
+
+namespace {
+
+// The codes of the whole host table from its FP32 values, tile by tile:
+// every quantization tile refit from its reached values (as a whole-table
+// load in HBM does), then encoded.
+int ooc_encode_all(pg_solver* s) {
+    Ooc& o = *s->ooc;
+    hipStream_t st = s->ctx->stream;
+    for (const OocTile& t : o.tiles) {
+        HIPCHECK(copy_interior<float>(s, o, t, o.dself[0], o.hV[o.cur], false, st));
+        HIPCHECK(pgk_launch_interior_stats(o.dself[0], t.t0, t.tn, s->K, s->Q, s->d_tstat, st));
+        HIPCHECK(pgk_launch_interior_refit(s->store, t.t0, t.tn, s->Q, s->d_tstat, s->d_tp, s->d_tp, s->d_trefit, 1,
+                                           st));
+        HIPCHECK(pgk_launch_interior_encode(s->store, o.dself[0], t.t0, t.tn, s->K, s->Q, s->d_tp, o.dcode[0], st));
+        HIPCHECK(copy_interior<uint16_t>(s, o, t, o.dcode[0], o.hM[o.cur], true, st));
+        HIPCHECK(hipStreamSynchronize(st));
+    }
+    s->f16_active = true;
+    return PG_OK;
+}
+
+// One precision-form sweep over the tiles (the 16-bit twin of the FP32
+// pipeline below): window codes and interior values in on one side stream,
+// the sweep, refit and encode on the solver stream, values and codes out on
+// the other side stream.
+int ooc_sweep16(pg_solver* s, double* sum_decrease, float* linf) {
+    Ooc& o = *s->ooc;
+    hipStream_t st = s->ctx->stream;
+    const float* hin = o.hV[o.cur];
+    float* hout = o.hV[1 - o.cur];
+    const uint16_t* min_ = o.hM[o.cur];
+    uint16_t* mout = o.hM[1 - o.cur];
+    const int64_t nt = (int64_t)o.tiles.size();
+    HIPCHECK(hipMemsetAsync(s->d_linf, 0, sizeof(unsigned), st));
+    HIPCHECK(hipMemsetAsync(o.d_miss, 0, sizeof(unsigned), st));
+    HIPCHECK(hipEventRecord(s->ev0, st));
+    auto load = [&](int64_t k, int j) -> hipError_t {
+        const OocTile& t = o.tiles[(size_t)k];
+        hipError_t e = copy_window_in<uint16_t>(s, o, t, min_, o.dwin16[j], o.h2d);
+        if (e == hipSuccess) e = copy_interior<float>(s, o, t, o.dself[j], const_cast<float*>(hin), false, o.h2d);
+        return e;
+    };
+    HIPCHECK(hipStreamWaitEvent(o.h2d, s->ev0, 0));
+    HIPCHECK(load(0, 0));
+    HIPCHECK(hipEventRecord(o.ev_h2d[0], o.h2d));
+    for (int64_t k = 0; k < nt; ++k) {
+        const int j = (int)(k & 1);
+        const OocTile& t = o.tiles[(size_t)k];
+        HIPCHECK(hipStreamWaitEvent(st, o.ev_h2d[j], 0));
+        if (k >= 2) HIPCHECK(hipStreamWaitEvent(st, o.ev_d2h[j], 0));
+        HIPCHECK(pgk_launch_sweep_fast_tile(s->shape, s->diag ? 1 : 0, s->store, s->K, s->F, t.G, s->Q, o.dwin16[j],
+                                            s->d_tp, t.wo, t.wl, o.d_miss, o.dself[j], o.dout[j], s->d_occ, s->d_bu,
+                                            s->d_worig, s->d_ffree, s->d_part, s->d_linf, s->d_tstat, st));
+        HIPCHECK(pgk_launch_reduce(s->d_part, pgk::box_blocks(t.G.nbox), o.d_tsum + k, nullptr, st));
+        // this tile's quantization tiles: refit into the next sweep's array, encode
+        HIPCHECK(pgk_launch_interior_refit(s->store, t.t0, t.tn, s->Q, s->d_tstat, s->d_tp, o.d_tpn, s->d_trefit, 0,
+                                           st));
+        HIPCHECK(pgk_launch_interior_encode(s->store, o.dout[j], t.t0, t.tn, s->K, s->Q, o.d_tpn, o.dcode[j], st));
+        HIPCHECK(hipEventRecord(o.ev_k[j], st));
+        if (k + 1 < nt) {
+            const int jn = 1 - j;
+            if (k >= 1) HIPCHECK(hipStreamWaitEvent(o.h2d, o.ev_k[jn], 0));
+            HIPCHECK(load(k + 1, jn));
+            HIPCHECK(hipEventRecord(o.ev_h2d[jn], o.h2d));
+        }
+        HIPCHECK(hipStreamWaitEvent(o.d2h, o.ev_k[j], 0));
+        HIPCHECK(copy_interior<float>(s, o, t, o.dout[j], hout, true, o.d2h));
+        HIPCHECK(copy_interior<uint16_t>(s, o, t, o.dcode[j], mout, true, o.d2h));
+        HIPCHECK(hipEventRecord(o.ev_d2h[j], o.d2h));
+    }
+    std::vector<double> tsum((size_t)nt);
+    unsigned hl = 0, miss = 0;
+    HIPCHECK(hipMemcpyAsync(tsum.data(), o.d_tsum, (size_t)nt * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(&hl, s->d_linf, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(&miss, o.d_miss, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    HIPCHECK(hipStreamSynchronize(o.d2h));
+    HIPCHECK(hipEventRecord(s->ev1, st));
+    HIPCHECK(hipEventSynchronize(s->ev1));
+    HIPCHECK(hipEventElapsedTime(&s->last_ms, s->ev0, s->ev1));
+    if (miss) return fail(PG_ERR_STATE, "a backup read a cell outside its tile's window");
+    // seeds hold 0 in both tables already (a backup never goes below 0)
+    std::swap(s->d_tp, o.d_tpn);
+    o.cur = 1 - o.cur;
+    double d = 0.0;
+    for (double v : tsum) d += v;
+    float lf;
+    std::memcpy(&lf, &hl, 4);
+    if (sum_decrease) *sum_decrease = d;
+    if (linf) *linf = lf;
+    return PG_OK;
+}
+
+}  // namespace
 
 int ooc_fill_seed(pg_solver* s) {
     Ooc& o = *s->ooc;
     o.cur = 0;
     std::fill(o.hV[0], o.hV[0] + s->ncell, s->p.cap);
     for (int64_t i : s->h_seeds) o.hV[0][i] = 0.0f;
+    if (s->vstore != PG_VSTORE_F32) return ooc_encode_all(s);
     return PG_OK;
 }
 
 int ooc_sweep(pg_solver* s, double* sum_decrease, float* linf) {
+    if (s->f16_active) return ooc_sweep16(s, sum_decrease, linf);
     Ooc& o = *s->ooc;
     hipStream_t st = s->ctx->stream;
     const float* hin = o.hV[o.cur];
@@ -314,7 +481,7 @@ int ooc_sweep(pg_solver* s, double* sum_decrease, float* linf) {
     HIPCHECK(hipEventRecord(s->ev0, st));
     // the first window; later ones are issued one tile ahead
     HIPCHECK(hipStreamWaitEvent(o.h2d, s->ev0, 0));
-    HIPCHECK(copy_window_in(s, o, o.tiles[0], hin, o.dwin[0], o.h2d));
+    HIPCHECK(copy_window_in<float>(s, o, o.tiles[0], hin, o.dwin[0], o.h2d));
     HIPCHECK(hipEventRecord(o.ev_h2d[0], o.h2d));
     for (int64_t k = 0; k < nt; ++k) {
         const int j = (int)(k & 1);
@@ -330,7 +497,7 @@ int ooc_sweep(pg_solver* s, double* sum_decrease, float* linf) {
         if (k + 1 < nt) {
             const int jn = 1 - j;
             if (k >= 1) HIPCHECK(hipStreamWaitEvent(o.h2d, o.ev_k[jn], 0));
-            HIPCHECK(copy_window_in(s, o, o.tiles[(size_t)k + 1], hin, o.dwin[jn], o.h2d));
+            HIPCHECK(copy_window_in<float>(s, o, o.tiles[(size_t)k + 1], hin, o.dwin[jn], o.h2d));
             HIPCHECK(hipEventRecord(o.ev_h2d[jn], o.h2d));
         }
         // tile k's interior out
@@ -369,6 +536,17 @@ int ooc_upload(pg_solver* s, const float* src, int64_t off, int64_t n) {
     Ooc& o = *s->ooc;
     if (!s->seeded) std::fill(o.hV[o.cur], o.hV[o.cur] + s->ncell, s->p.cap);
     std::memcpy(o.hV[o.cur] + off, src, (size_t)n * sizeof(float));
+    if (s->vstore != PG_VSTORE_F32 && (s->f16_active || !s->seeded)) return ooc_encode_all(s);
+    return PG_OK;
+}
+
+const uint16_t* ooc_host_codes(const pg_solver* s) { return s->ooc->hM[s->ooc->cur]; }
+
+// The precision form's end: the host FP32 table is the table, the codes
+// are dropped and the FP32 tiles sweep from here on.
+int ooc_polish_start(pg_solver* s) {
+    s->f16_active = false;
+    s->kernel = PG_KERNEL_LDS;
     return PG_OK;
 }
 

@@ -66,23 +66,88 @@ __device__ __forceinline__ int64_t tile_of_coords(const QuantGeom& Q, const int*
     return t;
 }
 
-// Reads of V in either store.
-// (no __restrict__ on the table: in place, a sweep writes what it reads)
+__device__ __forceinline__ int64_t flat_of(const Consts& K, const int* gi) {
+    int64_t fl = 0;
+#pragma unroll
+    for (int a = 0; a < 6; ++a) fl += (int64_t)gi[a] * K.stride[a];
+    return fl;
+}
+
+// Reads of V in either store: stage() for a window cell, read() for a value
+// a backup uses. (no __restrict__ on the table: in place, a sweep writes
+// what it reads)
 template <int STORE>
 struct VRead {
     const uint16_t* V;
     const float2* __restrict__ tp;
-    __device__ __forceinline__ float operator()(const Consts& K, const QuantGeom& Q, const int* gi,
-                                                int64_t fl) const {
-        return Codec<STORE>::decode(V[fl], tp[tile_of_coords(Q, gi)], K.cap);
+    __device__ __forceinline__ float read(const Consts& K, const QuantGeom& Q, const int* gi) const {
+        return Codec<STORE>::decode(V[flat_of(K, gi)], tp[tile_of_coords(Q, gi)], K.cap);
+    }
+    __device__ __forceinline__ float stage(const Consts& K, const QuantGeom& Q, const int* gi) const {
+        return read(K, Q, gi);
     }
 };
 
 template <>
 struct VRead<PG_STORE_F32> {
     const float* V;
-    __device__ __forceinline__ float operator()(const Consts&, const QuantGeom&, const int*, int64_t fl) const {
-        return V[fl];
+    __device__ __forceinline__ float read(const Consts& K, const QuantGeom&, const int* gi) const {
+        return V[flat_of(K, gi)];
+    }
+    __device__ __forceinline__ float stage(const Consts& K, const QuantGeom& Q, const int* gi) const {
+        return read(K, Q, gi);
+    }
+};
+
+// An out-of-core tile's window of 16-bit codes: cells x in [o0, o0 + L0),
+// y in [o1, o1 + L1), heading in [o2, o2 + L2) modulo n3, every velocity
+// cell; omega fastest. Offsets/scales stay those of the whole grid. The
+// window holds every cell the tile's backups read, so a read outside it is a
+// defect: it returns cap and raises *miss. Staged cells outside it are never
+// read as corners and stage cap quietly.
+template <int STORE>
+struct VReadTile {
+    const uint16_t* __restrict__ W;
+    const float2* __restrict__ tp;
+    int32_t o[3], L[3];
+    int64_t ws[6];
+    unsigned* miss;
+    __device__ __forceinline__ bool local(const Consts& K, const int* gi, int64_t& fl) const {
+        int l2 = gi[2] - o[2];
+        l2 = l2 < 0 ? l2 + K.n[2] : l2;
+        const int l0 = gi[0] - o[0], l1 = gi[1] - o[1];
+        fl = (int64_t)l0 * ws[0] + (int64_t)l1 * ws[1] + (int64_t)l2 * ws[2] + (int64_t)gi[3] * ws[3] +
+             (int64_t)gi[4] * ws[4] + gi[5];
+        return ((unsigned)l0 < (unsigned)L[0]) & ((unsigned)l1 < (unsigned)L[1]) & ((unsigned)l2 < (unsigned)L[2]);
+    }
+    __device__ __forceinline__ float stage(const Consts& K, const QuantGeom& Q, const int* gi) const {
+        int64_t fl;
+        return local(K, gi, fl) ? Codec<STORE>::decode(W[fl], tp[tile_of_coords(Q, gi)], K.cap) : K.cap;
+    }
+    __device__ __forceinline__ float read(const Consts& K, const QuantGeom& Q, const int* gi) const {
+        int64_t fl;
+        if (local(K, gi, fl)) return Codec<STORE>::decode(W[fl], tp[tile_of_coords(Q, gi)], K.cap);
+        atomicOr(miss, 1u);
+        return K.cap;
+    }
+};
+
+// The states' own FP32 values and the sweep's FP32 results: the whole grid,
+// or an out-of-core tile's interior (x, y, heading from t0; omega fastest).
+struct SelfGlobal {
+    const float* self;
+    float* out;
+    __device__ __forceinline__ int64_t index(const Consts& K, const int* i) const { return flat_of(K, i); }
+};
+
+struct SelfTile {
+    const float* __restrict__ self;
+    float* __restrict__ out;
+    int32_t t0[3];
+    int64_t os[6];
+    __device__ __forceinline__ int64_t index(const Consts&, const int* i) const {
+        return (int64_t)(i[0] - t0[0]) * os[0] + (int64_t)(i[1] - t0[1]) * os[1] + (int64_t)(i[2] - t0[2]) * os[2] +
+               (int64_t)i[3] * os[3] + (int64_t)i[4] * os[4] + i[5];
     }
 };
 
@@ -101,9 +166,9 @@ struct BoxShared {
 // (value, control) in registers (strict <, so the lowest control wins a
 // tie) and folds it into the state's LDS minimum once. Threads 0..ns-1
 // then write the states.
-template <class W, bool DIAG, int STORE, int MAXS, bool WRITE_V, bool WRITE_POL>
+template <class W, bool DIAG, int STORE, int MAXS, bool WRITE_V, bool WRITE_POL, class RD, class IO>
 __global__ __launch_bounds__(256, 4) void sweep_fast_kernel(
-    fastk::VRead<STORE> vin, const float* vself, float* Vout, uint16_t* codes_out, const uint8_t* __restrict__ occ,
+    RD vin, IO io, uint16_t* codes_out, const uint8_t* __restrict__ occ,
     const float* __restrict__ bu_g, Consts K, FastConsts F, BoxGeom G, QuantGeom Q,
     const int16_t* __restrict__ worig, const uint8_t* __restrict__ pfree, uint8_t* __restrict__ pol,
     double* __restrict__ part, unsigned* __restrict__ linf_bits, unsigned* __restrict__ tstat, ActiveG A) {
@@ -128,12 +193,7 @@ __global__ __launch_bounds__(256, 4) void sweep_fast_kernel(
             int l[6], gi[6];
             const bool on = window_cell<W>(K.n, o, j, l, gi);
             float v = K.cap;
-            if (on) {
-                int64_t fl = 0;
-#pragma unroll
-                for (int a = 0; a < 6; ++a) fl += (int64_t)gi[a] * K.stride[a];
-                v = vin(K, Q, gi, fl);
-            }
+            if (on) v = vin.stage(K, Q, gi);
             int lo = 0;
 #pragma unroll
             for (int a = 0; a < 6; ++a) lo += l[a] * W::S[a];
@@ -169,13 +229,9 @@ __global__ __launch_bounds__(256, 4) void sweep_fast_kernel(
             unsigned pk = 0;
             float vmir = 0.0f;
             if (SELF) {
-                int64_t fl = 0;
 #pragma unroll
-                for (int a = 0; a < 6; ++a) {
-                    fl += (int64_t)i[a] * K.stride[a];
-                    pk |= (unsigned)(i[a] - B.s0[a]) << (3 * a);
-                }
-                vmir = vin(K, Q, i, fl);
+                for (int a = 0; a < 6; ++a) pk |= (unsigned)(i[a] - B.s0[a]) << (3 * a);
+                vmir = vin.read(K, Q, i);
             }
             float bv = __uint_as_float(0x7f800000u);
             int bk = 0;
@@ -229,7 +285,6 @@ __global__ __launch_bounds__(256, 4) void sweep_fast_kernel(
                         acc = tensor_interp_grouped(
                             [&](int d) {
                                 int gi[6];
-                                int64_t fl = 0;
 #pragma unroll
                                 for (int a = 0; a < 6; ++a) {
                                     int j = si[a] + ((d >> a) & 1);
@@ -240,9 +295,8 @@ __global__ __launch_bounds__(256, 4) void sweep_fast_kernel(
                                         j = j < K.n[a] - 1 ? j : K.n[a] - 1;
                                     }
                                     gi[a] = j;
-                                    fl += (int64_t)j * K.stride[a];
                                 }
-                                return vin(K, Q, gi, fl);
+                                return vin.read(K, Q, gi);
                             },
                             t);
                     }
@@ -282,7 +336,8 @@ __global__ __launch_bounds__(256, 4) void sweep_fast_kernel(
             const unsigned long long key = sm.best[s];
             const float best = __uint_as_float((unsigned)(key >> 32));
             if (WRITE_V) {
-                const float old = vself[idx];
+                const int64_t io_i = io.index(K, i);
+                const float old = io.self[io_i];
                 float out = old;
                 if (best < old) {
                     out = best;
@@ -290,7 +345,7 @@ __global__ __launch_bounds__(256, 4) void sweep_fast_kernel(
                     dsum += (double)d;
                     dmax = fmaxf(dmax, d);
                 }
-                Vout[idx] = out;
+                io.out[io_i] = out;
                 // in place: this state's code now, for the boxes swept after it
                 if constexpr (STORE != PG_STORE_F32) {
                     if (codes_out) codes_out[idx] = Codec<STORE>::encode(out, vin.tp[tile_of_coords(Q, i)], K.cap);
@@ -310,7 +365,7 @@ __global__ __launch_bounds__(256, 4) void sweep_fast_kernel(
             int64_t idx = 0;
 #pragma unroll
             for (int a = 0; a < 6; ++a) idx += (int64_t)i[a] * K.stride[a];
-            const float out = vself[idx];
+            const float out = io.self[io.index(K, i)];
             if (out < K.cap) {
                 lmin = min(lmin, __float_as_uint(out));
                 lmax = max(lmax, __float_as_uint(out));
@@ -372,19 +427,18 @@ __global__ __launch_bounds__(256) void tile_stats_kernel(const float* __restrict
 // representable range [off, off + ncode*scale] (or always, with force), and
 // flag the refit tiles; clear the statistics. A refit leaves headroom of a
 // quarter of the span below the minimum, since values only decrease.
-__global__ __launch_bounds__(256) void tile_refit_kernel(unsigned* __restrict__ tstat,
-                                                         float2* __restrict__ tp,
-                                                         uint8_t* __restrict__ trefit, int64_t ntile,
-                                                         float ncode, int force) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ntile) return;
+// Refit of tile t from its statistics (which it resets): the offset/scale
+// read from tpo, the result written to tpn (the same array in HBM; the
+// previous and the next sweep's arrays out of core).
+__device__ __forceinline__ void refit_tile(int64_t t, unsigned* __restrict__ tstat, const float2* tpo, float2* tpn,
+                                           uint8_t* __restrict__ trefit, float ncode, int force) {
     const unsigned mnb = tstat[2 * t], mxb = tstat[2 * t + 1];
     tstat[2 * t] = 0x7f800000u;
     tstat[2 * t + 1] = 0u;
     uint8_t flag = 0;
     if (mnb <= mxb) {
         const float mn = __uint_as_float(mnb), mx = __uint_as_float(mxb);
-        const float2 p = tp[t];
+        const float2 p = tpo[t];
         const bool unset = !(p.y > 0.0f);
         const float top = p.x + p.y * ncode;
         if (force || unset || mn < p.x || mx > top) {
@@ -400,14 +454,92 @@ __global__ __launch_bounds__(256) void tile_refit_kernel(unsigned* __restrict__ 
                 off = fmaxf(mn - mag * (1.0f / 1024.0f), 0.0f);
                 scale = (mn + mag * (1.0f / 1024.0f) - off) / ncode;
             }
-            tp[t] = make_float2(off, scale);
+            tpn[t] = make_float2(off, scale);
             flag = 1;
+        } else {
+            tpn[t] = p;
         }
     } else if (force) {
-        tp[t] = make_float2(0.0f, 0.0f);   // nothing reached: unset
+        tpn[t] = make_float2(0.0f, 0.0f);   // nothing reached: unset
         flag = 1;
+    } else {
+        tpn[t] = tpo[t];
     }
     trefit[t] = flag;
+}
+
+__global__ __launch_bounds__(256) void tile_refit_kernel(unsigned* __restrict__ tstat, float2* tp,
+                                                         uint8_t* __restrict__ trefit, int64_t ntile,
+                                                         float ncode, int force) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntile) return;
+    refit_tile(t, tstat, tp, tp, trefit, ncode, force);
+}
+
+// An out-of-core tile's interior: x, y, heading cells t0 .. t0 + tn, every
+// velocity cell, omega fastest; its quantization tiles lie inside it (the
+// interior is cut on multiples of the tile edge).
+struct Interior {
+    int32_t t0[3], tn[3];
+    __device__ __forceinline__ void coords(const Consts& K, int64_t j, int* gi) const {
+        for (int a = 5; a >= 3; --a) {
+            gi[a] = (int)(j % K.n[a]);
+            j /= K.n[a];
+        }
+        gi[2] = t0[2] + (int)(j % tn[2]);
+        j /= tn[2];
+        gi[1] = t0[1] + (int)(j % tn[1]);
+        gi[0] = t0[0] + (int)(j / tn[1]);
+    }
+};
+
+// Statistics of an interior buffer's reached values into the grid's tiles.
+__global__ __launch_bounds__(256) void interior_stats_kernel(const float* __restrict__ V, int64_t n, Interior I,
+                                                             Consts K, QuantGeom Q, unsigned* __restrict__ tstat) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const float v = V[j];
+    if (!(v < K.cap)) return;
+    int gi[6];
+    I.coords(K, j, gi);
+    const int64_t t = fastk::tile_of_coords(Q, gi);
+    atomicMin(&tstat[2 * t], __float_as_uint(v));
+    atomicMax(&tstat[2 * t + 1], __float_as_uint(v));
+}
+
+// Refit of the tiles under an interior.
+__global__ __launch_bounds__(256) void interior_refit_kernel(Interior I, QuantGeom Q, unsigned* __restrict__ tstat,
+                                                             const float2* __restrict__ tpo, float2* __restrict__ tpn,
+                                                             uint8_t* __restrict__ trefit, float ncode, int force) {
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int32_t q0[3], qn[3];
+    int64_t nv = 1, n = 1;
+    for (int a = 0; a < 3; ++a) {
+        q0[a] = I.t0[a] >> kQuantShift;
+        qn[a] = ((I.t0[a] + I.tn[a] - 1) >> kQuantShift) - q0[a] + 1;
+        n *= qn[a];
+    }
+    for (int a = 3; a < 6; ++a) nv *= Q.nt[a];
+    if (j >= n * nv) return;
+    const int64_t v = j % nv;
+    j /= nv;
+    const int32_t c2 = (int32_t)(j % qn[2]);
+    j /= qn[2];
+    const int32_t c1 = (int32_t)(j % qn[1]), c0 = (int32_t)(j / qn[1]);
+    const int64_t t = ((((int64_t)(q0[0] + c0) * Q.nt[1] + (q0[1] + c1)) * Q.nt[2] + (q0[2] + c2)) * nv) + v;
+    refit_tile(t, tstat, tpo, tpn, trefit, ncode, force);
+}
+
+// Codes of an interior buffer.
+template <int STORE>
+__global__ __launch_bounds__(256) void interior_encode_kernel(const float* __restrict__ V, int64_t n, Interior I,
+                                                              Consts K, QuantGeom Q, const float2* __restrict__ tp,
+                                                              uint16_t* __restrict__ codes) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    int gi[6];
+    I.coords(K, j, gi);
+    codes[j] = fastk::Codec<STORE>::encode(V[j], tp[fastk::tile_of_coords(Q, gi)], K.cap);
 }
 
 // FP32 table -> codes. In a refit tile every cell is encoded afresh; in
@@ -484,23 +616,26 @@ hipError_t launch_fast_w(const pgk::Consts& K, const pgk::FastConsts& F, const p
                          uint16_t* codes_out,
                          const uint8_t* occ, const float* bu, const int16_t* worig, const uint8_t* pfree, uint8_t* pol,
                          double* part, unsigned* linf, unsigned* tstat, int write_v, const pgk::ActiveG& A, hipStream_t st) {
-    pgk::fastk::VRead<STORE> rd;
+    using RD = pgk::fastk::VRead<STORE>;
+    using IO = pgk::fastk::SelfGlobal;
+    RD rd;
     if constexpr (STORE == PG_STORE_F32) {
         rd.V = (const float*)Vin;
     } else {
         rd.V = (const uint16_t*)Vin;
         rd.tp = tp;
     }
+    const IO io{vself, Vout};
     return pgk::for_box_chunks(G, [&](const pgk::BoxGeom& g, unsigned nb) {
         if (write_v && pol)
-            hipLaunchKernelGGL((pgk::sweep_fast_kernel<W, DIAG, STORE, 128, true, true>), dim3(nb), dim3(256), 0, st,
-                               rd, vself, Vout, codes_out, occ, bu, K, F, g, Q, worig, pfree, pol, part, linf, tstat, A);
+            hipLaunchKernelGGL((pgk::sweep_fast_kernel<W, DIAG, STORE, 128, true, true, RD, IO>), dim3(nb), dim3(256),
+                               0, st, rd, io, codes_out, occ, bu, K, F, g, Q, worig, pfree, pol, part, linf, tstat, A);
         else if (write_v)
-            hipLaunchKernelGGL((pgk::sweep_fast_kernel<W, DIAG, STORE, 128, true, false>), dim3(nb), dim3(256), 0, st,
-                               rd, vself, Vout, codes_out, occ, bu, K, F, g, Q, worig, pfree, pol, part, linf, tstat, A);
+            hipLaunchKernelGGL((pgk::sweep_fast_kernel<W, DIAG, STORE, 128, true, false, RD, IO>), dim3(nb), dim3(256),
+                               0, st, rd, io, codes_out, occ, bu, K, F, g, Q, worig, pfree, pol, part, linf, tstat, A);
         else
-            hipLaunchKernelGGL((pgk::sweep_fast_kernel<W, DIAG, STORE, 128, false, true>), dim3(nb), dim3(256), 0, st,
-                               rd, vself, Vout, codes_out, occ, bu, K, F, g, Q, worig, pfree, pol, part, linf, tstat, A);
+            hipLaunchKernelGGL((pgk::sweep_fast_kernel<W, DIAG, STORE, 128, false, true, RD, IO>), dim3(nb), dim3(256),
+                               0, st, rd, io, codes_out, occ, bu, K, F, g, Q, worig, pfree, pol, part, linf, tstat, A);
     });
 }
 
@@ -539,6 +674,119 @@ hipError_t pgk_launch_sweep_fast(int shape, int diag, int store, const pgk::Cons
         case PG_STORE_U16 * 2 + 1: return launch_fast_shape<true, PG_STORE_U16>(shape, K, F, G, Q, Vin, tp, vself, Vout, codes_out, occ, bu, worig, pfree, pol, part, linf, tstat, write_v, A, st);
         default: return hipErrorInvalidValue;
     }
+}
+
+namespace {
+
+template <class W, bool DIAG, int STORE>
+hipError_t launch_fast_tile_w(const pgk::Consts& K, const pgk::FastConsts& F, const pgk::BoxGeom& G,
+                              const pgk::QuantGeom& Q, const pgk::fastk::VReadTile<STORE>& rd,
+                              const pgk::fastk::SelfTile& io, const uint8_t* occ, const float* bu,
+                              const int16_t* worig, const uint8_t* pfree, double* part, unsigned* linf,
+                              unsigned* tstat, hipStream_t st) {
+    using RD = pgk::fastk::VReadTile<STORE>;
+    using IO = pgk::fastk::SelfTile;
+    return pgk::for_box_chunks(G, [&](const pgk::BoxGeom& g, unsigned nb) {
+        hipLaunchKernelGGL((pgk::sweep_fast_kernel<W, DIAG, STORE, 128, true, false, RD, IO>), dim3(nb), dim3(256), 0,
+                           st, rd, io, nullptr, occ, bu, K, F, g, Q, worig, pfree, nullptr, part, linf, tstat,
+                           pgk::ActiveG{});
+    });
+}
+
+template <bool DIAG, int STORE>
+hipError_t launch_fast_tile(int shape, const pgk::Consts& K, const pgk::FastConsts& F, const pgk::BoxGeom& G,
+                            const pgk::QuantGeom& Q, const uint16_t* win, const float2* tp, const int32_t wo[3],
+                            const int32_t wl[3], unsigned* miss, const float* self, float* out, const uint8_t* occ,
+                            const float* bu, const int16_t* worig, const uint8_t* pfree, double* part, unsigned* linf,
+                            unsigned* tstat, hipStream_t st) {
+    pgk::fastk::VReadTile<STORE> rd{};
+    rd.W = win;
+    rd.tp = tp;
+    rd.miss = miss;
+    const int64_t colv = (int64_t)K.n[3] * K.n[4] * K.n[5];
+    for (int a = 0; a < 3; ++a) { rd.o[a] = wo[a]; rd.L[a] = wl[a]; }
+    rd.ws[5] = 1;
+    rd.ws[4] = K.n[5];
+    rd.ws[3] = (int64_t)K.n[4] * K.n[5];
+    rd.ws[2] = colv;
+    rd.ws[1] = colv * wl[2];
+    rd.ws[0] = colv * wl[2] * wl[1];
+    pgk::fastk::SelfTile io{};
+    io.self = self;
+    io.out = out;
+    for (int a = 0; a < 3; ++a) io.t0[a] = G.org[a];
+    io.os[5] = 1;
+    io.os[4] = K.n[5];
+    io.os[3] = (int64_t)K.n[4] * K.n[5];
+    io.os[2] = colv;
+    io.os[1] = colv * G.rn[2];
+    io.os[0] = colv * G.rn[2] * G.rn[1];
+    switch (shape) {
+        case 0: return launch_fast_tile_w<pgk::Win0, DIAG, STORE>(K, F, G, Q, rd, io, occ, bu, worig, pfree, part, linf, tstat, st);
+        case 1: return launch_fast_tile_w<pgk::Win1, DIAG, STORE>(K, F, G, Q, rd, io, occ, bu, worig, pfree, part, linf, tstat, st);
+        case 2: return launch_fast_tile_w<pgk::Win2, DIAG, STORE>(K, F, G, Q, rd, io, occ, bu, worig, pfree, part, linf, tstat, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace
+
+hipError_t pgk_launch_sweep_fast_tile(int shape, int diag, int store, const pgk::Consts& K, const pgk::FastConsts& F,
+                                      const pgk::BoxGeom& G, const pgk::QuantGeom& Q, const uint16_t* win,
+                                      const float2* tp, const int32_t wo[3], const int32_t wl[3], unsigned* miss,
+                                      const float* self, float* out, const uint8_t* occ, const float* bu,
+                                      const int16_t* worig, const uint8_t* pfree, double* part, unsigned* linf,
+                                      unsigned* tstat, hipStream_t st) {
+    switch (store * 2 + (diag ? 1 : 0)) {
+        case PG_STORE_F16 * 2 + 0: return launch_fast_tile<false, PG_STORE_F16>(shape, K, F, G, Q, win, tp, wo, wl, miss, self, out, occ, bu, worig, pfree, part, linf, tstat, st);
+        case PG_STORE_F16 * 2 + 1: return launch_fast_tile<true, PG_STORE_F16>(shape, K, F, G, Q, win, tp, wo, wl, miss, self, out, occ, bu, worig, pfree, part, linf, tstat, st);
+        case PG_STORE_U16 * 2 + 0: return launch_fast_tile<false, PG_STORE_U16>(shape, K, F, G, Q, win, tp, wo, wl, miss, self, out, occ, bu, worig, pfree, part, linf, tstat, st);
+        case PG_STORE_U16 * 2 + 1: return launch_fast_tile<true, PG_STORE_U16>(shape, K, F, G, Q, win, tp, wo, wl, miss, self, out, occ, bu, worig, pfree, part, linf, tstat, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t pgk_launch_interior_stats(const float* V, const int32_t t0[3], const int32_t tn[3], const pgk::Consts& K,
+                                     const pgk::QuantGeom& Q, unsigned* tstat, hipStream_t st) {
+    pgk::Interior I{};
+    for (int a = 0; a < 3; ++a) { I.t0[a] = t0[a]; I.tn[a] = tn[a]; }
+    const int64_t n = (int64_t)tn[0] * tn[1] * tn[2] * K.n[3] * K.n[4] * K.n[5];
+    hipLaunchKernelGGL(pgk::interior_stats_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, V, n, I, K, Q,
+                       tstat);
+    return hipGetLastError();
+}
+
+hipError_t pgk_launch_interior_refit(int store, const int32_t t0[3], const int32_t tn[3], const pgk::QuantGeom& Q,
+                                     unsigned* tstat, const float2* tpo, float2* tpn, uint8_t* trefit, int force,
+                                     hipStream_t st) {
+    pgk::Interior I{};
+    int64_t n = 1;
+    for (int a = 0; a < 3; ++a) {
+        I.t0[a] = t0[a];
+        I.tn[a] = tn[a];
+        n *= ((t0[a] + tn[a] - 1) >> pgk::kQuantShift) - (t0[a] >> pgk::kQuantShift) + 1;
+    }
+    for (int a = 3; a < 6; ++a) n *= Q.nt[a];
+    const float ncode = store == PG_STORE_U16 ? pgk::fastk::kNcodeU16 : pgk::fastk::kNcodeF16;
+    hipLaunchKernelGGL(pgk::interior_refit_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, I, Q, tstat,
+                       tpo, tpn, trefit, ncode, force);
+    return hipGetLastError();
+}
+
+hipError_t pgk_launch_interior_encode(int store, const float* V, const int32_t t0[3], const int32_t tn[3],
+                                      const pgk::Consts& K, const pgk::QuantGeom& Q, const float2* tp,
+                                      uint16_t* codes, hipStream_t st) {
+    pgk::Interior I{};
+    for (int a = 0; a < 3; ++a) { I.t0[a] = t0[a]; I.tn[a] = tn[a]; }
+    const int64_t n = (int64_t)tn[0] * tn[1] * tn[2] * K.n[3] * K.n[4] * K.n[5];
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    if (store == PG_STORE_U16)
+        hipLaunchKernelGGL(pgk::interior_encode_kernel<PG_STORE_U16>, dim3(nb), dim3(256), 0, st, V, n, I, K, Q, tp,
+                           codes);
+    else
+        hipLaunchKernelGGL(pgk::interior_encode_kernel<PG_STORE_F16>, dim3(nb), dim3(256), 0, st, V, n, I, K, Q, tp,
+                           codes);
+    return hipGetLastError();
 }
 
 hipError_t pgk_launch_tile_stats(const float* V, const pgk::Consts& K, const pgk::QuantGeom& Q,

@@ -122,7 +122,7 @@ class HipSolver:
             abi.check(self.lib, "pg_solver_create_ooc",
                       self.lib.pg_solver_create_ooc(self.ctx._h, ctypes.byref(self._gc), ctypes.byref(self._mc),
                                                     ctypes.byref(self._pc), soa.ctypes.data, nctl,
-                                                    occ.ctypes.data, int(ooc_budget), tile.ctypes.data,
+                                                    occ.ctypes.data, int(vstore), int(ooc_budget), tile.ctypes.data,
                                                     ctypes.byref(self._h)))
         self.ncells = g.ncells
 

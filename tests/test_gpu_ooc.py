@@ -83,3 +83,40 @@ def test_budget_too_small_is_an_error(pg):
     with pytest.raises(pg.abi.PeregrineError):
         pg.HipSolver(prob.grid, prob.model, prob.params, prob.ctl, prob.occ, ooc_budget=1 << 16,
                      ooc_tile=(0, 0, 0))
+
+
+def test_binary16_store_refused_out_of_core(pg):
+    prob = pg.configs.problem(1)
+    with pytest.raises(pg.abi.PeregrineError, match="binary16"):
+        pg.HipSolver(prob.grid, prob.model, prob.params, prob.ctl, prob.occ, vstore=pg.abi.PG_VSTORE_F16_TILE,
+                     ooc_budget=8 << 30, ooc_tile=(8, 8, 8))
+
+
+@pytest.mark.parametrize("store", ["u16"])
+def test_precision_form_out_of_core(pg, store):
+    """The precision form out of core (codes streamed, offsets/scales refit
+    tile by tile into the next sweep's array) against the in-HBM precision
+    form: 6 sweeps with x, y and wrapping heading cuts, each bit for bit,
+    then 2 FP32 polish sweeps."""
+    vs = pg.abi.PG_VSTORE_U16_TILE if store == "u16" else pg.abi.PG_VSTORE_F16_TILE
+    prob = pg.configs.problem(2, shape=(24, 24, 16, 9, 9, 11))
+    with pg.HipSolver(prob.grid, prob.model, prob.params, prob.ctl, prob.occ, vstore=vs) as ref, \
+         pg.HipSolver(prob.grid, prob.model, prob.params, prob.ctl, prob.occ, vstore=vs,
+                      ooc_budget=8 << 30, ooc_tile=(8, 8, 8)) as ooc:
+        ref.fill_seed(prob.seeds)
+        ooc.fill_seed(prob.seeds)
+        info = ooc.ooc_info()
+        print(f"{store}: tiles {info[0]} x {info[1]}, amplification {info[3]:.2f}", flush=True)
+        assert info[0] == (8, 8, 8) and info[1] == 18
+        for it in range(6):
+            r = ref.sweep(it, it % 2 == 1)
+            o = ooc.sweep(it, it % 2 == 1)
+            a, b = ref.download(), ooc.download()
+            bad = np.nonzero(bits(a) != bits(b))[0]
+            assert bad.size == 0, f"sweep {it}: {bad.size} cells differ, first {bad[:5].tolist()}"
+            assert o.linf == r.linf
+            assert o.sum_decrease == pytest.approx(r.sum_decrease, rel=1e-9, abs=1e-6)
+        ra = ref.polish(2)
+        oa = ooc.polish(2)
+        assert np.array_equal(bits(ref.download()), bits(ooc.download()))
+        assert ra[1] == oa[1]
