@@ -34,7 +34,13 @@ def test_in_place_iterates_stay_below_jacobi(pg, kernel):
             tabs[mode] = out
     lower = 0
     for a, b in zip(tabs[pg.abi.PG_SWEEP_IN_PLACE], tabs[pg.abi.PG_SWEEP_JACOBI]):
-        assert np.all(a <= b)
+        if kernel == "fast":
+            # the contracted kernel interpolates by lerps a + t (b - a), which
+            # are monotone in each corner only up to rounding: a few ulps
+            over = (a.astype(np.float64) - b) / np.maximum(np.abs(b.astype(np.float64)), 1.0)
+            assert over.max() <= 4 * 2.0 ** -23, f"in place above Jacobi by {over.max():.3e} (relative)"
+        else:
+            assert np.all(a <= b)
         lower = np.count_nonzero(a < b)
     assert lower > 0   # in place does propagate within a sweep
 
