@@ -270,6 +270,9 @@ struct CellStart {
     float sx, sy, sw, dx, dy, dw, w2;
     float tau, hs;     // the cell's horizon and substep
     int32_t nsub;
+    // the control-independent part of the first Euler substep: position and
+    // heading after it, their partial left-fold sum and the heading's sincos
+    float x1, y1, h1, s1, sh1, ch1;
 };
 
 // cfl_tau (PeregrineSolver.jl:954-980) of a cell with field velocity (vfx,
@@ -300,6 +303,18 @@ __device__ __forceinline__ int substeps_dev(const Consts& K, float tau) {
     return k > 1 ? k : 1;
 }
 
+// Position and heading after the first Euler substep, and their partial
+// left-fold sum (the control does not enter them).
+__device__ __forceinline__ void cell_sub1(CellStart& c) {
+    const float px = c.ch * c.vx - c.sh * c.vy;
+    const float py = c.sh * c.vx + c.ch * c.vy;
+    c.x1 = c.x + px * c.hs;
+    c.y1 = c.y + py * c.hs;
+    c.h1 = c.h + c.w * c.hs;
+    const float sum = c.x1 + c.y1;
+    c.s1 = sum + c.h1;
+}
+
 __device__ __forceinline__ void cell_start(const Consts& K, int i0, int i1, int i2, int i3, int i4,
                                            int i5, CellStart& c) {
     c.x = K.lo[0] + (float)i0 * K.step[0];
@@ -327,6 +342,8 @@ __device__ __forceinline__ void cell_start(const Consts& K, int i0, int i1, int 
         c.nsub = K.nsub;
         c.hs = K.hs;
     }
+    cell_sub1(c);
+    sincos_ref(c.h1, c.sh1, c.ch1);
 }
 
 // Kuhn / Freudenthal simplex interpolation (PeregrineSolver.jl:571-665): the
@@ -371,6 +388,41 @@ __device__ __forceinline__ float kuhn_interp(const float f[6], const Corner& cor
 // Forward Euler from a cell start: substep 1 reuses the cell's state-only
 // terms (a grid state is finite, so the divergence test of substep 1 always
 // passes), substeps 2..nsub are PeregrineSolver.jl:208-221 verbatim.
+// One Euler substep (step_state's loop body, PeregrineSolver.jl:208-223)
+// from a state whose finite-check prefix (x + y) + h and heading sincos are
+// given.
+template <bool DIAG>
+__device__ __forceinline__ void euler_substep(const Consts& K, float bux, float buy, float buw, float hs, float pre,
+                                              float sh, float ch, bool& ok, float& x, float& y, float& h, float& vx,
+                                              float& vy, float& w) {
+    float sum = pre + vx;
+    sum = sum + vy;
+    sum = sum + w;
+    ok = ok && finite_f(sum);
+    const float w2 = DIAG ? 0.0f : w * w;
+    const float sx = clamp1(qdiv(vx, K.reps[0]));
+    const float sy = clamp1(qdiv(vy, K.reps[1]));
+    const float sw = clamp1(qdiv(w, K.reps[2]));
+    const float dx = __builtin_fabsf(vx) * vx;
+    const float dy = __builtin_fabsf(vy) * vy;
+    const float dw = __builtin_fabsf(w) * w;
+    float ax, ay, al;
+    accel_rows<DIAG>(K, bux, buy, buw, vx, vy, w, w2, sx, sy, sw, dx, dy, dw, ax, ay, al);
+    const float dvx = ax + w * vy;
+    const float dvy = ay - w * vx;
+    const float px = ch * vx - sh * vy;
+    const float py = sh * vx + ch * vy;
+    x = x + px * hs;
+    y = y + py * hs;
+    h = h + w * hs;
+    vx = vx + dvx * hs;
+    vy = vy + dvy * hs;
+    w = w + al * hs;
+}
+
+// Forward Euler from a cell start: the first substep's position, heading and
+// the second substep's sincos are the start's (they do not depend on the
+// control), the rest as step_state does them.
 template <bool DIAG>
 __device__ __forceinline__ bool euler_from(const Consts& K, const CellStart& c, float bux, float buy,
                                            float buw, float& x, float& y, float& h, float& vx,
@@ -383,43 +435,20 @@ __device__ __forceinline__ bool euler_from(const Consts& K, const CellStart& c, 
                          ax, ay, al);
         const float dvx = ax + c.w * c.vy;
         const float dvy = ay - c.w * c.vx;
-        const float px = c.ch * c.vx - c.sh * c.vy;
-        const float py = c.sh * c.vx + c.ch * c.vy;
-        x = c.x + px * hs;
-        y = c.y + py * hs;
-        h = c.h + c.w * hs;
+        x = c.x1;
+        y = c.y1;
+        h = c.h1;
         vx = c.vx + dvx * hs;
         vy = c.vy + dvy * hs;
         w = c.w + al * hs;
     }
-    for (int k = 1; k < c.nsub; ++k) {
-        float sum = x + y;
-        sum = sum + h;
-        sum = sum + vx;
-        sum = sum + vy;
-        sum = sum + w;
-        ok = ok && finite_f(sum);
-        const float w2 = DIAG ? 0.0f : w * w;
-        const float sx = clamp1(qdiv(vx, K.reps[0]));
-        const float sy = clamp1(qdiv(vy, K.reps[1]));
-        const float sw = clamp1(qdiv(w, K.reps[2]));
-        const float dx = __builtin_fabsf(vx) * vx;
-        const float dy = __builtin_fabsf(vy) * vy;
-        const float dw = __builtin_fabsf(w) * w;
-        float ax, ay, al;
-        accel_rows<DIAG>(K, bux, buy, buw, vx, vy, w, w2, sx, sy, sw, dx, dy, dw, ax, ay, al);
-        const float dvx = ax + w * vy;
-        const float dvy = ay - w * vx;
+    if (c.nsub > 1) euler_substep<DIAG>(K, bux, buy, buw, hs, c.s1, c.sh1, c.ch1, ok, x, y, h, vx, vy, w);
+    for (int k = 2; k < c.nsub; ++k) {
+        float pre = x + y;
+        pre = pre + h;
         float sh, ch;
         sincos_ref(h, sh, ch);
-        const float px = ch * vx - sh * vy;
-        const float py = sh * vx + ch * vy;
-        x = x + px * hs;
-        y = y + py * hs;
-        h = h + w * hs;
-        vx = vx + dvx * hs;
-        vy = vy + dvy * hs;
-        w = w + al * hs;
+        euler_substep<DIAG>(K, bux, buy, buw, hs, pre, sh, ch, ok, x, y, h, vx, vy, w);
     }
     return ok;
 }

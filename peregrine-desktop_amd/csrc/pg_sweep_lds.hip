@@ -22,6 +22,7 @@ struct BoxShared {
     float x[MAXS], y[MAXS], h[MAXS], vx[MAXS], vy[MAXS], w[MAXS];
     float sh[MAXS], ch[MAXS], sx[MAXS], sy[MAXS], sw[MAXS], dx[MAXS], dy[MAXS], dw[MAXS], w2[MAXS];
     float tau[MAXS], hs[MAXS];   // per-cell horizon (cfl > 0)
+    float sh1[MAXS], ch1[MAXS];  // sincos of the heading after the first substep
     unsigned long long best[MAXS];
     uint8_t blocked[MAXS];
     uint8_t nsub[MAXS];
@@ -38,6 +39,8 @@ __device__ __forceinline__ void load_cell_start(const Consts& K, const BoxShared
     } else {
         c.tau = K.tau; c.hs = K.hs; c.nsub = K.nsub;
     }
+    cell_sub1(c);
+    c.sh1 = sm.sh1[s]; c.ch1 = sm.ch1[s];
 }
 
 template <int MAXS>
@@ -53,6 +56,7 @@ __device__ __forceinline__ void setup_box(const Consts& K, const BoxInfo& B,
         sm.sh[s] = c.sh; sm.ch[s] = c.ch; sm.sx[s] = c.sx; sm.sy[s] = c.sy; sm.sw[s] = c.sw;
         sm.dx[s] = c.dx; sm.dy[s] = c.dy; sm.dw[s] = c.dw; sm.w2[s] = c.w2;
         sm.tau[s] = c.tau; sm.hs[s] = c.hs; sm.nsub[s] = (uint8_t)c.nsub;
+        sm.sh1[s] = c.sh1; sm.ch1[s] = c.ch1;
         const bool blk = occ[((int64_t)i[0] * K.n[1] + i[1]) * K.n[2] + i[2]] != 0;
         sm.blocked[s] = blk ? 1 : 0;
         // blocked cells keep (cap, coast) (PeregrineSolver.jl:1226-1228)
