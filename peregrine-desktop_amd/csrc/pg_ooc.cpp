@@ -152,8 +152,25 @@ void build_tiles(const pg_solver* s, Ooc& o, const std::vector<PlanRef>& plans, 
 int64_t window_cells(const OocTile& t, int64_t colv) { return (int64_t)t.wl[0] * t.wl[1] * t.wl[2] * colv; }
 int64_t interior_cells(const OocTile& t, int64_t colv) { return (int64_t)t.tn[0] * t.tn[1] * t.tn[2] * colv; }
 
-// Host table rows -> window buffer, x plane by x plane; a heading range that
-// wraps is two copies.
+// One strided block between the host table and a device buffer: depth x
+// planes of height y rows of width bytes; host rows n3 * colv cells apart,
+// host planes n1 rows apart.
+template <class T>
+hipError_t copy_block(const pg_solver* s, const Ooc& o, T* host, T* dev, size_t dev_row, size_t dev_rows,
+                      size_t width, size_t height, size_t depth, bool to_host, hipStream_t st) {
+    const size_t host_row = (size_t)s->g.n[2] * (size_t)o.colv * sizeof(T);
+    hipPitchedPtr hp = make_hipPitchedPtr(host, host_row, width, (size_t)s->g.n[1]);
+    hipPitchedPtr dp = make_hipPitchedPtr(dev, dev_row, width, dev_rows);
+    hipMemcpy3DParms p{};
+    p.srcPtr = to_host ? dp : hp;
+    p.dstPtr = to_host ? hp : dp;
+    p.extent = make_hipExtent(width, height, depth);
+    p.kind = to_host ? hipMemcpyDeviceToHost : hipMemcpyHostToDevice;
+    return hipMemcpy3DAsync(&p, st);
+}
+
+// Host table -> window buffer: one block per heading piece (a heading range
+// that wraps is two).
 template <class T>
 hipError_t copy_window_in(const pg_solver* s, const Ooc& o, const OocTile& t, const T* hV, T* dw,
                           hipStream_t st) {
@@ -161,41 +178,29 @@ hipError_t copy_window_in(const pg_solver* s, const Ooc& o, const OocTile& t, co
     const int64_t colv = o.colv;
     const int n3 = g.n[2];
     const int32_t h_first = std::min(t.wl[2], n3 - t.wo[2]);
-    for (int32_t lx = 0; lx < t.wl[0]; ++lx) {
-        const int64_t x = t.wo[0] + lx;
-        for (int piece = 0; piece < 2; ++piece) {
-            const int32_t hs = piece == 0 ? t.wo[2] : 0;
-            const int32_t hl = piece == 0 ? h_first : t.wl[2] - h_first;
-            const int32_t lh = piece == 0 ? 0 : h_first;
-            if (hl <= 0) continue;
-            const T* src = hV + ((x * g.n[1] + t.wo[1]) * n3 + hs) * colv;
-            T* dst = dw + (((int64_t)lx * t.wl[1]) * t.wl[2] + lh) * colv;
-            hipError_t e = hipMemcpy2DAsync(dst, (size_t)t.wl[2] * colv * sizeof(T), src,
-                                            (size_t)n3 * colv * sizeof(T), (size_t)hl * colv * sizeof(T),
-                                            (size_t)t.wl[1], hipMemcpyHostToDevice, st);
-            if (e != hipSuccess) return e;
-        }
+    for (int piece = 0; piece < 2; ++piece) {
+        const int32_t hs = piece == 0 ? t.wo[2] : 0;
+        const int32_t hl = piece == 0 ? h_first : t.wl[2] - h_first;
+        const int32_t lh = piece == 0 ? 0 : h_first;
+        if (hl <= 0) continue;
+        T* src = const_cast<T*>(hV) + (((int64_t)t.wo[0] * g.n[1] + t.wo[1]) * n3 + hs) * colv;
+        T* dst = dw + (int64_t)lh * colv;
+        hipError_t e = copy_block<T>(s, o, src, dst, (size_t)t.wl[2] * colv * sizeof(T), (size_t)t.wl[1],
+                                     (size_t)hl * colv * sizeof(T), (size_t)t.wl[1], (size_t)t.wl[0], false, st);
+        if (e != hipSuccess) return e;
     }
     return hipSuccess;
 }
 
-// Interior buffer <-> host table, x plane by x plane.
+// Interior buffer <-> host table, one block.
 template <class T>
 hipError_t copy_interior(const pg_solver* s, const Ooc& o, const OocTile& t, T* dbuf, T* hV, bool to_host,
                          hipStream_t st) {
     const pg_grid& g = s->g;
     const int64_t colv = o.colv;
-    const int n3 = g.n[2];
-    for (int32_t lx = 0; lx < t.tn[0]; ++lx) {
-        const int64_t x = t.t0[0] + lx;
-        T* h = hV + ((x * g.n[1] + t.t0[1]) * n3 + t.t0[2]) * colv;
-        T* d = dbuf + ((int64_t)lx * t.tn[1] * t.tn[2]) * colv;
-        const size_t hp = (size_t)n3 * colv * sizeof(T), dp = (size_t)t.tn[2] * colv * sizeof(T);
-        hipError_t e = to_host ? hipMemcpy2DAsync(h, hp, d, dp, dp, (size_t)t.tn[1], hipMemcpyDeviceToHost, st)
-                               : hipMemcpy2DAsync(d, dp, h, hp, dp, (size_t)t.tn[1], hipMemcpyHostToDevice, st);
-        if (e != hipSuccess) return e;
-    }
-    return hipSuccess;
+    T* h = hV + (((int64_t)t.t0[0] * g.n[1] + t.t0[1]) * g.n[2] + t.t0[2]) * colv;
+    const size_t row = (size_t)t.tn[2] * colv * sizeof(T);
+    return copy_block<T>(s, o, h, dbuf, row, (size_t)t.tn[1], row, (size_t)t.tn[1], (size_t)t.tn[0], to_host, st);
 }
 
 hipError_t copy_interior_out(const pg_solver* s, const Ooc& o, const OocTile& t, float* dout, float* hV,
