@@ -339,18 +339,25 @@ def solve_value(solver: HipSolver, tidx, iters: int, tol: float, tol_linf: float
                 on_progress=None):
     """solve_value! (Run.jl:1434-1474): returns (iterations_done, final_delta).
 
-    `tidx` holds 0-based seed indices. The quiet rule is the reference's
-    (summed decrease <= tol, one quiet sweep for a full control scan); with
-    tol_linf > 0 the sup-norm rule ||dV||_inf < tol_linf stops as well.
+    `tidx` holds 0-based seed indices. Sweep i takes phase i - 1 (the
+    rotating slice's offset) and alternates direction. The quiet rule is the
+    reference's: stop after quiet_needed sweeps in a row whose summed
+    decrease is <= tol, one for a full control scan and a whole turn of the
+    slice otherwise (Run.jl:1440-1441); with tol_linf > 0 a sweep whose
+    ||dV||_inf < tol_linf also counts as quiet. on_progress(it, result) is
+    called on sweep 1, every 5th and the last.
     """
+    p = solver.p
+    need = 1 if p.ncoarse <= 0 else max(1, -(-(solver.nctl - 1) // p.ncoarse))
     solver.fill_seed(tidx)
-    done, last = 0, float("inf")
+    done, last, quiet = 0, float("inf"), 0
     for it in range(1, iters + 1):
         r = solver.sweep(it - 1, it % 2 == 1)
         done, last = it, r.sum_decrease
-        quiet = (tol >= 0 and r.sum_decrease <= tol) or (tol_linf > 0 and r.linf < tol_linf)
-        if on_progress is not None and (it % 5 == 0 or it == 1 or quiet):
+        q = (tol >= 0 and r.sum_decrease <= tol) or (tol_linf > 0 and r.linf < tol_linf)
+        quiet = quiet + 1 if q else 0
+        if on_progress is not None and (it % 5 == 0 or it == 1 or quiet >= need):
             on_progress(it, r)
-        if quiet:
+        if quiet >= need:
             break
     return done, last
