@@ -267,12 +267,13 @@ int pg_solver_ncells(pg_solver* s, int64_t* n);
  * pg_upload_V, pg_last_sweep_ms as usual (at most 128 controls; no policy,
  * escape pass or FP64 tables).
  *
- * vstore = PG_VSTORE_U16_TILE runs the precision form out of core: the host also holds the Jacobi pair of 16-bit codes, windows
+ * vstore = PG_VSTORE_U16_TILE or PG_VSTORE_F16_TILE runs the precision form
+ * out of core: the host also holds the Jacobi pair of 16-bit codes, windows
  * stream in as codes (half the bytes) with the interior's own FP32 values,
  * and each tile's quantization tiles are refit and its interior encoded
  * before values and codes stream out; tiles are cut on multiples of 4 cells.
  * Every sweep equals the in-HBM precision-form sweep bit for bit; pg_polish
- * then continues with FP32 tiles. PG_VSTORE_F16_TILE is refused out of core. */
+ * then continues with FP32 tiles. */
 int pg_solver_create_ooc(pg_ctx* ctx, const pg_grid* g, const pg_model* m, const pg_params* p,
                          const float* ctl_soa, int32_t nctl, const uint8_t* occ, int32_t vstore,
                          int64_t budget_bytes, const int32_t tile[3], pg_solver** out);

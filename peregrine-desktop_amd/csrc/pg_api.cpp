@@ -679,8 +679,6 @@ int pg_solver_create_ooc(pg_ctx* ctx, const pg_grid* g, const pg_model* m, const
     if (p && (p->rounds > 0 || p->ntau > 1))
         return fail(PG_ERR_UNSUPPORTED, "the out-of-core sweep does not run refine_rounds or tau_levels (%s set)",
                     p->rounds > 0 ? "refine_rounds" : "tau_levels");
-    if (vstore == PG_VSTORE_F16_TILE)
-        return fail(PG_ERR_UNSUPPORTED, "the binary16 store is not built out of core (PG_VSTORE_U16_TILE is)");
     if (!tile) return fail(PG_ERR_ARG, "null tile");
     int rc = create_solver(ctx, g, m, p, ctl_soa, nctl, occ, vstore, true, out);
     if (rc) return rc;

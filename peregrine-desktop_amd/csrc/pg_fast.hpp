@@ -7,7 +7,10 @@
 // tensor product of 63 lerps (identical weights in exact arithmetic,
 // PeregrineSolver.jl:399-466). Results agree with the exact path to FP32
 // rounding, not bit for bit; the precision form is measured against the
-// FP64 restatement instead.
+// FP64 restatement instead. Every fused multiply-add is written out
+// (fmaf), never left to the compiler's contraction, so that each kernel
+// built from these functions (the in-HBM and the out-of-core instantiations)
+// computes the same bits.
 #pragma once
 #include "pg_device.hpp"
 
@@ -20,7 +23,7 @@ struct FastConsts {
     float reps[3];    // 1 / eps[k]
 };
 
-#pragma clang fp contract(fast)
+__device__ __forceinline__ float fma_(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
 __device__ __forceinline__ void sincos_fast(float x, float& s, float& c) {
     // v_sin_f32 / v_cos_f32 take the angle in revolutions
@@ -40,17 +43,23 @@ __device__ __forceinline__ void accel_fast(const Consts& K, const FastConsts& F,
     const float dy = __builtin_fabsf(vy) * vy;
     const float dw = __builtin_fabsf(w) * w;
     if (DIAG) {
-        ax = bux + K.A[0] * vx + K.S[0] * sx + K.D[0] * dx + K.c[0];
-        ay = buy + K.A[4] * vy + K.S[4] * sy + K.D[4] * dy + K.c[1];
-        al = buw + K.A[8] * w + K.S[8] * sw + K.D[8] * dw + K.c[2];
+        ax = fma_(K.D[0], dx, fma_(K.S[0], sx, fma_(K.A[0], vx, bux))) + K.c[0];
+        ay = fma_(K.D[4], dy, fma_(K.S[4], sy, fma_(K.A[4], vy, buy))) + K.c[1];
+        al = fma_(K.D[8], dw, fma_(K.S[8], sw, fma_(K.A[8], w, buw))) + K.c[2];
     } else {
         const float w2 = w * w;
-        ax = bux + K.A[0] * vx + K.A[1] * vy + K.A[2] * w + K.q[0] * w2 + K.S[0] * sx + K.S[1] * sy +
-             K.S[2] * sw + K.D[0] * dx + K.D[1] * dy + K.D[2] * dw + K.c[0];
-        ay = buy + K.A[3] * vx + K.A[4] * vy + K.A[5] * w + K.q[1] * w2 + K.S[3] * sx + K.S[4] * sy +
-             K.S[5] * sw + K.D[3] * dx + K.D[4] * dy + K.D[5] * dw + K.c[1];
-        al = buw + K.A[6] * vx + K.A[7] * vy + K.A[8] * w + K.q[2] * w2 + K.S[6] * sx + K.S[7] * sy +
-             K.S[8] * sw + K.D[6] * dx + K.D[7] * dy + K.D[8] * dw + K.c[2];
+        const float z[11] = {vx, vy, w, w2, sx, sy, sw, dx, dy, dw, 0.0f};
+        float r[3] = {bux, buy, buw};
+#pragma unroll
+        for (int row = 0; row < 3; ++row) {
+            const float co[10] = {K.A[3 * row], K.A[3 * row + 1], K.A[3 * row + 2], K.q[row], K.S[3 * row],
+                                  K.S[3 * row + 1], K.S[3 * row + 2], K.D[3 * row], K.D[3 * row + 1],
+                                  K.D[3 * row + 2]};
+#pragma unroll
+            for (int j = 0; j < 10; ++j) r[row] = fma_(co[j], z[j], r[row]);
+            r[row] = r[row] + K.c[row];
+        }
+        ax = r[0]; ay = r[1]; al = r[2];
     }
 }
 
@@ -72,12 +81,12 @@ __device__ __forceinline__ void fast_start(const Consts& K, const FastConsts& F,
     S.x0 = c.x;
     S.y0 = c.y;
     S.h0 = c.h;
-    S.x1 = c.x + (c.ch * c.vx - c.sh * c.vy) * hs;
-    S.y1 = c.y + (c.sh * c.vx + c.ch * c.vy) * hs;
-    S.h1 = c.h + c.w * hs;
-    S.vx1 = c.vx + (ax + c.w * c.vy) * hs;
-    S.vy1 = c.vy + (ay - c.w * c.vx) * hs;
-    S.w1 = c.w + al * hs;
+    S.x1 = fma_(fma_(c.ch, c.vx, -(c.sh * c.vy)), hs, c.x);
+    S.y1 = fma_(fma_(c.sh, c.vx, c.ch * c.vy), hs, c.y);
+    S.h1 = fma_(c.w, hs, c.h);
+    S.vx1 = fma_(fma_(c.w, c.vy, ax), hs, c.vx);
+    S.vy1 = fma_(fma_(-c.w, c.vx, ay), hs, c.vy);
+    S.w1 = fma_(al, hs, c.w);
     sincos_fast(S.h1, S.sh1, S.ch1);
 }
 
@@ -90,29 +99,29 @@ __device__ __forceinline__ void euler_fast(const Consts& K, const FastConsts& F,
                                            float& h, float& vx, float& vy, float& w) {
     const float hs = K.hs;
     x = S.x1; y = S.y1; h = S.h1;
-    vx = S.vx1 + bux * hs;
-    vy = S.vy1 + buy * hs;
-    w = S.w1 + buw * hs;
+    vx = fma_(bux, hs, S.vx1);
+    vy = fma_(buy, hs, S.vy1);
+    w = fma_(buw, hs, S.w1);
     float sh = S.sh1, ch = S.ch1;
     for (int k = 1; k < K.nsub; ++k) {
         if (k > 1) sincos_fast(h, sh, ch);
         float ax, ay, al;
         accel_fast<DIAG>(K, F, bux, buy, buw, vx, vy, w, ax, ay, al);
-        const float dvx = ax + w * vy;
-        const float dvy = ay - w * vx;
-        const float px = ch * vx - sh * vy;
-        const float py = sh * vx + ch * vy;
-        x = x + px * hs;
-        y = y + py * hs;
-        h = h + w * hs;
-        vx = vx + dvx * hs;
-        vy = vy + dvy * hs;
-        w = w + al * hs;
+        const float dvx = fma_(w, vy, ax);
+        const float dvy = fma_(-w, vx, ay);
+        const float px = fma_(ch, vx, -(sh * vy));
+        const float py = fma_(sh, vx, ch * vy);
+        x = fma_(px, hs, x);
+        y = fma_(py, hs, y);
+        h = fma_(w, hs, h);
+        vx = fma_(dvx, hs, vx);
+        vy = fma_(dvy, hs, vy);
+        w = fma_(al, hs, w);
     }
 }
 
 __device__ __forceinline__ float wrap_fast(const Consts& K, const FastConsts& F, float f) {
-    return f - __builtin_floorf(f * F.rn3) * K.n3f;
+    return fma_(-__builtin_floorf(f * F.rn3), K.n3f, f);
 }
 
 template <class OccRead>
@@ -120,9 +129,9 @@ __device__ __forceinline__ bool probes_fast(const Consts& K, const FastConsts& F
                                             float x, float y, float h, float dx, float dy, float dh) {
     for (int sc = 0; sc < K.nchk; ++sc) {
         const float a = K.probe_a[sc];
-        const float gx = (x + dx * a - K.lo[0]) * F.rstep[0];
-        const float gy = (y + dy * a - K.lo[1]) * F.rstep[1];
-        const float gh = wrap_fast(K, F, (h + dh * a - K.lo[2]) * F.rstep[2]);
+        const float gx = (fma_(dx, a, x) - K.lo[0]) * F.rstep[0];
+        const float gy = (fma_(dy, a, y) - K.lo[1]) * F.rstep[1];
+        const float gh = wrap_fast(K, F, (fma_(dh, a, h) - K.lo[2]) * F.rstep[2]);
         const bool in = (gx >= 0.0f) & (gy >= 0.0f) & (gx <= K.lim[0]) & (gy <= K.lim[1]) &
                         (gh >= 0.0f) & (gh <= K.n3f);
         if (!in) return false;
@@ -150,8 +159,8 @@ __device__ __forceinline__ bool successor_fast(const Consts& K, const FastConsts
     if (probe && !probes_fast(K, F, og, S.x0, S.y0, S.h0, dx, dy, dh)) return false;
     float snh, cnh;
     sincos_fast(nh, snh, cnh);
-    const float nvfx = cnh * nvx - snh * nvy;
-    const float nvfy = snh * nvx + cnh * nvy;
+    const float nvfx = fma_(cnh, nvx, -(snh * nvy));
+    const float nvfy = fma_(snh, nvx, cnh * nvy);
     f[0] = (nx - K.lo[0]) * F.rstep[0];
     f[1] = (ny - K.lo[1]) * F.rstep[1];
     f[2] = wrap_fast(K, F, (nh - K.lo[2]) * F.rstep[2]);
@@ -178,7 +187,7 @@ __device__ __forceinline__ bool successor_reach_fast(const Consts& K, const Fast
     return finite_f(g[0]) & finite_f(g[1]) & finite_f(g[2]) & finite_f(nvx) & finite_f(nvy) & finite_f(nw);
 }
 
-__device__ __forceinline__ float lerp_fast(float a, float b, float t) { return a + t * (b - a); }
+__device__ __forceinline__ float lerp_fast(float a, float b, float t) { return fma_(t, b - a, a); }
 
 // Multilinear interpolation as a tensor product: w first (the corner pairs
 // are adjacent in memory), then vy, vx, h, y, x. vals[d] holds corner d with
@@ -225,7 +234,5 @@ __device__ __forceinline__ float tensor_interp_grouped(const Fetch& fetch, const
     const float y0 = lerp_fast(h0, h2, t[1]), y1 = lerp_fast(h1, h3, t[1]);
     return lerp_fast(y0, y1, t[0]);
 }
-
-#pragma clang fp contract(off)
 
 }  // namespace pgk

@@ -104,28 +104,28 @@ struct VRead<PG_STORE_F32> {
 // cell; omega fastest. Offsets/scales stay those of the whole grid. The
 // window holds every cell the tile's backups read, so a read outside it is a
 // defect: it returns cap and raises *miss. Staged cells outside it are never
-// read as corners and stage cap quietly.
+// read as corners and stage cap quietly. A window holds fewer than 2^31
+// cells (the host checks), so its indices are 32-bit.
 template <int STORE>
 struct VReadTile {
     const uint16_t* __restrict__ W;
     const float2* __restrict__ tp;
     int32_t o[3], L[3];
-    int64_t ws[6];
+    int32_t ws[6];
     unsigned* miss;
-    __device__ __forceinline__ bool local(const Consts& K, const int* gi, int64_t& fl) const {
+    __device__ __forceinline__ bool local(const Consts& K, const int* gi, int32_t& fl) const {
         int l2 = gi[2] - o[2];
         l2 = l2 < 0 ? l2 + K.n[2] : l2;
         const int l0 = gi[0] - o[0], l1 = gi[1] - o[1];
-        fl = (int64_t)l0 * ws[0] + (int64_t)l1 * ws[1] + (int64_t)l2 * ws[2] + (int64_t)gi[3] * ws[3] +
-             (int64_t)gi[4] * ws[4] + gi[5];
+        fl = l0 * ws[0] + l1 * ws[1] + l2 * ws[2] + gi[3] * ws[3] + gi[4] * ws[4] + gi[5];
         return ((unsigned)l0 < (unsigned)L[0]) & ((unsigned)l1 < (unsigned)L[1]) & ((unsigned)l2 < (unsigned)L[2]);
     }
     __device__ __forceinline__ float stage(const Consts& K, const QuantGeom& Q, const int* gi) const {
-        int64_t fl;
+        int32_t fl;
         return local(K, gi, fl) ? Codec<STORE>::decode(W[fl], tp[tile_of_coords(Q, gi)], K.cap) : K.cap;
     }
     __device__ __forceinline__ float read(const Consts& K, const QuantGeom& Q, const int* gi) const {
-        int64_t fl;
+        int32_t fl;
         if (local(K, gi, fl)) return Codec<STORE>::decode(W[fl], tp[tile_of_coords(Q, gi)], K.cap);
         atomicOr(miss, 1u);
         return K.cap;
@@ -133,7 +133,8 @@ struct VReadTile {
 };
 
 // The states' own FP32 values and the sweep's FP32 results: the whole grid,
-// or an out-of-core tile's interior (x, y, heading from t0; omega fastest).
+// or an out-of-core tile's interior (x, y, heading from t0; omega fastest;
+// fewer than 2^31 cells).
 struct SelfGlobal {
     const float* self;
     float* out;
@@ -144,10 +145,10 @@ struct SelfTile {
     const float* __restrict__ self;
     float* __restrict__ out;
     int32_t t0[3];
-    int64_t os[6];
-    __device__ __forceinline__ int64_t index(const Consts&, const int* i) const {
-        return (int64_t)(i[0] - t0[0]) * os[0] + (int64_t)(i[1] - t0[1]) * os[1] + (int64_t)(i[2] - t0[2]) * os[2] +
-               (int64_t)i[3] * os[3] + (int64_t)i[4] * os[4] + i[5];
+    int32_t os[6];
+    __device__ __forceinline__ int32_t index(const Consts&, const int* i) const {
+        return (i[0] - t0[0]) * os[0] + (i[1] - t0[1]) * os[1] + (i[2] - t0[2]) * os[2] + i[3] * os[3] +
+               i[4] * os[4] + i[5];
     }
 };
 
@@ -703,11 +704,11 @@ hipError_t launch_fast_tile(int shape, const pgk::Consts& K, const pgk::FastCons
     rd.W = win;
     rd.tp = tp;
     rd.miss = miss;
-    const int64_t colv = (int64_t)K.n[3] * K.n[4] * K.n[5];
+    const int32_t colv = K.n[3] * K.n[4] * K.n[5];
     for (int a = 0; a < 3; ++a) { rd.o[a] = wo[a]; rd.L[a] = wl[a]; }
     rd.ws[5] = 1;
     rd.ws[4] = K.n[5];
-    rd.ws[3] = (int64_t)K.n[4] * K.n[5];
+    rd.ws[3] = K.n[4] * K.n[5];
     rd.ws[2] = colv;
     rd.ws[1] = colv * wl[2];
     rd.ws[0] = colv * wl[2] * wl[1];
@@ -717,7 +718,7 @@ hipError_t launch_fast_tile(int shape, const pgk::Consts& K, const pgk::FastCons
     for (int a = 0; a < 3; ++a) io.t0[a] = G.org[a];
     io.os[5] = 1;
     io.os[4] = K.n[5];
-    io.os[3] = (int64_t)K.n[4] * K.n[5];
+    io.os[3] = K.n[4] * K.n[5];
     io.os[2] = colv;
     io.os[1] = colv * G.rn[2];
     io.os[0] = colv * G.rn[2] * G.rn[1];

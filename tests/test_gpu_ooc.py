@@ -85,14 +85,7 @@ def test_budget_too_small_is_an_error(pg):
                      ooc_tile=(0, 0, 0))
 
 
-def test_binary16_store_refused_out_of_core(pg):
-    prob = pg.configs.problem(1)
-    with pytest.raises(pg.abi.PeregrineError, match="binary16"):
-        pg.HipSolver(prob.grid, prob.model, prob.params, prob.ctl, prob.occ, vstore=pg.abi.PG_VSTORE_F16_TILE,
-                     ooc_budget=8 << 30, ooc_tile=(8, 8, 8))
-
-
-@pytest.mark.parametrize("store", ["u16"])
+@pytest.mark.parametrize("store", ["u16", "f16"])
 def test_precision_form_out_of_core(pg, store):
     """The precision form out of core (codes streamed, offsets/scales refit
     tile by tile into the next sweep's array) against the in-HBM precision
