@@ -684,11 +684,81 @@ def runtime_estimate(g: Grid6, sp: SolverParams, esc_iters: int, ntargets: int) 
             "lookaheads_per_state": lookaheads_per_state(p, sp.nctl, sp.warm)}
 
 
+def suggest_heading_deg(reach_cm: float, xy_cm: float) -> float:
+    """Heading bins such that the footprint's outer point moves about one
+    position cell per bin: 2 pi reach / xy bins, clamped to [8, 180]
+    (Advise.jl:361-365)."""
+    r = max(float(reach_cm), 1.0e-6)
+    nb = min(max(math.ceil(2 * math.pi * r / float(xy_cm)), 8), 180)
+    return 360.0 / nb
+
+
+def suggest_resolution(span_x: float, span_y: float, reach_cm: float, vmax: float, wmax: float, elem_bytes: int,
+                       ntargets: int, budget_bytes: float, finest_cm: float = 1.0, vram_bytes: float = math.inf,
+                       bytes_per_cell: int = 16):
+    """The finest position cell (from finest_cm, 0.1 cm steps below 10 cm,
+    0.5 cm above, up to 60 cm) whose card bytes fit budget_bytes and whose
+    device bytes fit vram_bytes; heading from the footprint, velocity and
+    spin a tenth of their envelope (21 samples each). bound_by says which
+    budget stopped it (Advise.jl:400-434)."""
+    v_cm_s, w_rad_s = vmax / 10, wmax / 10
+    nv = nw = 21
+    xy = float(finest_cm)
+    while xy <= 60.0:
+        hdeg = suggest_heading_deg(reach_cm, xy)
+        nh = max(4, round(360.0 / hdeg))
+        n1 = max(2, math.ceil(span_x / xy - 1.0e-9) + 1)
+        n2 = max(2, math.ceil(span_y / xy - 1.0e-9) + 1)
+        cells = n1 * n2 * nh * nv * nv * nw
+        nbytes = float(cells) * elem_bytes * max(ntargets, 1)
+        vram = float(cells) * bytes_per_cell
+        if nbytes <= budget_bytes and vram <= vram_bytes:
+            return {"xy_cm": xy, "heading_deg": hdeg, "v_cm_s": v_cm_s, "w_rad_s": w_rad_s,
+                    "n": [n1, n2, nh, nv, nv, nw], "cells": cells, "bytes": nbytes, "vram_bytes": vram,
+                    "bound_by": "vram" if vram > 0.5 * vram_bytes else "card"}
+        xy += 0.1 if xy < 10 else 0.5
+    return None
+
+
+def shape_for(span_x: float, span_y: float, vmax: float, wmax: float, xy: float, hdeg: float, v: float, w: float):
+    """The sample counts build_grid gives a resolution (Advise.jl:436-447)."""
+    n1 = max(2, math.ceil(span_x / xy - 1.0e-9) + 1)
+    n2 = max(2, math.ceil(span_y / xy - 1.0e-9) + 1)
+    nh = max(4, round(360.0 / hdeg))
+    nv = 2 * max(1, math.ceil(vmax / v - 1.0e-9)) + 1
+    nw = 2 * max(1, math.ceil(wmax / w - 1.0e-9)) + 1
+    return [n1, n2, nh, nv, nv, nw], n1 * n2 * nh * nv * nv * nw
+
+
+def coarsen_to_fit(span_x: float, span_y: float, vmax: float, wmax: float, res: dict, budget_bytes: float,
+                   bytes_per_cell: int = 16):
+    """The smallest common scale (1 to 16, steps of 0.05) of all four cell
+    sizes at which the grid holds whole on the device (Advise.jl:469-511)."""
+    k = 1.0
+    while k <= 16.0:
+        n, cells = shape_for(span_x, span_y, vmax, wmax, res["xy_cm"] * k, res["heading_deg"] * k,
+                             res["v_cm_s"] * k, res["w_rad_s"] * k)
+        if float(cells) * bytes_per_cell <= budget_bytes:
+            return {"scale": k, "xy_cm": res["xy_cm"] * k, "heading_deg": res["heading_deg"] * k,
+                    "v_cm_s": res["v_cm_s"] * k, "w_rad_s": res["w_rad_s"] * k, "n": n, "cells": cells,
+                    "vram_bytes": float(cells) * bytes_per_cell}
+        k += 0.05
+    return None
+
+
+def device_bytes_per_cell(warm: bool) -> int:
+    """Device bytes per state of a whole-grid FP32 solve: the Jacobi pair of
+    tables and, with the warm start, its 3-byte store."""
+    return 8 + (3 if warm else 0)
+
+
 def plan(cfg) -> dict:
     """Size and feasibility report: grid, bounds, resolution, occupancy,
     target checks, bytes on the card, whether the table fits the device, the
-    horizon in force and a run-time bound from measured kernel costs. The
-    reference's resolution and tiling advice (Advise.jl) is not part of it."""
+    horizon in force, a run-time bound from measured kernel costs, and the
+    resolution advice (the finest resolution within the card and device
+    budgets; the common coarsening that holds the grid whole). The reference's
+    horizon/tiling advice for its own tiled driver is not part of it."""
     bounds, polys, robot, gcfg, gi, g = _setup(cfg)
     m, _, _ = _model_for(cfg)
     names, states, _ = load_targets(str(cfg["targets"]))
@@ -705,6 +775,24 @@ def plan(cfg) -> dict:
     eb = card.DTYPES[dtype][0]
     avail, fits, need, free = _device_fit(g)
     res = grid_resolution(gcfg)
+    # resolution advice (Run.jl:800-833): a size budget on the card, the
+    # footprint's radius for the heading rule, and the device's free bytes
+    rec_budget = float(getc(cfg, "size_budget_bytes", 8 * 2 ** 30))
+    reach_cm = 0.0 if robot is None or robot.shape[1] < 3 else float(np.max(np.hypot(robot[0], robot[1])))
+    vmax = float(getc(gcfg, "vmax", DEFAULT_VMAX))
+    wmax = float(getc(gcfg, "wmax", DEFAULT_WMAX))
+    bpc = device_bytes_per_cell(sp.warm)
+    dev_budget = free * 2 ** 30 if avail and free > 0 else 0.0
+    span_x = float(g.step[0]) * (g.n[0] - 1)
+    span_y = float(g.step[1]) * (g.n[1] - 1)
+    sugg = suggest_resolution(span_x, span_y, reach_cm, vmax, wmax, card.DTYPES[dtype][0], len(names), rec_budget,
+                              vram_bytes=dev_budget if dev_budget > 0 else math.inf, bytes_per_cell=bpc)
+    incore = None
+    if avail and not fits and dev_budget > 0:
+        incore = coarsen_to_fit(span_x, span_y, vmax, wmax,
+                                {"xy_cm": float(min(g.step[0], g.step[1])),
+                                 "heading_deg": math.degrees(float(g.step[2])), "v_cm_s": float(g.step[3]),
+                                 "w_rad_s": float(g.step[5])}, dev_budget, bytes_per_cell=bpc)
     bal = grid_balance(g, m)
     full = (bounds[2] - bounds[0]) * (bounds[3] - bounds[1])
     span = (grid_max(g, 0) - float(g.lo[0])) * (grid_max(g, 1) - float(g.lo[1]))
@@ -737,6 +825,7 @@ def plan(cfg) -> dict:
         "cell_cm": float(min(g.step[0], g.step[1])), "cell_cm_s": float(min(g.step[3], g.step[4])),
         "controls": sp.nctl, "unsupported_options": unsupported_options(sp, avail and not fits),
         "recommend": {
+            "size_budget_bytes": rec_budget, "resolution": sugg, "in_core": incore,
             "tau_applied": float(sp.p.tau_max), "tau_applied_cost": value_cost(sp.p.tau_max, TAU_REF),
             "tau_auto": sp.tau_auto, "tau_reference": TAU_REF,
             "tau_pinned_waste": (not sp.tau_auto) and bool(fits) and sp.p.tau_max < TAU_REF,

@@ -218,3 +218,27 @@ def test_solve_refuses_a_cpu_backend(tmp_path):
     r = cli(path, "solve")
     assert r.returncode == 1 and r.stderr.startswith("ERROR ") and "backend 'cpu'" in r.stderr
     assert not os.path.exists(cfg["out_dir"])
+
+
+def test_resolution_advice_is_the_finest_that_fits(run):
+    """suggest_resolution returns a grid within both budgets whose next finer
+    position cell would not fit; the heading count follows the footprint
+    (one position cell of arc per bin, 8 to 180 bins); coarsen_to_fit is the
+    smallest common scale that holds the grid whole."""
+    assert run.suggest_heading_deg(0.0, 8.0) == 360.0 / 8
+    assert run.suggest_heading_deg(25.0, 1.0) == 360.0 / 158      # ceil(2 pi 25)
+    assert run.suggest_heading_deg(1000.0, 1.0) == 360.0 / 180
+    budget = 2.0 * 2 ** 30
+    r = run.suggest_resolution(320.0, 320.0, 25.0, 170.0, 8.0, 2, 3, budget, vram_bytes=40e9, bytes_per_cell=11)
+    assert r["bytes"] <= budget and r["vram_bytes"] <= 40e9 and r["n"][3] == 21
+    assert r["cells"] == math.prod(r["n"])
+    finer = r["xy_cm"] - (0.1 if r["xy_cm"] <= 10 else 0.5)
+    n1 = max(2, math.ceil(320.0 / finer - 1e-9) + 1)
+    nh = max(4, round(360.0 / run.suggest_heading_deg(25.0, finer)))
+    assert n1 * n1 * nh * 21 ** 3 * 2 * 3 > budget
+    res = {"xy_cm": 4.0, "heading_deg": 10.0, "v_cm_s": 20.0, "w_rad_s": 1.0}
+    c = run.coarsen_to_fit(320.0, 320.0, 170.0, 8.0, res, 5e9, bytes_per_cell=11)
+    assert c["vram_bytes"] <= 5e9 and c["scale"] > 1
+    k = c["scale"] - 0.05
+    _, cells = run.shape_for(320.0, 320.0, 170.0, 8.0, 4.0 * k, 10.0 * k, 20.0 * k, 1.0 * k)
+    assert cells * 11 > 5e9
