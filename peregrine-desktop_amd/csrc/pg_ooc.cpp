@@ -131,6 +131,15 @@ void build_tiles(const pg_solver* s, Ooc& o, const std::vector<PlanRef>& plans, 
                         lo[a] = std::min(lo[a], l);
                         hi[a] = std::max(hi[a], l + t.tn[a]);
                     }
+                // 16-bit codes with an odd velocity block: an even first
+                // heading and an even heading count keep every host offset,
+                // pitch and row of the window's copy 4-byte aligned. The DMA
+                // engines move only such copies; any other is done by a copy
+                // kernel that takes CUs from the sweep running beside it.
+                if (interior && (o.colv & 1) && !(n3 & 1) && lo[2] <= hi[2]) {
+                    if ((h0 + lo[2]) & 1) --lo[2];
+                    if ((hi[2] - lo[2]) & 1) ++hi[2];
+                }
                 for (int a = 0; a < 2; ++a) {
                     if (full[a] || lo[a] > hi[a]) { lo[a] = 0; hi[a] = g.n[a]; }
                     lo[a] = std::max(lo[a], 0);
