@@ -161,7 +161,10 @@ int pg_solve(pg_solver* s, int32_t max_iters, double tol_sum, float tol_linf,
  * beats the cell's escape time. Jacobi, global-gather kernel, bit-exact with
  * the CPU restatement. Returns the summed and the largest improvement.
  * pg_solve_escape stops as pg_solve does. A 16-bit store must be polished
- * first; not built out of core. */
+ * first. Out of core (solve_escape_ooc!, solver/src/OutOfCore.jl:609-700) the
+ * tiles sweep the same kernel through windows planned for the escape reads
+ * (every state, no probe rejections) on the first escape sweep, bit for bit
+ * equal to the in-HBM pass; the plain scheme only there. */
 int pg_escape_sweep(pg_solver* s, int32_t phase, int32_t rev, double* sum_improve, float* linf);
 int pg_solve_escape(pg_solver* s, int32_t max_iters, double tol_sum, float tol_linf, int32_t* iters,
                     double* last_sum, float* last_linf);
@@ -264,8 +267,8 @@ int pg_solver_ncells(pg_solver* s, int64_t* n);
  * {0, 0, 0} for the largest x-y tile with the whole heading axis whose
  * buffers fit budget_bytes of device memory. PG_ERR_NOMEM if they do not.
  * The solver then takes pg_fill_seed, pg_sweep, pg_solve, pg_download_V,
- * pg_upload_V, pg_last_sweep_ms as usual (at most 128 controls; no policy,
- * escape pass or FP64 tables).
+ * pg_upload_V, pg_escape_sweep, pg_solve_escape, pg_last_sweep_ms as usual
+ * (at most 128 controls; no policy or FP64 tables).
  *
  * vstore = PG_VSTORE_U16_TILE or PG_VSTORE_F16_TILE runs the precision form
  * out of core: the host also holds the Jacobi pair of 16-bit codes, windows

@@ -865,11 +865,11 @@ int pg_solve(pg_solver* s, int32_t max_iters, double tol_sum, float tol_linf, in
 int pg_escape_sweep(pg_solver* s, int32_t phase, int32_t rev, double* sum_improve, float* linf) {
     (void)rev;
     if (!s) return fail(PG_ERR_ARG, "null solver");
-    if (s->ooc) return fail(PG_ERR_UNSUPPORTED, "the escape pass is not built for an out-of-core solver");
     if (!s->seeded) return fail(PG_ERR_STATE, "no value table yet");
     if (s->f16_active) return fail(PG_ERR_STATE, "a 16-bit store must be polished (pg_polish) before the escape pass");
     HIPCHECK(hipSetDevice(s->ctx->device));
     s->K.off = slice_offset(s, phase);
+    if (s->ooc) return pgh::ooc_escape_sweep(s, sum_improve, linf);
     hipStream_t st = s->ctx->stream;
     const int64_t nb = (s->ncell + 255) / 256;
     int rc = ensure_partials(s, nb);

@@ -119,6 +119,8 @@ int ooc_create(pg_solver* s, int64_t budget_bytes, const int32_t tile[3]);
 void ooc_release(pg_solver* s);
 int ooc_fill_seed(pg_solver* s);
 int ooc_sweep(pg_solver* s, double* sum_decrease, float* linf);
+// One escape sweep over the tiles (the phase's slice offset already in s->K).
+int ooc_escape_sweep(pg_solver* s, double* sum_improve, float* linf);
 int ooc_download(pg_solver* s, float* dst, int64_t off, int64_t n);
 int ooc_upload(pg_solver* s, const float* src, int64_t off, int64_t n);
 int ooc_polish_start(pg_solver* s);

@@ -19,6 +19,14 @@ constexpr int kReduceBlocks = 1024;
 hipError_t pgk_launch_escape(const pgk::Consts& K, const float* Vin, float* Vout, const uint8_t* occ,
                              const float* bu, const float* uraw, uint8_t* pol, double* part, unsigned* linf_bits,
                              hipStream_t st, int64_t* nblocks);
+// The same over an out-of-core tile's interior (t0, tn: x, y, heading),
+// reading V from the tile's window (origin wo, extent wl; heading modulo n3)
+// and writing the interior to `out`; a read outside the window raises *miss.
+// Plain scheme only (no production candidates).
+hipError_t pgk_launch_escape_tile(const pgk::Consts& K, const int32_t t0[3], const int32_t tn[3],
+                                  const float* win, const int32_t wo[3], const int32_t wl[3], float* out,
+                                  const uint8_t* occ, const float* bu, double* part, unsigned* linf, unsigned* miss,
+                                  hipStream_t st, int64_t* nblocks);
 hipError_t pgk_launch_reduce(const double* part, int64_t n, double* out, double* mid, hipStream_t st);
 hipError_t pgk_launch_fill(float* V, int64_t n, float v, hipStream_t st);
 hipError_t pgk_launch_seed(float* V, const int64_t* seeds, int64_t n, hipStream_t st);

@@ -34,6 +34,7 @@ namespace pgh {
 struct OocTile {
     int32_t t0[3], tn[3];   // interior origin and extent: x, y, heading
     int32_t wo[3], wl[3];   // window origin (heading modulo n3) and extent
+    int32_t ewo[3], ewl[3]; // the escape pass's window (set on its first sweep)
     pgk::BoxGeom G;         // the interior's boxes
 };
 
@@ -58,6 +59,9 @@ struct Ooc {
     int64_t device_bytes = 0;
     double amplification = 0.0;
     int64_t colv = 0;        // cells per (x, y, heading): the velocity block
+    int64_t budget = 0;      // device bytes allowed
+    int64_t win_floats = 0;  // capacity of each FP32 window buffer, in cells
+    bool escape_ready = false;
 };
 
 namespace {
@@ -69,8 +73,79 @@ struct PlanRef {
     const std::vector<uint8_t>* ext;
 };
 
-// Boxes and window of every tile: the union of its boxes' corner ranges
-// under each plan given, and with `interior` the interior itself.
+// A tile's window: the union of its boxes' corner ranges under each plan
+// given, with `interior` the interior itself, and with `align16` widened so
+// that copies of 16-bit codes stay 4-byte aligned (below).
+void tile_window(const pg_solver* s, const Ooc& o, const OocTile& t, const std::vector<PlanRef>& plans,
+                 bool interior, bool align16, int32_t wo[3], int32_t wl[3]) {
+    const pg_grid& g = s->g;
+    const pgk::BoxGeom& GG = s->G;
+    const pgk::BoxGeom& G = t.G;
+    const int n3 = g.n[2];
+    const int32_t h0 = t.t0[2];
+    // union of the boxes' corner ranges; heading relative to h0
+    int32_t lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
+    bool full[3] = {false, false, false};
+    const int32_t b0 = G.org[0] / G.b[0], b1 = G.org[1] / G.b[1], b2 = G.org[2] / G.b[2];
+    const int64_t per = (int64_t)GG.nb[3] * GG.nb[4] * GG.nb[5];
+    for (int32_t i0 = 0; i0 < G.nb[0]; ++i0)
+        for (int32_t i1 = 0; i1 < G.nb[1]; ++i1)
+            for (int32_t i2 = 0; i2 < G.nb[2]; ++i2) {
+                const int64_t head = (((int64_t)(b0 + i0) * GG.nb[1] + (b1 + i1)) * GG.nb[2] + (b2 + i2));
+                const int32_t sh = (b2 + i2) * G.b[2];   // the box's first heading
+                for (const PlanRef& pr : plans)
+                    for (int64_t v = 0; v < per; ++v) {
+                        const size_t bx = (size_t)(head * per + v) * 6;
+                        for (int a = 0; a < 3; ++a) {
+                            const int e = (*pr.ext)[bx + a];
+                            if (e >= 255) full[a] = true;
+                            int32_t l = (*pr.orig)[bx + a];
+                            if (a == 2) {
+                                // the plan measures a box's heading range from
+                                // the box's own first heading, in (-n3/2, n3/2]
+                                l = (l - sh) % n3;
+                                if (l < 0) l += n3;
+                                if (2 * l > n3) l -= n3;
+                                l += sh - h0;
+                            }
+                            lo[a] = std::min(lo[a], l);
+                            hi[a] = std::max(hi[a], l + e);
+                        }
+                    }
+            }
+    if (interior)
+        for (int a = 0; a < 3; ++a) {
+            const int32_t l = a == 2 ? 0 : t.t0[a];
+            lo[a] = std::min(lo[a], l);
+            hi[a] = std::max(hi[a], l + t.tn[a]);
+        }
+    // 16-bit codes with an odd velocity block: an even first heading and an
+    // even heading count keep every host offset, pitch and row of the
+    // window's copy 4-byte aligned. The DMA engines move only such copies;
+    // any other is done by a copy kernel that takes CUs from the sweep
+    // running beside it.
+    if (align16 && (o.colv & 1) && !(n3 & 1) && lo[2] <= hi[2]) {
+        if ((h0 + lo[2]) & 1) --lo[2];
+        if ((hi[2] - lo[2]) & 1) ++hi[2];
+    }
+    for (int a = 0; a < 2; ++a) {
+        if (full[a] || lo[a] > hi[a]) { lo[a] = 0; hi[a] = g.n[a]; }
+        lo[a] = std::max(lo[a], 0);
+        hi[a] = std::min(hi[a], g.n[a]);
+        wo[a] = lo[a];
+        wl[a] = hi[a] - lo[a];
+    }
+    if (full[2] || lo[2] > hi[2] || hi[2] - lo[2] >= n3) {
+        wo[2] = 0;
+        wl[2] = n3;
+    } else {
+        wo[2] = ((h0 + lo[2]) % n3 + n3) % n3;
+        wl[2] = hi[2] - lo[2];
+    }
+}
+
+// Boxes and window of every tile under the plans given (with `interior` the
+// 16-bit store's windows: they hold the interior too, and are aligned).
 void build_tiles(const pg_solver* s, Ooc& o, const std::vector<PlanRef>& plans, bool interior) {
     const pg_grid& g = s->g;
     const pgk::BoxGeom& GG = s->G;
@@ -95,65 +170,7 @@ void build_tiles(const pg_solver* s, Ooc& o, const std::vector<PlanRef>& plans, 
                     nbox *= G.nb[a];
                 }
                 G.nbox = nbox;
-                // union of the boxes' corner ranges; heading relative to h0
-                int32_t lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
-                bool full[3] = {false, false, false};
-                const int32_t b0 = G.org[0] / G.b[0], b1 = G.org[1] / G.b[1], b2 = G.org[2] / G.b[2];
-                for (int32_t i0 = 0; i0 < G.nb[0]; ++i0)
-                    for (int32_t i1 = 0; i1 < G.nb[1]; ++i1)
-                        for (int32_t i2 = 0; i2 < G.nb[2]; ++i2) {
-                            const int64_t head = (((int64_t)(b0 + i0) * GG.nb[1] + (b1 + i1)) * GG.nb[2] + (b2 + i2));
-                            const int32_t sh = (b2 + i2) * G.b[2];   // the box's first heading
-                            const int64_t per = (int64_t)GG.nb[3] * GG.nb[4] * GG.nb[5];
-                            for (const PlanRef& pr : plans)
-                            for (int64_t v = 0; v < per; ++v) {
-                                const size_t bx = (size_t)(head * per + v) * 6;
-                                for (int a = 0; a < 3; ++a) {
-                                    const int e = (*pr.ext)[bx + a];
-                                    if (e >= 255) full[a] = true;
-                                    int32_t l = (*pr.orig)[bx + a];
-                                    if (a == 2) {
-                                        // the plan measures a box's heading range from
-                                        // the box's own first heading, in (-n3/2, n3/2]
-                                        l = (l - sh) % n3;
-                                        if (l < 0) l += n3;
-                                        if (2 * l > n3) l -= n3;
-                                        l += sh - h0;
-                                    }
-                                    lo[a] = std::min(lo[a], l);
-                                    hi[a] = std::max(hi[a], l + e);
-                                }
-                            }
-                        }
-                if (interior)
-                    for (int a = 0; a < 3; ++a) {
-                        const int32_t l = a == 2 ? 0 : t.t0[a];
-                        lo[a] = std::min(lo[a], l);
-                        hi[a] = std::max(hi[a], l + t.tn[a]);
-                    }
-                // 16-bit codes with an odd velocity block: an even first
-                // heading and an even heading count keep every host offset,
-                // pitch and row of the window's copy 4-byte aligned. The DMA
-                // engines move only such copies; any other is done by a copy
-                // kernel that takes CUs from the sweep running beside it.
-                if (interior && (o.colv & 1) && !(n3 & 1) && lo[2] <= hi[2]) {
-                    if ((h0 + lo[2]) & 1) --lo[2];
-                    if ((hi[2] - lo[2]) & 1) ++hi[2];
-                }
-                for (int a = 0; a < 2; ++a) {
-                    if (full[a] || lo[a] > hi[a]) { lo[a] = 0; hi[a] = g.n[a]; }
-                    lo[a] = std::max(lo[a], 0);
-                    hi[a] = std::min(hi[a], g.n[a]);
-                    t.wo[a] = lo[a];
-                    t.wl[a] = hi[a] - lo[a];
-                }
-                if (full[2] || lo[2] > hi[2] || hi[2] - lo[2] >= n3) {
-                    t.wo[2] = 0;
-                    t.wl[2] = n3;
-                } else {
-                    t.wo[2] = ((h0 + lo[2]) % n3 + n3) % n3;
-                    t.wl[2] = hi[2] - lo[2];
-                }
+                tile_window(s, o, t, plans, interior, interior, t.wo, t.wl);
                 o.tiles.push_back(t);
             }
 }
@@ -257,6 +274,7 @@ int ooc_create(pg_solver* s, int64_t budget_bytes, const int32_t tile_req[3]) {
     Ooc* o = new Ooc;
     s->ooc = o;
     o->colv = (int64_t)g.n[3] * g.n[4] * g.n[5];
+    o->budget = budget_bytes;
     const int64_t fixed = fixed_device_bytes(s);
     // partial sums: one per workgroup of the largest tile, one total per tile
     auto need = [&](Ooc& oc) {
@@ -331,6 +349,7 @@ int ooc_create(pg_solver* s, int64_t budget_bytes, const int32_t tile_req[3]) {
     for (int j = 0; j < 2; ++j) {
         chk(hipHostMalloc((void**)&o->hV[j], (size_t)s->ncell * sizeof(float), hipHostMallocDefault));
         chk(hipMalloc(&o->dwin[j], (size_t)win * sizeof(float)));
+        o->win_floats = win;
         chk(hipMalloc(&o->dout[j], (size_t)in * sizeof(float)));
         chk(hipEventCreateWithFlags(&o->ev_h2d[j], hipEventDisableTiming));
         chk(hipEventCreateWithFlags(&o->ev_k[j], hipEventDisableTiming));
@@ -477,6 +496,155 @@ int ooc_sweep16(pg_solver* s, double* sum_decrease, float* linf) {
 }
 
 }  // namespace
+
+namespace {
+
+// The escape pass's windows, on its first sweep. The escape backups read
+// from blocked cells too and without the probes' rejections (a blocked cell
+// moves through obstacle space), so the windows come from a plan over an
+// empty occupancy -- every state, every successor: a superset of what the
+// escape pass reads -- joined with the interior (each cell reads its own
+// value). The FP32 window buffers grow if a window needs more room, within
+// the device budget.
+int ooc_escape_prepare(pg_solver* s) {
+    Ooc& o = *s->ooc;
+    if (o.escape_ready) return PG_OK;
+    const pg_grid& g = s->g;
+    hipStream_t st = s->ctx->stream;
+    const int64_t nbox = s->G.nbox;
+    const size_t nocc = (size_t)g.n[0] * g.n[1] * g.n[2];
+    uint8_t* zocc = nullptr;
+    int16_t* d_orig = nullptr;
+    uint8_t* d_ext = nullptr;
+    std::vector<int16_t> orig((size_t)nbox * 6);
+    std::vector<uint8_t> ext((size_t)nbox * 6);
+    hipError_t e = hipMalloc(&zocc, nocc);
+    if (e == hipSuccess) e = hipMalloc(&d_orig, (size_t)nbox * 6 * sizeof(int16_t));
+    if (e == hipSuccess) e = hipMalloc(&d_ext, (size_t)nbox * 6);
+    if (e == hipSuccess) e = hipMemsetAsync(zocc, 0, nocc, st);
+    if (e == hipSuccess)
+        e = pgk_launch_plan(s->K, nullptr, s->G, zocc, s->d_bu, d_orig, d_ext, nullptr, s->diag ? 1 : 0, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(orig.data(), d_orig, orig.size() * sizeof(int16_t), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(ext.data(), d_ext, ext.size(), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(zocc);
+    (void)hipFree(d_orig);
+    (void)hipFree(d_ext);
+    if (e != hipSuccess)
+        return fail(e == hipErrorOutOfMemory ? PG_ERR_NOMEM : PG_ERR_HIP, "escape window plan: %s",
+                    hipGetErrorString(e));
+    const std::vector<PlanRef> plans{{&orig, &ext}};
+    int64_t win = 0, blocks = 0;
+    for (OocTile& t : o.tiles) {
+        tile_window(s, o, t, plans, true, false, t.ewo, t.ewl);
+        win = std::max(win, (int64_t)t.ewl[0] * t.ewl[1] * t.ewl[2] * o.colv);
+        blocks = std::max(blocks, ceil_div(interior_cells(t, o.colv), 256));
+    }
+    int rc = ensure_partials(s, blocks);
+    if (rc) return rc;
+    if (win > o.win_floats) {
+        const int64_t grown = o.device_bytes + 2 * (win - o.win_floats) * (int64_t)sizeof(float);
+        if (grown > o.budget)
+            return fail(PG_ERR_NOMEM, "the escape pass's tile windows need %lld device bytes, over the budget of %lld",
+                        (long long)grown, (long long)o.budget);
+        for (int j = 0; j < 2; ++j) {
+            (void)hipFree(o.dwin[j]);
+            o.dwin[j] = nullptr;
+        }
+        hipError_t a = hipMalloc(&o.dwin[0], (size_t)win * sizeof(float));
+        if (a == hipSuccess) a = hipMalloc(&o.dwin[1], (size_t)win * sizeof(float));
+        if (a != hipSuccess) {
+            // leave the solver whole: value sweeps need their own windows back
+            (void)hipFree(o.dwin[0]);
+            (void)hipFree(o.dwin[1]);
+            o.dwin[0] = o.dwin[1] = nullptr;
+            hipError_t b = hipMalloc(&o.dwin[0], (size_t)o.win_floats * sizeof(float));
+            if (b == hipSuccess) b = hipMalloc(&o.dwin[1], (size_t)o.win_floats * sizeof(float));
+            if (b != hipSuccess) return fail(PG_ERR_NOMEM, "out-of-core window buffers lost: %s", hipGetErrorString(b));
+            return fail(a == hipErrorOutOfMemory ? PG_ERR_NOMEM : PG_ERR_HIP, "escape window buffers: %s",
+                        hipGetErrorString(a));
+        }
+        o.device_bytes = grown;
+        o.win_floats = win;
+    }
+    o.escape_ready = true;
+    return PG_OK;
+}
+
+// The escape window of tile t in, host FP32 table -> buffer (the value
+// pass's copy with the escape window's origin and extent).
+hipError_t copy_escape_window_in(const pg_solver* s, const Ooc& o, const OocTile& t, const float* hV, float* dw,
+                                 hipStream_t st) {
+    OocTile e = t;
+    for (int a = 0; a < 3; ++a) {
+        e.wo[a] = t.ewo[a];
+        e.wl[a] = t.ewl[a];
+    }
+    return copy_window_in<float>(s, o, e, hV, dw, st);
+}
+
+}  // namespace
+
+// One escape sweep over the tiles (Jacobi, as in HBM): the pipeline of the
+// FP32 value sweep with the escape kernel.
+int ooc_escape_sweep(pg_solver* s, double* sum_improve, float* linf) {
+    if (s->f16_active) return fail(PG_ERR_STATE, "a 16-bit store must be polished (pg_polish) before the escape pass");
+    if (s->K.rounds > 0 || s->K.ntau > 1)
+        return fail(PG_ERR_UNSUPPORTED, "the out-of-core escape pass runs the plain scheme only");
+    int rc = ooc_escape_prepare(s);
+    if (rc) return rc;
+    Ooc& o = *s->ooc;
+    hipStream_t st = s->ctx->stream;
+    const float* hin = o.hV[o.cur];
+    float* hout = o.hV[1 - o.cur];
+    const int64_t nt = (int64_t)o.tiles.size();
+    HIPCHECK(hipMemsetAsync(s->d_linf, 0, sizeof(unsigned), st));
+    HIPCHECK(hipMemsetAsync(o.d_miss, 0, sizeof(unsigned), st));
+    HIPCHECK(hipEventRecord(s->ev0, st));
+    HIPCHECK(hipStreamWaitEvent(o.h2d, s->ev0, 0));
+    HIPCHECK(copy_escape_window_in(s, o, o.tiles[0], hin, o.dwin[0], o.h2d));
+    HIPCHECK(hipEventRecord(o.ev_h2d[0], o.h2d));
+    for (int64_t k = 0; k < nt; ++k) {
+        const int j = (int)(k & 1);
+        const OocTile& t = o.tiles[(size_t)k];
+        HIPCHECK(hipStreamWaitEvent(st, o.ev_h2d[j], 0));
+        if (k >= 2) HIPCHECK(hipStreamWaitEvent(st, o.ev_d2h[j], 0));
+        int64_t nblk = 0;
+        HIPCHECK(pgk_launch_escape_tile(s->K, t.t0, t.tn, o.dwin[j], t.ewo, t.ewl, o.dout[j], s->d_occ, s->d_bu,
+                                        s->d_part, s->d_linf, o.d_miss, st, &nblk));
+        HIPCHECK(pgk_launch_reduce(s->d_part, nblk, o.d_tsum + k, nullptr, st));
+        HIPCHECK(hipEventRecord(o.ev_k[j], st));
+        if (k + 1 < nt) {
+            const int jn = 1 - j;
+            if (k >= 1) HIPCHECK(hipStreamWaitEvent(o.h2d, o.ev_k[jn], 0));
+            HIPCHECK(copy_escape_window_in(s, o, o.tiles[(size_t)k + 1], hin, o.dwin[jn], o.h2d));
+            HIPCHECK(hipEventRecord(o.ev_h2d[jn], o.h2d));
+        }
+        HIPCHECK(hipStreamWaitEvent(o.d2h, o.ev_k[j], 0));
+        HIPCHECK(copy_interior_out(s, o, t, o.dout[j], hout, o.d2h));
+        HIPCHECK(hipEventRecord(o.ev_d2h[j], o.d2h));
+    }
+    std::vector<double> tsum((size_t)nt);
+    unsigned hl = 0, miss = 0;
+    HIPCHECK(hipMemcpyAsync(tsum.data(), o.d_tsum, (size_t)nt * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(&hl, s->d_linf, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(&miss, o.d_miss, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    HIPCHECK(hipStreamSynchronize(o.d2h));
+    HIPCHECK(hipEventRecord(s->ev1, st));
+    HIPCHECK(hipEventSynchronize(s->ev1));
+    HIPCHECK(hipEventElapsedTime(&s->last_ms, s->ev0, s->ev1));
+    if (miss) return fail(PG_ERR_STATE, "an escape backup read a cell outside its tile's window");
+    o.cur = 1 - o.cur;
+    double d = 0.0;
+    for (double v : tsum) d += v;
+    float lf;
+    std::memcpy(&lf, &hl, 4);
+    if (sum_improve) *sum_improve = d;
+    if (linf) *linf = lf;
+    return PG_OK;
+}
 
 int ooc_fill_seed(pg_solver* s) {
     Ooc& o = *s->ooc;

@@ -25,8 +25,48 @@ namespace pgk {
 // 836-842): a corner whose (x, y, h) cell is an obstacle reads cap unless
 // `ignore` (the updating cell is itself blocked); otherwise a negative value
 // reads as its escape time, a real route (0 <= v < cap) as 0, the rest as cap.
-template <bool ESC>
-__device__ __forceinline__ float interp_global(const Consts& K, const float* __restrict__ V,
+// Where a backup reads V: the whole table (VGlobal), or an out-of-core
+// tile's window (VWindow). off() is a corner index's offset along axis k,
+// load() the value at a summed offset.
+struct VGlobal {
+    const float* V;
+    __device__ __forceinline__ int64_t off(const Consts& K, int k, int j) const { return (int64_t)j * K.stride[k]; }
+    __device__ __forceinline__ float load(const Consts&, int64_t a) const { return V[a]; }
+};
+
+// A window: cells x in [o0, o0 + L0), y in [o1, o1 + L1), heading in
+// [o2, o2 + L2) modulo n3, every velocity cell; omega fastest. An index
+// outside it has an offset so negative that every sum holding it stays
+// negative; load() reads only offsets inside the window, and any other
+// returns cap and raises *miss (the window was built to hold every read).
+constexpr int64_t kOutsideWindow = -((int64_t)1 << 60);
+
+struct VWindow {
+    const float* __restrict__ W;
+    int32_t o[3], L[3];
+    int64_t ws[6];
+    int64_t size;
+    unsigned* miss;
+    __device__ __forceinline__ int64_t off(const Consts& K, int k, int j) const {
+        if (k < 3) {
+            int l = j - o[k];
+            if (k == 2 && l < 0) l += K.n[2];
+            if ((unsigned)l >= (unsigned)L[k]) return kOutsideWindow;
+            return (int64_t)l * ws[k];
+        }
+        return (int64_t)j * ws[k];
+    }
+    __device__ __forceinline__ float load(const Consts& K, int64_t a) const {
+        if ((uint64_t)a >= (uint64_t)size) {
+            atomicOr(miss, 1u);
+            return K.cap;
+        }
+        return W[a];
+    }
+};
+
+template <bool ESC, class VR>
+__device__ __forceinline__ float interp_global(const Consts& K, const VR& V,
                                                const uint8_t* __restrict__ occ, bool ignore,
                                                float x, float y, float h, float vx, float vy,
                                                float w) {
@@ -66,8 +106,8 @@ __device__ __forceinline__ float interp_global(const Consts& K, const float* __r
             j0 = i;
             j1 = (i + 1 < K.n[k] - 1) ? i + 1 : K.n[k] - 1;
         }
-        o0[k] = (int64_t)j0 * K.stride[k];
-        o1[k] = (int64_t)j1 * K.stride[k];
+        o0[k] = V.off(K, k, j0);
+        o1[k] = V.off(K, k, j1);
         if (k < 3) { jc[k][0] = j0; jc[k][1] = j1; }
     }
     if (K.nearest) {
@@ -81,7 +121,7 @@ __device__ __forceinline__ float interp_global(const Consts& K, const float* __r
             a += up ? o1[k] : o0[k];
             if (k < 3) c |= (up ? 1 : 0) << k;
         }
-        float v = V[a];
+        float v = V.load(K, a);
         if (ESC) {
             const bool wall = !ignore && occ[((int64_t)jc[0][c & 1] * K.n[1] + jc[1][(c >> 1) & 1]) * K.n[2] +
                                              jc[2][(c >> 2) & 1]] != 0;
@@ -99,7 +139,7 @@ __device__ __forceinline__ float interp_global(const Consts& K, const float* __r
                 a += up ? o1[k] : o0[k];
                 if (k < 3) c |= (up ? 1 : 0) << k;
             }
-            float v = V[a];
+            float v = V.load(K, a);
             if (ESC) {
                 const bool wall = !ignore && occ[((int64_t)jc[0][c & 1] * K.n[1] + jc[1][(c >> 1) & 1]) * K.n[2] +
                                                  jc[2][(c >> 2) & 1]] != 0;
@@ -135,7 +175,7 @@ __device__ __forceinline__ float interp_global(const Consts& K, const float* __r
     }
     float vals[64];
 #pragma unroll
-    for (int d = 0; d < 64; ++d) vals[d] = V[a5[d & 31] + ((d & 32) ? o1[5] : o0[5])];
+    for (int d = 0; d < 64; ++d) vals[d] = V.load(K, a5[d & 31] + ((d & 32) ? o1[5] : o0[5]));
     if (ESC) {
         bool wall[8];
 #pragma unroll
@@ -170,8 +210,8 @@ struct OccGlobal {
 };
 
 // lookahead (PeregrineSolver.jl:1100-1153) for one control.
-template <bool ESC>
-__device__ __forceinline__ float lookahead_global(const Consts& K, const float* __restrict__ V,
+template <bool ESC, class VR>
+__device__ __forceinline__ float lookahead_global(const Consts& K, const VR& V,
                                                   const OccGlobal& occ, float tau, float hs, int nsub,
                                                   float x, float y,
                                                   float h, float vx, float vy, float w,
@@ -338,7 +378,8 @@ __global__ __launch_bounds__(256) void sweep_global_kernel(
                     bx = lx; by = ly; bw = lw;
                     ladder = true;
                 }
-                const float c = lookahead_global<ESC>(K, Vin, og, t, ths, tn, x, y, h, vx, vy, w, bx, by, bw);
+                const float c = lookahead_global<ESC>(K, VGlobal{Vin}, og, t, ths, tn, x, y, h, vx, vy, w, bx, by,
+                                                      bw);
                 if (c < best) {
                     best = c;
                     if (!ladder) { bk = kk; b1 = c1; b2 = c2; b3 = c3; }
@@ -364,6 +405,78 @@ __global__ __launch_bounds__(256) void sweep_global_kernel(
         if (WRITE_POL) pol[idx] = (uint8_t)bk;
     }
     if (WRITE_V) block_reduce_sum_max((double)dec, dec, part_sum, linf_bits);
+}
+
+// The escape pass over an out-of-core tile's interior (cells x, y, heading
+// from t0 over tn, every velocity cell; omega fastest), reading V through the
+// tile's window: sweep_global_kernel's ESC form without the production
+// candidates (coast, then the lattice or this phase's slice), the same
+// arithmetic in the same order, so a tiled escape sweep equals the in-HBM
+// one bit for bit.
+struct TileCells {
+    int32_t t0[3], tn[3];
+    int64_t ncell;
+};
+
+__global__ __launch_bounds__(256) void escape_tile_kernel(VWindow Vw, float* __restrict__ out, TileCells T,
+                                                          const uint8_t* __restrict__ occ,
+                                                          const float* __restrict__ bu, Consts K,
+                                                          double* __restrict__ part_sum,
+                                                          unsigned* __restrict__ linf_bits) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    float dec = 0.0f;
+    if (idx < T.ncell) {
+        int64_t r = idx;
+        int32_t i[6];
+#pragma unroll
+        for (int k = 5; k >= 3; --k) {
+            i[k] = (int32_t)(r % K.n[k]);
+            r /= K.n[k];
+        }
+        i[2] = T.t0[2] + (int32_t)(r % T.tn[2]);
+        r /= T.tn[2];
+        i[1] = T.t0[1] + (int32_t)(r % T.tn[1]);
+        i[0] = T.t0[0] + (int32_t)(r / T.tn[1]);
+        const bool blocked = occ[((int64_t)i[0] * K.n[1] + i[1]) * K.n[2] + i[2]] != 0;
+        int64_t self = 0;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) self += Vw.off(K, k, i[k]);
+        const float old = Vw.load(K, self);
+        const bool terminal = old >= 0.0f && old < K.cap;
+        float best = K.cap;
+        if (!terminal) {
+            const float x = K.lo[0] + (float)i[0] * K.step[0];
+            const float y = K.lo[1] + (float)i[1] * K.step[1];
+            const float h = K.lo[2] + (float)i[2] * K.step[2];
+            const float vfx = K.lo[3] + (float)i[3] * K.step[3];
+            const float vfy = K.lo[4] + (float)i[4] * K.step[4];
+            const float w = K.lo[5] + (float)i[5] * K.step[5];
+            float sh0, ch0;
+            sincos_ref(h, sh0, ch0);
+            const float vx = ch0 * vfx + sh0 * vfy;
+            const float vy = -sh0 * vfx + ch0 * vfy;
+            const OccGlobal og{occ, K.n[1], K.n[2], blocked};
+            float tau = K.tau, hs = K.hs;
+            int nsub = K.nsub;
+            if (K.cfl) {
+                tau = cfl_tau_dev(K, vfx, vfy, w);
+                nsub = substeps_dev(K, tau);
+                hs = tau / (float)nsub;
+            }
+            // slot 0 coast, slots 1 .. span the lattice (or the slice)
+            for (int slot = 0; slot <= K.span; ++slot) {
+                const int kk = slot == 0 ? 0 : slice_control(K, slot);
+                const float c = lookahead_global<true>(K, Vw, og, tau, hs, nsub, x, y, h, vx, vy, w, bu[3 * kk],
+                                                       bu[3 * kk + 1], bu[3 * kk + 2]);
+                if (c < best) best = c;
+            }
+        }
+        float o = old;
+        const float e_old = old < 0.0f ? -old : K.cap;
+        if (!terminal && best < e_old) { o = -best; dec = e_old - best; }
+        out[idx] = o;
+    }
+    block_reduce_sum_max((double)dec, dec, part_sum, linf_bits);
 }
 
 // Deterministic sum of the per-block partials (a fixed order for each n):
@@ -462,6 +575,38 @@ hipError_t pgk_launch_escape(const pgk::Consts& K, const float* Vin, float* Vout
         launch_global<true, true, true>(K, nb, st, Vin, Vout, occ, bu, uraw, pol, nullptr, part, linf_bits);
     else
         launch_global<true, false, true>(K, nb, st, Vin, Vout, occ, bu, uraw, pol, nullptr, part, linf_bits);
+    return hipGetLastError();
+}
+
+hipError_t pgk_launch_escape_tile(const pgk::Consts& K, const int32_t t0[3], const int32_t tn[3],
+                                  const float* win, const int32_t wo[3], const int32_t wl[3], float* out,
+                                  const uint8_t* occ, const float* bu, double* part, unsigned* linf, unsigned* miss,
+                                  hipStream_t st, int64_t* nblocks) {
+    const int64_t colv = (int64_t)K.n[3] * K.n[4] * K.n[5];
+    pgk::TileCells T;
+    pgk::VWindow Vw;
+    Vw.W = win;
+    Vw.miss = miss;
+    for (int a = 0; a < 3; ++a) {
+        T.t0[a] = t0[a];
+        T.tn[a] = tn[a];
+        Vw.o[a] = wo[a];
+        Vw.L[a] = wl[a];
+    }
+    T.ncell = (int64_t)tn[0] * tn[1] * tn[2] * colv;
+    Vw.ws[5] = 1;
+    Vw.ws[4] = K.n[5];
+    Vw.ws[3] = (int64_t)K.n[4] * K.n[5];
+    Vw.ws[2] = colv;
+    Vw.ws[1] = colv * wl[2];
+    Vw.ws[0] = Vw.ws[1] * wl[1];
+    Vw.size = Vw.ws[0] * wl[0];
+    // a dispatch counts its work-items in 32 bits
+    if (T.ncell >= ((int64_t)1 << 31)) return hipErrorInvalidValue;
+    const int64_t nb = (T.ncell + 255) / 256;
+    *nblocks = nb;
+    hipLaunchKernelGGL(pgk::escape_tile_kernel, dim3((unsigned)nb), dim3(256), 0, st, Vw, out, T, occ, bu, K, part,
+                       linf);
     return hipGetLastError();
 }
 

@@ -14,8 +14,8 @@ Run.jl:1434-2133) with its PROGRESS stream, MANIFEST.JSON and MODEL.JSON
 The value solve runs on the device through libperegrine_hip; there is no
 CPU backend (`backend: "cpu"` is refused). Scheme options the tiled
 (out-of-core) driver does not implement (`tau_levels` > 1,
-`refine_rounds` > 0, `warm_start`, the escape pass) are refused there with
-the setting that avoids them, never silently dropped.
+`refine_rounds` > 0, `warm_start`) are refused there with the setting that
+avoids them, never silently dropped; the escape pass runs there too.
 """
 from __future__ import annotations
 
@@ -937,7 +937,7 @@ def run_solve(cfg) -> dict:
                          "whole table would come out unreachable:\n" + "\n".join(lines))
     fits, _, _ = solver.gpu_fits(g)
     ooc = not fits
-    bad = unsupported_options(sp, ooc) + (["the escape pass (set escape: false)"] if ooc and do_escape else [])
+    bad = unsupported_options(sp, ooc)
     if bad:
         raise SolveError("the grid does not fit the device whole, and the tiled driver does not run "
                          + ", ".join(bad))

@@ -113,3 +113,33 @@ def test_precision_form_out_of_core(pg, store):
         oa = ooc.polish(2)
         assert np.array_equal(bits(ref.download()), bits(ooc.download()))
         assert ra[1] == oa[1]
+
+
+@pytest.mark.parametrize("shape,tile", [((24, 24, 16, 9, 9, 11), (8, 8, 4)), (None, (5, 7, 3))])
+def test_escape_pass_out_of_core(pg, shape, tile):
+    """The escape pass out of core (windows planned for the escape reads:
+    blocked cells and probe-rejected successors included) against the
+    in-HBM escape pass: 4 value sweeps, then 4 escape sweeps, each bit for
+    bit, on a reduced problem-2 grid with wrapping heading windows and on
+    problem 1 with partial edge tiles."""
+    prob = pg.configs.problem(2, shape=shape) if shape else pg.configs.problem(1)
+    with pg.HipSolver(prob.grid, prob.model, prob.params, prob.ctl, prob.occ) as ref, \
+         pg.HipSolver(prob.grid, prob.model, prob.params, prob.ctl, prob.occ,
+                      ooc_budget=8 << 30, ooc_tile=tile) as ooc:
+        ref.fill_seed(prob.seeds)
+        ooc.fill_seed(prob.seeds)
+        print(f"escape: grid {prob.grid.n}, tiles {ooc.ooc_info()[:2]}", flush=True)
+        for it in range(4):
+            ref.sweep(it, it % 2 == 1)
+            ooc.sweep(it, it % 2 == 1)
+        assert np.array_equal(bits(ref.download()), bits(ooc.download()))
+        for it in range(4):
+            r = ref.escape_sweep(it, it % 2 == 1)
+            o = ooc.escape_sweep(it, it % 2 == 1)
+            a, b = ref.download(), ooc.download()
+            bad = np.nonzero(bits(a) != bits(b))[0]
+            assert bad.size == 0, f"escape sweep {it}: {bad.size} cells differ, first {bad[:5].tolist()}"
+            assert o.linf == r.linf
+            assert o.sum_decrease == pytest.approx(r.sum_decrease, rel=1e-9, abs=1e-6)
+        V = ooc.download()
+        assert (V < 0).sum() > 0
