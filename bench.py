@@ -300,10 +300,31 @@ def run_ooc(pg, prob, args) -> dict:
             del a, b
             equal.append(same and rr.linf == r.linf)
             note(f"sweep {i}: tables {'bit-identical' if equal[-1] else 'DIFFER'}")
+    esc_ms, esc_equal = [], []
+    if args.ooc_escape > 0:
+        if args.ooc_store != "f32":
+            s.polish(0)
+            if ref is not None:
+                ref.polish(0)
+        for i in range(args.ooc_escape):
+            t = time.perf_counter()
+            r = s.escape_sweep(i, i % 2 == 1)
+            esc_ms.append((time.perf_counter() - t) * 1e3)
+            print(f"# ooc escape sweep {i}: {esc_ms[-1]:.0f} ms, linf {r.linf:.4f}", file=sys.stderr, flush=True)
+            if ref is not None:
+                rr = ref.escape_sweep(i, i % 2 == 1)
+                note(f"in-HBM escape sweep {i}: {rr.kernel_ms:.0f} ms")
+                same = bool(np.array_equal(s.download().view(np.uint32), ref.download().view(np.uint32)))
+                esc_equal.append(same and rr.linf == r.linf)
+                note(f"escape sweep {i}: tables {'bit-identical' if esc_equal[-1] else 'DIFFER'}")
     s.close()
     if ref is not None:
         ref.close()
-    return {"metric": "out-of-core tiled sweep", "store": args.ooc_store, "value": g.ncells / (float(np.mean(ms)) / 1e3),
+    extra = {}
+    if esc_ms:
+        # the first escape sweep also plans the escape windows
+        extra["escape"] = {"ms_per_step": esc_ms, "bit_identical_to_in_hbm": (all(esc_equal) if esc_equal else None)}
+    return {**extra, "metric": "out-of-core tiled sweep", "store": args.ooc_store, "value": g.ncells / (float(np.mean(ms)) / 1e3),
             "unit": "state-updates/s", "ms_per_step": float(np.mean(ms)), "steps": steps,
             "config": {"workload": f"problem {args.config}", "grid": list(g.n), "states": g.ncells,
                        "controls": prob.nctl, "budget_bytes": int(args.ooc_budget), "tile": list(info[0]),
@@ -373,6 +394,8 @@ def main() -> int:
     ap.add_argument("--ooc-store", choices=["f32", "u16"], default="f32",
                     help="the tiled sweep's table: FP32, or the precision form's 16-bit codes")
     ap.add_argument("--no-ooc-compare", action="store_true")
+    ap.add_argument("--ooc-escape", type=int, default=0,
+                    help="escape sweeps after the tiled value sweeps (the first plans the escape windows)")
     ap.add_argument("--f64-out", default=None)
     ap.add_argument("--f64-sample", default=None)
     ap.add_argument("--no-cpu", action="store_true")
