@@ -339,9 +339,9 @@ int ooc_create(pg_solver* s, int64_t budget_bytes, const int32_t tile_req[3]) {
         in = std::max(in, interior_cells(t, o->colv));
         loaded += window_cells(t, o->colv);
     }
-    // the 16-bit tile kernel indexes windows and interiors in 32 bits
-    if (f16 && win >= ((int64_t)1 << 31))
-        return fail(PG_ERR_UNSUPPORTED, "a tile window of %lld cells: the 16-bit store's tiles hold fewer than 2^31 "
+    // the tile kernels index windows and interiors in 32 bits
+    if (win >= ((int64_t)1 << 31))
+        return fail(PG_ERR_UNSUPPORTED, "a tile window of %lld cells: tiles hold fewer than 2^31 "
                     "(give a smaller tile)", (long long)win);
     o->amplification = (double)loaded / (double)s->ncell;
     hipError_t e = hipSuccess;

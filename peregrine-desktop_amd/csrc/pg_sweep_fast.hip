@@ -117,7 +117,11 @@ struct VReadTile {
         int l2 = gi[2] - o[2];
         l2 = l2 < 0 ? l2 + K.n[2] : l2;
         const int l0 = gi[0] - o[0], l1 = gi[1] - o[1];
-        fl = l0 * ws[0] + l1 * ws[1] + l2 * ws[2] + gi[3] * ws[3] + gi[4] * ws[4] + gi[5];
+        // unsigned arithmetic: an index outside the window may wrap, and is
+        // never used
+        fl = (int32_t)((unsigned)l0 * (unsigned)ws[0] + (unsigned)l1 * (unsigned)ws[1] +
+                       (unsigned)l2 * (unsigned)ws[2] + (unsigned)gi[3] * (unsigned)ws[3] +
+                       (unsigned)gi[4] * (unsigned)ws[4] + (unsigned)gi[5]);
         return ((unsigned)l0 < (unsigned)L[0]) & ((unsigned)l1 < (unsigned)L[1]) & ((unsigned)l2 < (unsigned)L[2]);
     }
     __device__ __forceinline__ float stage(const Consts& K, const QuantGeom& Q, const int* gi) const {

@@ -277,22 +277,25 @@ struct DevDst {
 struct TileSrc {
     const float* __restrict__ W;
     int32_t o[3], L[3];
-    int64_t ws[6];
+    int32_t ws[6];
     unsigned* miss;
-    __device__ __forceinline__ bool local(const Consts& K, const int gi[6], int64_t& fl) const {
+    __device__ __forceinline__ bool local(const Consts& K, const int gi[6], int32_t& fl) const {
         int l2 = gi[2] - o[2];
         l2 = l2 < 0 ? l2 + K.n[2] : l2;
         const int l0 = gi[0] - o[0], l1 = gi[1] - o[1];
-        fl = (int64_t)l0 * ws[0] + (int64_t)l1 * ws[1] + (int64_t)l2 * ws[2] + (int64_t)gi[3] * ws[3] +
-             (int64_t)gi[4] * ws[4] + gi[5];
+        // unsigned arithmetic: an index outside the window may wrap, and is
+        // never used
+        fl = (int32_t)((unsigned)l0 * (unsigned)ws[0] + (unsigned)l1 * (unsigned)ws[1] +
+                       (unsigned)l2 * (unsigned)ws[2] + (unsigned)gi[3] * (unsigned)ws[3] +
+                       (unsigned)gi[4] * (unsigned)ws[4] + (unsigned)gi[5]);
         return ((unsigned)l0 < (unsigned)L[0]) & ((unsigned)l1 < (unsigned)L[1]) & ((unsigned)l2 < (unsigned)L[2]);
     }
     __device__ __forceinline__ float stage(const Consts& K, const int gi[6]) const {
-        int64_t fl;
+        int32_t fl;
         return local(K, gi, fl) ? W[fl] : K.cap;
     }
     __device__ __forceinline__ float read(const Consts& K, const int gi[6]) const {
-        int64_t fl;
+        int32_t fl;
         if (local(K, gi, fl)) return W[fl];
         atomicOr(miss, 1u);
         return K.cap;
@@ -305,10 +308,10 @@ struct TileDst {
     float* __restrict__ V;
     uint8_t* __restrict__ pol;
     int32_t t0[3];
-    int64_t os[6];
+    int32_t os[6];
     __device__ __forceinline__ void put(const Consts& K, const int i[6], float v, int k) const {
-        const int64_t fl = (int64_t)(i[0] - t0[0]) * os[0] + (int64_t)(i[1] - t0[1]) * os[1] +
-                           (int64_t)(i[2] - t0[2]) * os[2] + (int64_t)i[3] * os[3] + (int64_t)i[4] * os[4] + i[5];
+        const int32_t fl = (i[0] - t0[0]) * os[0] + (i[1] - t0[1]) * os[1] + (i[2] - t0[2]) * os[2] +
+                           i[3] * os[3] + i[4] * os[4] + i[5];
         if (V) V[fl] = v;
         if (pol) pol[fl] = (uint8_t)k;
     }
@@ -636,15 +639,20 @@ hipError_t pgk_launch_sweep_tile(int shape, int diag, const pgk::Consts& K, cons
         dst.t0[a] = G.org[a];
     }
     // window strides: L0 x L1 x L2 x n4 x n5 x n6; interior: rn0 x rn1 x rn2 x ...
+    // window and interior are indexed in 32 bits
+    const int64_t colv = (int64_t)K.n[3] * K.n[4] * K.n[5];
+    if ((int64_t)wl[0] * wl[1] * wl[2] * colv >= ((int64_t)1 << 31) ||
+        (int64_t)G.rn[0] * G.rn[1] * G.rn[2] * colv >= ((int64_t)1 << 31))
+        return hipErrorInvalidValue;
     src.ws[5] = 1;
     src.ws[4] = K.n[5];
-    src.ws[3] = (int64_t)K.n[4] * K.n[5];
+    src.ws[3] = K.n[4] * K.n[5];
     src.ws[2] = src.ws[3] * K.n[3];
     src.ws[1] = src.ws[2] * wl[2];
     src.ws[0] = src.ws[1] * wl[1];
     dst.os[5] = 1;
     dst.os[4] = K.n[5];
-    dst.os[3] = (int64_t)K.n[4] * K.n[5];
+    dst.os[3] = K.n[4] * K.n[5];
     dst.os[2] = dst.os[3] * K.n[3];
     dst.os[1] = dst.os[2] * G.rn[2];
     dst.os[0] = dst.os[1] * G.rn[1];
