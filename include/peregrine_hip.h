@@ -267,8 +267,15 @@ int pg_solver_ncells(pg_solver* s, int64_t* n);
  * {0, 0, 0} for the largest x-y tile with the whole heading axis whose
  * buffers fit budget_bytes of device memory. PG_ERR_NOMEM if they do not.
  * The solver then takes pg_fill_seed, pg_sweep, pg_solve, pg_download_V,
- * pg_upload_V, pg_escape_sweep, pg_solve_escape, pg_last_sweep_ms as usual
- * (at most 128 controls; no policy or FP64 tables).
+ * pg_upload_V, pg_escape_sweep, pg_solve_escape, pg_solver_set_warm_start,
+ * pg_download_warm_start, pg_last_sweep_ms as usual (at most 128 controls;
+ * no policy or FP64 tables). With the FP32 store the production scheme
+ * (refine_rounds, tau_levels, the warm start) runs tiled on the global-gather
+ * kernel through windows from the scheme's reach (every grid heading,
+ * velocity and spin, every lattice command and commands sampled over the
+ * octahedron, every horizon, two cells of margin), the warm-start store held
+ * on the host and streamed beside each interior; a read outside a window
+ * fails the sweep (PG_ERR_STATE) rather than read a wrong value.
  *
  * vstore = PG_VSTORE_U16_TILE or PG_VSTORE_F16_TILE runs the precision form
  * out of core: the host also holds the Jacobi pair of 16-bit codes, windows

@@ -676,9 +676,9 @@ int pg_solver_create_ooc(pg_ctx* ctx, const pg_grid* g, const pg_model* m, const
                          const int32_t tile[3], pg_solver** out) {
     if (nctl > kLdsMaxCtl)
         return fail(PG_ERR_UNSUPPORTED, "the out-of-core sweep holds at most %d controls", kLdsMaxCtl);
-    if (p && (p->rounds > 0 || p->ntau > 1))
-        return fail(PG_ERR_UNSUPPORTED, "the out-of-core sweep does not run refine_rounds or tau_levels (%s set)",
-                    p->rounds > 0 ? "refine_rounds" : "tau_levels");
+    if (p && (p->rounds > 0 || p->ntau > 1) && vstore != PG_VSTORE_F32)
+        return fail(PG_ERR_UNSUPPORTED, "the out-of-core precision form does not run refine_rounds or tau_levels "
+                    "(%s set)", p->rounds > 0 ? "refine_rounds" : "tau_levels");
     if (!tile) return fail(PG_ERR_ARG, "null tile");
     int rc = create_solver(ctx, g, m, p, ctl_soa, nctl, occ, vstore, true, out);
     if (rc) return rc;
@@ -1373,9 +1373,10 @@ int pg_solver_set_active(pg_solver* s, int32_t on) {
 
 int pg_solver_set_warm_start(pg_solver* s, int32_t on) {
     if (!s) return fail(PG_ERR_ARG, "null solver");
-    if (on && (s->ooc || s->vstore != PG_VSTORE_F32))
-        return fail(PG_ERR_UNSUPPORTED, "the warm start runs on an in-HBM FP32 solver only");
+    if (on && s->vstore != PG_VSTORE_F32)
+        return fail(PG_ERR_UNSUPPORTED, "the warm start runs on an FP32 solver only");
     HIPCHECK(hipSetDevice(s->ctx->device));
+    if (s->ooc) return pgh::ooc_set_warm_start(s, on != 0);
     if (!on) {
         if (s->d_wpol) HIPCHECK(hipFree(s->d_wpol));
         s->d_wpol = nullptr;
@@ -1398,6 +1399,7 @@ int pg_solver_set_warm_start(pg_solver* s, int32_t on) {
 
 int pg_download_warm_start(pg_solver* s, int8_t* dst) {
     if (!s || !dst) return fail(PG_ERR_ARG, "null argument");
+    if (s->ooc) return pgh::ooc_download_warm_start(s, dst);
     if (!s->d_wpol) return fail(PG_ERR_STATE, "the warm start is off");
     HIPCHECK(hipSetDevice(s->ctx->device));
     HIPCHECK(hipMemcpyAsync(dst, s->d_wpol, 3 * (size_t)s->ncell, hipMemcpyDeviceToHost, s->ctx->stream));

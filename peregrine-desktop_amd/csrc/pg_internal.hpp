@@ -119,6 +119,12 @@ int ooc_create(pg_solver* s, int64_t budget_bytes, const int32_t tile[3]);
 void ooc_release(pg_solver* s);
 int ooc_fill_seed(pg_solver* s);
 int ooc_sweep(pg_solver* s, double* sum_decrease, float* linf);
+// Whether a tiled solver's sweeps take the production candidates (pattern
+// search, horizon ladder, warm start): then they run the global-gather kernel
+// through windows from the scheme's reach.
+bool ooc_production(const pg_solver* s);
+int ooc_set_warm_start(pg_solver* s, bool on);
+int ooc_download_warm_start(const pg_solver* s, int8_t* dst);
 // One escape sweep over the tiles (the phase's slice offset already in s->K).
 int ooc_escape_sweep(pg_solver* s, double* sum_improve, float* linf);
 int ooc_download(pg_solver* s, float* dst, int64_t off, int64_t n);

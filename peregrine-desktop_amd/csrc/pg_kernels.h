@@ -22,11 +22,25 @@ hipError_t pgk_launch_escape(const pgk::Consts& K, const float* Vin, float* Vout
 // The same over an out-of-core tile's interior (t0, tn: x, y, heading),
 // reading V from the tile's window (origin wo, extent wl; heading modulo n3)
 // and writing the interior to `out`; a read outside the window raises *miss.
-// Plain scheme only (no production candidates).
+// With the production candidates when K asks for them (as pgk_launch_escape).
 hipError_t pgk_launch_escape_tile(const pgk::Consts& K, const int32_t t0[3], const int32_t tn[3],
                                   const float* win, const int32_t wo[3], const int32_t wl[3], float* out,
-                                  const uint8_t* occ, const float* bu, double* part, unsigned* linf, unsigned* miss,
-                                  hipStream_t st, int64_t* nblocks);
+                                  const uint8_t* occ, const float* bu, const float* uraw, double* part,
+                                  unsigned* linf, unsigned* miss, hipStream_t st, int64_t* nblocks);
+// One value sweep of the global-gather kernel over an out-of-core tile's
+// interior through its window (as pgk_launch_escape_tile), with the
+// production candidates when K or wpol asks for them; wpol is the interior's
+// warm-start store (3 per cell, interior order) or null.
+hipError_t pgk_launch_sweep_global_tile(const pgk::Consts& K, const int32_t t0[3], const int32_t tn[3],
+                                        const float* win, const int32_t wo[3], const int32_t wl[3], float* out,
+                                        const uint8_t* occ, const float* bu, const float* uraw, int8_t* wpol,
+                                        double* part, unsigned* linf, unsigned* miss, hipStream_t st,
+                                        int64_t* nblocks);
+// The production scheme's reach in cells (lo/hi for x, y, heading into out[6],
+// which the caller pre-sets to 0): every lattice command and the ns raw
+// commands in raw (3 each), at every horizon the scheme uses.
+hipError_t pgk_launch_reach(const pgk::Consts& K, const float* bu, const float* raw, int ns, int* out,
+                            hipStream_t st);
 hipError_t pgk_launch_reduce(const double* part, int64_t n, double* out, double* mid, hipStream_t st);
 hipError_t pgk_launch_fill(float* V, int64_t n, float v, hipStream_t st);
 hipError_t pgk_launch_seed(float* V, const int64_t* seeds, int64_t n, hipStream_t st);

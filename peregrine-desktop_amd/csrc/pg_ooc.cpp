@@ -35,6 +35,7 @@ struct OocTile {
     int32_t t0[3], tn[3];   // interior origin and extent: x, y, heading
     int32_t wo[3], wl[3];   // window origin (heading modulo n3) and extent
     int32_t ewo[3], ewl[3]; // the escape pass's window (set on its first sweep)
+    int32_t pwo[3], pwl[3]; // the production scheme's window (set on its first sweep)
     pgk::BoxGeom G;         // the interior's boxes
 };
 
@@ -62,6 +63,11 @@ struct Ooc {
     int64_t budget = 0;      // device bytes allowed
     int64_t win_floats = 0;  // capacity of each FP32 window buffer, in cells
     bool escape_ready = false;
+    bool prod_ready = false;
+    // the Int8 warm-start store (3 per cell) on the host, and the interiors'
+    // on the device (null: the warm start is off)
+    int8_t* hW = nullptr;
+    int8_t* dwp[2] = {nullptr, nullptr};
 };
 
 namespace {
@@ -398,6 +404,9 @@ void ooc_release(pg_solver* s) {
     (void)hipFree(o->d_miss);
     (void)hipFree(o->d_tsum);
     (void)hipFree(o->d_tpn);
+    (void)hipHostFree(o->hW);
+    (void)hipFree(o->dwp[0]);
+    (void)hipFree(o->dwp[1]);
     delete o;
     s->ooc = nullptr;
 }
@@ -499,6 +508,40 @@ int ooc_sweep16(pg_solver* s, double* sum_decrease, float* linf) {
 
 namespace {
 
+// FP32 window buffers of `win` cells each, if they hold fewer, within the
+// device budget given at creation.
+int grow_windows(pg_solver* s, int64_t win, const char* what) {
+    Ooc& o = *s->ooc;
+    if (win <= o.win_floats) return PG_OK;
+    if (win >= ((int64_t)1 << 31))
+        return fail(PG_ERR_UNSUPPORTED, "%s tile windows of %lld cells: tiles hold fewer than 2^31 (give a smaller "
+                    "tile)", what, (long long)win);
+    const int64_t grown = o.device_bytes + 2 * (win - o.win_floats) * (int64_t)sizeof(float);
+    if (grown > o.budget)
+        return fail(PG_ERR_NOMEM, "%s tile windows need %lld device bytes, over the budget of %lld", what,
+                    (long long)grown, (long long)o.budget);
+    for (int j = 0; j < 2; ++j) {
+        (void)hipFree(o.dwin[j]);
+        o.dwin[j] = nullptr;
+    }
+    hipError_t a = hipMalloc(&o.dwin[0], (size_t)win * sizeof(float));
+    if (a == hipSuccess) a = hipMalloc(&o.dwin[1], (size_t)win * sizeof(float));
+    if (a != hipSuccess) {
+        // leave the solver whole: the other sweeps need their windows back
+        (void)hipFree(o.dwin[0]);
+        (void)hipFree(o.dwin[1]);
+        o.dwin[0] = o.dwin[1] = nullptr;
+        hipError_t b = hipMalloc(&o.dwin[0], (size_t)o.win_floats * sizeof(float));
+        if (b == hipSuccess) b = hipMalloc(&o.dwin[1], (size_t)o.win_floats * sizeof(float));
+        if (b != hipSuccess) return fail(PG_ERR_NOMEM, "out-of-core window buffers lost: %s", hipGetErrorString(b));
+        return fail(a == hipErrorOutOfMemory ? PG_ERR_NOMEM : PG_ERR_HIP, "%s window buffers: %s", what,
+                    hipGetErrorString(a));
+    }
+    o.device_bytes = grown;
+    o.win_floats = win;
+    return PG_OK;
+}
+
 // The escape pass's windows, on its first sweep. The escape backups read
 // from blocked cells too and without the probes' rejections (a blocked cell
 // moves through obstacle space), so the windows come from a plan over an
@@ -509,6 +552,7 @@ namespace {
 int ooc_escape_prepare(pg_solver* s) {
     Ooc& o = *s->ooc;
     if (o.escape_ready) return PG_OK;
+    int rc = PG_OK;
     const pg_grid& g = s->g;
     hipStream_t st = s->ctx->stream;
     const int64_t nbox = s->G.nbox;
@@ -541,33 +585,10 @@ int ooc_escape_prepare(pg_solver* s) {
         win = std::max(win, (int64_t)t.ewl[0] * t.ewl[1] * t.ewl[2] * o.colv);
         blocks = std::max(blocks, ceil_div(interior_cells(t, o.colv), 256));
     }
-    int rc = ensure_partials(s, blocks);
+    rc = ensure_partials(s, blocks);
     if (rc) return rc;
-    if (win > o.win_floats) {
-        const int64_t grown = o.device_bytes + 2 * (win - o.win_floats) * (int64_t)sizeof(float);
-        if (grown > o.budget)
-            return fail(PG_ERR_NOMEM, "the escape pass's tile windows need %lld device bytes, over the budget of %lld",
-                        (long long)grown, (long long)o.budget);
-        for (int j = 0; j < 2; ++j) {
-            (void)hipFree(o.dwin[j]);
-            o.dwin[j] = nullptr;
-        }
-        hipError_t a = hipMalloc(&o.dwin[0], (size_t)win * sizeof(float));
-        if (a == hipSuccess) a = hipMalloc(&o.dwin[1], (size_t)win * sizeof(float));
-        if (a != hipSuccess) {
-            // leave the solver whole: value sweeps need their own windows back
-            (void)hipFree(o.dwin[0]);
-            (void)hipFree(o.dwin[1]);
-            o.dwin[0] = o.dwin[1] = nullptr;
-            hipError_t b = hipMalloc(&o.dwin[0], (size_t)o.win_floats * sizeof(float));
-            if (b == hipSuccess) b = hipMalloc(&o.dwin[1], (size_t)o.win_floats * sizeof(float));
-            if (b != hipSuccess) return fail(PG_ERR_NOMEM, "out-of-core window buffers lost: %s", hipGetErrorString(b));
-            return fail(a == hipErrorOutOfMemory ? PG_ERR_NOMEM : PG_ERR_HIP, "escape window buffers: %s",
-                        hipGetErrorString(a));
-        }
-        o.device_bytes = grown;
-        o.win_floats = win;
-    }
+    rc = grow_windows(s, win, "the escape pass's");
+    if (rc) return rc;
     o.escape_ready = true;
     return PG_OK;
 }
@@ -586,13 +607,19 @@ hipError_t copy_escape_window_in(const pg_solver* s, const Ooc& o, const OocTile
 
 }  // namespace
 
+namespace {
+int ooc_prod_prepare(pg_solver* s);
+hipError_t copy_prod_window_in(const pg_solver* s, const Ooc& o, const OocTile& t, const float* hV, float* dw,
+                               hipStream_t st);
+}  // namespace
+
 // One escape sweep over the tiles (Jacobi, as in HBM): the pipeline of the
 // FP32 value sweep with the escape kernel.
 int ooc_escape_sweep(pg_solver* s, double* sum_improve, float* linf) {
     if (s->f16_active) return fail(PG_ERR_STATE, "a 16-bit store must be polished (pg_polish) before the escape pass");
-    if (s->K.rounds > 0 || s->K.ntau > 1)
-        return fail(PG_ERR_UNSUPPORTED, "the out-of-core escape pass runs the plain scheme only");
-    int rc = ooc_escape_prepare(s);
+    // with the production candidates the reads follow the scheme's reach
+    const bool prod = s->K.rounds > 0 || s->K.ntau > 1;
+    int rc = prod ? ooc_prod_prepare(s) : ooc_escape_prepare(s);
     if (rc) return rc;
     Ooc& o = *s->ooc;
     hipStream_t st = s->ctx->stream;
@@ -603,7 +630,10 @@ int ooc_escape_sweep(pg_solver* s, double* sum_improve, float* linf) {
     HIPCHECK(hipMemsetAsync(o.d_miss, 0, sizeof(unsigned), st));
     HIPCHECK(hipEventRecord(s->ev0, st));
     HIPCHECK(hipStreamWaitEvent(o.h2d, s->ev0, 0));
-    HIPCHECK(copy_escape_window_in(s, o, o.tiles[0], hin, o.dwin[0], o.h2d));
+    auto load = [&](const OocTile& t, float* dw) {
+        return prod ? copy_prod_window_in(s, o, t, hin, dw, o.h2d) : copy_escape_window_in(s, o, t, hin, dw, o.h2d);
+    };
+    HIPCHECK(load(o.tiles[0], o.dwin[0]));
     HIPCHECK(hipEventRecord(o.ev_h2d[0], o.h2d));
     for (int64_t k = 0; k < nt; ++k) {
         const int j = (int)(k & 1);
@@ -611,14 +641,15 @@ int ooc_escape_sweep(pg_solver* s, double* sum_improve, float* linf) {
         HIPCHECK(hipStreamWaitEvent(st, o.ev_h2d[j], 0));
         if (k >= 2) HIPCHECK(hipStreamWaitEvent(st, o.ev_d2h[j], 0));
         int64_t nblk = 0;
-        HIPCHECK(pgk_launch_escape_tile(s->K, t.t0, t.tn, o.dwin[j], t.ewo, t.ewl, o.dout[j], s->d_occ, s->d_bu,
-                                        s->d_part, s->d_linf, o.d_miss, st, &nblk));
+        HIPCHECK(pgk_launch_escape_tile(s->K, t.t0, t.tn, o.dwin[j], prod ? t.pwo : t.ewo, prod ? t.pwl : t.ewl,
+                                        o.dout[j], s->d_occ, s->d_bu, s->d_uraw, s->d_part, s->d_linf, o.d_miss, st,
+                                        &nblk));
         HIPCHECK(pgk_launch_reduce(s->d_part, nblk, o.d_tsum + k, nullptr, st));
         HIPCHECK(hipEventRecord(o.ev_k[j], st));
         if (k + 1 < nt) {
             const int jn = 1 - j;
             if (k >= 1) HIPCHECK(hipStreamWaitEvent(o.h2d, o.ev_k[jn], 0));
-            HIPCHECK(copy_escape_window_in(s, o, o.tiles[(size_t)k + 1], hin, o.dwin[jn], o.h2d));
+            HIPCHECK(load(o.tiles[(size_t)k + 1], o.dwin[jn]));
             HIPCHECK(hipEventRecord(o.ev_h2d[jn], o.h2d));
         }
         HIPCHECK(hipStreamWaitEvent(o.d2h, o.ev_k[j], 0));
@@ -646,9 +677,214 @@ int ooc_escape_sweep(pg_solver* s, double* sum_improve, float* linf) {
     return PG_OK;
 }
 
+namespace {
+
+// The production scheme's windows, on its first sweep: interior plus a halo
+// from the scheme's reach (every grid heading, velocity and spin, every
+// lattice command, commands sampled over the octahedron -- the pattern
+// search and the warm start evaluate commands on it -- and every horizon),
+// two cells of margin each side for the interpolation corners. Pattern-search
+// and warm-start commands are data-dependent, so the windows are a bound, not
+// a plan: a read outside one raises the miss flag and fails the sweep.
+int ooc_prod_prepare(pg_solver* s) {
+    Ooc& o = *s->ooc;
+    if (o.prod_ready) return PG_OK;
+    const pg_grid& g = s->g;
+    hipStream_t st = s->ctx->stream;
+    // octahedron |u1| + |u2| + |u3| = 1: every face, barycentric step 1/12
+    std::vector<float> raw;
+    const int M = 12;
+    for (int f = 0; f < 8; ++f)
+        for (int a = 0; a <= M; ++a)
+            for (int b = 0; a + b <= M; ++b) {
+                const int c = M - a - b;
+                raw.push_back((f & 1 ? -1.0f : 1.0f) * (float)a / M);
+                raw.push_back((f & 2 ? -1.0f : 1.0f) * (float)b / M);
+                raw.push_back((f & 4 ? -1.0f : 1.0f) * (float)c / M);
+            }
+    const int ns = (int)(raw.size() / 3);
+    float* d_raw = nullptr;
+    int* d_out = nullptr;
+    int h_out[6] = {0, 0, 0, 0, 0, 0};
+    hipError_t e = hipMalloc(&d_raw, raw.size() * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&d_out, sizeof h_out);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_raw, raw.data(), raw.size() * sizeof(float), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_out, h_out, sizeof h_out, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = pgk_launch_reach(s->K, s->d_bu, d_raw, ns, d_out, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(h_out, d_out, sizeof h_out, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(d_raw);
+    (void)hipFree(d_out);
+    if (e != hipSuccess)
+        return fail(e == hipErrorOutOfMemory ? PG_ERR_NOMEM : PG_ERR_HIP, "reach pass: %s", hipGetErrorString(e));
+    const int n3 = g.n[2];
+    int64_t win = 0, blocks = 0;
+    for (OocTile& t : o.tiles) {
+        for (int a = 0; a < 2; ++a) {
+            const int64_t lo = std::max<int64_t>(0, (int64_t)t.t0[a] + h_out[2 * a] - 2);
+            const int64_t hi = std::min<int64_t>(g.n[a], (int64_t)t.t0[a] + t.tn[a] + h_out[2 * a + 1] + 2);
+            t.pwo[a] = (int32_t)lo;
+            t.pwl[a] = (int32_t)(hi - lo);
+        }
+        const int64_t lo = (int64_t)t.t0[2] + h_out[4] - 2, hi = (int64_t)t.t0[2] + t.tn[2] + h_out[5] + 2;
+        if (hi - lo >= n3) {
+            t.pwo[2] = 0;
+            t.pwl[2] = n3;
+        } else {
+            t.pwo[2] = (int32_t)(((lo % n3) + n3) % n3);
+            t.pwl[2] = (int32_t)(hi - lo);
+        }
+        win = std::max(win, (int64_t)t.pwl[0] * t.pwl[1] * t.pwl[2] * o.colv);
+        blocks = std::max(blocks, ceil_div(interior_cells(t, o.colv), 256));
+    }
+    int rc = ensure_partials(s, blocks);
+    if (rc) return rc;
+    rc = grow_windows(s, win, "the production scheme's");
+    if (rc) return rc;
+    o.prod_ready = true;
+    return PG_OK;
+}
+
+// A tile's warm-start entries (3 bytes per cell) between the host store and
+// an interior buffer: one strided block.
+hipError_t copy_warm(const pg_solver* s, const Ooc& o, const OocTile& t, int8_t* dbuf, bool to_host,
+                     hipStream_t st) {
+    const pg_grid& g = s->g;
+    const size_t cell = 3 * (size_t)o.colv;   // bytes per (x, y, heading)
+    int8_t* h = o.hW + ((((int64_t)t.t0[0] * g.n[1] + t.t0[1]) * g.n[2] + t.t0[2]) * (int64_t)cell);
+    const size_t row = (size_t)t.tn[2] * cell;
+    hipPitchedPtr hp = make_hipPitchedPtr(h, (size_t)g.n[2] * cell, row, (size_t)g.n[1]);
+    hipPitchedPtr dp = make_hipPitchedPtr(dbuf, row, row, (size_t)t.tn[1]);
+    hipMemcpy3DParms p{};
+    p.srcPtr = to_host ? dp : hp;
+    p.dstPtr = to_host ? hp : dp;
+    p.extent = make_hipExtent(row, (size_t)t.tn[1], (size_t)t.tn[0]);
+    p.kind = to_host ? hipMemcpyDeviceToHost : hipMemcpyHostToDevice;
+    return hipMemcpy3DAsync(&p, st);
+}
+
+hipError_t copy_prod_window_in(const pg_solver* s, const Ooc& o, const OocTile& t, const float* hV, float* dw,
+                               hipStream_t st) {
+    OocTile e = t;
+    for (int a = 0; a < 3; ++a) {
+        e.wo[a] = t.pwo[a];
+        e.wl[a] = t.pwl[a];
+    }
+    return copy_window_in<float>(s, o, e, hV, dw, st);
+}
+
+// One production-scheme sweep over the tiles (Jacobi, as in HBM): the FP32
+// pipeline with the global-gather kernel, the tile's warm-start entries in
+// and out beside its window and interior (each cell reads and writes only
+// its own entry).
+int ooc_sweep_prod(pg_solver* s, double* sum_decrease, float* linf) {
+    int rc = ooc_prod_prepare(s);
+    if (rc) return rc;
+    Ooc& o = *s->ooc;
+    hipStream_t st = s->ctx->stream;
+    const float* hin = o.hV[o.cur];
+    float* hout = o.hV[1 - o.cur];
+    const int64_t nt = (int64_t)o.tiles.size();
+    const bool warm = o.hW != nullptr;
+    HIPCHECK(hipMemsetAsync(s->d_linf, 0, sizeof(unsigned), st));
+    HIPCHECK(hipMemsetAsync(o.d_miss, 0, sizeof(unsigned), st));
+    HIPCHECK(hipEventRecord(s->ev0, st));
+    auto load = [&](int64_t k, int j) -> hipError_t {
+        const OocTile& t = o.tiles[(size_t)k];
+        hipError_t e = copy_prod_window_in(s, o, t, hin, o.dwin[j], o.h2d);
+        if (e == hipSuccess && warm) e = copy_warm(s, o, t, o.dwp[j], false, o.h2d);
+        return e;
+    };
+    HIPCHECK(hipStreamWaitEvent(o.h2d, s->ev0, 0));
+    HIPCHECK(load(0, 0));
+    HIPCHECK(hipEventRecord(o.ev_h2d[0], o.h2d));
+    for (int64_t k = 0; k < nt; ++k) {
+        const int j = (int)(k & 1);
+        const OocTile& t = o.tiles[(size_t)k];
+        HIPCHECK(hipStreamWaitEvent(st, o.ev_h2d[j], 0));
+        if (k >= 2) HIPCHECK(hipStreamWaitEvent(st, o.ev_d2h[j], 0));
+        int64_t nblk = 0;
+        HIPCHECK(pgk_launch_sweep_global_tile(s->K, t.t0, t.tn, o.dwin[j], t.pwo, t.pwl, o.dout[j], s->d_occ, s->d_bu,
+                                              s->d_uraw, warm ? o.dwp[j] : nullptr, s->d_part, s->d_linf, o.d_miss,
+                                              st, &nblk));
+        HIPCHECK(pgk_launch_reduce(s->d_part, nblk, o.d_tsum + k, nullptr, st));
+        HIPCHECK(hipEventRecord(o.ev_k[j], st));
+        if (k + 1 < nt) {
+            const int jn = 1 - j;
+            if (k >= 1) HIPCHECK(hipStreamWaitEvent(o.h2d, o.ev_k[jn], 0));
+            // the warm-start buffer jn is also waiting on its copy out
+            if (k >= 1 && warm) HIPCHECK(hipStreamWaitEvent(o.h2d, o.ev_d2h[jn], 0));
+            HIPCHECK(load(k + 1, jn));
+            HIPCHECK(hipEventRecord(o.ev_h2d[jn], o.h2d));
+        }
+        HIPCHECK(hipStreamWaitEvent(o.d2h, o.ev_k[j], 0));
+        HIPCHECK(copy_interior_out(s, o, t, o.dout[j], hout, o.d2h));
+        if (warm) HIPCHECK(copy_warm(s, o, t, o.dwp[j], true, o.d2h));
+        HIPCHECK(hipEventRecord(o.ev_d2h[j], o.d2h));
+    }
+    std::vector<double> tsum((size_t)nt);
+    unsigned hl = 0, miss = 0;
+    HIPCHECK(hipMemcpyAsync(tsum.data(), o.d_tsum, (size_t)nt * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(&hl, s->d_linf, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(&miss, o.d_miss, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    HIPCHECK(hipStreamSynchronize(o.d2h));
+    HIPCHECK(hipEventRecord(s->ev1, st));
+    HIPCHECK(hipEventSynchronize(s->ev1));
+    HIPCHECK(hipEventElapsedTime(&s->last_ms, s->ev0, s->ev1));
+    if (miss) return fail(PG_ERR_STATE, "a backup read a cell outside its tile's window (reach bound)");
+    for (int64_t i : s->h_seeds) hout[i] = 0.0f;
+    o.cur = 1 - o.cur;
+    double d = 0.0;
+    for (double v : tsum) d += v;
+    float lf;
+    std::memcpy(&lf, &hl, 4);
+    if (sum_decrease) *sum_decrease = d;
+    if (linf) *linf = lf;
+    return PG_OK;
+}
+
+}  // namespace
+
+bool ooc_production(const pg_solver* s) {
+    return s->ooc && (s->K.rounds > 0 || s->K.ntau > 1 || s->ooc->hW != nullptr);
+}
+
+int ooc_set_warm_start(pg_solver* s, bool on) {
+    Ooc& o = *s->ooc;
+    if (!on) {
+        (void)hipHostFree(o.hW);
+        (void)hipFree(o.dwp[0]);
+        (void)hipFree(o.dwp[1]);
+        o.hW = nullptr;
+        o.dwp[0] = o.dwp[1] = nullptr;
+        return PG_OK;
+    }
+    if (o.hW) return PG_OK;
+    int64_t in = 0;
+    for (const OocTile& t : o.tiles) in = std::max(in, interior_cells(t, o.colv));
+    hipError_t e = hipHostMalloc((void**)&o.hW, 3 * (size_t)s->ncell, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMalloc(&o.dwp[0], 3 * (size_t)in);
+    if (e == hipSuccess) e = hipMalloc(&o.dwp[1], 3 * (size_t)in);
+    if (e != hipSuccess) {
+        ooc_set_warm_start(s, false);
+        return fail(e == hipErrorOutOfMemory ? PG_ERR_NOMEM : PG_ERR_HIP, "warm-start store allocation failed: %s",
+                    hipGetErrorString(e));
+    }
+    std::memset(o.hW, 0, 3 * (size_t)s->ncell);
+    return PG_OK;
+}
+
+int ooc_download_warm_start(const pg_solver* s, int8_t* dst) {
+    if (!s->ooc->hW) return fail(PG_ERR_STATE, "the warm start is off");
+    std::memcpy(dst, s->ooc->hW, 3 * (size_t)s->ncell);
+    return PG_OK;
+}
+
 int ooc_fill_seed(pg_solver* s) {
     Ooc& o = *s->ooc;
     o.cur = 0;
+    if (o.hW) std::memset(o.hW, 0, 3 * (size_t)s->ncell);
     std::fill(o.hV[0], o.hV[0] + s->ncell, s->p.cap);
     for (int64_t i : s->h_seeds) o.hV[0][i] = 0.0f;
     if (s->vstore != PG_VSTORE_F32) return ooc_encode_all(s);
@@ -657,6 +893,7 @@ int ooc_fill_seed(pg_solver* s) {
 
 int ooc_sweep(pg_solver* s, double* sum_decrease, float* linf) {
     if (s->f16_active) return ooc_sweep16(s, sum_decrease, linf);
+    if (ooc_production(s)) return ooc_sweep_prod(s, sum_decrease, linf);
     Ooc& o = *s->ooc;
     hipStream_t st = s->ctx->stream;
     const float* hin = o.hV[o.cur];

@@ -32,6 +32,8 @@ struct VGlobal {
     const float* V;
     __device__ __forceinline__ int64_t off(const Consts& K, int k, int j) const { return (int64_t)j * K.stride[k]; }
     __device__ __forceinline__ float load(const Consts&, int64_t a) const { return V[a]; }
+    // the value of cell i (flat index idx on the whole grid)
+    __device__ __forceinline__ float self(const Consts&, int64_t idx, const int32_t*) const { return V[idx]; }
 };
 
 // A window: cells x in [o0, o0 + L0), y in [o1, o1 + L1), heading in
@@ -63,6 +65,20 @@ struct VWindow {
         }
         return W[a];
     }
+    __device__ __forceinline__ float self(const Consts& K, int64_t, const int32_t* i) const {
+        int64_t a = 0;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) a += off(K, k, i[k]);
+        return load(K, a);
+    }
+};
+
+// The cells a launch backs up: x, y, heading from t0 over tn, every velocity
+// cell; omega fastest; ncell of them (the whole grid, or an out-of-core
+// tile's interior).
+struct TileCells {
+    int32_t t0[3], tn[3];
+    int64_t ncell;
 };
 
 template <bool ESC, class VR>
@@ -282,24 +298,27 @@ __device__ __forceinline__ int8_t pol_quant(float v) {
 // that the lookahead is inlined once. When the value improves the winning
 // command is written back to the cell's own wpol entry (only the cell's own
 // entry is read, so the in-place update is the Jacobi one).
-template <bool WRITE_V, bool WRITE_POL, bool ESC, bool PROD>
+template <bool WRITE_V, bool WRITE_POL, bool ESC, bool PROD, class VR>
 __global__ __launch_bounds__(256) void sweep_global_kernel(
-    const float* Vin, float* Vout, const uint8_t* __restrict__ occ,
+    VR Vin, float* Vout, TileCells T, const uint8_t* __restrict__ occ,
     const float* __restrict__ bu, const float* __restrict__ uraw, Consts K, uint8_t* __restrict__ pol,
     int8_t* __restrict__ wpol, double* __restrict__ part_sum, unsigned* __restrict__ linf_bits) {
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     float dec = 0.0f;
-    if (idx < K.ncell) {
+    if (idx < T.ncell) {
         int64_t r = idx;
         int32_t i[6];
 #pragma unroll
-        for (int k = 5; k >= 1; --k) {
+        for (int k = 5; k >= 3; --k) {
             i[k] = (int32_t)(r % K.n[k]);
             r /= K.n[k];
         }
-        i[0] = (int32_t)r;
+        i[2] = T.t0[2] + (int32_t)(r % T.tn[2]);
+        r /= T.tn[2];
+        i[1] = T.t0[1] + (int32_t)(r % T.tn[1]);
+        i[0] = T.t0[0] + (int32_t)(r / T.tn[1]);
         const bool blocked = occ[((int64_t)i[0] * K.n[1] + i[1]) * K.n[2] + i[2]] != 0;
-        const float old = Vin[idx];
+        const float old = Vin.self(K, idx, i);
         const bool terminal = old >= 0.0f && old < K.cap;
         float best = K.cap;
         int bk = 0;
@@ -378,8 +397,7 @@ __global__ __launch_bounds__(256) void sweep_global_kernel(
                     bx = lx; by = ly; bw = lw;
                     ladder = true;
                 }
-                const float c = lookahead_global<ESC>(K, VGlobal{Vin}, og, t, ths, tn, x, y, h, vx, vy, w, bx, by,
-                                                      bw);
+                const float c = lookahead_global<ESC>(K, Vin, og, t, ths, tn, x, y, h, vx, vy, w, bx, by, bw);
                 if (c < best) {
                     best = c;
                     if (!ladder) { bk = kk; b1 = c1; b2 = c2; b3 = c3; }
@@ -407,76 +425,95 @@ __global__ __launch_bounds__(256) void sweep_global_kernel(
     if (WRITE_V) block_reduce_sum_max((double)dec, dec, part_sum, linf_bits);
 }
 
-// The escape pass over an out-of-core tile's interior (cells x, y, heading
-// from t0 over tn, every velocity cell; omega fastest), reading V through the
-// tile's window: sweep_global_kernel's ESC form without the production
-// candidates (coast, then the lattice or this phase's slice), the same
-// arithmetic in the same order, so a tiled escape sweep equals the in-HBM
-// one bit for bit.
-struct TileCells {
-    int32_t t0[3], tn[3];
-    int64_t ncell;
-};
-
-__global__ __launch_bounds__(256) void escape_tile_kernel(VWindow Vw, float* __restrict__ out, TileCells T,
-                                                          const uint8_t* __restrict__ occ,
-                                                          const float* __restrict__ bu, Consts K,
-                                                          double* __restrict__ part_sum,
-                                                          unsigned* __restrict__ linf_bits) {
+// Reach of the production scheme, for out-of-core windows: the successor's
+// displacement from every (heading, velocity, spin) grid state -- position
+// does not enter the dynamics -- under every lattice command and `ns` raw
+// commands sampled over the octahedron (the pattern search and the warm
+// start evaluate commands on it), at the cell's horizon, every ladder rung
+// and the dt rung. Probes are not applied (a superset). Out: the smallest and
+// largest displacement in cells, floor/ceil, for x, y and heading (ints
+// lo0, hi0, lo1, hi1, lo2, hi2; pre-set by the caller).
+__global__ __launch_bounds__(256) void reach_kernel(Consts K, const float* __restrict__ bu, const float* __restrict__ raw,
+                                                    int ns, int* __restrict__ out) {
+    const int64_t nst = (int64_t)K.n[2] * K.n[3] * K.n[4] * K.n[5];
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    float dec = 0.0f;
-    if (idx < T.ncell) {
+    float mn[3] = {0.0f, 0.0f, 0.0f}, mx[3] = {0.0f, 0.0f, 0.0f};
+    if (idx < nst) {
         int64_t r = idx;
-        int32_t i[6];
+        int i[6];
 #pragma unroll
         for (int k = 5; k >= 3; --k) {
-            i[k] = (int32_t)(r % K.n[k]);
+            i[k] = (int)(r % K.n[k]);
             r /= K.n[k];
         }
-        i[2] = T.t0[2] + (int32_t)(r % T.tn[2]);
-        r /= T.tn[2];
-        i[1] = T.t0[1] + (int32_t)(r % T.tn[1]);
-        i[0] = T.t0[0] + (int32_t)(r / T.tn[1]);
-        const bool blocked = occ[((int64_t)i[0] * K.n[1] + i[1]) * K.n[2] + i[2]] != 0;
-        int64_t self = 0;
-#pragma unroll
-        for (int k = 0; k < 6; ++k) self += Vw.off(K, k, i[k]);
-        const float old = Vw.load(K, self);
-        const bool terminal = old >= 0.0f && old < K.cap;
-        float best = K.cap;
-        if (!terminal) {
-            const float x = K.lo[0] + (float)i[0] * K.step[0];
-            const float y = K.lo[1] + (float)i[1] * K.step[1];
-            const float h = K.lo[2] + (float)i[2] * K.step[2];
-            const float vfx = K.lo[3] + (float)i[3] * K.step[3];
-            const float vfy = K.lo[4] + (float)i[4] * K.step[4];
-            const float w = K.lo[5] + (float)i[5] * K.step[5];
-            float sh0, ch0;
-            sincos_ref(h, sh0, ch0);
-            const float vx = ch0 * vfx + sh0 * vfy;
-            const float vy = -sh0 * vfx + ch0 * vfy;
-            const OccGlobal og{occ, K.n[1], K.n[2], blocked};
-            float tau = K.tau, hs = K.hs;
-            int nsub = K.nsub;
-            if (K.cfl) {
-                tau = cfl_tau_dev(K, vfx, vfy, w);
-                nsub = substeps_dev(K, tau);
-                hs = tau / (float)nsub;
+        i[2] = (int)r;
+        const float h = K.lo[2] + (float)i[2] * K.step[2];
+        const float vfx = K.lo[3] + (float)i[3] * K.step[3];
+        const float vfy = K.lo[4] + (float)i[4] * K.step[4];
+        const float w = K.lo[5] + (float)i[5] * K.step[5];
+        float sh0, ch0;
+        sincos_ref(h, sh0, ch0);
+        const float vx = ch0 * vfx + sh0 * vfy;
+        const float vy = -sh0 * vfx + ch0 * vfy;
+        float tau = K.tau;
+        if (K.cfl) tau = cfl_tau_dev(K, vfx, vfy, w);
+        const int nrung = K.ntau > 1 ? K.ntau : 0;
+        for (int c = 0; c < K.nctl + ns; ++c) {
+            float bx, by, bw;
+            if (c < K.nctl) {
+                bx = bu[3 * c]; by = bu[3 * c + 1]; bw = bu[3 * c + 2];
+            } else {
+                const float* u = raw + 3 * (c - K.nctl);
+                bu_of(K, u[0], u[1], u[2], bx, by, bw);
             }
-            // slot 0 coast, slots 1 .. span the lattice (or the slice)
-            for (int slot = 0; slot <= K.span; ++slot) {
-                const int kk = slot == 0 ? 0 : slice_control(K, slot);
-                const float c = lookahead_global<true>(K, Vw, og, tau, hs, nsub, x, y, h, vx, vy, w, bu[3 * kk],
-                                                       bu[3 * kk + 1], bu[3 * kk + 2]);
-                if (c < best) best = c;
+            float tl = tau * 0.5f;
+            for (int q = -2; q < nrung; ++q) {
+                // q = -2: the cell's horizon; -1: the dt rung; 0..: ladder rungs
+                float t;
+                if (q == -2) {
+                    t = tau;
+                } else if (q == -1) {
+                    if (!K.cfl) continue;
+                    t = K.tau;
+                } else {
+                    t = tl;
+                    tl = tl * K.tau_ratio;
+                }
+                if (q >= -1) {
+                    t = t > K.tau_min ? t : K.tau_min;
+                    t = t < K.tau_max ? t : K.tau_max;
+                }
+                // the horizons and substeps the sweep takes for them
+                int tn = K.nsub;
+                float ths = K.hs;
+                if (q >= -1 || K.cfl) {
+                    tn = substeps_dev(K, t);
+                    ths = t / (float)tn;
+                }
+                float nx = 0.0f, ny = 0.0f, nh = h, nvx = vx, nvy = vy, nw = w;
+                const bool ok = K.rk2 ? rk2_state(K, ths, tn, bx, by, bw, nx, ny, nh, nvx, nvy, nw)
+                                      : euler(K, ths, tn, bx, by, bw, nx, ny, nh, nvx, nvy, nw);
+                const float d[3] = {nx, ny, nh - h};
+                if (!(ok & finite_f(d[0]) & finite_f(d[1]) & finite_f(d[2]) & finite_f(nvx) & finite_f(nvy) &
+                      finite_f(nw)))
+                    continue;
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    mn[a] = d[a] < mn[a] ? d[a] : mn[a];
+                    mx[a] = d[a] > mx[a] ? d[a] : mx[a];
+                }
             }
         }
-        float o = old;
-        const float e_old = old < 0.0f ? -old : K.cap;
-        if (!terminal && best < e_old) { o = -best; dec = e_old - best; }
-        out[idx] = o;
     }
-    block_reduce_sum_max((double)dec, dec, part_sum, linf_bits);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        // in cells, far values held at +-1e6 (they reach the whole axis)
+        const float lo = __builtin_floorf(mn[a] / K.step[a]), hi = __builtin_ceilf(mx[a] / K.step[a]);
+        const int il = lo < -1.0e6f ? -1000000 : (int)lo;
+        const int ih = hi > 1.0e6f ? 1000000 : (int)hi;
+        atomicMin(&out[2 * a], il);
+        atomicMax(&out[2 * a + 1], ih);
+    }
 }
 
 // Deterministic sum of the per-block partials (a fixed order for each n):
@@ -540,16 +577,28 @@ static bool prod_scheme(const pgk::Consts& K, const int8_t* wpol) {
     return wpol != nullptr || K.rounds > 0 || K.ntau > 1;
 }
 
+static pgk::TileCells whole_grid(const pgk::Consts& K) {
+    pgk::TileCells T;
+    for (int a = 0; a < 3; ++a) {
+        T.t0[a] = 0;
+        T.tn[a] = K.n[a];
+    }
+    T.ncell = K.ncell;
+    return T;
+}
+
 template <bool WV, bool WP, bool ESC>
 static void launch_global(const pgk::Consts& K, int64_t nb, hipStream_t st, const float* Vin, float* Vout,
                           const uint8_t* occ, const float* bu, const float* uraw, uint8_t* pol, int8_t* wpol,
                           double* part, unsigned* linf_bits) {
+    const pgk::VGlobal vin{Vin};
+    const pgk::TileCells T = whole_grid(K);
     if (prod_scheme(K, wpol))
-        hipLaunchKernelGGL((pgk::sweep_global_kernel<WV, WP, ESC, true>), dim3((unsigned)nb), dim3(256), 0, st, Vin,
-                           Vout, occ, bu, uraw, K, pol, wpol, part, linf_bits);
+        hipLaunchKernelGGL((pgk::sweep_global_kernel<WV, WP, ESC, true, pgk::VGlobal>), dim3((unsigned)nb), dim3(256),
+                           0, st, vin, Vout, T, occ, bu, uraw, K, pol, wpol, part, linf_bits);
     else
-        hipLaunchKernelGGL((pgk::sweep_global_kernel<WV, WP, ESC, false>), dim3((unsigned)nb), dim3(256), 0, st, Vin,
-                           Vout, occ, bu, uraw, K, pol, wpol, part, linf_bits);
+        hipLaunchKernelGGL((pgk::sweep_global_kernel<WV, WP, ESC, false, pgk::VGlobal>), dim3((unsigned)nb), dim3(256),
+                           0, st, vin, Vout, T, occ, bu, uraw, K, pol, wpol, part, linf_bits);
 }
 
 hipError_t pgk_launch_sweep(const pgk::Consts& K, const float* Vin, float* Vout,
@@ -578,13 +627,12 @@ hipError_t pgk_launch_escape(const pgk::Consts& K, const float* Vin, float* Vout
     return hipGetLastError();
 }
 
-hipError_t pgk_launch_escape_tile(const pgk::Consts& K, const int32_t t0[3], const int32_t tn[3],
-                                  const float* win, const int32_t wo[3], const int32_t wl[3], float* out,
-                                  const uint8_t* occ, const float* bu, double* part, unsigned* linf, unsigned* miss,
-                                  hipStream_t st, int64_t* nblocks) {
+// A tile's window reader and cells; false if either needs 2^31 or more cells
+// (a dispatch counts its work-items in 32 bits).
+static bool tile_reader(const pgk::Consts& K, const int32_t t0[3], const int32_t tn[3], const float* win,
+                        const int32_t wo[3], const int32_t wl[3], unsigned* miss, pgk::VWindow& Vw,
+                        pgk::TileCells& T) {
     const int64_t colv = (int64_t)K.n[3] * K.n[4] * K.n[5];
-    pgk::TileCells T;
-    pgk::VWindow Vw;
     Vw.W = win;
     Vw.miss = miss;
     for (int a = 0; a < 3; ++a) {
@@ -601,12 +649,51 @@ hipError_t pgk_launch_escape_tile(const pgk::Consts& K, const int32_t t0[3], con
     Vw.ws[1] = colv * wl[2];
     Vw.ws[0] = Vw.ws[1] * wl[1];
     Vw.size = Vw.ws[0] * wl[0];
-    // a dispatch counts its work-items in 32 bits
-    if (T.ncell >= ((int64_t)1 << 31)) return hipErrorInvalidValue;
+    return T.ncell < ((int64_t)1 << 31);
+}
+
+hipError_t pgk_launch_escape_tile(const pgk::Consts& K, const int32_t t0[3], const int32_t tn[3],
+                                  const float* win, const int32_t wo[3], const int32_t wl[3], float* out,
+                                  const uint8_t* occ, const float* bu, const float* uraw, double* part,
+                                  unsigned* linf, unsigned* miss, hipStream_t st, int64_t* nblocks) {
+    pgk::VWindow Vw;
+    pgk::TileCells T;
+    if (!tile_reader(K, t0, tn, win, wo, wl, miss, Vw, T)) return hipErrorInvalidValue;
     const int64_t nb = (T.ncell + 255) / 256;
     *nblocks = nb;
-    hipLaunchKernelGGL(pgk::escape_tile_kernel, dim3((unsigned)nb), dim3(256), 0, st, Vw, out, T, occ, bu, K, part,
-                       linf);
+    if (prod_scheme(K, nullptr))
+        hipLaunchKernelGGL((pgk::sweep_global_kernel<true, false, true, true, pgk::VWindow>), dim3((unsigned)nb),
+                           dim3(256), 0, st, Vw, out, T, occ, bu, uraw, K, nullptr, nullptr, part, linf);
+    else
+        hipLaunchKernelGGL((pgk::sweep_global_kernel<true, false, true, false, pgk::VWindow>), dim3((unsigned)nb),
+                           dim3(256), 0, st, Vw, out, T, occ, bu, uraw, K, nullptr, nullptr, part, linf);
+    return hipGetLastError();
+}
+
+hipError_t pgk_launch_sweep_global_tile(const pgk::Consts& K, const int32_t t0[3], const int32_t tn[3],
+                                        const float* win, const int32_t wo[3], const int32_t wl[3], float* out,
+                                        const uint8_t* occ, const float* bu, const float* uraw, int8_t* wpol,
+                                        double* part, unsigned* linf, unsigned* miss, hipStream_t st,
+                                        int64_t* nblocks) {
+    pgk::VWindow Vw;
+    pgk::TileCells T;
+    if (!tile_reader(K, t0, tn, win, wo, wl, miss, Vw, T)) return hipErrorInvalidValue;
+    const int64_t nb = (T.ncell + 255) / 256;
+    *nblocks = nb;
+    if (prod_scheme(K, wpol))
+        hipLaunchKernelGGL((pgk::sweep_global_kernel<true, false, false, true, pgk::VWindow>), dim3((unsigned)nb),
+                           dim3(256), 0, st, Vw, out, T, occ, bu, uraw, K, nullptr, wpol, part, linf);
+    else
+        hipLaunchKernelGGL((pgk::sweep_global_kernel<true, false, false, false, pgk::VWindow>), dim3((unsigned)nb),
+                           dim3(256), 0, st, Vw, out, T, occ, bu, uraw, K, nullptr, wpol, part, linf);
+    return hipGetLastError();
+}
+
+hipError_t pgk_launch_reach(const pgk::Consts& K, const float* bu, const float* raw, int ns, int* out,
+                            hipStream_t st) {
+    const int64_t nst = (int64_t)K.n[2] * K.n[3] * K.n[4] * K.n[5];
+    const int64_t nb = (nst + 255) / 256;
+    hipLaunchKernelGGL(pgk::reach_kernel, dim3((unsigned)nb), dim3(256), 0, st, K, bu, raw, ns, out);
     return hipGetLastError();
 }
 

@@ -12,10 +12,8 @@ Run.jl:1434-2133) with its PROGRESS stream, MANIFEST.JSON and MODEL.JSON
 (Run.jl:1397-1432) and a `plan` report (Run.jl:699-959).
 
 The value solve runs on the device through libperegrine_hip; there is no
-CPU backend (`backend: "cpu"` is refused). Scheme options the tiled
-(out-of-core) driver does not implement (`tau_levels` > 1,
-`refine_rounds` > 0, `warm_start`) are refused there with the setting that
-avoids them, never silently dropped; the escape pass runs there too.
+CPU backend (`backend: "cpu"` is refused). The tiled (out-of-core) driver
+runs the same scheme and the escape pass.
 """
 from __future__ import annotations
 
@@ -346,18 +344,11 @@ def solver_params(cfg, g: Grid6) -> SolverParams:
 
 def unsupported_options(sp: SolverParams, ooc: bool) -> list:
     """Scheme settings the chosen driver does not implement, each with the
-    setting that avoids it: the tiled driver runs neither the pattern search,
-    the horizon ladder nor the warm start."""
-    out = []
-    if not ooc:
-        return out
-    if sp.p.ntau > 1:
-        out.append(f"tau_levels {sp.p.ntau} (set tau_levels: 1)")
-    if sp.p.rounds > 0:
-        out.append(f"refine_rounds {sp.p.rounds} (set refine_rounds: 0)")
-    if sp.warm:
-        out.append("warm_start (set warm_start: false)")
-    return out
+    setting that avoids it. Both drivers run every setting: the tiled one
+    sweeps the pattern search, the horizon ladder and the warm start through
+    windows from the scheme's reach, so the list is empty; the plan report
+    keeps the field."""
+    return []
 
 
 def quiet_needed(p: Params, nctl: int) -> int:

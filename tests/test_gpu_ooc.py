@@ -143,3 +143,41 @@ def test_escape_pass_out_of_core(pg, shape, tile):
             assert o.sum_decrease == pytest.approx(r.sum_decrease, rel=1e-9, abs=1e-6)
         V = ooc.download()
         assert (V < 0).sum() > 0
+
+
+def test_production_scheme_out_of_core(pg):
+    """The reference's default scheme out of core (pattern search, horizon
+    ladder, Int8 warm start, Kuhn lookups, RK2, CFL horizon, adaptive probes,
+    rotating slice; windows from the scheme's reach, warm-start entries
+    streamed beside each interior) against the in-HBM global-gather kernel:
+    5 value sweeps, V and the warm-start store bit for bit, then 2 escape
+    sweeps with the production candidates."""
+    import dataclasses
+    prob = pg.configs.problem(2, shape=(24, 24, 16, 5, 5, 7))
+    prob = dataclasses.replace(prob, model=dataclasses.replace(prob.model, knee=0.6))
+    prob = dataclasses.replace(prob, params=dataclasses.replace(
+        prob.params, ntau=5, tau_ratio=2.0, cfl=2.0, tau_min=0.004, tau_max=0.5, rounds=2, delta0=0.35, ncoarse=8,
+        simplex=True, rk2=True, adaptive_checks=True))
+    with pg.HipSolver(prob.grid, prob.model, prob.params, prob.ctl, prob.occ) as ref, \
+         pg.HipSolver(prob.grid, prob.model, prob.params, prob.ctl, prob.occ,
+                      ooc_budget=8 << 30, ooc_tile=(8, 8, 4)) as ooc:
+        ref.set_warm_start(True)
+        ooc.set_warm_start(True)
+        ref.fill_seed(prob.seeds)
+        ooc.fill_seed(prob.seeds)
+        print(f"production out of core: grid {prob.grid.n}, tiles {ooc.ooc_info()[:2]}", flush=True)
+        for it in range(5):
+            r = ref.sweep(it, it % 2 == 1)
+            o = ooc.sweep(it, it % 2 == 1)
+            a, b = ref.download(), ooc.download()
+            bad = np.nonzero(bits(a) != bits(b))[0]
+            assert bad.size == 0, f"sweep {it}: {bad.size} cells differ, first {bad[:5].tolist()}"
+            assert o.linf == r.linf
+            assert o.sum_decrease == pytest.approx(r.sum_decrease, rel=1e-9, abs=1e-6)
+            assert np.array_equal(ref.download_warm_start(), ooc.download_warm_start()), f"sweep {it}: store differs"
+        assert np.count_nonzero(ooc.download_warm_start()) > 0
+        for it in range(2):
+            r = ref.escape_sweep(it, it % 2 == 1)
+            o = ooc.escape_sweep(it, it % 2 == 1)
+            assert np.array_equal(bits(ref.download()), bits(ooc.download())), f"escape sweep {it} differs"
+            assert o.linf == r.linf
