@@ -250,8 +250,14 @@ def run_ooc(pg, prob, args) -> dict:
     tile = tuple(int(v) for v in args.ooc_tile.split(","))
     t = time.perf_counter()
     vstore = pg.abi.PG_VSTORE_U16_TILE if args.ooc_store == "u16" else pg.abi.PG_VSTORE_F32
+    if args.reference_scheme:
+        # the reference's default solve.jl scheme, tiled (as run_reference_scheme in HBM)
+        from dataclasses import replace
+        prob = replace(prob, params=reference_scheme_params(prob.params), ctl=pg.control_set(4))
     s = pg.HipSolver(g, prob.model, prob.params, prob.ctl, prob.occ, vstore=vstore, ooc_budget=int(args.ooc_budget),
                      ooc_tile=tile)
+    if args.reference_scheme:
+        s.set_warm_start(True)
     info = s.ooc_info()
     print(f"# ooc: tiles {info[0]} x {info[1]}, {info[2] / 1e9:.2f} GB on the device, amplification {info[3]:.2f}, "
           f"set up in {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
@@ -276,7 +282,9 @@ def run_ooc(pg, prob, args) -> dict:
     ref = None
     if not args.no_ooc_compare:
         ref = pg.HipSolver(g, prob.model, prob.params, prob.ctl, prob.occ, vstore=vstore)
-        if vstore == pg.abi.PG_VSTORE_F32:
+        if args.reference_scheme:
+            ref.set_warm_start(True)
+        elif vstore == pg.abi.PG_VSTORE_F32:
             ref.set_kernel(pg.abi.PG_KERNEL_LDS)
         ref.set_active(False)
         ref.fill_seed(prob.seeds)
@@ -328,8 +336,17 @@ def run_ooc(pg, prob, args) -> dict:
             "unit": "state-updates/s", "ms_per_step": float(np.mean(ms)), "steps": steps,
             "config": {"workload": f"problem {args.config}", "grid": list(g.n), "states": g.ncells,
                        "controls": prob.nctl, "budget_bytes": int(args.ooc_budget), "tile": list(info[0]),
-                       "tiles": info[1], "device_bytes": info[2], "amplification": info[3]},
+                       "tiles": info[1], "device_bytes": info[2], "amplification": info[3],
+                       "scheme": "reference default (solve.jl)" if args.reference_scheme else "plain"},
             "bit_identical_to_in_hbm": (all(equal) if equal else None)}
+
+
+def reference_scheme_params(params):
+    """The reference's default solve.jl scheme on top of a problem's
+    parameters (Run.jl:203-294 defaults)."""
+    from dataclasses import replace
+    return replace(params, adaptive_checks=True, ntau=5, tau_ratio=2.0, cfl=2.0, tau_min=0.004, tau_max=0.5,
+                   ncoarse=8, rounds=2, delta0=0.35, rk2=True, vclamp=False, simplex=True)
 
 
 def run_reference_scheme(pg, prob, args) -> dict:
@@ -338,10 +355,8 @@ def run_reference_scheme(pg, prob, args) -> dict:
     rotating slice of 8, 2 pattern-search rounds, the Int8 warm start, Kuhn
     lookups, RK2, a per-cell CFL horizon (cfl 2) with a 5-rung ladder and
     adaptive probes (Run.jl:203-294 defaults)."""
-    from dataclasses import replace
     g = prob.grid
-    p = replace(prob.params, adaptive_checks=True, ntau=5, tau_ratio=2.0, cfl=2.0, tau_min=0.004, tau_max=0.5,
-                ncoarse=8, rounds=2, delta0=0.35, rk2=True, vclamp=False, simplex=True)
+    p = reference_scheme_params(prob.params)
     ctl = pg.control_set(4)
     s = pg.HipSolver(g, prob.model, p, ctl, prob.occ)
     s.set_warm_start(True)
