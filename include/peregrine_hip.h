@@ -275,7 +275,9 @@ int pg_solver_ncells(pg_solver* s, int64_t* n);
  * velocity and spin, every lattice command and commands sampled over the
  * octahedron, every horizon, two cells of margin), the warm-start store held
  * on the host and streamed beside each interior; a read outside a window
- * fails the sweep (PG_ERR_STATE) rather than read a wrong value.
+ * fails the sweep (PG_ERR_STATE) rather than read a wrong value. Those
+ * windows (and the escape pass's) are sized on their first sweep, within
+ * budget_bytes: PG_ERR_NOMEM there asks for a smaller tile.
  *
  * vstore = PG_VSTORE_U16_TILE or PG_VSTORE_F16_TILE runs the precision form
  * out of core: the host also holds the Jacobi pair of 16-bit codes, windows
