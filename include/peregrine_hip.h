@@ -163,8 +163,9 @@ int pg_solve(pg_solver* s, int32_t max_iters, double tol_sum, float tol_linf,
  * pg_solve_escape stops as pg_solve does. A 16-bit store must be polished
  * first. Out of core (solve_escape_ooc!, solver/src/OutOfCore.jl:609-700) the
  * tiles sweep the same kernel through windows planned for the escape reads
- * (every state, no probe rejections) on the first escape sweep, bit for bit
- * equal to the in-HBM pass; the plain scheme only there. */
+ * (every state, no probe rejections) on the first escape sweep -- with the
+ * production candidates, through the production scheme's windows -- bit for
+ * bit equal to the in-HBM pass. */
 int pg_escape_sweep(pg_solver* s, int32_t phase, int32_t rev, double* sum_improve, float* linf);
 int pg_solve_escape(pg_solver* s, int32_t max_iters, double tol_sum, float tol_linf, int32_t* iters,
                     double* last_sum, float* last_linf);
