@@ -242,3 +242,14 @@ def test_resolution_advice_is_the_finest_that_fits(run):
     k = c["scale"] - 0.05
     _, cells = run.shape_for(320.0, 320.0, 170.0, 8.0, 4.0 * k, 10.0 * k, 20.0 * k, 1.0 * k)
     assert cells * 11 > 5e9
+
+
+def test_tiled_driver_runs_the_default_scheme(run, tmp_path):
+    """The reference's default scheme (pattern search, horizon ladder, warm
+    start) is refused by neither driver: the tiled one sweeps it through
+    windows from the scheme's reach."""
+    cfg, _ = config(tmp_path)
+    g = run._setup(cfg)[-1]
+    sp = run.solver_params(cfg, g)
+    assert sp.p.rounds > 0 and sp.p.ntau > 1 and sp.warm
+    assert run.unsupported_options(sp, True) == [] and run.unsupported_options(sp, False) == []
