@@ -282,7 +282,9 @@ __device__ __forceinline__ int8_t pol_quant(float v) {
     return (int8_t)(int32_t)__builtin_rintf(c * 127.0f);
 }
 
-// One thread per cell. ESC: the escape pass (gpu_escape_kernel!, Gpu.jl:51-67)
+// One thread per cell of T (the whole grid, or an out-of-core tile's interior:
+// then Vout, pol and wpol are interior-shaped and V is read through the
+// tile's window). ESC: the escape pass (gpu_escape_kernel!, Gpu.jl:51-67)
 // in Jacobi form: a cell with a real route (0 <= v < cap) is the boundary and
 // keeps its value; any other cell, blocked ones included (through obstacle
 // space, PeregrineSolver.jl:864-874), takes -c when its backup c beats the
