@@ -32,6 +32,8 @@ SIG = {
     "orc_step_state_f32": (None, [_P, _P, _P, _F, _I32, _I32, _P]),
     "orc_step_state_f64": (None, [_P, _P, _P, _D, _I32, _I32, _P]),
     "orc_traction_gain": (_F, [_F, _F, _F, _F]),
+    "orc_body_accel_f64": (None, [_P, _P, _P, _P]),
+    "orc_body_accel_f32": (None, [_P, _P, _P, _P]),
     "orc_lookahead_f32": (_F, [_P, _P, _P, _P, _P, _P, _F, _I32, _P]),
     "orc_substeps_for": (_I32, [_F, _P]),
     "orc_cfl_tau": (_F, [_P, _P, _F, _F, _F, _P]),

@@ -261,6 +261,23 @@ float orc_traction_gain(float u1, float u2, float u3, float knee) {
     return traction_gain<float>(u1, u2, u3, knee);
 }
 
+// Body-frame acceleration of the model at raw command u and body velocity v
+// (saturate + body_accel_sat, PeregrineSolver.jl:139-162); a[3] receives the
+// traction gain.
+void orc_body_accel_f64(const pg_model* m, const double* u, const double* v, double* a) {
+    double u1, u2, u3;
+    saturate<double>(*m, u[0], u[1], u[2], u1, u2, u3);
+    body_accel_sat<double>(*m, u1, u2, u3, v[0], v[1], v[2], a[0], a[1], a[2]);
+    a[3] = traction_gain<double>(u[0], u[1], u[2], m->knee);
+}
+
+void orc_body_accel_f32(const pg_model* m, const float* u, const float* v, float* a) {
+    float u1, u2, u3;
+    saturate<float>(*m, u[0], u[1], u[2], u1, u2, u3);
+    body_accel_sat<float>(*m, u1, u2, u3, v[0], v[1], v[2], a[0], a[1], a[2]);
+    a[3] = traction_gain<float>(u[0], u[1], u[2], m->knee);
+}
+
 float orc_lookahead_f32(const float* V, const uint8_t* occ, const pg_grid* g, const pg_model* m,
                         const float* s, const float* u, float tau, int32_t nsub,
                         const pg_params* p) {
