@@ -44,6 +44,7 @@ SIG = {
     "orc_sweep_f32": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _I32, _P, _I32, _I32, _I32, _I32, _P, _P, _P, _I32]),
     "orc_sweep_warm_f32": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _I32, _P, _I32, _I32, _I32, _I32, _P, _P, _P]),
     "orc_tanh_f32": (_F, [_F]),
+    "orc_sweep_range_f32": (ctypes.c_int, [_P, _P, _P, _P, _P, _I32, _P, _I64, _I64, _I32, _I32, _I32, _P, _P]),
     "orc_sweep_f64": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _I32, _P, _I32, _I32, _I32, _I32, _P, _P, _P, _I32]),
     "orc_target_cells": (_I64, [_P, _P, _P, _P, _I64]),
     "orc_check_quotient": (_I64, [_F, _I32]),
@@ -99,6 +100,18 @@ class Problem:
         rc = f(_ptr(Vin), _ptr(Vout), _ptr(self.occ), ctypes.byref(self.gc), ctypes.byref(self.mc),
                _ptr(self.ctl), self.nctl, ctypes.byref(self.pc), mode, rev, phase, nthreads,
                _ptr(pol), ctypes.byref(d), ctypes.byref(l), esc)
+        if rc != 0:
+            raise RuntimeError(f"oracle sweep failed ({rc})")
+        return d.value, l.value
+
+    def sweep_range(self, V: np.ndarray, lo: int, hi: int, rev: int = 0, phase: int = 0, nthreads: int = 0):
+        """In place, threaded, over the cells [lo, hi) of float V only."""
+        assert V.dtype == np.float32 and V.flags.c_contiguous and V.size == self.ncell
+        d = ctypes.c_double()
+        l = ctypes.c_double()
+        rc = lib().orc_sweep_range_f32(_ptr(V), _ptr(self.occ), ctypes.byref(self.gc), ctypes.byref(self.mc),
+                                       _ptr(self.ctl), self.nctl, ctypes.byref(self.pc), int(lo), int(hi),
+                                       int(rev), int(phase), int(nthreads), ctypes.byref(d), ctypes.byref(l))
         if rc != 0:
             raise RuntimeError(f"oracle sweep failed ({rc})")
         return d.value, l.value

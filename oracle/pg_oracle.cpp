@@ -24,14 +24,16 @@ template <class T>
 int sweep_impl(const T* Vin, T* Vout, const uint8_t* occ, const pg_grid* g, const pg_model* m,
                const float* ctl, int32_t nctl, const pg_params* p, int32_t mode, int32_t rev,
                int32_t phase, int32_t nthreads, uint8_t* pol, double* sum_out, double* linf_out,
-               int32_t esc, int8_t* wpol = nullptr) {
+               int32_t esc, int8_t* wpol = nullptr, int64_t lo = 0, int64_t hi = -1) {
     if (!params_supported(p) || nctl < 2) return -3;
-    const int64_t n = ncells(*g);
+    if (hi < 0) hi = ncells(*g);
+    if (lo < 0 || lo > hi || hi > ncells(*g)) return -2;
+    const int64_t n = hi - lo;
     double sum = 0.0, linf = 0.0;
     if (mode == 0) {
         if (nthreads > 0) omp_set_num_threads(nthreads);
 #pragma omp parallel for schedule(dynamic, 512) reduction(+ : sum) reduction(max : linf)
-        for (int64_t i = 0; i < n; ++i) {
+        for (int64_t i = lo; i < hi; ++i) {
             T old = Vin[i];
             if (esc) {
                 // gpu_escape_kernel! update rule (solver/src/Gpu.jl:51-67)
@@ -70,7 +72,7 @@ int sweep_impl(const T* Vin, T* Vout, const uint8_t* occ, const pg_grid* g, cons
         T* V = Vout;
         if (mode == 2 && nthreads > 0) omp_set_num_threads(nthreads);
         auto body = [&](int64_t ii, double& s, double& l) {
-            int64_t idx = rev ? (n - 1 - ii) : ii;
+            int64_t idx = rev ? (hi - 1 - ii) : lo + ii;
             T old = V[idx];
             int32_t k = 0;
             if (esc) {
@@ -354,6 +356,16 @@ int orc_sweep_warm_f32(const float* Vin, float* Vout, const uint8_t* occ, const 
                        double* sum, double* linf) {
     return sweep_impl<float>(Vin, Vout, occ, g, m, ctl, nctl, p, mode, rev, phase, nthreads, nullptr,
                              sum, linf, 0, wpol);
+}
+
+// An in-place, OpenMP-threaded sweep (mode 2) of the cells [lo, hi) only:
+// the reference CPU path's update order over a contiguous slab, for timing
+// it on a bounded part of a grid too large to sweep whole on the host.
+int orc_sweep_range_f32(float* V, const uint8_t* occ, const pg_grid* g, const pg_model* m,
+                        const float* ctl, int32_t nctl, const pg_params* p, int64_t lo, int64_t hi,
+                        int32_t rev, int32_t phase, int32_t nthreads, double* sum, double* linf) {
+    return sweep_impl<float>(V, V, occ, g, m, ctl, nctl, p, 2, rev, phase, nthreads, nullptr, sum, linf, 0,
+                             nullptr, lo, hi);
 }
 
 // The oracle's float tanh of a non-negative argument (the traction gain's).

@@ -131,6 +131,34 @@ def algorithmic_bytes_per_update(nctl: int, bv: int) -> int:
     return nctl * 64 * bv + 2 * bv + 1
 
 
+# Floating-point operations of the reference's plain backup, counted on its
+# source (solver/src/PeregrineSolver.jl): every +, -, *, / and sqrt, with
+# comparisons, clamps, floor/round, abs and integer index arithmetic free.
+SINCOS_FLOPS = 28          # Float32 sincos: reduction 5, sin kernel 12, cos kernel 11 (double)
+SUBSTEP_FLOPS = 140        # diverged 5, body_accel_sat 85, w x v 4, sincos 28, update 18
+PROBE_FLOPS = 16           # chord point 7, grid coordinates 6, heading wrap 3
+INTERP_FLOPS = 475         # coordinates 12 + wrap 3, fractions 6 + 6, 64 corners x (5 mul + fma 2)
+
+
+def algorithmic_flops_per_lookahead(nsub: int, checks: int, knee: bool = False) -> int:
+    """One `lookahead` of the plain scheme (PeregrineSolver.jl:1100-1153):
+    hs = dt/nsub (1) and the command scaled by the traction gain (3, plus
+    the gain's sqrt, products and quotients, 10, when a knee is set; the
+    tanh is not counted), nsub forward-Euler substeps (:208-221),
+    displacement (3), `checks` swept probes (:1117-1135), the successor's
+    velocity rotated to the field frame (sincos + 6), the 64-corner
+    multilinear read (:399-466) and tau + V (1), min with cap (free)."""
+    return 1 + 3 + (10 if knee else 0) + nsub * SUBSTEP_FLOPS + 3 + checks * PROBE_FLOPS + SINCOS_FLOPS + 6 + \
+        INTERP_FLOPS + 1
+
+
+def algorithmic_flops_per_update(nctl: int, nsub: int, checks: int, knee: bool = False) -> int:
+    """One state-update of the plain scheme: the cell's coordinates (6 axis
+    values, 12), its field velocity rotated into the body frame (sincos + 6),
+    and one lookahead per control (`cell_update`, :1205-1344)."""
+    return 12 + SINCOS_FLOPS + 6 + nctl * algorithmic_flops_per_lookahead(nsub, checks, knee)
+
+
 def adjoint_starts(prob: Problem, n: int = 4096) -> np.ndarray:
     """The adjoint objective's start states: x, y uniform over the field,
     heading uniform, velocity and spin zero, redrawn while the start's
