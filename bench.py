@@ -499,7 +499,7 @@ def main() -> int:
                     help="most exact polish sweeps after the precision form's 16-bit sweeps")
     ap.add_argument("--f64-only", action="store_true")
     ap.add_argument("--adjoint", action="store_true")
-    ap.add_argument("--fd-params", default="0,4,8,9,13,17,19,21,25,27,28,29")
+    ap.add_argument("--fd-params", default=",".join(str(j) for j in range(30)))
     ap.add_argument("--fd-step", type=float, default=1e-3)
     ap.add_argument("--ooc", action="store_true")
     ap.add_argument("--reference-scheme", action="store_true",

@@ -631,6 +631,62 @@ def _setup(cfg):
     return bounds, polys, robot, gcfg, gi, g
 
 
+def _device_budget(headroom: float):
+    """Free device bytes times headroom, or None without a device."""
+    from . import solver
+    try:
+        return solver.gpu_budget(headroom)
+    except (abi.PeregrineError, OSError):
+        return None
+
+
+def _settle(cfg, g: Grid6, m: Model, sp: SolverParams) -> dict:
+    """settle + decompose (Run.jl:316-354, 650-682) with this build's device
+    budget, and the solver parameters with the resolved tau_max."""
+    from . import tiling
+    st = tiling.settle(cfg, g, m, sp.p, sp.ctl, sp.nctl, sp.warm, sp.tau_auto, _device_budget)
+    st["sp"] = replace(sp, p=st["p"])
+    return st
+
+
+def ooc_report(cfg, g: Grid6, dec: dict, iters: int, ntargets: int) -> dict:
+    """PLAN's out_of_core block (Run.jl:721-781): the mode, and when tiled
+    either the tile plan (halo, tile, loaded extent, tiles per round,
+    resident bytes, amplification, reach, store bytes, bytes moved per round)
+    or, when no tiling fits, the advice that would make one fit."""
+    from . import tiling
+    ooc = {"mode": dec["mode"], "budget_bytes": dec["budget"], "whole_bytes": dec["whole_bytes"]}
+    if dec["mode"] != "ooc":
+        return ooc
+    cell_cm = min(float(g.step[0]), float(g.step[1]))
+    tp = dec["tp"]
+    if tp is None:
+        a = dec["advice"]
+        ooc.update(fits=False, reach_cm=dec["dxy_cm"], reach_cells=dec["dxy_cm"] / cell_cm,
+                   halo_cells=[dec["hx"], dec["hy"]], column_bytes=a["column_bytes"],
+                   max_halo_cells=a["max_halo"], min_window_cells=1 + 2 * max(dec["hx"], dec["hy"]),
+                   warm_start_would_help=a["warm_helps"])
+        rounds = max(1, iters)
+        for pre, o in (("suggest", a["tau_max"]), ("comfortable", a["tau_comfortable"])):
+            if o is None:
+                continue
+            ooc.update({pre + "_tau_max": o["tau"], pre + "_halo_cells": o["halo"], pre + "_reach_cm": o["reach_cm"],
+                        pre + "_tile_cells": o["tile"], pre + "_amplification": o["amplification"],
+                        pre + "_value_cost": o["cost"], pre + "_total_s": o["round_s"] * rounds * ntargets})
+        return ooc
+    wt = bool(getc(cfg, "warm_start_tiled", True))
+    sb = tiling.cell_bytes(wt)
+    store = g.ncells * sb
+    per_round = int(round(dec["col"] * sb * tp["nxl"] * tp["nyl"] * -(-g.n[0] // tp["wx"]) *
+                          -(-g.n[1] // tp["wy"]))) + store
+    ooc.update(fits=True, halo_cells=[tp["hx"], tp["hy"]], tile_cells=[tp["wx"], tp["wy"]],
+               loaded_cells=[tp["nxl"], tp["nyl"]], tiles_per_round=tp["ntiles"], resident_bytes=tp["bytes"],
+               amplification=tp["amplification"], reach_cm=tp["dxy_cm"], reach_rad=tp["dh_rad"],
+               reach_cells=tp["dxy_cm"] / cell_cm, store_bytes=store, io_bytes_per_round=per_round, tile_sweeps=1,
+               store_in="pinned host memory")
+    return ooc
+
+
 def _device_fit(g: Grid6):
     """(available, fits whole, need GiB, free GiB) from the library."""
     from . import solver
@@ -758,6 +814,8 @@ def plan(cfg) -> dict:
     if dtype not in card.DTYPES:
         raise SolveError(f"unknown dtype '{dtype}'")
     chunk_elements = int(getc(cfg, "chunk_elements", 1 << 23))
+    st = _settle(cfg, g, m, sp)
+    sp, dec, rtau = st["sp"], st["dec"], st["rec"]
     occ = build_occupancy(g, polys, sp.margin, robot, gi.hsub, sp.clearance, bounds=bounds,
                           wall_clearance_cm=gi.wall_clearance)
     osum = occupancy_summary(g, occ)
@@ -773,13 +831,13 @@ def plan(cfg) -> dict:
     vmax = float(getc(gcfg, "vmax", DEFAULT_VMAX))
     wmax = float(getc(gcfg, "wmax", DEFAULT_WMAX))
     bpc = device_bytes_per_cell(sp.warm)
-    dev_budget = free * 2 ** 30 if avail and free > 0 else 0.0
+    dev_budget = float(dec["budget"]) if dec["budget"] > 0 else 0.0
     span_x = float(g.step[0]) * (g.n[0] - 1)
     span_y = float(g.step[1]) * (g.n[1] - 1)
     sugg = suggest_resolution(span_x, span_y, reach_cm, vmax, wmax, card.DTYPES[dtype][0], len(names), rec_budget,
                               vram_bytes=dev_budget if dev_budget > 0 else math.inf, bytes_per_cell=bpc)
     incore = None
-    if avail and not fits and dev_budget > 0:
+    if dec["mode"] == "ooc" and dev_budget > 0:
         incore = coarsen_to_fit(span_x, span_y, vmax, wmax,
                                 {"xy_cm": float(min(g.step[0], g.step[1])),
                                  "heading_deg": math.degrees(float(g.step[2])), "v_cm_s": float(g.step[3]),
@@ -811,17 +869,22 @@ def plan(cfg) -> dict:
         "dtype": dtype, "elem_bytes": eb, "bytes_per_target": cells * eb, "n_targets": len(names),
         "bytes_total": cells * eb * len(names), "chunks_per_target": -(-cells // chunk_elements),
         "gpu_available": avail, "gpu_fits": bool(fits), "gpu_need_gb": need, "gpu_free_gb": free,
-        "out_of_core": {"mode": ("incore" if fits else "ooc") if avail else None},
+        "out_of_core": ooc_report(cfg, g, dec, sp.iters, len(names)),
         "grid_balance": bal["ratio"], "dv_distance_cm": bal["dv_distance_cm"],
         "cell_cm": float(min(g.step[0], g.step[1])), "cell_cm_s": float(min(g.step[3], g.step[4])),
-        "controls": sp.nctl, "unsupported_options": unsupported_options(sp, avail and not fits),
+        "controls": sp.nctl, "unsupported_options": unsupported_options(sp, dec["mode"] == "ooc"),
         "recommend": {
             "size_budget_bytes": rec_budget, "resolution": sugg, "in_core": incore,
             "tau_applied": float(sp.p.tau_max), "tau_applied_cost": value_cost(sp.p.tau_max, TAU_REF),
             "tau_auto": sp.tau_auto, "tau_reference": TAU_REF,
-            "tau_pinned_waste": (not sp.tau_auto) and bool(fits) and sp.p.tau_max < TAU_REF,
-            "tau_reason": "the grid is held whole in HBM, so the full reference horizon costs no halo"
-                          if fits else None,
+            "tau_pinned_waste": (not sp.tau_auto) and dec["mode"] != "ooc" and sp.p.tau_max < TAU_REF,
+            "tau_max": rtau["tau_max"] if rtau else None, "tau_value_cost": rtau["cost"] if rtau else None,
+            "tau_current": rtau["current"] if rtau else float(sp.p.tau_max),
+            "tau_current_cost": rtau["current_cost"] if rtau else None,
+            "tau_io_minor": rtau["io_minor"] if rtau else None,
+            "tau_reason": rtau["reason"] if rtau else None,
+            "tau_options": [{k: o[k] for k in ("tau", "cost", "round_s", "compute_s", "io_s", "amplification",
+                                               "reach_cm")} for o in (rtau["options"] if rtau else [])],
         },
         "runtime": runtime_estimate(g, sp, int(getc(cfg, "escape_iterations", 60))
                                     if bool(getc(cfg, "escape", True)) else 0, len(names)),
@@ -889,7 +952,7 @@ def run_solve(cfg) -> dict:
     if chunk_elements <= 0 or chunk_elements & (chunk_elements - 1):
         raise SolveError("chunk_elements must be a power of two")
     sp = solver_params(cfg, g)
-    p, cap = sp.p, sp.cap
+    cap = sp.cap
     backend = str(getc(cfg, "backend", "auto"))
     if backend not in ("auto", "cuda", "hip"):
         raise SolveError(f"backend '{backend}' is not built: this solver runs the sweep on the GPU only")
@@ -926,8 +989,13 @@ def run_solve(cfg) -> dict:
                 lines.append(f"  '{t['name']}' " + "; ".join(t["blocked_by"]))
         raise SolveError("these targets are states the robot cannot be in, so they would seed nothing and the "
                          "whole table would come out unreachable:\n" + "\n".join(lines))
-    fits, _, _ = solver.gpu_fits(g)
-    ooc = not fits
+    st = _settle(cfg, g, m, sp)
+    sp, dec = st["sp"], st["dec"]
+    p = sp.p
+    if dec["mode"] == "cpu":
+        raise SolveError("no HIP device is visible and no vram_budget_bytes is set: this solver runs the sweep "
+                         "on the GPU only")
+    ooc = dec["mode"] == "ooc"
     bad = unsupported_options(sp, ooc)
     if bad:
         raise SolveError("the grid does not fit the device whole, and the tiled driver does not run "
@@ -945,10 +1013,23 @@ def run_solve(cfg) -> dict:
                 + (ntargets - ti) * (sp.iters * solve_unit + esc_units * esc_unit))
 
     entries = []
-    kw = dict(ooc_budget=solver.gpu_budget()) if ooc else {}
+    # tile_cells (this build's key): the tiled driver's interior (x, y,
+    # heading cells); 0s let the budget choose the largest that fits
+    tile = [int(v) for v in getc(cfg, "tile_cells", [0, 0, 0])]
+    if len(tile) != 3 or min(tile) < 0:
+        raise SolveError(f"tile_cells must be three non-negative cell counts (x, y, heading), got {tile}")
+    kw = dict(ooc_budget=int(dec["budget"]), ooc_tile=tuple(tile)) if ooc else {}
+    tiling = None
     with solver.HipSolver(g, m, p, sp.ctl, occ, **kw) as s:
         if sp.warm:
             s.set_warm_start(True)
+        if ooc:
+            tile, ntiles, dev_bytes, amp = s.ooc_info()
+            tiling = {"tile_cells": list(tile), "tiles_per_round": ntiles, "tile_sweeps": 1,
+                      "device_bytes": dev_bytes, "budget_bytes": int(dec["budget"]), "amplification": amp,
+                      "windows": "each tile's exact read range", "warm_start_tiled": sp.warm,
+                      "store": "pinned host memory"}
+            progress(phase="tiling", **tiling)
         for ti0, st in enumerate(states):
             ti = ti0 + 1
             seeds = target_cells(g, st, sp.ttol)
@@ -989,22 +1070,12 @@ def run_solve(cfg) -> dict:
             escf = info.escaped / info.cells
             dead = info.cells - info.reached
             eou = info.escaped / dead if dead > 0 else 0.0
-            entries.append({
-                "index": ti0, "name": names[ti0], "state": [float(v) for v in st],
-                "file_pattern": f"TABLES/T{ti0:02d}C%04d.BIN", "n_chunks": info.n_chunks, "bytes": info.bytes,
-                "sha256": info.sha256, "iterations": done, "final_delta": last, "reached_frac": reached,
-                "escape_frac": escf, "escape_of_unreached_frac": eou, "escape_iterations": esc_done,
-                "escape_final_delta": esc_delta})
+            entries.append(card_entry(ti0, names[ti0], st, info, done, last, esc_done, esc_delta))
             progress(phase="target_done", target=ti0, target_name=names[ti0], chunks=info.n_chunks,
                      bytes=info.bytes, reached_frac=reached, escape_frac=escf, escape_of_unreached_frac=eou,
                      elapsed_s=time.time() - t0)
-    man = manifest(cfg, g, gi, bounds, sp, dtype, scale, chunk_elements, do_escape, esc_iters, esc_tol,
-                   escape_scale, robot, ooc, entries, reg_path, field_path)
-    with open(os.path.join(out_dir, "MANIFEST.JSON"), "w") as fh:
-        json.dump(man, fh, indent=2)
-    mj = model_json(m, reg_path, zeroed, cfg)
-    with open(os.path.join(out_dir, "MODEL.JSON"), "w") as fh:
-        json.dump(mj, fh, indent=2)
+    man, mj = write_card_files(out_dir, cfg, g, gi, bounds, sp, dtype, scale, chunk_elements, do_escape, esc_iters,
+                               esc_tol, escape_scale, robot, ooc, entries, reg_path, field_path, m, zeroed, tiling)
     progress(phase="model", file="MODEL.JSON", nonzero_blocks=mj["nonzero_blocks"],
              honing_omega=mj["honing"]["omega"], honing_budget_frac=mj["honing"]["config"]["budget_frac"],
              honing_bound_by=mj["honing"]["omega_limits"]["bound_by"])
@@ -1012,8 +1083,33 @@ def run_solve(cfg) -> dict:
     return man
 
 
+def card_entry(ti0: int, name: str, st, info, done: int, last: float, esc_done: int, esc_delta: float) -> dict:
+    """One target's MANIFEST.JSON entry from its written table (Run.jl:2040-2060)."""
+    reached = info.reached / info.cells
+    dead = info.cells - info.reached
+    return {"index": ti0, "name": name, "state": [float(v) for v in st],
+            "file_pattern": f"TABLES/T{ti0:02d}C%04d.BIN", "n_chunks": info.n_chunks, "bytes": info.bytes,
+            "sha256": info.sha256, "iterations": done, "final_delta": last, "reached_frac": reached,
+            "escape_frac": info.escaped / info.cells,
+            "escape_of_unreached_frac": info.escaped / dead if dead > 0 else 0.0, "escape_iterations": esc_done,
+            "escape_final_delta": esc_delta}
+
+
+def write_card_files(out_dir, cfg, g, gi, bounds, sp, dtype, scale, chunk_elements, do_escape, esc_iters, esc_tol,
+                     escape_scale, robot, ooc, entries, reg_path, field_path, m, zeroed, tiling=None):
+    """MANIFEST.JSON and MODEL.JSON beside the written TABLES/."""
+    man = manifest(cfg, g, gi, bounds, sp, dtype, scale, chunk_elements, do_escape, esc_iters, esc_tol,
+                   escape_scale, robot, ooc, entries, reg_path, field_path, tiling)
+    with open(os.path.join(out_dir, "MANIFEST.JSON"), "w") as fh:
+        json.dump(man, fh, indent=2)
+    mj = model_json(m, reg_path, zeroed, cfg)
+    with open(os.path.join(out_dir, "MODEL.JSON"), "w") as fh:
+        json.dump(mj, fh, indent=2)
+    return man, mj
+
+
 def manifest(cfg, g, gi, bounds, sp, dtype, scale, chunk_elements, do_escape, esc_iters, esc_tol, escape_scale,
-             robot, ooc, entries, reg_path, field_path) -> dict:
+             robot, ooc, entries, reg_path, field_path, tiling=None) -> dict:
     """MANIFEST.JSON, schema 1 (Run.jl:1998-2117, TABLE_FORMAT.md section 2)."""
     p = sp.p
     nbytes, sentinel, _, ebase = card.DTYPES[dtype]
@@ -1062,7 +1158,7 @@ def manifest(cfg, g, gi, bounds, sp, dtype, scale, chunk_elements, do_escape, es
             "substep_max": p.hmax, "adaptive_checks": p.adaptive_checks, "rk2": p.rk2, "simplex": p.simplex,
             "interpolant": "kuhn_simplex_7pt" if p.simplex else "multilinear_64pt",
             "velocity_clamp": p.vclamp, "integrator": "midpoint_rk2" if p.rk2 else "semi_implicit_euler",
-            "driver": "tiled" if ooc else "whole_grid", "tiling": None,
+            "driver": "tiled" if ooc else "whole_grid", "tiling": tiling,
             "sweep": "jacobi",
             "sweep_checks": p.checks, "zero_c": bool(getc(cfg, "zero_c", True)), "margin_cm": sp.margin,
             "clearance_cm": sp.clearance, "wall_clearance_cm": gi.wall_clearance, "value_cap_s": sp.cap,

@@ -39,13 +39,18 @@ def test_device_adjoint_matches_cpu_fp64_adjoint(pg, orc):
     assert np.max(np.abs(grad - g_cpu)) <= 1e-8 * scale, (grad, g_cpu)
 
 
+NAMES = ([f"B{r}{c}" for r in (1, 2, 3) for c in (1, 2, 3)] + [f"A{r}{c}" for r in (1, 2, 3) for c in (1, 2, 3)]
+         + ["q1", "q2", "q3", "S11", "S22", "S33", "D11", "D22", "D33", "c1", "c2", "c3"])
+
+
 def test_device_adjoint_against_central_differences(pg):
+    """All 30 parameters, h = 1e-3 of the parameter (1e-5 for a zero one)."""
     prob = pg.configs.problem(5, shape=(12, 12, 8, 5, 5, 7))
     starts = pg.configs.adjoint_starts(prob, 1024)
     L, grad, it, _ = fp64_L(pg, prob, prob.model, starts)
     th = prob.model.theta()
-    for j in (0, 4, 8, 9, 13, 17, 19, 21, 25, 27, 29):
-        h = 1e-4 * max(abs(th[j]), 1e-2)
+    for j in range(30):
+        h = 1e-3 * max(abs(th[j]), 1e-2)
         tp, tm = th.copy(), th.copy()
         tp[j] += h
         tm[j] -= h
@@ -54,7 +59,7 @@ def test_device_adjoint_against_central_differences(pg):
         Lp = fp64_L(pg, prob, prob.model.with_theta(tp), starts)[0]
         Lm = fp64_L(pg, prob, prob.model.with_theta(tm), starts)[0]
         fd = (Lp - Lm) / h_eff
-        print(f"theta[{j}] {th[j]:+.4g}: adjoint {grad[j]:+.6e} fd {fd:+.6e}")
+        print(f"theta[{j}] {NAMES[j]} {th[j]:+.4g}: adjoint {grad[j]:+.6e} fd {fd:+.6e}", flush=True)
         assert abs(grad[j] - fd) <= 1e-3 * max(abs(fd), 1e-3 * np.max(np.abs(grad))), (j, grad[j], fd)
 
 

@@ -88,6 +88,42 @@ def test_cli_tables_equal_the_oracle(pg, orc, solved):
         assert escaped > 0
 
 
+def test_cli_card_is_the_card_the_reference_checker_passed(solved):
+    """tests/golden/cli/card_verdict.json holds the reference's verifier
+    (wizard/verify_tables.py, run unchanged on the host-written card of this
+    same job) passing, and that card's digests: the device card has them."""
+    cfg, run = solved
+    verdict = json.loads((GOLD / "card_verdict.json").read_text())
+    assert verdict["exit_status"] == 0 and any("PASS" in ln for ln in verdict["stdout"])
+    man = json.load(open(os.path.join(cfg["out_dir"], "MANIFEST.JSON")))
+    assert [t["sha256"] for t in man["targets"]] == [t["sha256"] for t in verdict["targets"]]
+    assert man["grid"]["n"] == verdict["grid_n"]
+
+
+def test_forced_tiling_writes_the_same_card(pg, solved, tmp_path):
+    """out_of_core = "always" with a device budget and tiles of 4 x 4 cells
+    over the whole heading axis: the tiled driver (windows in pinned host
+    memory) writes every chunk byte for byte as the in-HBM solve did, and
+    the manifest records the tiling."""
+    cfg, run = solved
+    cfg2, path2 = config(tmp_path, iterations=ITERS, tolerance=-1.0, escape_iterations=ESC_ITERS,
+                         escape_tolerance=-1.0, out_of_core="always", vram_budget_bytes=1 << 30,
+                         tile_cells=[4, 4, 8], tau_max=0.5)
+    print(f"cli tiled solve: {path2}", flush=True)
+    assert run.main([str(path2), "solve"]) == 0
+    man = json.load(open(os.path.join(cfg["out_dir"], "MANIFEST.JSON")))
+    man2 = json.load(open(os.path.join(cfg2["out_dir"], "MANIFEST.JSON")))
+    assert man["solver"]["driver"] == "whole_grid" and man2["solver"]["driver"] == "tiled"
+    til = man2["solver"]["tiling"]
+    assert til["tiles_per_round"] > 1 and til["tile_cells"][:2] == [4, 4]
+    for t, t2 in zip(man["targets"], man2["targets"]):
+        assert t["sha256"] == t2["sha256"] and t["n_chunks"] == t2["n_chunks"]
+        for c in range(t["n_chunks"]):
+            a = open(os.path.join(cfg["out_dir"], t["file_pattern"] % c), "rb").read()
+            b = open(os.path.join(cfg2["out_dir"], t2["file_pattern"] % c), "rb").read()
+            assert a == b, f"target {t['name']} chunk {c} differs"
+
+
 class Card:
     """A card reader over the manifest (TABLE_FORMAT.md sections 3-6)."""
 
