@@ -98,7 +98,7 @@ def load_model(path: str):
     with open(path, "rb") as fh:
         cfg = tomli.load(fh)
     if "mecanum_basis" not in cfg:
-        raise SolveError(f"{path} has no [mecanum_basis]; is it a drivetrain_fit.toml?")
+        raise SolveError(f"{path}: no [mecanum_basis] table; the solve reads its model from a drivetrain fit's mecanum_basis")
     mb = cfg["mecanum_basis"]
 
     def mat(name):
@@ -107,10 +107,11 @@ def load_model(path: str):
         return tuple(float(mb[name][r][c]) for r in range(3) for c in range(3))
 
     if mb.get("traction_knee") is None:
-        raise SolveError("regression has no traction_knee; refit with fit_drivetrain.py --traction-knee auto")
+        raise SolveError("the drivetrain fit carries no traction_knee; the gains assume a saturated command, so fit one "
+                         "(fit_drivetrain.py --traction-knee auto)")
     knee = F32(mb["traction_knee"])
     if math.isnan(knee) or knee <= 0:
-        raise SolveError(f"regression traction_knee is {mb['traction_knee']}; a positive knee is required")
+        raise SolveError(f"traction_knee {mb['traction_knee']} is not a positive number; the command saturation needs one")
     eps = tuple(float(v) for v in mb["coulomb_eps"]) if "coulomb_eps" in mb else (5.0, 5.0, 0.15)
     units = cfg.get("units", {}).get("distance", "cm")
     if units != "cm":
@@ -122,11 +123,11 @@ def load_model(path: str):
 
 def _poly(pts, what: str) -> np.ndarray:
     if len(pts) < 3:
-        raise SolveError(f"{what} has fewer than 3 vertices")
+        raise SolveError(f"{what}: a polygon needs three or more vertices")
     M = np.empty((2, len(pts)), np.float64)
     for i, p in enumerate(pts):
         if len(p) != 2:
-            raise SolveError(f"{what} has a vertex that is not [x, y]")
+            raise SolveError(f"{what}: every vertex has to be an [x, y] pair")
         M[0, i], M[1, i] = float(p[0]), float(p[1])
     return M
 
@@ -151,11 +152,11 @@ def load_targets(path: str):
     for t in cfg["targets"]:
         s = t["state"]
         if len(s) != 6:
-            raise SolveError(f"target '{t.get('name', '?')}' state must be 6 numbers")
+            raise SolveError(f"target '{t.get('name', '?')}': a state is six numbers (x, y, heading, vx, vy, omega)")
         names.append(str(t.get("name", f"target{len(names)}")))
         states.append(tuple(F32(v) for v in s))
     if not states:
-        raise SolveError(f"{path} lists no targets")
+        raise SolveError(f"{path}: the target list is empty")
     return names, states, cfg
 
 
@@ -193,7 +194,7 @@ def feasible_bounds(bounds, robot, clearance: float, nh: int, substeps: int):
 def axis_samples(span: float, cell: float):
     """Samples covering `span` at spacing <= `cell`, and the span they cover."""
     if not cell > 0:
-        raise SolveError(f"resolution must be positive, got {cell}")
+        raise SolveError(f"a cell size of {cell} is not positive")
     n = max(2, math.ceil(span / cell - 1.0e-9) + 1)
     return n, (n - 1) * cell
 
@@ -201,7 +202,7 @@ def axis_samples(span: float, cell: float):
 def symmetric_samples(limit: float, cell: float):
     """An odd sample count over [-limit, limit] with a node at zero."""
     if not cell > 0:
-        raise SolveError(f"resolution must be positive, got {cell}")
+        raise SolveError(f"a cell size of {cell} is not positive")
     half = max(1, math.ceil(limit / cell - 1.0e-9))
     return 2 * half + 1, half * cell
 
@@ -383,7 +384,7 @@ def check_targets(g: Grid6, occ: np.ndarray, names, states, ttol, bounds, polys,
             th = float(s[2])
             bxlo, bxhi, bylo, byhi = wall_box(bounds, robot, wall_clearance, th)
             if float(s[0]) < bxlo or float(s[0]) > bxhi or float(s[1]) < bylo or float(s[1]) > byhi:
-                why.append("the footprint crosses the field boundary")
+                why.append("its footprint reaches past the field edge (wall clearance included)")
             point = robot is None or robot.shape[1] < 3
             body = None
             if not point:
@@ -393,11 +394,11 @@ def check_targets(g: Grid6, occ: np.ndarray, names, states, ttol, bounds, polys,
                 hit = point_too_close(float(s[0]), float(s[1]), P, clearance) if point \
                     else polys_too_close(body, P, clearance)
                 if hit:
-                    why.append(f"it is within {clearance} cm of obstacle {pi_}")
+                    why.append(f"it comes closer than {clearance} cm to obstacle {pi_}")
             if not why:
-                why.append(f"it is legal at its own heading but its heading bin is not. The bin spans "
-                           f"{math.degrees(float(g.step[2])):.1f} deg and counts as blocked if any angle in it "
-                           f"is, so more heading samples would separate the two")
+                why.append(f"its own heading is clear but its heading bin is not: a bin of "
+                           f"{math.degrees(float(g.step[2])):.1f} deg is blocked when any angle in it is, and "
+                           f"finer heading bins would tell the two apart")
         out.append({"index": i, "name": names[i], "seed_cells": int(seeds.size), "free_seed_cells": nfree,
                     "ok": nfree > 0, "off_axes": off, "blocked_by": why})
     return out
@@ -420,8 +421,8 @@ def inv3(M):
     (a, b, c), (d, e, f), (g, h, i) = M
     det = a * (e * i - f * h) - b * (d * i - f * g) + c * (d * h - e * g)
     if not (math.isfinite(det) and abs(det) > 1e-12):
-        raise SolveError(f"A_u is singular (det = {det}); the drivetrain fit does not give the three axes "
-                         f"independent authority, so no controller can be derived from it")
+        raise SolveError(f"the command block of the fit is singular (determinant {det}): the three axes lack independent "
+                         f"control authority, and no approach controller exists for such a model")
     return [[(e * i - f * h) / det, (c * h - b * i) / det, (b * f - c * e) / det],
             [(f * g - d * i) / det, (a * i - c * g) / det, (c * d - a * f) / det],
             [(d * h - e * g) / det, (b * g - a * h) / det, (a * e - b * d) / det]]
@@ -456,13 +457,12 @@ def honing_gains(m: Model, cfg) -> dict:
     scale = float(getc(cfg, "honing_bandwidth_scale", 1.0))
     ishare = float(getc(cfg, "honing_integral_share", 0.25))
     if not (budget > 0 and frac > 0 and hz > 0 and scale > 0 and 0 < ishare < 1):
-        raise SolveError(f"honing config must be positive (honing_budget={budget}, honing_budget_frac={frac}, "
-                         f"honing_loop_hz={hz}, honing_bandwidth_scale={scale}, honing_integral_share={ishare}), "
-                         f"with the integral share below 1")
+        raise SolveError(f"the honing settings need positive values and an integral share under 1; got "
+                         f"honing_budget={budget}, honing_budget_frac={frac}, honing_loop_hz={hz}, "
+                         f"honing_bandwidth_scale={scale}, honing_integral_share={ishare}")
     if budget > 1:
-        raise SolveError(f"honing_budget is {budget}, outside the octahedron |fwd| + |strafe| + |turn| <= 1; the "
-                         f"wheels would clip the command the gains are sized against (honing_budget_frac={frac} "
-                         f"at a knee of {knee})")
+        raise SolveError(f"a honing_budget of {budget} exceeds the command set |fwd| + |strafe| + |turn| <= 1, so the "
+                         f"mixer would clip what the gains are sized for (honing_budget_frac={frac}, knee {knee})")
     r_op = budget / knee
     tgain = 1.0 if r_op < 1e-6 else math.tanh(r_op) / r_op
     Binv = inv3([[tgain * B[r][c] for c in range(3)] for r in range(3)])
@@ -500,8 +500,8 @@ def honing_gains(m: Model, cfg) -> dict:
     w_loop = 2 * math.pi * hz / 10
     w = scale * min(w_sat, w_loop)
     if not (math.isfinite(w) and w > 0):
-        raise SolveError(f"honing bandwidth came out as {w}; check honing_loop_hz, honing_budget and "
-                         f"honing_bandwidth_scale are positive")
+        raise SolveError(f"the honing bandwidth evaluates to {w}; honing_loop_hz, honing_budget and "
+                         f"honing_bandwidth_scale must all be positive")
     Kp, Ki, Kd, p, kd = gains_at(w)
     ilim = []
     for c in range(3):
@@ -608,12 +608,11 @@ def honing_only(cfg, write_dir=None) -> dict:
     if write_dir is not None:
         path = os.path.join(str(write_dir), "MODEL.JSON")
         if not os.path.isfile(path):
-            raise SolveError(f"no MODEL.JSON in {write_dir}; there is nothing to update there, so solve the run first")
+            raise SolveError(f"{write_dir} holds no MODEL.JSON to update; run the solve that writes it first")
         old = _readjson(path)
         if str(getc(old, "regression_sha256", "")) != mj["regression_sha256"]:
-            raise SolveError(f"{path} was written from a different regression than {reg_path}; its tables were "
-                             f"solved from that one, so writing these gains beside them would ship a model the "
-                             f"tables disagree with")
+            raise SolveError(f"{path} belongs to tables solved from a different regression than {reg_path}; gains "
+                             f"from this fit would not match them")
         with open(path, "w") as fh:
             json.dump(mj, fh, indent=2)
     return mj["honing"]
@@ -812,7 +811,7 @@ def plan(cfg) -> dict:
     sp = solver_params(cfg, g)
     dtype = str(getc(cfg, "dtype", "u16"))
     if dtype not in card.DTYPES:
-        raise SolveError(f"unknown dtype '{dtype}'")
+        raise SolveError(f"dtype '{dtype}' is not one of {', '.join(card.DTYPES)}")
     chunk_elements = int(getc(cfg, "chunk_elements", 1 << 23))
     st = _settle(cfg, g, m, sp)
     sp, dec, rtau = st["sp"], st["dec"], st["rec"]
@@ -946,11 +945,11 @@ def run_solve(cfg) -> dict:
     cells = g.ncells
     dtype = str(getc(cfg, "dtype", "u16"))
     if dtype not in card.DTYPES:
-        raise SolveError(f"unknown dtype '{dtype}'")
+        raise SolveError(f"dtype '{dtype}' is not one of {', '.join(card.DTYPES)}")
     scale = float(getc(cfg, "scale", card.DTYPES[dtype][2]))
     chunk_elements = int(getc(cfg, "chunk_elements", 1 << 23))
     if chunk_elements <= 0 or chunk_elements & (chunk_elements - 1):
-        raise SolveError("chunk_elements must be a power of two")
+        raise SolveError(f"chunk_elements {chunk_elements} is not a power of two")
     sp = solver_params(cfg, g)
     cap = sp.cap
     backend = str(getc(cfg, "backend", "auto"))
@@ -962,9 +961,9 @@ def run_solve(cfg) -> dict:
     escape_scale = card.escape_scale_for(dtype, cap) if do_escape else 0.0
     ebase = card.DTYPES[dtype][3]
     if do_escape and ebase > 0 and cap / scale >= ebase:
-        raise SolveError(f"value_cap {cap} s at scale {scale} needs codes up to {round(cap / scale)}, which runs "
-                         f"into the escape band at {ebase}. Lower value_cap below {round(ebase * scale, 2)} s, use "
-                         f"a coarser scale, or set escape to false")
+        raise SolveError(f"with scale {scale}, value_cap {cap} s encodes as {round(cap / scale)}, inside the escape "
+                         f"codes that start at {ebase}: keep value_cap under {round(ebase * scale, 2)} s, coarsen "
+                         f"the scale, or turn the escape pass off")
     progress(phase="setup", cells=cells, n=list(g.n), controls=sp.nctl, dtype=dtype, escape=do_escape,
              escape_scale=escape_scale, escape_base=ebase, escape_codes=card.escape_codes(dtype),
              bytes_per_target=cells * card.DTYPES[dtype][0], iterations=sp.iters,
@@ -983,12 +982,12 @@ def run_solve(cfg) -> dict:
             if t["off_axes"]:
                 span = ", ".join(f"{AXES[k]} {float(g.lo[k]):.1f}..{float(g.lo[k]) + float(g.step[k]) * (g.n[k] - 1):.1f}"
                                  for k in (0, 1, 3, 4, 5))
-                lines.append(f"  '{t['name']}' is outside the table on {', '.join(t['off_axes'])} "
-                             f"(the table spans {span})")
+                lines.append(f"  '{t['name']}' lies off the table on {', '.join(t['off_axes'])}; the table "
+                             f"covers {span}")
             else:
                 lines.append(f"  '{t['name']}' " + "; ".join(t["blocked_by"]))
-        raise SolveError("these targets are states the robot cannot be in, so they would seed nothing and the "
-                         "whole table would come out unreachable:\n" + "\n".join(lines))
+        raise SolveError("no seed cell of these targets is a state the robot can be in, so every table entry "
+                         "would come out unreachable:\n" + "\n".join(lines))
     st = _settle(cfg, g, m, sp)
     sp, dec = st["sp"], st["dec"]
     p = sp.p
