@@ -343,6 +343,28 @@ int pg_release_f64(pg_solver* s);
 int pg_adjoint(pg_solver* s, const double* starts, int64_t nstarts, int32_t max_iters, double tol,
                double* dL_dtheta, double* L, int32_t* iters);
 
+/* One Jacobi sweep V' = min(V, T V) over caller-owned device buffers that
+ * allocates nothing. No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). It runs the global-gather sweep of
+ * pg_sweep with the same results bit for bit (V, summed decrease, sup-norm),
+ * for any scheme pg_sweep runs on that kernel except the warm-start store;
+ * `phase` picks the rotating slice's offset as in pg_sweep. The caller
+ * passes, as device pointers, occ_dev (n0*n1*n2 bytes, nonzero = blocked),
+ * Vin_dev and Vout_dev (ncell floats each, distinct buffers); ctl_soa is a
+ * host array (3*nctl floats, entry 0 coast). Seeds are not reasserted: a
+ * Jacobi backup never raises a seed held at 0. The scratch it needs (control
+ * tables, per-workgroup partial sums, the reduction) is a static device array
+ * of PG_NOALLOC_SCRATCH_BYTES on each device, held until the process ends and
+ * shared by every call on that device: calls that overlap on one device race.
+ * Grids above PG_NOALLOC_MAX_CELLS cells are refused (PG_ERR_ARG) before
+ * anything runs. Returns when the sweep is done. */
+#define PG_NOALLOC_SCRATCH_BYTES (32 << 20)
+#define PG_NOALLOC_MAX_CELLS ((int64_t)(((PG_NOALLOC_SCRATCH_BYTES) - 16384) / 8) * 256)
+int pg_sweep_noalloc(pg_ctx* ctx, const pg_grid* g, const pg_model* m, const pg_params* p,
+                     const float* ctl_soa, int32_t nctl, const uint8_t* occ_dev, const float* Vin_dev,
+                     float* Vout_dev, int32_t phase, double* sum_decrease, float* linf);
+
 #ifdef __cplusplus
 }
 #endif

@@ -132,7 +132,13 @@ SIGNATURES = {
     "pg_solver_set_warm_start": (ctypes.c_int, [_P, _I32]),
     "pg_download_warm_start": (ctypes.c_int, [_P, _P]),
     "pg_solver_ncells": (ctypes.c_int, [_P, ctypes.POINTER(_I64)]),
+    "pg_sweep_noalloc": (ctypes.c_int, [_P, _P, _P, _P, _P, _I32, _P, _P, _P, _I32, ctypes.POINTER(ctypes.c_double),
+                                        ctypes.POINTER(ctypes.c_float)]),
 }
+
+# pg_sweep_noalloc's static scratch per device, and the largest grid it takes
+PG_NOALLOC_SCRATCH_BYTES = 32 << 20
+PG_NOALLOC_MAX_CELLS = ((PG_NOALLOC_SCRATCH_BYTES - 16384) // 8) * 256
 
 
 class PeregrineError(RuntimeError):

@@ -491,6 +491,15 @@ int pg_create(int32_t device, pg_ctx** out) {
         delete ctx;
         return fail(PG_ERR_HIP, "hipStreamCreate failed: %s", hipGetErrorString(e));
     }
+    // this device's copy of the no-allocation scratch, looked up once here
+    void* sp = nullptr;
+    e = pgk_noalloc_scratch(&sp);
+    if (e != hipSuccess || !sp) {
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return fail(PG_ERR_HIP, "scratch address lookup failed: %s", hipGetErrorString(e));
+    }
+    ctx->noalloc = static_cast<unsigned char*>(sp);
     *out = ctx;
     return PG_OK;
 }
@@ -1361,6 +1370,80 @@ int pg_solver_kernel_info(pg_solver* s, int32_t* kernel, double* window_coverage
 int pg_last_sweep_ms(pg_solver* s, float* ms) {
     if (!s || !ms) return fail(PG_ERR_ARG, "null argument");
     *ms = s->last_ms;
+    return PG_OK;
+}
+
+int pg_sweep_noalloc(pg_ctx* ctx, const pg_grid* g, const pg_model* m, const pg_params* p,
+                     const float* ctl_soa, int32_t nctl, const uint8_t* occ_dev, const float* Vin_dev,
+                     float* Vout_dev, int32_t phase, double* sum_decrease, float* linf) {
+    if (!ctx || !g || !m || !p || !ctl_soa || !occ_dev || !Vin_dev || !Vout_dev)
+        return fail(PG_ERR_ARG, "null argument");
+    if (Vin_dev == Vout_dev) return fail(PG_ERR_ARG, "Vin_dev and Vout_dev must be distinct (a Jacobi sweep)");
+    int rc = check_grid(g);
+    if (rc) return rc;
+    if (nctl < 2 || nctl > 255) return fail(PG_ERR_ARG, "nctl must be in [2, 255], got %d", nctl);
+    if (ctl_soa[0] != 0.0f || ctl_soa[nctl] != 0.0f || ctl_soa[2 * nctl] != 0.0f)
+        return fail(PG_ERR_ARG, "control 0 must be coast (0, 0, 0)");
+    rc = check_params(p, nctl);
+    if (rc) return rc;
+    for (int k = 0; k < 3; ++k)
+        if (!(m->eps[k] > 0.0f)) return fail(PG_ERR_ARG, "coulomb eps[%d] must be positive", k);
+    const int64_t ncell = grid_cells(*g);
+    if (ncell > PG_NOALLOC_MAX_CELLS)
+        return fail(PG_ERR_ARG, "%lld cells: the no-allocation sweep's scratch holds the partial sums of at most "
+                    "%lld cells", (long long)ncell, (long long)PG_NOALLOC_MAX_CELLS);
+    if (!ctx->noalloc) return fail(PG_ERR_STATE, "the context has no scratch address");
+    HIPCHECK(hipSetDevice(ctx->device));
+
+    // the constants a solver of this grid would hold; no device memory
+    pg_solver t;
+    t.ctx = ctx;
+    t.g = *g;
+    t.m = *m;
+    t.p = *p;
+    t.nctl = nctl;
+    t.ncell = ncell;
+    build_consts(&t);
+    t.K.off = slice_offset(&t, phase);
+    // B * u_sat and the raw commands, as pg_solver_create folds them
+    float bu[3 * 255], ur[3 * 255];
+    for (int k = 0; k < nctl; ++k) {
+        const float u1r = ctl_soa[k], u2r = ctl_soa[k + nctl], u3r = ctl_soa[k + 2 * nctl];
+        const float gn = traction_gain(u1r, u2r, u3r, m->knee);
+        const float u1 = u1r * gn, u2 = u2r * gn, u3 = u3r * gn;
+        for (int r = 0; r < 3; ++r) {
+            float a = m->B[3 * r] * u1;
+            a = a + m->B[3 * r + 1] * u2;
+            a = a + m->B[3 * r + 2] * u3;
+            bu[3 * k + r] = a;
+        }
+        for (int a = 0; a < 3; ++a) ur[3 * k + a] = ctl_soa[k + a * nctl];
+    }
+    unsigned char* S = ctx->noalloc;
+    double* d_sum = reinterpret_cast<double*>(S);              // + kReduceBlocks partials
+    unsigned* d_linf = reinterpret_cast<unsigned*>(S + 8448);
+    float* d_bu = reinterpret_cast<float*>(S + 8704);
+    float* d_uraw = reinterpret_cast<float*>(S + 11776);
+    double* d_part = reinterpret_cast<double*>(S + 16384);
+    static_assert(8 * (1 + kReduceBlocks) <= 8448, "reduction scratch overlaps");
+    static_assert(8704 + 3 * 255 * 4 <= 11776 && 11776 + 3 * 255 * 4 <= 16384, "control tables overlap");
+    hipStream_t st = ctx->stream;
+    HIPCHECK(hipMemcpyAsync(d_bu, bu, 3 * (size_t)nctl * sizeof(float), hipMemcpyHostToDevice, st));
+    HIPCHECK(hipMemcpyAsync(d_uraw, ur, 3 * (size_t)nctl * sizeof(float), hipMemcpyHostToDevice, st));
+    HIPCHECK(pgk_launch_zero_u32(d_linf, st));
+    int64_t nblk = 0;
+    HIPCHECK(pgk_launch_sweep(t.K, Vin_dev, Vout_dev, occ_dev, d_bu, d_uraw, nullptr, nullptr, d_part, d_linf, 1,
+                              st, &nblk));
+    HIPCHECK(pgk_launch_reduce(d_part, nblk, d_sum, d_sum + 1, st));
+    double hsum = 0.0;
+    unsigned hl = 0;
+    HIPCHECK(hipMemcpyAsync(&hsum, d_sum, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(&hl, d_linf, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    float lf;
+    std::memcpy(&lf, &hl, 4);
+    if (sum_decrease) *sum_decrease = hsum;
+    if (linf) *linf = lf;
     return PG_OK;
 }
 

@@ -29,6 +29,8 @@ struct Ooc;
 struct pg_ctx {
     int32_t device = 0;
     hipStream_t stream = nullptr;
+    // device address of this device's pg_sweep_noalloc scratch (pg_noalloc.hip)
+    unsigned char* noalloc = nullptr;
 };
 
 struct pg_solver {

@@ -146,3 +146,9 @@ int64_t pgk_adj_grad_blocks(const pgk::Consts64& K);
 hipError_t pgk_launch_adj_grad(const double* V, const double* lam, const uint8_t* pol, const double* tcoord,
                                const uint8_t* occ, const double* bu, const double* usat, const pgk::Consts64& K,
                                double* part, hipStream_t st);
+
+// The no-allocation sweep's scratch (pg_noalloc.hip): the current device's
+// address of the static array, and a one-word clear that is a kernel (a
+// memset node misbehaves in captured graphs).
+hipError_t pgk_noalloc_scratch(void** ptr);
+hipError_t pgk_launch_zero_u32(unsigned* p, hipStream_t st);
