@@ -62,8 +62,11 @@ def test_in_place_reaches_the_jacobi_table(pg):
     e = np.abs(a.astype(np.float64) - b)[both]
     print(f"sweeps to 1e-3: Jacobi {res[pg.abi.PG_SWEEP_JACOBI][1]}, in place {res[pg.abi.PG_SWEEP_IN_PLACE][1]}; "
           f"stagnated tables: max {e.max():.2e} p99.9 {np.quantile(e, 0.999):.2e}")
+    worst = np.argsort(-np.abs(a.astype(np.float64) - b) * both)[:5]
+    print("largest differences (cell, in place, Jacobi):",
+          [(int(i), float(a[i]), float(b[i])) for i in worst], flush=True)
     assert np.array_equal(a < cap, b < cap)
-    assert np.quantile(e, 0.999) < 1e-4
+    assert e.max() < 1e-4
     assert res[pg.abi.PG_SWEEP_IN_PLACE][1] <= res[pg.abi.PG_SWEEP_JACOBI][1]
 
 
