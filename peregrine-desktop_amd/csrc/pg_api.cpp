@@ -112,6 +112,8 @@ void release(pg_solver* s) {
     (void)hipFree(s->d_forig);
     (void)hipFree(s->d_fext);
     (void)hipFree(s->d_ffree);
+    (void)hipFree(s->d_worig_esc);
+    (void)hipFree(s->d_occ0);
     (void)hipFree(s->d_act[0]);
     (void)hipFree(s->d_act[1]);
     (void)hipFree(s->d_nwork);
@@ -344,10 +346,51 @@ int plan(pg_solver* s, bool keep_host, std::vector<int16_t>* h_orig, std::vector
     return ensure_partials(s, lds_blocks(s));
 }
 
+// The escape sweep's window plan, once per solver: the exact plan pass over
+// an empty occupancy (every state counts as free and no probe rejects), so
+// each box's window holds every corner any escape backup of it reads.
+int plan_escape(pg_solver* s) {
+    if (s->esc_planned) return PG_OK;
+    hipStream_t st = s->ctx->stream;
+    box_geometry(s);
+    const int64_t nbox = s->G.nbox;
+    const int64_t nocc = (int64_t)s->g.n[0] * s->g.n[1] * s->g.n[2];
+    if (!s->d_occ0) {
+        HIPCHECK(hipMalloc(&s->d_occ0, (size_t)nocc));
+        HIPCHECK(hipMemsetAsync(s->d_occ0, 0, (size_t)nocc, st));
+    }
+    if (!s->d_worig_esc) HIPCHECK(hipMalloc(&s->d_worig_esc, (size_t)nbox * 6 * sizeof(int16_t)));
+    uint8_t* d_ext = nullptr;
+    HIPCHECK(hipMalloc(&d_ext, (size_t)nbox * 6));
+    hipError_t e = pgk_launch_plan(s->K, nullptr, s->G, s->d_occ0, s->d_bu, s->d_worig_esc, d_ext, nullptr,
+                                   s->diag ? 1 : 0, st);
+    std::vector<uint8_t> ext((size_t)nbox * 6);
+    if (e == hipSuccess) e = hipMemcpyAsync(ext.data(), d_ext, ext.size(), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(d_ext);
+    if (e != hipSuccess) return fail(PG_ERR_HIP, "escape window plan failed: %s", hipGetErrorString(e));
+    const int (*shapes)[6] = nullptr;
+    const int nsh = pgk_window_shapes(&shapes);
+    int64_t best = -1;
+    for (int i = 0; i < nsh; ++i) {
+        int64_t cov = 0;
+        for (int64_t b = 0; b < nbox; ++b) {
+            bool ok = true;
+            for (int k = 0; k < 6; ++k) ok = ok && ext[(size_t)b * 6 + k] <= shapes[i][k];
+            cov += ok ? 1 : 0;
+        }
+        if (cov > best) { best = cov; s->esc_shape = i; }
+    }
+    s->esc_coverage = nbox > 0 ? (double)best / (double)nbox : 1.0;
+    s->esc_planned = true;
+    return ensure_partials(s, lds_blocks(s));
+}
+
 }  // namespace pgh
 
 using pgh::ensure_partials;
 using pgh::plan;
+using pgh::plan_escape;
 
 namespace {
 
@@ -889,11 +932,23 @@ int pg_escape_sweep(pg_solver* s, int32_t phase, int32_t rev, double* sum_improv
             return fail(e == hipErrorOutOfMemory ? PG_ERR_NOMEM : PG_ERR_HIP, "escape buffer allocation failed: %s",
                         hipGetErrorString(e));
     }
+    // LDS-staged unless a setting needs the global-gather kernel (the
+    // production candidates, more controls than the LDS table holds) or the
+    // caller chose that kernel
+    const bool lds = s->kernel != PG_KERNEL_GLOBAL && s->p.rounds <= 0 && s->p.ntau <= 1 && s->nctl <= kLdsMaxCtl;
+    if (lds) {
+        rc = plan_escape(s);
+        if (rc) return rc;
+    }
     HIPCHECK(hipMemsetAsync(s->d_linf, 0, sizeof(unsigned), st));
-    int64_t nblk = nb;
+    int64_t nblk = lds ? lds_blocks(s) : nb;
     HIPCHECK(hipEventRecord(s->ev0, st));
-    HIPCHECK(pgk_launch_escape(s->K, s->dV[s->cur], s->dV[1 - s->cur], s->d_occ, s->d_bu, s->d_uraw, nullptr, s->d_part,
-                               s->d_linf, st, &nblk));
+    if (lds)
+        HIPCHECK(pgk_launch_escape_lds(s->esc_shape, s->diag ? 1 : 0, s->K, s->G, s->dV[s->cur], s->dV[1 - s->cur],
+                                       s->d_occ, s->d_bu, s->d_worig_esc, s->d_part, s->d_linf, st));
+    else
+        HIPCHECK(pgk_launch_escape(s->K, s->dV[s->cur], s->dV[1 - s->cur], s->d_occ, s->d_bu, s->d_uraw, nullptr,
+                                   s->d_part, s->d_linf, st, &nblk));
     HIPCHECK(hipEventRecord(s->ev1, st));
     HIPCHECK(pgk_launch_reduce(s->d_part, nblk, s->d_sum, s->d_sum + 1, st));
     s->cur = 1 - s->cur;

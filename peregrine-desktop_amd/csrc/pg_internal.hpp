@@ -90,6 +90,13 @@ struct pg_solver {
     int16_t* d_forig = nullptr;     // precision-form read ranges
     uint8_t* d_fext = nullptr;
     uint8_t* d_ffree = nullptr;     // precision form: boxes whose probes only meet free cells
+    // the LDS-staged escape sweep: window origins from a plan over an empty
+    // occupancy (every state's successors, no probe rejections)
+    int16_t* d_worig_esc = nullptr;
+    uint8_t* d_occ0 = nullptr;
+    bool esc_planned = false;
+    int esc_shape = 0;
+    double esc_coverage = 0.0;
     unsigned* d_nwork = nullptr;
     int64_t last_work = -1;         // boxes the last sweep computed (-1: every box)
     // FP64 tables (allocated on first use)

@@ -60,6 +60,11 @@ hipError_t pgk_launch_sweep_lds(int shape, int diag, const pgk::Consts& K, const
                                 const float* Vin, float* Vout, const uint8_t* occ, const float* bu,
                                 const int16_t* worig, uint8_t* pol, double* part, unsigned* linf,
                                 int write_v, const pgk::ActiveG& A, hipStream_t st);
+// One LDS-staged Jacobi escape sweep of the whole grid (gpu_escape_kernel!,
+// Gpu.jl:51-67), window origins from a plan over an empty occupancy.
+hipError_t pgk_launch_escape_lds(int shape, int diag, const pgk::Consts& K, const pgk::BoxGeom& G, const float* Vin,
+                                 float* Vout, const uint8_t* occ, const float* bu, const int16_t* worig, double* part,
+                                 unsigned* linf, hipStream_t st);
 // One tile of an out-of-core sweep: G is the tile's interior region; the
 // window (cells wo[a] .. wo[a] + wl[a] - 1 on x, y and heading modulo n3,
 // every velocity cell) holds the old V; results go to the interior buffer

@@ -70,8 +70,8 @@ __device__ __forceinline__ void setup_box(const Consts& K, const BoxInfo& B,
 // scheme option as K sets it (explicit midpoint substeps, adaptive probes,
 // velocity clamp); otherwise the default scheme, compiled without them.
 // Returns 0 if rejected (value cap), 1 if the successor must be interpolated.
-template <bool DIAG, bool GEN>
-__device__ __forceinline__ int successor_t(const Consts& K, const OccG& og, const CellStart& c,
+template <bool DIAG, bool GEN, class OCC = OccG>
+__device__ __forceinline__ int successor_t(const Consts& K, const OCC& og, const CellStart& c,
                                            float bux, float buy, float buw, float f[6]) {
     float nx, ny, nh, nvx, nvy, nw;
     const bool ok = (GEN && K.rk2) ? rk2_from<DIAG>(K, c, bux, buy, buw, nx, ny, nh, nvx, nvy, nw)
@@ -80,7 +80,7 @@ __device__ __forceinline__ int successor_t(const Consts& K, const OccG& og, cons
     if (!(ok & finite_f(dx) & finite_f(dy) & finite_f(dh) & finite_f(nvx) & finite_f(nvy) &
           finite_f(nw)))
         return 0;
-    if (!probes_free<OccG, GEN>(K, og, c.x, c.y, c.h, dx, dy, dh)) return 0;
+    if (!probes_free<OCC, GEN>(K, og, c.x, c.y, c.h, dx, dy, dh)) return 0;
     float snh, cnh;
     sincos_ref(nh, snh, cnh);
     const float nvfx = cnh * nvx - snh * nvy;
@@ -536,6 +536,190 @@ __global__ __launch_bounds__(256, 4) void sweep_lds_kernel(
     }
 }
 
+// ---------------------------------------------------------------------------
+// Escape sweep (gpu_escape_kernel!, Gpu.jl:51-67; EscapeView / OccView,
+// PeregrineSolver.jl:807-874), LDS-staged, Jacobi. A cell with a real route
+// (0 <= v < cap) keeps its value; every other cell, blocked ones included,
+// takes -c when its backup c beats the escape time it holds (-v if v < 0,
+// else cap). Corners read as the escape pass reads them: a stored escape as
+// its time, a real route as 0, the rest as cap, and -- for a free updating
+// cell -- an obstacle (x, y, h) cell as cap; a blocked updating cell's probes
+// and corners ignore the obstacles. The window is staged already read that
+// way, walls applied when the box holds a free state; a blocked state of
+// such a box reads its corners through global memory. Results equal the
+// global-gather escape sweep's bit for bit.
+// ---------------------------------------------------------------------------
+struct OccE {
+    const uint8_t* __restrict__ occ;
+    int32_t n2, n3;
+    bool ignore;
+    __device__ __forceinline__ bool operator()(int ci, int cj, int ck) const {
+        return !ignore && occ[((int64_t)ci * n2 + cj) * n3 + ck] != 0;
+    }
+};
+
+struct EscRead {
+    const float* V;
+    const uint8_t* __restrict__ occ;
+    bool walls;
+    __device__ __forceinline__ float read(const Consts& K, const int gi[6]) const {
+        if (walls && occ[((int64_t)gi[0] * K.n[1] + gi[1]) * K.n[2] + gi[2]] != 0) return K.cap;
+        int64_t fl = 0;
+#pragma unroll
+        for (int a = 0; a < 6; ++a) fl += (int64_t)gi[a] * K.stride[a];
+        const float v = V[fl];
+        return v < 0.0f ? -v : (v < K.cap ? 0.0f : K.cap);
+    }
+};
+
+template <class W, bool DIAG, bool GEN, int MAXS>
+__global__ __launch_bounds__(256, 4) void escape_lds_kernel(
+    const float* Vin, float* __restrict__ Vout, const uint8_t* __restrict__ occ, const float* __restrict__ bu_g,
+    Consts K, BoxGeom G, const int16_t* __restrict__ worig, double* __restrict__ part,
+    unsigned* __restrict__ linf_bits) {
+    __shared__ float win[W::SIZE];
+    __shared__ BoxShared<MAXS> sm;
+    __shared__ uint8_t term[MAXS];
+    const int64_t box = box_of_block(G);
+    const bool live = box < G.nbox;
+    BoxInfo B;
+    int o[6];
+    bool free_here = false;
+    if (live) {
+        box_info(G, box, B);
+#pragma unroll
+        for (int a = 0; a < 6; ++a) o[a] = worig[B.gbox * 6 + a];
+        setup_box<MAXS>(K, B, occ, bu_g, sm);
+        // the same states as setup_box's loop, so no barrier is needed
+        for (int s = threadIdx.x; s < B.ns; s += kThreads) {
+            int i[6];
+            state_coords(B, s, i);
+            int64_t fl = 0;
+#pragma unroll
+            for (int a = 0; a < 6; ++a) fl += (int64_t)i[a] * K.stride[a];
+            const float old = Vin[fl];
+            term[s] = (old >= 0.0f && old < K.cap) ? 1 : 0;
+            sm.best[s] = (unsigned long long)0x7f800000u << 32;
+            free_here |= sm.blocked[s] == 0;
+        }
+    }
+    // walls in the staged window when any state of the box is free
+    const bool walls = __syncthreads_or(free_here ? 1 : 0) != 0;
+    if (live) {
+        const EscRead er{Vin, occ, walls};
+        for (int j = threadIdx.x; j < W::CELLS; j += kThreads) {
+            int l[6], gi[6];
+            const bool on = window_cell<W>(K.n, o, j, l, gi);
+            const float v = on ? er.read(K, gi) : K.cap;
+            int lo = 0;
+#pragma unroll
+            for (int a = 0; a < 6; ++a) lo += l[a] * W::S[a];
+            win[lo] = v;
+        }
+    }
+    __syncthreads();
+    if (live) {
+        const int ns = B.ns;
+        const int total = ns * (K.span + 1);
+        int s = threadIdx.x % ns, r = threadIdx.x / ns;
+        const int ds = kThreads % ns, dr = kThreads / ns;
+        for (int p = threadIdx.x; p < total; p += kThreads) {
+            if (!term[s]) {
+                const bool blk = sm.blocked[s] != 0;
+                const OccE og{occ, K.n[1], K.n[2], blk};
+                const EscRead er{Vin, occ, !blk};
+                CellStart c;
+                load_cell_start<MAXS, GEN>(K, sm, s, c);
+                float f[6];
+                float val = K.cap;
+                const int k = slice_control(K, r);
+                if (successor_t<DIAG, GEN>(K, og, c, sm.bu[3 * k], sm.bu[3 * k + 1], sm.bu[3 * k + 2], f)) {
+                    float t[6], u[6];
+                    int i[6];
+#pragma unroll
+                    for (int a = 0; a < 6; ++a) {
+                        const float b = __builtin_floorf(f[a]);
+                        t[a] = f[a] - b;
+                        u[a] = 1.0f - t[a];
+                        i[a] = (int)b;
+                    }
+                    int l[6];
+                    bool inwin = !(blk && walls);   // the staged walls apply to free states only
+#pragma unroll
+                    for (int a = 0; a < 6; ++a) {
+                        int d = i[a] - o[a];
+                        if (a == 2) {
+                            d = d < 0 ? d + K.n[2] : d;
+                            d = d >= K.n[2] ? d - K.n[2] : d;
+                        }
+                        l[a] = d;
+                        inwin = inwin & ((unsigned)d <= (unsigned)(W::E[a] - 2));
+                    }
+                    float acc;
+                    if (GEN && K.nearest) {
+                        acc = nearest_value(K, er, win, W{}, l, i, f, inwin);
+                    } else if (GEN && K.simplex) {
+                        acc = kuhn_value(K, er, win, W{}, l, i, f, inwin);
+                    } else {
+                        Weights Wt;
+                        corner_weights(t, u, Wt);
+                        float vals[64];
+                        if (inwin) {
+                            int base = 0;
+#pragma unroll
+                            for (int a = 0; a < 6; ++a) base += l[a] * W::S[a];
+                            const float* wp = win + base;
+#pragma unroll
+                            for (int d = 0; d < 64; ++d) {
+                                const int off = ((d >> 0) & 1) * W::S1 + ((d >> 1) & 1) * W::S2 +
+                                                ((d >> 2) & 1) * W::S3 + ((d >> 3) & 1) * W::S4 +
+                                                ((d >> 4) & 1) * W::S5 + ((d >> 5) & 1) * W::S6;
+                                vals[d] = wp[off];
+                            }
+                        } else {
+                            fetch_src(K, er, i, vals);
+                        }
+                        acc = corner_sum(Wt, vals);
+                    }
+                    const float nv = (acc < K.cap) ? acc : K.cap;
+                    const float cc = c.tau + nv;
+                    val = (cc < K.cap) ? cc : K.cap;
+                }
+                const unsigned long long key = ((unsigned long long)__float_as_uint(val) << 32) | (unsigned)r;
+                atomicMin(&sm.best[s], key);
+            }
+            s += ds; r += dr;
+            if (s >= ns) { s -= ns; ++r; }
+        }
+    }
+    __syncthreads();
+    double dsum = 0.0;
+    float dmax = 0.0f;
+    if (live) {
+        for (int s = threadIdx.x; s < B.ns; s += kThreads) {
+            int i[6];
+            state_coords(B, s, i);
+            int64_t fl = 0;
+#pragma unroll
+            for (int a = 0; a < 6; ++a) fl += (int64_t)i[a] * K.stride[a];
+            const float old = Vin[fl];
+            float out = old;
+            if (!term[s]) {
+                const float best = __uint_as_float((unsigned)(sm.best[s] >> 32));
+                const float e_old = old < 0.0f ? -old : K.cap;
+                if (best < e_old) {
+                    out = -best;
+                    const float d = e_old - best;
+                    dsum += (double)d;
+                    dmax = fmaxf(dmax, d);
+                }
+            }
+            Vout[fl] = out;
+        }
+    }
+    block_reduce_sum_max(dsum, dmax, part + G.blk0, linf_bits);
+}
+
 }  // namespace pgk
 
 // ---------------------------------------------------------------------------
@@ -621,6 +805,32 @@ hipError_t pgk_launch_sweep_lds(int shape, int diag, const pgk::Consts& K, const
     const pgk::DevSrc src{Vin};
     const pgk::DevDst dst{write_v ? Vout : nullptr, pol};
     return launch_shape(shape, diag, K, G, src, dst, occ, bu, worig, part, linf, write_v, pol != nullptr, A, st);
+}
+
+hipError_t pgk_launch_escape_lds(int shape, int diag, const pgk::Consts& K, const pgk::BoxGeom& G, const float* Vin,
+                                 float* Vout, const uint8_t* occ, const float* bu, const int16_t* worig, double* part,
+                                 unsigned* linf, hipStream_t st) {
+    const bool gen = K.rk2 || K.nearest || K.simplex || K.cfl || K.adaptive || K.vclamp;
+    return pgk::for_box_chunks(G, [&](const pgk::BoxGeom& g, unsigned nb) {
+        const dim3 grid(nb), blk(256);
+#define PGK_ESC(WS, D, GN) hipLaunchKernelGGL((pgk::escape_lds_kernel<WS, D, GN, 128>), grid, blk, 0, st, Vin, Vout, \
+                                             occ, bu, K, g, worig, part, linf)
+        switch ((shape * 2 + (diag ? 1 : 0)) * 2 + (gen ? 1 : 0)) {
+            case 0: PGK_ESC(pgk::Win0, false, false); break;
+            case 1: PGK_ESC(pgk::Win0, false, true); break;
+            case 2: PGK_ESC(pgk::Win0, true, false); break;
+            case 3: PGK_ESC(pgk::Win0, true, true); break;
+            case 4: PGK_ESC(pgk::Win1, false, false); break;
+            case 5: PGK_ESC(pgk::Win1, false, true); break;
+            case 6: PGK_ESC(pgk::Win1, true, false); break;
+            case 7: PGK_ESC(pgk::Win1, true, true); break;
+            case 8: PGK_ESC(pgk::Win2, false, false); break;
+            case 9: PGK_ESC(pgk::Win2, false, true); break;
+            case 10: PGK_ESC(pgk::Win2, true, false); break;
+            default: PGK_ESC(pgk::Win2, true, true); break;
+        }
+#undef PGK_ESC
+    });
 }
 
 hipError_t pgk_launch_sweep_tile(int shape, int diag, const pgk::Consts& K, const pgk::BoxGeom& G,

@@ -14,14 +14,24 @@ from test_gpu_parity import bits_equal, diff_report
 pytestmark = pytest.mark.gpu
 
 
-def test_escape_sweeps_bit_exact(pg, orc):
+OPTIONS = dict(rk2=True, simplex=True, adaptive_checks=True, cfl=2.0, tau_min=0.004, tau_max=0.5, ncoarse=16)
+
+
+@pytest.mark.parametrize("kernel,options", [("lds", False), ("global", False), ("lds", True)])
+def test_escape_sweeps_bit_exact(pg, orc, kernel, options):
     """A reduced problem-2 grid (six obstacles, 125 controls): 30 value sweeps,
-    then 8 escape sweeps compared one by one."""
+    then 8 escape sweeps compared one by one; the LDS-staged escape kernel
+    and the global-gather one, and the LDS-staged one with the scheme
+    options its windows can plan for (RK2, Kuhn lookups, adaptive probes,
+    the per-cell horizon, a rotating slice)."""
+    import dataclasses
     prob = pg.configs.problem(2, shape=(16, 16, 12, 7, 7, 9))
+    if options:
+        prob = dataclasses.replace(prob, params=dataclasses.replace(prob.params, **OPTIONS))
     op = oracle_problem(orc, prob)
     cap = np.float32(prob.params.cap)
     with pg.HipSolver(prob.grid, prob.model, prob.params, prob.ctl, prob.occ) as s:
-        s.set_kernel(pg.abi.PG_KERNEL_LDS)
+        s.set_kernel(pg.abi.PG_KERNEL_LDS if kernel == "lds" else pg.abi.PG_KERNEL_GLOBAL)
         s.fill_seed(prob.seeds)
         s.solve(30, -1.0, 0.0)
         V = s.download()
@@ -31,7 +41,7 @@ def test_escape_sweeps_bit_exact(pg, orc):
             r = s.escape_sweep(it, it % 2 == 1)
             got = s.download()
             want = np.empty_like(V)
-            d, l = op.sweep(V, want, nthreads=nthreads(), esc=1)
+            d, l = op.sweep(V, want, phase=it, nthreads=nthreads(), esc=1)
             assert bits_equal(got, want), f"escape sweep {it}: " + diff_report(got, want)
             assert r.linf == l
             assert r.sum_decrease == pytest.approx(d, rel=1e-9, abs=1e-6)
