@@ -109,6 +109,7 @@ void release(pg_solver* s) {
     (void)hipFree(s->d_pol);
     (void)hipFree(s->d_worig);
     (void)hipFree(s->d_wext);
+    (void)hipFree(s->d_pfree);
     (void)hipFree(s->d_forig);
     (void)hipFree(s->d_fext);
     (void)hipFree(s->d_ffree);
@@ -316,8 +317,9 @@ int plan(pg_solver* s, bool keep_host, std::vector<int16_t>* h_orig, std::vector
     const int64_t nbox = s->G.nbox;
     if (!s->d_worig) HIPCHECK(hipMalloc(&s->d_worig, (size_t)nbox * 6 * sizeof(int16_t)));
     if (!s->d_wext) HIPCHECK(hipMalloc(&s->d_wext, (size_t)nbox * 6));
+    if (!s->d_pfree) HIPCHECK(hipMalloc(&s->d_pfree, (size_t)nbox));
     uint8_t* d_ext = s->d_wext;
-    hipError_t e = pgk_launch_plan(s->K, nullptr, s->G, s->d_occ, s->d_bu, s->d_worig, d_ext, nullptr,
+    hipError_t e = pgk_launch_plan(s->K, nullptr, s->G, s->d_occ, s->d_bu, s->d_worig, d_ext, s->d_pfree,
                                    s->diag ? 1 : 0, st);
     std::vector<uint8_t> ext_local;
     std::vector<uint8_t>& ext = h_ext ? *h_ext : ext_local;
@@ -851,7 +853,7 @@ int pg_sweep(pg_solver* s, int32_t phase, int32_t rev, double* sum_decrease, flo
                                        s->d_worig, s->d_ffree, nullptr, s->d_part, s->d_linf, nullptr, 1, A, st));
     } else if (kern == PG_KERNEL_LDS) {
         HIPCHECK(pgk_launch_sweep_lds(s->shape, s->diag ? 1 : 0, s->K, s->G, s->dV[s->cur], Vout, s->d_occ,
-                                      s->d_bu, s->d_worig, nullptr, s->d_part, s->d_linf, 1, A, st));
+                                      s->d_bu, s->d_worig, s->d_pfree, nullptr, s->d_part, s->d_linf, 1, A, st));
     } else {
         HIPCHECK(pgk_launch_sweep(s->K, s->dV[s->cur], Vout, s->d_occ, s->d_bu, s->d_uraw, nullptr, s->d_wpol,
                                   s->d_part, s->d_linf, 1, st, &nblk));
@@ -945,7 +947,8 @@ int pg_escape_sweep(pg_solver* s, int32_t phase, int32_t rev, double* sum_improv
     HIPCHECK(hipEventRecord(s->ev0, st));
     if (lds)
         HIPCHECK(pgk_launch_escape_lds(s->esc_shape, s->diag ? 1 : 0, s->K, s->G, s->dV[s->cur], s->dV[1 - s->cur],
-                                       s->d_occ, s->d_bu, s->d_worig_esc, s->d_part, s->d_linf, st));
+                                       s->d_occ, s->d_bu, s->d_worig_esc, s->planned ? s->d_pfree : nullptr,
+                                       s->d_part, s->d_linf, st));
     else
         HIPCHECK(pgk_launch_escape(s->K, s->dV[s->cur], s->dV[1 - s->cur], s->d_occ, s->d_bu, s->d_uraw, nullptr,
                                    s->d_part, s->d_linf, st, &nblk));
@@ -1058,7 +1061,8 @@ int pg_policy(pg_solver* s, uint8_t* pol_host) {
                                        st));
     } else if (kern == PG_KERNEL_LDS) {
         HIPCHECK(pgk_launch_sweep_lds(s->shape, s->diag ? 1 : 0, s->K, s->G, s->dV[s->cur], nullptr, s->d_occ,
-                                      s->d_bu, s->d_worig, s->d_pol, nullptr, nullptr, 0, pgk::ActiveG{}, st));
+                                      s->d_bu, s->d_worig, s->d_pfree, s->d_pol, nullptr, nullptr, 0, pgk::ActiveG{},
+                                      st));
     } else {
         HIPCHECK(pgk_launch_sweep(s->K, s->dV[s->cur], nullptr, s->d_occ, s->d_bu, s->d_uraw, s->d_pol, nullptr,
                                   nullptr, nullptr, 0, st, &nblk));

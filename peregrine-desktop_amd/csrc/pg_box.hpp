@@ -200,10 +200,15 @@ __device__ __forceinline__ int64_t box_of_cell(const BoxGeom& G, const int i[6])
     return g;
 }
 
+// Occupancy of an (x, y, heading) cell. skip: the caller has shown that no
+// cell it can ask about holds an obstacle (a box whose probes reach only
+// free cells), so the lookup is not made.
 struct OccG {
     const uint8_t* __restrict__ occ;
     int32_t n2, n3;
+    bool skip = false;
     __device__ __forceinline__ bool operator()(int ci, int cj, int ck) const {
+        if (skip) return false;
         return occ[((int64_t)ci * n2 + cj) * n3 + ck] != 0;
     }
 };

@@ -87,6 +87,7 @@ struct pg_solver {
     int act_cur = 0;
     int act_for = -1;
     uint8_t* d_wext = nullptr;      // exact read ranges: cells per axis from d_worig
+    uint8_t* d_pfree = nullptr;     // exact plan: boxes whose probe chords meet free cells only
     int16_t* d_forig = nullptr;     // precision-form read ranges
     uint8_t* d_fext = nullptr;
     uint8_t* d_ffree = nullptr;     // precision form: boxes whose probes only meet free cells

@@ -51,20 +51,24 @@ hipError_t pgk_launch_seed_mark(float* V, const int64_t* seeds, int64_t n, const
 // LDS-staged path (pg_sweep_lds.hip).
 int pgk_window_shapes(const int (**shapes)[6]);
 // Per-box corner ranges (origin, cells per axis): with F null those of the
-// exact successor, else those of the precision form's, with pfree[box] set
-// where the box's probes can only touch free cells (then skipped).
+// exact successor, else those of the precision form's. With pfree (always
+// for F), pfree[box] is set where every probe chord of the box meets free
+// cells only: the precision form then skips the box's probes, the exact
+// sweeps its occupancy lookups.
 hipError_t pgk_launch_plan(const pgk::Consts& K, const pgk::FastConsts* F, const pgk::BoxGeom& G,
                            const uint8_t* occ, const float* bu, int16_t* worig, uint8_t* wext, uint8_t* pfree,
                            int diag, hipStream_t st);
 hipError_t pgk_launch_sweep_lds(int shape, int diag, const pgk::Consts& K, const pgk::BoxGeom& G,
                                 const float* Vin, float* Vout, const uint8_t* occ, const float* bu,
-                                const int16_t* worig, uint8_t* pol, double* part, unsigned* linf,
-                                int write_v, const pgk::ActiveG& A, hipStream_t st);
+                                const int16_t* worig, const uint8_t* pfree, uint8_t* pol, double* part,
+                                unsigned* linf, int write_v, const pgk::ActiveG& A, hipStream_t st);
 // One LDS-staged Jacobi escape sweep of the whole grid (gpu_escape_kernel!,
-// Gpu.jl:51-67), window origins from a plan over an empty occupancy.
+// Gpu.jl:51-67), window origins from a plan over an empty occupancy; pfree
+// (may be null) is the value plan's per-box flag of probe chords that meet
+// free cells only.
 hipError_t pgk_launch_escape_lds(int shape, int diag, const pgk::Consts& K, const pgk::BoxGeom& G, const float* Vin,
-                                 float* Vout, const uint8_t* occ, const float* bu, const int16_t* worig, double* part,
-                                 unsigned* linf, hipStream_t st);
+                                 float* Vout, const uint8_t* occ, const float* bu, const int16_t* worig,
+                                 const uint8_t* pfree, double* part, unsigned* linf, hipStream_t st);
 // One tile of an out-of-core sweep: G is the tile's interior region; the
 // window (cells wo[a] .. wo[a] + wl[a] - 1 on x, y and heading modulo n3,
 // every velocity cell) holds the old V; results go to the interior buffer
@@ -73,7 +77,7 @@ hipError_t pgk_launch_escape_lds(int shape, int diag, const pgk::Consts& K, cons
 hipError_t pgk_launch_sweep_tile(int shape, int diag, const pgk::Consts& K, const pgk::BoxGeom& G,
                                  const float* win, const int32_t wo[3], const int32_t wl[3], float* out,
                                  uint8_t* pol, const uint8_t* occ, const float* bu, const int16_t* worig,
-                                 double* part, unsigned* linf, unsigned* miss, hipStream_t st);
+                                 const uint8_t* pfree, double* part, unsigned* linf, unsigned* miss, hipStream_t st);
 
 // Precision form (pg_sweep_fast.hip). pfree: the fast plan's per-box flags
 // (probes skipped where set).

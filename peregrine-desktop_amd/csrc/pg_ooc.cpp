@@ -913,7 +913,8 @@ int ooc_sweep(pg_solver* s, double* sum_decrease, float* linf) {
         HIPCHECK(hipStreamWaitEvent(st, o.ev_h2d[j], 0));
         if (k >= 2) HIPCHECK(hipStreamWaitEvent(st, o.ev_d2h[j], 0));
         HIPCHECK(pgk_launch_sweep_tile(s->shape, s->diag ? 1 : 0, s->K, t.G, o.dwin[j], t.wo, t.wl, o.dout[j],
-                                       nullptr, s->d_occ, s->d_bu, s->d_worig, s->d_part, s->d_linf, o.d_miss, st));
+                                       nullptr, s->d_occ, s->d_bu, s->d_worig, s->d_pfree, s->d_part, s->d_linf,
+                                       o.d_miss, st));
         HIPCHECK(pgk_launch_reduce(s->d_part, pgk::box_blocks(t.G.nbox), o.d_tsum + k, nullptr, st));
         HIPCHECK(hipEventRecord(o.ev_k[j], st));
         // tile k+1's window, into the buffer tile k-1 has finished with

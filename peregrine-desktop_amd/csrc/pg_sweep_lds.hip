@@ -70,9 +70,12 @@ __device__ __forceinline__ void setup_box(const Consts& K, const BoxInfo& B,
 // scheme option as K sets it (explicit midpoint substeps, adaptive probes,
 // velocity clamp); otherwise the default scheme, compiled without them.
 // Returns 0 if rejected (value cap), 1 if the successor must be interpolated.
-template <bool DIAG, bool GEN, class OCC = OccG>
+// REACH: when the probes run, reach[] receives the grid x, y and unwrapped
+// heading coordinates of the probe chord's far end (c + d, as the probes
+// form it); every probe lies between the start and that end on each axis.
+template <bool DIAG, bool GEN, class OCC = OccG, bool REACH = false>
 __device__ __forceinline__ int successor_t(const Consts& K, const OCC& og, const CellStart& c,
-                                           float bux, float buy, float buw, float f[6]) {
+                                           float bux, float buy, float buw, float f[6], float* reach = nullptr) {
     float nx, ny, nh, nvx, nvy, nw;
     const bool ok = (GEN && K.rk2) ? rk2_from<DIAG>(K, c, bux, buy, buw, nx, ny, nh, nvx, nvy, nw)
                                    : euler_from<DIAG>(K, c, bux, buy, buw, nx, ny, nh, nvx, nvy, nw);
@@ -80,6 +83,11 @@ __device__ __forceinline__ int successor_t(const Consts& K, const OCC& og, const
     if (!(ok & finite_f(dx) & finite_f(dy) & finite_f(dh) & finite_f(nvx) & finite_f(nvy) &
           finite_f(nw)))
         return 0;
+    if constexpr (REACH) {
+        reach[0] = qdiv((c.x + dx) - K.lo[0], K.rstep[0]);
+        reach[1] = qdiv((c.y + dy) - K.lo[1], K.rstep[1]);
+        reach[2] = qdiv((c.h + dh) - K.lo[2], K.rstep[2]);
+    }
     if (!probes_free<OCC, GEN>(K, og, c.x, c.y, c.h, dx, dy, dh)) return 0;
     float snh, cnh;
     sincos_ref(nh, snh, cnh);
@@ -88,23 +96,56 @@ __device__ __forceinline__ int successor_t(const Consts& K, const OCC& og, const
     return interp_coords<GEN>(K, nx, ny, nh, nvfx, nvfy, nw, f) ? 1 : 0;
 }
 
-// The same with every option as K sets it (the plan pass).
-template <bool DIAG>
-__device__ __forceinline__ int successor(const Consts& K, const OccG& og, const CellStart& c,
-                                         float bux, float buy, float buw, float f[6]) {
-    return successor_t<DIAG, true>(K, og, c, bux, buy, buw, f);
-}
-
 // ---------------------------------------------------------------------------
 // Plan: per box, the smallest and largest base corner index per axis over
 // every (state, control) pair whose successor is interpolated. Heading is
 // measured relative to the box's first heading, in (-n3/2, n3/2]. FAST
 // plans with the precision form's successor (its read ranges; the window
-// origins always come from the exact successor) and first flags, in pfree,
-// the boxes whose swept probes can only touch free cells: the (x, y,
-// heading) cells between the box's states and any finite successor hold no
-// obstacle. Such a box's sweeps, and its read ranges here, skip the probes.
+// origins always come from the exact successor). With pfree, the plan also
+// flags the boxes whose swept probes can only meet free cells: no obstacle
+// lies in the (x, y, heading) cells between the box's states and the far end
+// of any probe chord. The precision form's sweeps, and its read ranges here,
+// skip such a box's probes; the exact sweeps skip only their occupancy
+// lookups (which all answer "free" there), so their results do not change.
 // ---------------------------------------------------------------------------
+
+// Whether an obstacle lies in x [x0, x1] and y [y0, y1] (clamped to the
+// grid) at the headings hb + [h0, h1] (modulo n3; every heading when the
+// span covers the circle). Every thread of the block calls it.
+__device__ __forceinline__ bool obstacle_in(const Consts& K, const OccG& og, int x0, int x1, int y0, int y1, int hb,
+                                            int h0, int h1) {
+    x0 = x0 < 0 ? 0 : x0;
+    x1 = x1 > K.n[0] - 1 ? K.n[0] - 1 : x1;
+    y0 = y0 < 0 ? 0 : y0;
+    y1 = y1 > K.n[1] - 1 ? K.n[1] - 1 : y1;
+    if (h1 - h0 + 1 >= K.n[2]) { h0 = 0; h1 = K.n[2] - 1; }
+    const int nx = x1 >= x0 ? x1 - x0 + 1 : 0, ny = y1 >= y0 ? y1 - y0 + 1 : 0, nh = h1 - h0 + 1;
+    int hit = 0;
+    for (int j = threadIdx.x; j < nx * ny * nh; j += kThreads) {
+        const int ih = j % nh, iy = (j / nh) % ny, ix = j / (nh * ny);
+        int ch = hb + h0 + ih;
+        ch %= K.n[2];
+        if (ch < 0) ch += K.n[2];
+        hit |= og(x0 + ix, y0 + iy, ch) ? 1 : 0;
+    }
+    return __syncthreads_or(hit) != 0;
+}
+
+// Widens the (x, y, heading relative to hb) cell range [mn, mx] by the cells
+// rint can take between floor(g) - margin and ceil(g) + margin; coordinates
+// far outside the grid are clamped (the bounds tests reject them).
+__device__ __forceinline__ void widen_reach(const float g[3], int hb, int margin, int mn[3], int mx[3]) {
+    const float lo[3] = {g[0], g[1], g[2] - (float)hb};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float fl = __builtin_floorf(lo[a]), ce = __builtin_ceilf(lo[a]);
+        const int il = (fl < -1.0e6f ? -1000000 : (int)fl) - margin;
+        const int ic = (ce > 1.0e6f ? 1000000 : (int)ce) + margin;
+        mn[a] = il < mn[a] ? il : mn[a];
+        mx[a] = ic > mx[a] ? ic : mx[a];
+    }
+}
+
 template <bool DIAG, bool FAST, int MAXS>
 __global__ __launch_bounds__(256) void plan_kernel(const uint8_t* __restrict__ occ,
                                                    const float* __restrict__ bu_g, Consts K, FastConsts F,
@@ -114,9 +155,11 @@ __global__ __launch_bounds__(256) void plan_kernel(const uint8_t* __restrict__ o
     if (box >= G.nbox) return;
     __shared__ BoxShared<MAXS> sm;
     __shared__ int mn[6], mx[6];
+    __shared__ int rn[3], rx[3];
     BoxInfo B;
     box_info(G, box, B);
     if (threadIdx.x < 6) { mn[threadIdx.x] = 1 << 30; mx[threadIdx.x] = -(1 << 30); }
+    if (threadIdx.x < 3) { rn[threadIdx.x] = 1 << 30; rx[threadIdx.x] = -(1 << 30); }
     setup_box<MAXS>(K, B, occ, bu_g, sm);
     __syncthreads();
     const OccG og{occ, K.n[1], K.n[2]};
@@ -124,10 +167,11 @@ __global__ __launch_bounds__(256) void plan_kernel(const uint8_t* __restrict__ o
     const int total = ns * K.nctl;
     int s = threadIdx.x % ns, k = threadIdx.x / ns;
     const int ds = kThreads % ns, dk = kThreads / ns;
+    // cells (x, y, heading relative to the box) the probes can reach: the
+    // box's own, widened by every chord's far end
+    int rmn[3] = {B.s0[0], B.s0[1], 0}, rmx[3] = {B.s0[0] + B.vb[0] - 1, B.s0[1] + B.vb[1] - 1, B.vb[2] - 1};
     bool probe = true;
     if constexpr (FAST) {
-        // cells (x, y, heading relative to the box) the probes can reach
-        int rmn[3] = {B.s0[0], B.s0[1], 0}, rmx[3] = {B.s0[0] + B.vb[0] - 1, B.s0[1] + B.vb[1] - 1, B.vb[2] - 1};
         for (int p = threadIdx.x; p < total; p += kThreads) {
             if (!sm.blocked[s]) {
                 CellStart c;
@@ -135,47 +179,20 @@ __global__ __launch_bounds__(256) void plan_kernel(const uint8_t* __restrict__ o
                 FastStart S;
                 fast_start<DIAG>(K, F, c, S);
                 float g[3];
-                if (successor_reach_fast<DIAG>(K, F, S, sm.bu[3 * k], sm.bu[3 * k + 1], sm.bu[3 * k + 2], g)) {
-                    // heading unwrapped, relative to the box: the swept
-                    // headings lie between the start and this
-                    const float lo[3] = {g[0], g[1], g[2] - (float)B.s0[2]};
-#pragma unroll
-                    for (int a = 0; a < 3; ++a) {
-                        const float fl = __builtin_floorf(lo[a]), ce = __builtin_ceilf(lo[a]);
-                        // far outside the grid: the bounds tests reject it
-                        const int il = fl < -1.0e6f ? -1000000 : (int)fl;
-                        const int ic = ce > 1.0e6f ? 1000000 : (int)ce;
-                        rmn[a] = il < rmn[a] ? il : rmn[a];
-                        rmx[a] = ic > rmx[a] ? ic : rmx[a];
-                    }
-                }
+                if (successor_reach_fast<DIAG>(K, F, S, sm.bu[3 * k], sm.bu[3 * k + 1], sm.bu[3 * k + 2], g))
+                    widen_reach(g, B.s0[2], 0, rmn, rmx);
             }
             s += ds; k += dk;
             if (s >= ns) { s -= ns; ++k; }
         }
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            atomicMin(&mn[a], rmn[a]);
-            atomicMax(&mx[a], rmx[a]);
+            atomicMin(&rn[a], rmn[a]);
+            atomicMax(&rx[a], rmx[a]);
         }
         __syncthreads();
-        int x0 = mn[0] < 0 ? 0 : mn[0], x1 = mx[0] > K.n[0] - 1 ? K.n[0] - 1 : mx[0];
-        int y0 = mn[1] < 0 ? 0 : mn[1], y1 = mx[1] > K.n[1] - 1 ? K.n[1] - 1 : mx[1];
-        int h0 = mn[2], h1 = mx[2];
-        if (h1 - h0 + 1 >= K.n[2]) { h0 = 0; h1 = K.n[2] - 1; }
-        const int nx = x1 >= x0 ? x1 - x0 + 1 : 0, ny = y1 >= y0 ? y1 - y0 + 1 : 0, nh = h1 - h0 + 1;
-        int hit = 0;
-        for (int j = threadIdx.x; j < nx * ny * nh; j += kThreads) {
-            const int ih = j % nh, iy = (j / nh) % ny, ix = j / (nh * ny);
-            int ch = B.s0[2] + h0 + ih;
-            ch %= K.n[2];
-            if (ch < 0) ch += K.n[2];
-            hit |= og(x0 + ix, y0 + iy, ch) ? 1 : 0;
-        }
-        probe = __syncthreads_or(hit) != 0;
-        if (threadIdx.x < 6) { mn[threadIdx.x] = 1 << 30; mx[threadIdx.x] = -(1 << 30); }
+        probe = obstacle_in(K, og, rn[0], rx[0], rn[1], rx[1], B.s0[2], rn[2], rx[2]);
         if (threadIdx.x == 0) pfree[B.gbox] = probe ? 0 : 1;
-        __syncthreads();
         s = threadIdx.x % ns;
         k = threadIdx.x / ns;
     }
@@ -194,7 +211,12 @@ __global__ __launch_bounds__(256) void plan_kernel(const uint8_t* __restrict__ o
                 interp = successor_fast<DIAG>(K, F, og, S, sm.bu[3 * k], sm.bu[3 * k + 1], sm.bu[3 * k + 2], probe,
                                               f);
             } else {
-                interp = successor<DIAG>(K, og, c, sm.bu[3 * k], sm.bu[3 * k + 1], sm.bu[3 * k + 2], f) != 0;
+                // the chord's far end as the probes form it; one cell of
+                // margin each side on top of rint's floor/ceil bracket
+                float g[3] = {__builtin_nanf(""), 0.0f, 0.0f};
+                interp = successor_t<DIAG, true, OccG, true>(K, og, c, sm.bu[3 * k], sm.bu[3 * k + 1],
+                                                             sm.bu[3 * k + 2], f, g) != 0;
+                if (finite_f(g[0]) & finite_f(g[1]) & finite_f(g[2])) widen_reach(g, B.s0[2], 1, rmn, rmx);
             }
             if (interp) {
 #pragma unroll
@@ -219,7 +241,18 @@ __global__ __launch_bounds__(256) void plan_kernel(const uint8_t* __restrict__ o
         atomicMin(&mn[a], lmn[a]);
         atomicMax(&mx[a], lmx[a]);
     }
+    if constexpr (!FAST) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            atomicMin(&rn[a], rmn[a]);
+            atomicMax(&rx[a], rmx[a]);
+        }
+    }
     __syncthreads();
+    if (!FAST && pfree) {
+        const bool hit = obstacle_in(K, og, rn[0], rx[0], rn[1], rx[1], B.s0[2], rn[2], rx[2]);
+        if (threadIdx.x == 0) pfree[B.gbox] = hit ? 0 : 1;
+    }
     if (threadIdx.x == 0) {
         for (int a = 0; a < 6; ++a) {
             int lo = mn[a], hi = mx[a];
@@ -404,7 +437,7 @@ template <class W, bool DIAG, bool GEN, int MAXS, bool WRITE_V, bool WRITE_POL, 
 __global__ __launch_bounds__(256, 4) void sweep_lds_kernel(
     SRC src, DST dst, const uint8_t* __restrict__ occ,
     const float* __restrict__ bu_g, Consts K, BoxGeom G, const int16_t* __restrict__ worig,
-    double* __restrict__ part, unsigned* __restrict__ linf_bits, ActiveG A) {
+    const uint8_t* __restrict__ pfree, double* __restrict__ part, unsigned* __restrict__ linf_bits, ActiveG A) {
     __shared__ float win[W::SIZE];
     __shared__ BoxShared<MAXS> sm;
     const int64_t box = box_of_block(G);
@@ -431,7 +464,9 @@ __global__ __launch_bounds__(256, 4) void sweep_lds_kernel(
     }
     __syncthreads();
     if (live) {
-        const OccG og{occ, K.n[1], K.n[2]};
+        // a box the plan found free of obstacles along every probe chord
+        // makes no occupancy lookups
+        const OccG og{occ, K.n[1], K.n[2], pfree != nullptr && pfree[B.gbox] != 0};
         const int ns = B.ns;
         // pairs (state, scan position r): coast, then the lattice (or this
         // sweep's slice of it) in scan order; the minimum key keeps the
@@ -575,8 +610,8 @@ struct EscRead {
 template <class W, bool DIAG, bool GEN, int MAXS>
 __global__ __launch_bounds__(256, 4) void escape_lds_kernel(
     const float* Vin, float* __restrict__ Vout, const uint8_t* __restrict__ occ, const float* __restrict__ bu_g,
-    Consts K, BoxGeom G, const int16_t* __restrict__ worig, double* __restrict__ part,
-    unsigned* __restrict__ linf_bits) {
+    Consts K, BoxGeom G, const int16_t* __restrict__ worig, const uint8_t* __restrict__ pfree,
+    double* __restrict__ part, unsigned* __restrict__ linf_bits) {
     __shared__ float win[W::SIZE];
     __shared__ BoxShared<MAXS> sm;
     __shared__ uint8_t term[MAXS];
@@ -623,10 +658,12 @@ __global__ __launch_bounds__(256, 4) void escape_lds_kernel(
         const int total = ns * (K.span + 1);
         int s = threadIdx.x % ns, r = threadIdx.x / ns;
         const int ds = kThreads % ns, dr = kThreads / ns;
+        // the value plan's flag: every probe chord of the box meets free cells only
+        const bool pclear = pfree != nullptr && pfree[B.gbox] != 0;
         for (int p = threadIdx.x; p < total; p += kThreads) {
             if (!term[s]) {
                 const bool blk = sm.blocked[s] != 0;
-                const OccE og{occ, K.n[1], K.n[2], blk};
+                const OccE og{occ, K.n[1], K.n[2], blk || pclear};
                 const EscRead er{Vin, occ, !blk};
                 CellStart c;
                 load_cell_start<MAXS, GEN>(K, sm, s, c);
@@ -730,40 +767,41 @@ namespace {
 
 template <class W, bool DIAG, bool GEN, int MAXS, class SRC, class DST>
 hipError_t launch_w(const pgk::Consts& K, const pgk::BoxGeom& G, const SRC& src, const DST& dst,
-                    const uint8_t* occ, const float* bu, const int16_t* worig, double* part, unsigned* linf,
-                    int write_v, int write_pol, const pgk::ActiveG& A, hipStream_t st) {
+                    const uint8_t* occ, const float* bu, const int16_t* worig, const uint8_t* pfree, double* part,
+                    unsigned* linf, int write_v, int write_pol, const pgk::ActiveG& A, hipStream_t st) {
     return pgk::for_box_chunks(G, [&](const pgk::BoxGeom& g, unsigned nb) {
         if (write_v && write_pol)
             hipLaunchKernelGGL((pgk::sweep_lds_kernel<W, DIAG, GEN, MAXS, true, true, SRC, DST>), dim3(nb), dim3(256), 0,
-                               st, src, dst, occ, bu, K, g, worig, part, linf, A);
+                               st, src, dst, occ, bu, K, g, worig, pfree, part, linf, A);
         else if (write_v)
             hipLaunchKernelGGL((pgk::sweep_lds_kernel<W, DIAG, GEN, MAXS, true, false, SRC, DST>), dim3(nb), dim3(256),
-                               0, st, src, dst, occ, bu, K, g, worig, part, linf, A);
+                               0, st, src, dst, occ, bu, K, g, worig, pfree, part, linf, A);
         else
             hipLaunchKernelGGL((pgk::sweep_lds_kernel<W, DIAG, GEN, MAXS, false, true, SRC, DST>), dim3(nb), dim3(256),
-                               0, st, src, dst, occ, bu, K, g, worig, part, linf, A);
+                               0, st, src, dst, occ, bu, K, g, worig, pfree, part, linf, A);
     });
 }
 
 template <class SRC, class DST>
 hipError_t launch_shape(int shape, int diag, const pgk::Consts& K, const pgk::BoxGeom& G, const SRC& src,
-                        const DST& dst, const uint8_t* occ, const float* bu, const int16_t* worig, double* part,
-                        unsigned* linf, int write_v, int write_pol, const pgk::ActiveG& A, hipStream_t st) {
+                        const DST& dst, const uint8_t* occ, const float* bu, const int16_t* worig,
+                        const uint8_t* pfree, double* part, unsigned* linf, int write_v, int write_pol,
+                        const pgk::ActiveG& A, hipStream_t st) {
     // the general instantiation when any scheme option is set
     const bool gen = K.rk2 || K.nearest || K.simplex || K.cfl || K.adaptive || K.vclamp;
     switch ((shape * 2 + (diag ? 1 : 0)) * 2 + (gen ? 1 : 0)) {
-        case 0: return launch_w<pgk::Win0, false, false, 128>(K, G, src, dst, occ, bu, worig, part, linf, write_v, write_pol, A, st);
-        case 1: return launch_w<pgk::Win0, false, true, 128>(K, G, src, dst, occ, bu, worig, part, linf, write_v, write_pol, A, st);
-        case 2: return launch_w<pgk::Win0, true, false, 128>(K, G, src, dst, occ, bu, worig, part, linf, write_v, write_pol, A, st);
-        case 3: return launch_w<pgk::Win0, true, true, 128>(K, G, src, dst, occ, bu, worig, part, linf, write_v, write_pol, A, st);
-        case 4: return launch_w<pgk::Win1, false, false, 128>(K, G, src, dst, occ, bu, worig, part, linf, write_v, write_pol, A, st);
-        case 5: return launch_w<pgk::Win1, false, true, 128>(K, G, src, dst, occ, bu, worig, part, linf, write_v, write_pol, A, st);
-        case 6: return launch_w<pgk::Win1, true, false, 128>(K, G, src, dst, occ, bu, worig, part, linf, write_v, write_pol, A, st);
-        case 7: return launch_w<pgk::Win1, true, true, 128>(K, G, src, dst, occ, bu, worig, part, linf, write_v, write_pol, A, st);
-        case 8: return launch_w<pgk::Win2, false, false, 128>(K, G, src, dst, occ, bu, worig, part, linf, write_v, write_pol, A, st);
-        case 9: return launch_w<pgk::Win2, false, true, 128>(K, G, src, dst, occ, bu, worig, part, linf, write_v, write_pol, A, st);
-        case 10: return launch_w<pgk::Win2, true, false, 128>(K, G, src, dst, occ, bu, worig, part, linf, write_v, write_pol, A, st);
-        case 11: return launch_w<pgk::Win2, true, true, 128>(K, G, src, dst, occ, bu, worig, part, linf, write_v, write_pol, A, st);
+        case 0: return launch_w<pgk::Win0, false, false, 128>(K, G, src, dst, occ, bu, worig, pfree, part, linf, write_v, write_pol, A, st);
+        case 1: return launch_w<pgk::Win0, false, true, 128>(K, G, src, dst, occ, bu, worig, pfree, part, linf, write_v, write_pol, A, st);
+        case 2: return launch_w<pgk::Win0, true, false, 128>(K, G, src, dst, occ, bu, worig, pfree, part, linf, write_v, write_pol, A, st);
+        case 3: return launch_w<pgk::Win0, true, true, 128>(K, G, src, dst, occ, bu, worig, pfree, part, linf, write_v, write_pol, A, st);
+        case 4: return launch_w<pgk::Win1, false, false, 128>(K, G, src, dst, occ, bu, worig, pfree, part, linf, write_v, write_pol, A, st);
+        case 5: return launch_w<pgk::Win1, false, true, 128>(K, G, src, dst, occ, bu, worig, pfree, part, linf, write_v, write_pol, A, st);
+        case 6: return launch_w<pgk::Win1, true, false, 128>(K, G, src, dst, occ, bu, worig, pfree, part, linf, write_v, write_pol, A, st);
+        case 7: return launch_w<pgk::Win1, true, true, 128>(K, G, src, dst, occ, bu, worig, pfree, part, linf, write_v, write_pol, A, st);
+        case 8: return launch_w<pgk::Win2, false, false, 128>(K, G, src, dst, occ, bu, worig, pfree, part, linf, write_v, write_pol, A, st);
+        case 9: return launch_w<pgk::Win2, false, true, 128>(K, G, src, dst, occ, bu, worig, pfree, part, linf, write_v, write_pol, A, st);
+        case 10: return launch_w<pgk::Win2, true, false, 128>(K, G, src, dst, occ, bu, worig, pfree, part, linf, write_v, write_pol, A, st);
+        case 11: return launch_w<pgk::Win2, true, true, 128>(K, G, src, dst, occ, bu, worig, pfree, part, linf, write_v, write_pol, A, st);
         default: return hipErrorInvalidValue;
     }
 }
@@ -800,21 +838,22 @@ hipError_t pgk_launch_plan(const pgk::Consts& K, const pgk::FastConsts* F, const
 
 hipError_t pgk_launch_sweep_lds(int shape, int diag, const pgk::Consts& K, const pgk::BoxGeom& G,
                                 const float* Vin, float* Vout, const uint8_t* occ, const float* bu,
-                                const int16_t* worig, uint8_t* pol, double* part, unsigned* linf,
-                                int write_v, const pgk::ActiveG& A, hipStream_t st) {
+                                const int16_t* worig, const uint8_t* pfree, uint8_t* pol, double* part,
+                                unsigned* linf, int write_v, const pgk::ActiveG& A, hipStream_t st) {
     const pgk::DevSrc src{Vin};
     const pgk::DevDst dst{write_v ? Vout : nullptr, pol};
-    return launch_shape(shape, diag, K, G, src, dst, occ, bu, worig, part, linf, write_v, pol != nullptr, A, st);
+    return launch_shape(shape, diag, K, G, src, dst, occ, bu, worig, pfree, part, linf, write_v, pol != nullptr, A,
+                        st);
 }
 
 hipError_t pgk_launch_escape_lds(int shape, int diag, const pgk::Consts& K, const pgk::BoxGeom& G, const float* Vin,
-                                 float* Vout, const uint8_t* occ, const float* bu, const int16_t* worig, double* part,
-                                 unsigned* linf, hipStream_t st) {
+                                 float* Vout, const uint8_t* occ, const float* bu, const int16_t* worig,
+                                 const uint8_t* pfree, double* part, unsigned* linf, hipStream_t st) {
     const bool gen = K.rk2 || K.nearest || K.simplex || K.cfl || K.adaptive || K.vclamp;
     return pgk::for_box_chunks(G, [&](const pgk::BoxGeom& g, unsigned nb) {
         const dim3 grid(nb), blk(256);
 #define PGK_ESC(WS, D, GN) hipLaunchKernelGGL((pgk::escape_lds_kernel<WS, D, GN, 128>), grid, blk, 0, st, Vin, Vout, \
-                                             occ, bu, K, g, worig, part, linf)
+                                             occ, bu, K, g, worig, pfree, part, linf)
         switch ((shape * 2 + (diag ? 1 : 0)) * 2 + (gen ? 1 : 0)) {
             case 0: PGK_ESC(pgk::Win0, false, false); break;
             case 1: PGK_ESC(pgk::Win0, false, true); break;
@@ -836,7 +875,7 @@ hipError_t pgk_launch_escape_lds(int shape, int diag, const pgk::Consts& K, cons
 hipError_t pgk_launch_sweep_tile(int shape, int diag, const pgk::Consts& K, const pgk::BoxGeom& G,
                                  const float* win, const int32_t wo[3], const int32_t wl[3], float* out,
                                  uint8_t* pol, const uint8_t* occ, const float* bu, const int16_t* worig,
-                                 double* part, unsigned* linf, unsigned* miss, hipStream_t st) {
+                                 const uint8_t* pfree, double* part, unsigned* linf, unsigned* miss, hipStream_t st) {
     pgk::TileSrc src;
     src.W = win;
     src.miss = miss;
@@ -866,6 +905,6 @@ hipError_t pgk_launch_sweep_tile(int shape, int diag, const pgk::Consts& K, cons
     dst.os[2] = dst.os[3] * K.n[3];
     dst.os[1] = dst.os[2] * G.rn[2];
     dst.os[0] = dst.os[1] * G.rn[1];
-    return launch_shape(shape, diag, K, G, src, dst, occ, bu, worig, part, linf, out != nullptr, pol != nullptr,
-                        pgk::ActiveG{}, st);
+    return launch_shape(shape, diag, K, G, src, dst, occ, bu, worig, pfree, part, linf, out != nullptr,
+                        pol != nullptr, pgk::ActiveG{}, st);
 }
