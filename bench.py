@@ -610,6 +610,7 @@ def main() -> int:
     value = g.ncells * steps / wall
     kavg = float(np.mean(kms))
     kern, coverage = s.kernel_info()
+    chords_free, probes_pass = s.probe_info()
     p = prob.params
     flops_update = pg.configs.algorithmic_flops_per_update(prob.nctl, p.substeps(), p.checks, prob.model.knee > 0)
     bytes_update = pg.configs.algorithmic_bytes_per_update(prob.nctl, 4)
@@ -663,6 +664,8 @@ def main() -> int:
             "parallelism": "single GPU",
             "kernel": {1: "global-gather", 2: "LDS-staged", 3: "LDS-staged, contracted FP32"}.get(kern, str(kern)),
             "window_coverage": coverage,
+            "boxes_probe_chords_free": chords_free,
+            "boxes_probes_skipped": probes_pass,
         },
         "roofline": roofline,
     }

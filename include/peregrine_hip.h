@@ -220,6 +220,14 @@ int pg_solver_set_kernel(pg_solver* s, int32_t kernel);
  * the rest read global memory for the corners outside). */
 int pg_solver_kernel_info(pg_solver* s, int32_t* kernel, double* window_coverage);
 
+/* No counterpart in the reference (its probes run for every pair, each
+ * sweep): of the exact plan's boxes, the fraction whose probe chords meet no
+ * obstacle (their sweeps make no occupancy lookups) and the fraction whose
+ * every (state, control) pair passes its probes (their sweeps run no probes:
+ * a pair's successor, and so the probes' outcome, is the same in every
+ * sweep). Both 0 before the LDS-staged kernel's first sweep. */
+int pg_solver_probe_info(pg_solver* s, double* chords_free, double* probes_pass);
+
 /* Duration of the last sweep kernel, from HIP events on the solver stream. */
 int pg_last_sweep_ms(pg_solver* s, float* ms);
 

@@ -123,6 +123,7 @@ SIGNATURES = {
     "pg_solver_set_kernel": (ctypes.c_int, [_P, _I32]),
     "pg_solver_set_sweep_mode": (ctypes.c_int, [_P, _I32]),
     "pg_solver_kernel_info": (ctypes.c_int, [_P, ctypes.POINTER(_I32), ctypes.POINTER(ctypes.c_double)]),
+    "pg_solver_probe_info": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     "pg_last_sweep_ms": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_float)]),
     "pg_solver_set_active": (ctypes.c_int, [_P, _I32]),
     "pg_escape_sweep": (ctypes.c_int, [_P, _I32, _I32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_float)]),

@@ -1426,6 +1426,27 @@ int pg_solver_kernel_info(pg_solver* s, int32_t* kernel, double* window_coverage
     return PG_OK;
 }
 
+int pg_solver_probe_info(pg_solver* s, double* chords_free, double* probes_pass) {
+    if (!s) return fail(PG_ERR_ARG, "null solver");
+    double f1 = 0.0, f2 = 0.0;
+    if (s->planned && s->d_pfree && s->G.nbox > 0) {
+        std::vector<uint8_t> h((size_t)s->G.nbox);
+        hipStream_t st = s->ctx->stream;
+        HIPCHECK(hipMemcpyAsync(h.data(), s->d_pfree, h.size(), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        int64_t n1 = 0, n2 = 0;
+        for (uint8_t v : h) {
+            n1 += v & 1;
+            n2 += (v >> 1) & 1;
+        }
+        f1 = (double)n1 / (double)h.size();
+        f2 = (double)n2 / (double)h.size();
+    }
+    if (chords_free) *chords_free = f1;
+    if (probes_pass) *probes_pass = f2;
+    return PG_OK;
+}
+
 int pg_last_sweep_ms(pg_solver* s, float* ms) {
     if (!s || !ms) return fail(PG_ERR_ARG, "null argument");
     *ms = s->last_ms;

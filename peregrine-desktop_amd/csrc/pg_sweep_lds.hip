@@ -70,12 +70,16 @@ __device__ __forceinline__ void setup_box(const Consts& K, const BoxInfo& B,
 // scheme option as K sets it (explicit midpoint substeps, adaptive probes,
 // velocity clamp); otherwise the default scheme, compiled without them.
 // Returns 0 if rejected (value cap), 1 if the successor must be interpolated.
-// REACH: when the probes run, reach[] receives the grid x, y and unwrapped
-// heading coordinates of the probe chord's far end (c + d, as the probes
-// form it); every probe lies between the start and that end on each axis.
+// noprobe: the caller knows the swept probes pass (a box whose every pair
+// passed them in the plan; a successor never depends on V), so they are not
+// run. REACH: when the probes are reached, reach[] receives the grid x, y
+// and unwrapped heading coordinates of the probe chord's far end (c + d, as
+// the probes form it; every probe lies between the start and that end on
+// each axis) and *probed whether they passed.
 template <bool DIAG, bool GEN, class OCC = OccG, bool REACH = false>
 __device__ __forceinline__ int successor_t(const Consts& K, const OCC& og, const CellStart& c,
-                                           float bux, float buy, float buw, float f[6], float* reach = nullptr) {
+                                           float bux, float buy, float buw, float f[6], bool noprobe = false,
+                                           float* reach = nullptr, int* probed = nullptr) {
     float nx, ny, nh, nvx, nvy, nw;
     const bool ok = (GEN && K.rk2) ? rk2_from<DIAG>(K, c, bux, buy, buw, nx, ny, nh, nvx, nvy, nw)
                                    : euler_from<DIAG>(K, c, bux, buy, buw, nx, ny, nh, nvx, nvy, nw);
@@ -88,7 +92,11 @@ __device__ __forceinline__ int successor_t(const Consts& K, const OCC& og, const
         reach[1] = qdiv((c.y + dy) - K.lo[1], K.rstep[1]);
         reach[2] = qdiv((c.h + dh) - K.lo[2], K.rstep[2]);
     }
-    if (!probes_free<OCC, GEN>(K, og, c.x, c.y, c.h, dx, dy, dh)) return 0;
+    if (!noprobe) {
+        const bool pass = probes_free<OCC, GEN>(K, og, c.x, c.y, c.h, dx, dy, dh);
+        if constexpr (REACH) *probed = pass ? 1 : 0;
+        if (!pass) return 0;
+    }
     float snh, cnh;
     sincos_ref(nh, snh, cnh);
     const float nvfx = cnh * nvx - snh * nvy;
@@ -105,8 +113,12 @@ __device__ __forceinline__ int successor_t(const Consts& K, const OCC& og, const
 // flags the boxes whose swept probes can only meet free cells: no obstacle
 // lies in the (x, y, heading) cells between the box's states and the far end
 // of any probe chord. The precision form's sweeps, and its read ranges here,
-// skip such a box's probes; the exact sweeps skip only their occupancy
-// lookups (which all answer "free" there), so their results do not change.
+// skip such a box's probes. The exact plan writes two bits: 1, the chords
+// meet free cells only (the exact sweeps skip the occupancy lookups, which
+// would all answer "free"); 2, every (state, control) pair of the box that
+// reaches its probes passes them (the exact sweeps skip the probes: a pair's
+// successor, and so its probes' outcome, is the same in every sweep, and the
+// plan computes it with the sweeps' own arithmetic). Neither changes a bit.
 // ---------------------------------------------------------------------------
 
 // Whether an obstacle lies in x [x0, x1] and y [y0, y1] (clamped to the
@@ -199,6 +211,7 @@ __global__ __launch_bounds__(256) void plan_kernel(const uint8_t* __restrict__ o
     int lmn[6], lmx[6];
 #pragma unroll
     for (int a = 0; a < 6; ++a) { lmn[a] = 1 << 30; lmx[a] = -(1 << 30); }
+    bool allpass = true;
     for (int p = threadIdx.x; p < total; p += kThreads) {
         if (!sm.blocked[s]) {
             CellStart c;
@@ -214,9 +227,11 @@ __global__ __launch_bounds__(256) void plan_kernel(const uint8_t* __restrict__ o
                 // the chord's far end as the probes form it; one cell of
                 // margin each side on top of rint's floor/ceil bracket
                 float g[3] = {__builtin_nanf(""), 0.0f, 0.0f};
+                int probed = -1;
                 interp = successor_t<DIAG, true, OccG, true>(K, og, c, sm.bu[3 * k], sm.bu[3 * k + 1],
-                                                             sm.bu[3 * k + 2], f, g) != 0;
+                                                             sm.bu[3 * k + 2], f, false, g, &probed) != 0;
                 if (finite_f(g[0]) & finite_f(g[1]) & finite_f(g[2])) widen_reach(g, B.s0[2], 1, rmn, rmx);
+                allpass = allpass && probed != 0;
             }
             if (interp) {
 #pragma unroll
@@ -251,7 +266,8 @@ __global__ __launch_bounds__(256) void plan_kernel(const uint8_t* __restrict__ o
     __syncthreads();
     if (!FAST && pfree) {
         const bool hit = obstacle_in(K, og, rn[0], rx[0], rn[1], rx[1], B.s0[2], rn[2], rx[2]);
-        if (threadIdx.x == 0) pfree[B.gbox] = hit ? 0 : 1;
+        const bool pass = __syncthreads_and(allpass ? 1 : 0) != 0;
+        if (threadIdx.x == 0) pfree[B.gbox] = (uint8_t)((hit ? 0 : 1) | (pass ? 2 : 0));
     }
     if (threadIdx.x == 0) {
         for (int a = 0; a < 6; ++a) {
@@ -464,9 +480,11 @@ __global__ __launch_bounds__(256, 4) void sweep_lds_kernel(
     }
     __syncthreads();
     if (live) {
-        // a box the plan found free of obstacles along every probe chord
-        // makes no occupancy lookups
-        const OccG og{occ, K.n[1], K.n[2], pfree != nullptr && pfree[B.gbox] != 0};
+        // the plan's flags: bit 1, no obstacle along any probe chord of the
+        // box (no occupancy lookups); bit 2, every pair's probes pass (none run)
+        const unsigned pf = pfree != nullptr ? pfree[B.gbox] : 0u;
+        const OccG og{occ, K.n[1], K.n[2], (pf & 1u) != 0};
+        const bool noprobe = (pf & 2u) != 0;
         const int ns = B.ns;
         // pairs (state, scan position r): coast, then the lattice (or this
         // sweep's slice of it) in scan order; the minimum key keeps the
@@ -481,7 +499,8 @@ __global__ __launch_bounds__(256, 4) void sweep_lds_kernel(
                 float f[6];
                 float val = K.cap;
                 const int k = slice_control(K, r);
-                if (successor_t<DIAG, GEN>(K, og, c, sm.bu[3 * k], sm.bu[3 * k + 1], sm.bu[3 * k + 2], f)) {
+                if (successor_t<DIAG, GEN>(K, og, c, sm.bu[3 * k], sm.bu[3 * k + 1], sm.bu[3 * k + 2], f,
+                                           noprobe)) {
                     float t[6], u[6];
                     int i[6];
 #pragma unroll
@@ -658,8 +677,11 @@ __global__ __launch_bounds__(256, 4) void escape_lds_kernel(
         const int total = ns * (K.span + 1);
         int s = threadIdx.x % ns, r = threadIdx.x / ns;
         const int ds = kThreads % ns, dr = kThreads / ns;
-        // the value plan's flag: every probe chord of the box meets free cells only
-        const bool pclear = pfree != nullptr && pfree[B.gbox] != 0;
+        // the value plan's flags (free states only: the plan skips blocked
+        // ones, whose probes here ignore the obstacles): bit 1, no obstacle
+        // along any probe chord; bit 2, every pair's probes pass
+        const unsigned pf = pfree != nullptr ? pfree[B.gbox] : 0u;
+        const bool pclear = (pf & 1u) != 0, pallpass = (pf & 2u) != 0;
         for (int p = threadIdx.x; p < total; p += kThreads) {
             if (!term[s]) {
                 const bool blk = sm.blocked[s] != 0;
@@ -670,7 +692,8 @@ __global__ __launch_bounds__(256, 4) void escape_lds_kernel(
                 float f[6];
                 float val = K.cap;
                 const int k = slice_control(K, r);
-                if (successor_t<DIAG, GEN>(K, og, c, sm.bu[3 * k], sm.bu[3 * k + 1], sm.bu[3 * k + 2], f)) {
+                if (successor_t<DIAG, GEN>(K, og, c, sm.bu[3 * k], sm.bu[3 * k + 1], sm.bu[3 * k + 2], f,
+                                           pallpass && !blk)) {
                     float t[6], u[6];
                     int i[6];
 #pragma unroll

@@ -154,6 +154,15 @@ class HipSolver:
                   self.lib.pg_solver_kernel_info(self._h, ctypes.byref(k), ctypes.byref(cov)))
         return k.value, cov.value
 
+    def probe_info(self):
+        """(fraction of boxes whose probe chords meet no obstacle, fraction
+        whose every pair passes its probes), from the exact plan."""
+        f1 = ctypes.c_double()
+        f2 = ctypes.c_double()
+        abi.check(self.lib, "pg_solver_probe_info",
+                  self.lib.pg_solver_probe_info(self._h, ctypes.byref(f1), ctypes.byref(f2)))
+        return f1.value, f2.value
+
     # -- the reference's loop pieces ---------------------------------------
     def fill_seed(self, seeds) -> None:
         s = np.ascontiguousarray(np.asarray(seeds, dtype=np.int64))

@@ -91,6 +91,28 @@ def test_obstacles_125_controls_bit_exact(pg, orc, kernel):
     step_parity(pg, orc, prob, 2, start=V, kernel=kernel)
 
 
+def test_plan_probe_flags(pg):
+    """The exact plan's per-box probe flags (pg_solver_probe_info), which let
+    the LDS-staged sweeps skip occupancy lookups or whole probe loops: with no
+    obstacle every box's chords are free and the interior boxes skip their
+    probes; with problem 2's obstacles some boxes keep their lookups. The
+    sweeps these flags shape are the bit-exact ones of the tests above."""
+    def flags(prob):
+        with pg.HipSolver(prob.grid, prob.model, prob.params, prob.ctl, prob.occ) as s:
+            s.set_kernel(pg.abi.PG_KERNEL_LDS)
+            assert s.probe_info() == (0.0, 0.0)      # before the plan
+            s.fill_seed(prob.seeds)
+            s.sweep(0, False)
+            return s.probe_info()
+
+    free1, pass1 = flags(pg.configs.problem(1))
+    print(f"problem 1: chords free {free1:.3f}, probes skipped {pass1:.3f}", flush=True)
+    assert free1 == 1.0 and 0.0 < pass1 < 1.0
+    free2, pass2 = flags(pg.configs.problem(2, shape=(16, 16, 12, 7, 7, 9)))
+    print(f"problem 2 (reduced): chords free {free2:.3f}, probes skipped {pass2:.3f}", flush=True)
+    assert 0.0 < free2 < 1.0 and 0.0 <= pass2 < 1.0
+
+
 @pytest.mark.parametrize("kernel", KERNELS)
 def test_full_solve_matches_oracle_solve(pg, orc, kernel):
     """Whole solve on a small grid: same iteration count, same table."""
