@@ -51,10 +51,11 @@ hipError_t pgk_launch_seed_mark(float* V, const int64_t* seeds, int64_t n, const
 // LDS-staged path (pg_sweep_lds.hip).
 int pgk_window_shapes(const int (**shapes)[6]);
 // Per-box corner ranges (origin, cells per axis): with F null those of the
-// exact successor, else those of the precision form's. With pfree (always
-// for F), pfree[box] is set where every probe chord of the box meets free
-// cells only: the precision form then skips the box's probes, the exact
-// sweeps its occupancy lookups.
+// exact successor, else those of the precision form's. With F, pfree[box] is
+// 1 where every probe chord of the box meets free cells only (the precision
+// form then skips the box's probes). With F null and pfree given, bit 1 says
+// the same (the exact sweeps skip their occupancy lookups) and bit 2 that
+// every pair of the box passed its probes (the exact sweeps run none).
 hipError_t pgk_launch_plan(const pgk::Consts& K, const pgk::FastConsts* F, const pgk::BoxGeom& G,
                            const uint8_t* occ, const float* bu, int16_t* worig, uint8_t* wext, uint8_t* pfree,
                            int diag, hipStream_t st);
