@@ -70,6 +70,15 @@ hipError_t pgk_launch_sweep_lds(int shape, int diag, const pgk::Consts& K, const
 hipError_t pgk_launch_escape_lds(int shape, int diag, const pgk::Consts& K, const pgk::BoxGeom& G, const float* Vin,
                                  float* Vout, const uint8_t* occ, const float* bu, const int16_t* worig,
                                  const uint8_t* pfree, double* part, unsigned* linf, hipStream_t st);
+// The same over one out-of-core tile: G is the tile's interior region, the
+// window (as pgk_launch_sweep_tile's) holds the old V of the interior and of
+// every corner the tile's escape backups read; results go to the interior
+// buffer `out`; *miss is raised if a backup needed a cell outside the window.
+hipError_t pgk_launch_escape_lds_tile(int shape, int diag, const pgk::Consts& K, const pgk::BoxGeom& G,
+                                      const float* win, const int32_t wo[3], const int32_t wl[3], float* out,
+                                      const uint8_t* occ, const float* bu, const int16_t* worig,
+                                      const uint8_t* pfree, double* part, unsigned* linf, unsigned* miss,
+                                      hipStream_t st);
 // One tile of an out-of-core sweep: G is the tile's interior region; the
 // window (cells wo[a] .. wo[a] + wl[a] - 1 on x, y and heading modulo n3,
 // every velocity cell) holds the old V; results go to the interior buffer

@@ -63,6 +63,7 @@ struct Ooc {
     int64_t budget = 0;      // device bytes allowed
     int64_t win_floats = 0;  // capacity of each FP32 window buffer, in cells
     bool escape_ready = false;
+    bool escape_lds = false;   // the escape pass runs the LDS-staged kernel
     bool prod_ready = false;
     // the Int8 warm-start store (3 per cell) on the host, and the interiors'
     // on the device (null: the warm start is off)
@@ -573,22 +574,52 @@ int ooc_escape_prepare(pg_solver* s) {
     if (e == hipSuccess) e = hipMemcpyAsync(ext.data(), d_ext, ext.size(), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     (void)hipFree(zocc);
-    (void)hipFree(d_orig);
     (void)hipFree(d_ext);
-    if (e != hipSuccess)
+    if (e != hipSuccess) {
+        (void)hipFree(d_orig);
         return fail(e == hipErrorOutOfMemory ? PG_ERR_NOMEM : PG_ERR_HIP, "escape window plan: %s",
                     hipGetErrorString(e));
+    }
     const std::vector<PlanRef> plans{{&orig, &ext}};
     int64_t win = 0, blocks = 0;
     for (OocTile& t : o.tiles) {
         tile_window(s, o, t, plans, true, false, t.ewo, t.ewl);
         win = std::max(win, (int64_t)t.ewl[0] * t.ewl[1] * t.ewl[2] * o.colv);
-        blocks = std::max(blocks, ceil_div(interior_cells(t, o.colv), 256));
+        blocks = std::max(blocks, std::max(ceil_div(interior_cells(t, o.colv), 256), pgk::box_blocks(t.G.nbox)));
     }
     rc = ensure_partials(s, blocks);
-    if (rc) return rc;
-    rc = grow_windows(s, win, "the escape pass's");
-    if (rc) return rc;
+    if (rc == PG_OK) rc = grow_windows(s, win, "the escape pass's");
+    if (rc) {
+        (void)hipFree(d_orig);
+        return rc;
+    }
+    // The LDS-staged escape kernel stages each box's window from the plan's
+    // origins, which then stay on the device (12 bytes per box): within the
+    // budget, and with the control table in LDS, the tiles run it; otherwise
+    // the global-gather escape kernel, which reads the tile window directly.
+    const int64_t orig_bytes = nbox * 6 * (int64_t)sizeof(int16_t);
+    o.escape_lds = s->nctl <= kLdsMaxCtl && o.device_bytes + orig_bytes <= o.budget;
+    if (o.escape_lds) {
+        (void)hipFree(s->d_worig_esc);
+        s->d_worig_esc = d_orig;
+        o.device_bytes += orig_bytes;
+        // the window shape that holds the most boxes' escape reads whole
+        const int (*shapes)[6] = nullptr;
+        const int nsh = pgk_window_shapes(&shapes);
+        int64_t best = -1;
+        for (int i = 0; i < nsh; ++i) {
+            int64_t cov = 0;
+            for (int64_t b = 0; b < nbox; ++b) {
+                bool ok = true;
+                for (int k = 0; k < 6; ++k) ok = ok && ext[(size_t)b * 6 + k] <= shapes[i][k];
+                cov += ok ? 1 : 0;
+            }
+            if (cov > best) { best = cov; s->esc_shape = i; }
+        }
+        s->esc_coverage = nbox > 0 ? (double)best / (double)nbox : 1.0;
+    } else {
+        (void)hipFree(d_orig);
+    }
     o.escape_ready = true;
     return PG_OK;
 }
@@ -641,9 +672,16 @@ int ooc_escape_sweep(pg_solver* s, double* sum_improve, float* linf) {
         HIPCHECK(hipStreamWaitEvent(st, o.ev_h2d[j], 0));
         if (k >= 2) HIPCHECK(hipStreamWaitEvent(st, o.ev_d2h[j], 0));
         int64_t nblk = 0;
-        HIPCHECK(pgk_launch_escape_tile(s->K, t.t0, t.tn, o.dwin[j], prod ? t.pwo : t.ewo, prod ? t.pwl : t.ewl,
-                                        o.dout[j], s->d_occ, s->d_bu, s->d_uraw, s->d_part, s->d_linf, o.d_miss, st,
-                                        &nblk));
+        if (!prod && o.escape_lds) {
+            HIPCHECK(pgk_launch_escape_lds_tile(s->esc_shape, s->diag ? 1 : 0, s->K, t.G, o.dwin[j], t.ewo, t.ewl,
+                                                o.dout[j], s->d_occ, s->d_bu, s->d_worig_esc, s->d_pfree, s->d_part,
+                                                s->d_linf, o.d_miss, st));
+            nblk = pgk::box_blocks(t.G.nbox);
+        } else {
+            HIPCHECK(pgk_launch_escape_tile(s->K, t.t0, t.tn, o.dwin[j], prod ? t.pwo : t.ewo,
+                                            prod ? t.pwl : t.ewl, o.dout[j], s->d_occ, s->d_bu, s->d_uraw,
+                                            s->d_part, s->d_linf, o.d_miss, st, &nblk));
+        }
         HIPCHECK(pgk_launch_reduce(s->d_part, nblk, o.d_tsum + k, nullptr, st));
         HIPCHECK(hipEventRecord(o.ev_k[j], st));
         if (k + 1 < nt) {

@@ -486,14 +486,22 @@ __global__ __launch_bounds__(256, 4) void sweep_lds_kernel(
         const OccG og{occ, K.n[1], K.n[2], (pf & 1u) != 0};
         const bool noprobe = (pf & 2u) != 0;
         const int ns = B.ns;
-        // pairs (state, scan position r): coast, then the lattice (or this
-        // sweep's slice of it) in scan order; the minimum key keeps the
-        // earliest position among equal values, as the strict `<` scan does
-        const int total = ns * (K.span + 1);
-        int s = threadIdx.x % ns, r = threadIdx.x / ns;
-        const int ds = kThreads % ns, dr = kThreads / ns;
-        for (int p = threadIdx.x; p < total; p += kThreads) {
-            if (!sm.blocked[s]) {
+        // tps = 256 / ns threads share a state (ns <= 128, so tps >= 2): each
+        // holds the state's start in registers and walks scan positions
+        // r0, r0 + tps, ... (coast, then the lattice or this sweep's slice of
+        // it in scan order), keeping its running minimum in registers with a
+        // strict `<` (the earliest position wins a tie); the state's minimum
+        // key over its threads then keeps the earliest position among equal
+        // values, as the reference's strict `<` scan does
+        const int tps = kThreads / ns;
+        const int s = threadIdx.x % ns, r0 = threadIdx.x / ns;
+        if (r0 < tps && !sm.blocked[s]) {
+            float bv = __uint_as_float(0x7f800000u);
+            int br = 0;
+            for (int r = r0; r <= K.span; r += tps) {
+                // the state's start is read from LDS every iteration: held
+                // in registers across the loop it would spill
+                asm volatile("" ::: "memory");
                 CellStart c;
                 load_cell_start<MAXS, GEN>(K, sm, s, c);
                 float f[6];
@@ -552,12 +560,14 @@ __global__ __launch_bounds__(256, 4) void sweep_lds_kernel(
                     const float cc = c.tau + nv;
                     val = (cc < K.cap) ? cc : K.cap;
                 }
-                const unsigned long long key =
-                    ((unsigned long long)__float_as_uint(val) << 32) | (unsigned)r;
-                atomicMin(&sm.best[s], key);
+                // the order of the (value bits, position) keys: for the
+                // non-negative values of a value sweep, the order of values
+                if (__float_as_uint(val) < __float_as_uint(bv)) {
+                    bv = val;
+                    br = r;
+                }
             }
-            s += ds; r += dr;
-            if (s >= ns) { s -= ns; ++r; }
+            atomicMin(&sm.best[s], ((unsigned long long)__float_as_uint(bv) << 32) | (unsigned)br);
         }
     }
     __syncthreads();
@@ -612,23 +622,68 @@ struct OccE {
     }
 };
 
-struct EscRead {
+// A stored value as the escape pass reads it: a stored escape as its time,
+// a real route as 0, the rest as cap.
+__device__ __forceinline__ float escape_view(const Consts& K, float v) {
+    return v < 0.0f ? -v : (v < K.cap ? 0.0f : K.cap);
+}
+
+__device__ __forceinline__ bool wall_at(const Consts& K, const uint8_t* __restrict__ occ, const int gi[6]) {
+    return occ[((int64_t)gi[0] * K.n[1] + gi[1]) * K.n[2] + gi[2]] != 0;
+}
+
+// Where an escape sweep reads V. raw: a state's own stored value; read: a
+// corner as the escape pass reads it, an obstacle (x, y, h) cell as cap
+// when walls apply; stage: the same for a window cell to stage. The whole
+// table in HBM:
+struct EscDev {
     const float* V;
     const uint8_t* __restrict__ occ;
-    bool walls;
-    __device__ __forceinline__ float read(const Consts& K, const int gi[6]) const {
-        if (walls && occ[((int64_t)gi[0] * K.n[1] + gi[1]) * K.n[2] + gi[2]] != 0) return K.cap;
+    __device__ __forceinline__ float raw(const Consts& K, const int gi[6]) const {
         int64_t fl = 0;
 #pragma unroll
         for (int a = 0; a < 6; ++a) fl += (int64_t)gi[a] * K.stride[a];
-        const float v = V[fl];
-        return v < 0.0f ? -v : (v < K.cap ? 0.0f : K.cap);
+        return V[fl];
+    }
+    __device__ __forceinline__ float read(const Consts& K, const int gi[6], bool walls) const {
+        if (walls && wall_at(K, occ, gi)) return K.cap;
+        return escape_view(K, raw(K, gi));
+    }
+    __device__ __forceinline__ float stage(const Consts& K, const int gi[6], bool walls) const {
+        return read(K, gi, walls);
     }
 };
 
-template <class W, bool DIAG, bool GEN, int MAXS>
+// ... or (out of core) one tile's window, which holds the tile's interior
+// and every corner its escape backups read: a read outside it is a defect
+// (cap, and *miss raised); a staged cell outside it is never read as a
+// corner and stages cap quietly.
+struct EscTile {
+    TileSrc t;
+    const uint8_t* __restrict__ occ;
+    __device__ __forceinline__ float raw(const Consts& K, const int gi[6]) const { return t.read(K, gi); }
+    __device__ __forceinline__ float read(const Consts& K, const int gi[6], bool walls) const {
+        if (walls && wall_at(K, occ, gi)) return K.cap;
+        return escape_view(K, t.read(K, gi));
+    }
+    __device__ __forceinline__ float stage(const Consts& K, const int gi[6], bool walls) const {
+        if (walls && wall_at(K, occ, gi)) return K.cap;
+        int32_t fl;
+        return t.local(K, gi, fl) ? escape_view(K, t.W[fl]) : K.cap;
+    }
+};
+
+// A source with its walls fixed, as the corner readers take it.
+template <class ESRC>
+struct EscView {
+    const ESRC& src;
+    bool walls;
+    __device__ __forceinline__ float read(const Consts& K, const int gi[6]) const { return src.read(K, gi, walls); }
+};
+
+template <class W, bool DIAG, bool GEN, int MAXS, class ESRC, class DST>
 __global__ __launch_bounds__(256, 4) void escape_lds_kernel(
-    const float* Vin, float* __restrict__ Vout, const uint8_t* __restrict__ occ, const float* __restrict__ bu_g,
+    ESRC src, DST dst, const uint8_t* __restrict__ occ, const float* __restrict__ bu_g,
     Consts K, BoxGeom G, const int16_t* __restrict__ worig, const uint8_t* __restrict__ pfree,
     double* __restrict__ part, unsigned* __restrict__ linf_bits) {
     __shared__ float win[W::SIZE];
@@ -648,10 +703,7 @@ __global__ __launch_bounds__(256, 4) void escape_lds_kernel(
         for (int s = threadIdx.x; s < B.ns; s += kThreads) {
             int i[6];
             state_coords(B, s, i);
-            int64_t fl = 0;
-#pragma unroll
-            for (int a = 0; a < 6; ++a) fl += (int64_t)i[a] * K.stride[a];
-            const float old = Vin[fl];
+            const float old = src.raw(K, i);
             term[s] = (old >= 0.0f && old < K.cap) ? 1 : 0;
             sm.best[s] = (unsigned long long)0x7f800000u << 32;
             free_here |= sm.blocked[s] == 0;
@@ -660,11 +712,10 @@ __global__ __launch_bounds__(256, 4) void escape_lds_kernel(
     // walls in the staged window when any state of the box is free
     const bool walls = __syncthreads_or(free_here ? 1 : 0) != 0;
     if (live) {
-        const EscRead er{Vin, occ, walls};
         for (int j = threadIdx.x; j < W::CELLS; j += kThreads) {
             int l[6], gi[6];
             const bool on = window_cell<W>(K.n, o, j, l, gi);
-            const float v = on ? er.read(K, gi) : K.cap;
+            const float v = on ? src.stage(K, gi, walls) : K.cap;
             int lo = 0;
 #pragma unroll
             for (int a = 0; a < 6; ++a) lo += l[a] * W::S[a];
@@ -674,19 +725,19 @@ __global__ __launch_bounds__(256, 4) void escape_lds_kernel(
     __syncthreads();
     if (live) {
         const int ns = B.ns;
-        const int total = ns * (K.span + 1);
-        int s = threadIdx.x % ns, r = threadIdx.x / ns;
-        const int ds = kThreads % ns, dr = kThreads / ns;
         // the value plan's flags (free states only: the plan skips blocked
         // ones, whose probes here ignore the obstacles): bit 1, no obstacle
         // along any probe chord; bit 2, every pair's probes pass
         const unsigned pf = pfree != nullptr ? pfree[B.gbox] : 0u;
         const bool pclear = (pf & 1u) != 0, pallpass = (pf & 2u) != 0;
+        const int total = ns * (K.span + 1);
+        int s = threadIdx.x % ns, r = threadIdx.x / ns;
+        const int ds = kThreads % ns, dr = kThreads / ns;
         for (int p = threadIdx.x; p < total; p += kThreads) {
             if (!term[s]) {
                 const bool blk = sm.blocked[s] != 0;
                 const OccE og{occ, K.n[1], K.n[2], blk || pclear};
-                const EscRead er{Vin, occ, !blk};
+                const EscView<ESRC> er{src, !blk};
                 CellStart c;
                 load_cell_start<MAXS, GEN>(K, sm, s, c);
                 float f[6];
@@ -759,10 +810,7 @@ __global__ __launch_bounds__(256, 4) void escape_lds_kernel(
         for (int s = threadIdx.x; s < B.ns; s += kThreads) {
             int i[6];
             state_coords(B, s, i);
-            int64_t fl = 0;
-#pragma unroll
-            for (int a = 0; a < 6; ++a) fl += (int64_t)i[a] * K.stride[a];
-            const float old = Vin[fl];
+            const float old = src.raw(K, i);
             float out = old;
             if (!term[s]) {
                 const float best = __uint_as_float((unsigned)(sm.best[s] >> 32));
@@ -774,7 +822,7 @@ __global__ __launch_bounds__(256, 4) void escape_lds_kernel(
                     dmax = fmaxf(dmax, d);
                 }
             }
-            Vout[fl] = out;
+            dst.put(K, i, out, 0);
         }
     }
     block_reduce_sum_max(dsum, dmax, part + G.blk0, linf_bits);
@@ -869,14 +917,17 @@ hipError_t pgk_launch_sweep_lds(int shape, int diag, const pgk::Consts& K, const
                         st);
 }
 
-hipError_t pgk_launch_escape_lds(int shape, int diag, const pgk::Consts& K, const pgk::BoxGeom& G, const float* Vin,
-                                 float* Vout, const uint8_t* occ, const float* bu, const int16_t* worig,
-                                 const uint8_t* pfree, double* part, unsigned* linf, hipStream_t st) {
+namespace {
+
+template <class ESRC, class DST>
+hipError_t launch_escape_shape(int shape, int diag, const pgk::Consts& K, const pgk::BoxGeom& G, const ESRC& src,
+                               const DST& dst, const uint8_t* occ, const float* bu, const int16_t* worig,
+                               const uint8_t* pfree, double* part, unsigned* linf, hipStream_t st) {
     const bool gen = K.rk2 || K.nearest || K.simplex || K.cfl || K.adaptive || K.vclamp;
     return pgk::for_box_chunks(G, [&](const pgk::BoxGeom& g, unsigned nb) {
         const dim3 grid(nb), blk(256);
-#define PGK_ESC(WS, D, GN) hipLaunchKernelGGL((pgk::escape_lds_kernel<WS, D, GN, 128>), grid, blk, 0, st, Vin, Vout, \
-                                             occ, bu, K, g, worig, pfree, part, linf)
+#define PGK_ESC(WS, D, GN) hipLaunchKernelGGL((pgk::escape_lds_kernel<WS, D, GN, 128, ESRC, DST>), grid, blk, 0, st, \
+                                             src, dst, occ, bu, K, g, worig, pfree, part, linf)
         switch ((shape * 2 + (diag ? 1 : 0)) * 2 + (gen ? 1 : 0)) {
             case 0: PGK_ESC(pgk::Win0, false, false); break;
             case 1: PGK_ESC(pgk::Win0, false, true); break;
@@ -895,14 +946,13 @@ hipError_t pgk_launch_escape_lds(int shape, int diag, const pgk::Consts& K, cons
     });
 }
 
-hipError_t pgk_launch_sweep_tile(int shape, int diag, const pgk::Consts& K, const pgk::BoxGeom& G,
-                                 const float* win, const int32_t wo[3], const int32_t wl[3], float* out,
-                                 uint8_t* pol, const uint8_t* occ, const float* bu, const int16_t* worig,
-                                 const uint8_t* pfree, double* part, unsigned* linf, unsigned* miss, hipStream_t st) {
-    pgk::TileSrc src;
+// A tile's window reader and interior writer (x, y, heading cut; every
+// velocity cell), both indexed in 32 bits: false when either holds 2^31
+// cells or more.
+bool tile_io(const pgk::Consts& K, const pgk::BoxGeom& G, const float* win, const int32_t wo[3],
+             const int32_t wl[3], unsigned* miss, float* out, uint8_t* pol, pgk::TileSrc& src, pgk::TileDst& dst) {
     src.W = win;
     src.miss = miss;
-    pgk::TileDst dst;
     dst.V = out;
     dst.pol = pol;
     for (int a = 0; a < 3; ++a) {
@@ -911,11 +961,10 @@ hipError_t pgk_launch_sweep_tile(int shape, int diag, const pgk::Consts& K, cons
         dst.t0[a] = G.org[a];
     }
     // window strides: L0 x L1 x L2 x n4 x n5 x n6; interior: rn0 x rn1 x rn2 x ...
-    // window and interior are indexed in 32 bits
     const int64_t colv = (int64_t)K.n[3] * K.n[4] * K.n[5];
     if ((int64_t)wl[0] * wl[1] * wl[2] * colv >= ((int64_t)1 << 31) ||
         (int64_t)G.rn[0] * G.rn[1] * G.rn[2] * colv >= ((int64_t)1 << 31))
-        return hipErrorInvalidValue;
+        return false;
     src.ws[5] = 1;
     src.ws[4] = K.n[5];
     src.ws[3] = K.n[4] * K.n[5];
@@ -928,6 +977,38 @@ hipError_t pgk_launch_sweep_tile(int shape, int diag, const pgk::Consts& K, cons
     dst.os[2] = dst.os[3] * K.n[3];
     dst.os[1] = dst.os[2] * G.rn[2];
     dst.os[0] = dst.os[1] * G.rn[1];
+    return true;
+}
+
+}  // namespace
+
+hipError_t pgk_launch_escape_lds(int shape, int diag, const pgk::Consts& K, const pgk::BoxGeom& G, const float* Vin,
+                                 float* Vout, const uint8_t* occ, const float* bu, const int16_t* worig,
+                                 const uint8_t* pfree, double* part, unsigned* linf, hipStream_t st) {
+    const pgk::EscDev src{Vin, occ};
+    const pgk::DevDst dst{Vout, nullptr};
+    return launch_escape_shape(shape, diag, K, G, src, dst, occ, bu, worig, pfree, part, linf, st);
+}
+
+hipError_t pgk_launch_escape_lds_tile(int shape, int diag, const pgk::Consts& K, const pgk::BoxGeom& G,
+                                      const float* win, const int32_t wo[3], const int32_t wl[3], float* out,
+                                      const uint8_t* occ, const float* bu, const int16_t* worig,
+                                      const uint8_t* pfree, double* part, unsigned* linf, unsigned* miss,
+                                      hipStream_t st) {
+    pgk::EscTile src;
+    pgk::TileDst dst;
+    if (!tile_io(K, G, win, wo, wl, miss, out, nullptr, src.t, dst)) return hipErrorInvalidValue;
+    src.occ = occ;
+    return launch_escape_shape(shape, diag, K, G, src, dst, occ, bu, worig, pfree, part, linf, st);
+}
+
+hipError_t pgk_launch_sweep_tile(int shape, int diag, const pgk::Consts& K, const pgk::BoxGeom& G,
+                                 const float* win, const int32_t wo[3], const int32_t wl[3], float* out,
+                                 uint8_t* pol, const uint8_t* occ, const float* bu, const int16_t* worig,
+                                 const uint8_t* pfree, double* part, unsigned* linf, unsigned* miss, hipStream_t st) {
+    pgk::TileSrc src;
+    pgk::TileDst dst;
+    if (!tile_io(K, G, win, wo, wl, miss, out, pol, src, dst)) return hipErrorInvalidValue;
     return launch_shape(shape, diag, K, G, src, dst, occ, bu, worig, pfree, part, linf, out != nullptr,
                         pol != nullptr, pgk::ActiveG{}, st);
 }
