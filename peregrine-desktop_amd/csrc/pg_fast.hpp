@@ -235,4 +235,41 @@ __device__ __forceinline__ float tensor_interp_grouped(const Fetch& fetch, const
     return lerp_fast(y0, y1, t[0]);
 }
 
+// The same lerp tree as a loop that is not unrolled, for the rare backups
+// whose corners lie outside the LDS window: eight passes, one group of
+// eight corners each (x, y, h fixed), in the order c = 0, 4, 2, 6, 1, 5,
+// 3, 7 (c is the bit reversal of the pass number), so that h-pairs (0, 4),
+// (2, 6), (1, 5), (3, 7) and then y-pairs (0, 2), (1, 3) complete in turn.
+// Every lerp has the operands and order of tensor_interp_grouped's, so the
+// value is identical, while at most eight corner reads are in flight --
+// unrolled, the addressed and decoded reads of all 64 corners would raise
+// the register pressure of the whole backup.
+template <class Fetch>
+__device__ __forceinline__ float tensor_interp_rolled(const Fetch& fetch, const float t[6]) {
+    float rp = 0.0f, hf = 0.0f, y0 = 0.0f, y1 = 0.0f;
+#pragma unroll 1
+    for (int i = 0; i < 8; ++i) {
+        const int c = ((i & 1) << 2) | (i & 2) | ((i >> 2) & 1);
+        const float a0 = lerp_fast(fetch(c), fetch(c | 32), t[5]);
+        const float a1 = lerp_fast(fetch(c | 16), fetch(c | 48), t[5]);
+        const float a2 = lerp_fast(fetch(c | 8), fetch(c | 40), t[5]);
+        const float a3 = lerp_fast(fetch(c | 24), fetch(c | 56), t[5]);
+        const float b0 = lerp_fast(a0, a1, t[4]);
+        const float b1 = lerp_fast(a2, a3, t[4]);
+        const float r = lerp_fast(b0, b1, t[3]);
+        if ((i & 1) == 0) {
+            rp = r;                                   // r[c] of an h-pair's first group
+        } else {
+            const float h = lerp_fast(rp, r, t[2]);   // h0, h2, h1, h3 in turn
+            if ((i & 2) == 0) {
+                hf = h;
+            } else {
+                const float y = lerp_fast(hf, h, t[1]);
+                if (i == 3) y0 = y; else y1 = y;
+            }
+        }
+    }
+    return lerp_fast(y0, y1, t[0]);
+}
+
 }  // namespace pgk

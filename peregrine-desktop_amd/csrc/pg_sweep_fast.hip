@@ -16,6 +16,8 @@
 #include "pg_fast.hpp"
 #include "pg_kernels.h"
 
+#include <type_traits>
+
 namespace pgk {
 
 namespace fastk {
@@ -76,12 +78,28 @@ __device__ __forceinline__ int64_t flat_of(const Consts& K, const int* gi) {
 // Reads of V in either store: stage() for a window cell, read() for a value
 // a backup uses. (no __restrict__ on the table: in place, a sweep writes
 // what it reads)
+// An in-HBM table holds fewer than 2^32 cells (the solver refuses larger
+// grids), so its cell and tile indices are 32-bit.
+__device__ __forceinline__ uint32_t flat32(const Consts& K, const int* gi) {
+    uint32_t fl = 0;
+#pragma unroll
+    for (int a = 0; a < 6; ++a) fl += (uint32_t)gi[a] * (uint32_t)K.stride[a];
+    return fl;
+}
+
+__device__ __forceinline__ uint32_t tile32(const QuantGeom& Q, const int* gi) {
+    uint32_t t = 0;
+#pragma unroll
+    for (int a = 0; a < 6; ++a) t = t * (uint32_t)Q.nt[a] + (uint32_t)(gi[a] >> kQuantShift);
+    return t;
+}
+
 template <int STORE>
 struct VRead {
     const uint16_t* V;
     const float2* __restrict__ tp;
     __device__ __forceinline__ float read(const Consts& K, const QuantGeom& Q, const int* gi) const {
-        return Codec<STORE>::decode(V[flat_of(K, gi)], tp[tile_of_coords(Q, gi)], K.cap);
+        return Codec<STORE>::decode(V[flat32(K, gi)], tp[tile32(Q, gi)], K.cap);
     }
     __device__ __forceinline__ float stage(const Consts& K, const QuantGeom& Q, const int* gi) const {
         return read(K, Q, gi);
@@ -92,7 +110,7 @@ template <>
 struct VRead<PG_STORE_F32> {
     const float* V;
     __device__ __forceinline__ float read(const Consts& K, const QuantGeom&, const int* gi) const {
-        return V[flat_of(K, gi)];
+        return V[flat32(K, gi)];
     }
     __device__ __forceinline__ float stage(const Consts& K, const QuantGeom& Q, const int* gi) const {
         return read(K, Q, gi);
@@ -238,88 +256,120 @@ __global__ __launch_bounds__(256, 4) void sweep_fast_kernel(
                 for (int a = 0; a < 6; ++a) pk |= (unsigned)(i[a] - B.s0[a]) << (3 * a);
                 vmir = vin.read(K, Q, i);
             }
-            float bv = __uint_as_float(0x7f800000u);
-            int bk = 0;
-            for (int k = k0; k < K.nctl; k += tps) {
+            // One backup: the value of control k, or false when a corner of
+            // its successor lies outside the window and OUTSIDE is not set
+            // (the main loop leaves those pairs to a second pass, so that
+            // their addressed reads do not raise the loop's register
+            // pressure).
+            auto backup = [&](int k, auto outside, float& val) -> bool {
+                constexpr bool OUTSIDE = decltype(outside)::value;
                 float f[6];
-                float val = K.cap;
-                if (successor_fast<DIAG>(K, F, og, S, sm.bu[3 * k], sm.bu[3 * k + 1], sm.bu[3 * k + 2], probe, f)) {
-                    float t[6];
-                    int si[6];
-                    int l[6];
-                    bool inwin = true;
+                val = K.cap;
+                if (!successor_fast<DIAG>(K, F, og, S, sm.bu[3 * k], sm.bu[3 * k + 1], sm.bu[3 * k + 2], probe, f))
+                    return true;
+                float t[6];
+                int si[6];
+                int l[6];
+                bool inwin = true;
+#pragma unroll
+                for (int a = 0; a < 6; ++a) {
+                    const float b = __builtin_floorf(f[a]);
+                    t[a] = f[a] - b;
+                    si[a] = (int)b;
+                    int d = si[a] - o[a];
+                    if (a == 2) {
+                        d = d < 0 ? d + K.n[2] : d;
+                        d = d >= K.n[2] ? d - K.n[2] : d;
+                    }
+                    l[a] = d;
+                    inwin = inwin & ((unsigned)d <= (unsigned)(W::E[a] - 2));
+                }
+                // Weight of the state's own cell in the successor's stencil.
+                float wself = 0.0f;
+                if (SELF) {
+                    wself = 1.0f;
 #pragma unroll
                     for (int a = 0; a < 6; ++a) {
-                        const float b = __builtin_floorf(f[a]);
-                        t[a] = f[a] - b;
-                        si[a] = (int)b;
-                        int d = si[a] - o[a];
-                        if (a == 2) {
-                            d = d < 0 ? d + K.n[2] : d;
-                            d = d >= K.n[2] ? d - K.n[2] : d;
-                        }
-                        l[a] = d;
-                        inwin = inwin & ((unsigned)d <= (unsigned)(W::E[a] - 2));
-                    }
-                    // Weight of the state's own cell in the successor's stencil.
-                    float wself = 0.0f;
-                    if (SELF) {
-                        wself = 1.0f;
-#pragma unroll
-                        for (int a = 0; a < 6; ++a) {
-                            int d = B.s0[a] + (int)((pk >> (3 * a)) & 7u) - si[a];
-                            if (a == 2) d = d < 0 ? d + K.n[2] : d;
-                            wself *= d == 0 ? 1.0f - t[a] : (d == 1 ? t[a] : 0.0f);
-                        }
-                    }
-                    float acc;
-                    if (inwin) {
-                        int base = 0;
-#pragma unroll
-                        for (int a = 0; a < 6; ++a) base += l[a] * W::S[a];
-                        const float* wp = win + base;
-                        acc = tensor_interp_grouped(
-                            [&](int d) {
-                                return wp[((d >> 0) & 1) * W::S1 + ((d >> 1) & 1) * W::S2 + ((d >> 2) & 1) * W::S3 +
-                                          ((d >> 3) & 1) * W::S4 + ((d >> 4) & 1) * W::S5 + ((d >> 5) & 1) * W::S6];
-                            },
-                            t);
-                    } else {
-                        // corners outside the window: global memory, the
-                        // reference's clamped / periodic indices
-                        acc = tensor_interp_grouped(
-                            [&](int d) {
-                                int gi[6];
-#pragma unroll
-                                for (int a = 0; a < 6; ++a) {
-                                    int j = si[a] + ((d >> a) & 1);
-                                    if (a == 2) {
-                                        j = j >= K.n[2] ? j - K.n[2] : j;
-                                        j = j >= K.n[2] ? j - K.n[2] : j;
-                                    } else {
-                                        j = j < K.n[a] - 1 ? j : K.n[a] - 1;
-                                    }
-                                    gi[a] = j;
-                                }
-                                return vin.read(K, Q, gi);
-                            },
-                            t);
-                    }
-                    if (SELF && wself > 0.0f) {
-                        // V = tau + w V + rest solved for V (Jacobi on the
-                        // policy system with its diagonal): same fixed point
-                        // as tau + min(acc, cap), without the 1/(1-w)-times
-                        // slower approach through the cell's own value.
-                        const float rest = acc - wself * vmir;
-                        const float cc = (K.tau + rest) * __builtin_amdgcn_rcpf(1.0f - wself);
-                        val = cc < K.cap ? cc : K.cap;
-                    } else {
-                        const float nv = acc < K.cap ? acc : K.cap;
-                        const float cc = K.tau + nv;
-                        val = cc < K.cap ? cc : K.cap;
+                        int d = B.s0[a] + (int)((pk >> (3 * a)) & 7u) - si[a];
+                        if (a == 2) d = d < 0 ? d + K.n[2] : d;
+                        wself *= d == 0 ? 1.0f - t[a] : (d == 1 ? t[a] : 0.0f);
                     }
                 }
+                float acc;
+                if constexpr (OUTSIDE) {
+                    // corners outside the window: global memory, the
+                    // reference's clamped / periodic indices
+                    acc = tensor_interp_rolled(
+                        [&](int d) {
+                            int gi[6];
+#pragma unroll
+                            for (int a = 0; a < 6; ++a) {
+                                int j = si[a] + ((d >> a) & 1);
+                                if (a == 2) {
+                                    j = j >= K.n[2] ? j - K.n[2] : j;
+                                    j = j >= K.n[2] ? j - K.n[2] : j;
+                                } else {
+                                    j = j < K.n[a] - 1 ? j : K.n[a] - 1;
+                                }
+                                gi[a] = j;
+                            }
+                            return vin.read(K, Q, gi);
+                        },
+                        t);
+                } else {
+                    if (!inwin) return false;
+                    int base = 0;
+#pragma unroll
+                    for (int a = 0; a < 6; ++a) base += l[a] * W::S[a];
+                    const float* wp = win + base;
+                    acc = tensor_interp_grouped(
+                        [&](int d) {
+                            return wp[((d >> 0) & 1) * W::S1 + ((d >> 1) & 1) * W::S2 + ((d >> 2) & 1) * W::S3 +
+                                      ((d >> 3) & 1) * W::S4 + ((d >> 4) & 1) * W::S5 + ((d >> 5) & 1) * W::S6];
+                        },
+                        t);
+                }
+                if (SELF && wself > 0.0f) {
+                    // V = tau + w V + rest solved for V (Jacobi on the
+                    // policy system with its diagonal): same fixed point
+                    // as tau + min(acc, cap), without the 1/(1-w)-times
+                    // slower approach through the cell's own value.
+                    const float rest = acc - wself * vmir;
+                    const float cc = (K.tau + rest) * __builtin_amdgcn_rcpf(1.0f - wself);
+                    val = cc < K.cap ? cc : K.cap;
+                } else {
+                    const float nv = acc < K.cap ? acc : K.cap;
+                    const float cc = K.tau + nv;
+                    val = cc < K.cap ? cc : K.cap;
+                }
+                return true;
+            };
+            float bv = __uint_as_float(0x7f800000u);
+            int bk = 0;
+            // bit j: the pair of the thread's j-th control was left for the
+            // second pass (a thread holds at most 128 / 2 = 64 controls)
+            unsigned long long deferred = 0;
+            int j = 0;
+            for (int k = k0; k < K.nctl; k += tps, ++j) {
+                float val;
+                if (!backup(k, std::false_type{}, val)) {
+                    deferred |= 1ull << j;
+                    continue;
+                }
                 if (val < bv) {
+                    bv = val;
+                    bk = k;
+                }
+            }
+            // The deferred pairs, merged as the scan would have taken them:
+            // the smallest value, the lowest control among equal values.
+            while (deferred) {
+                const int jj = __builtin_ctzll(deferred);
+                deferred &= deferred - 1;
+                const int k = k0 + jj * tps;
+                float val;
+                backup(k, std::true_type{}, val);
+                if (val < bv || (val == bv && k < bk)) {
                     bv = val;
                     bk = k;
                 }
