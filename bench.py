@@ -78,14 +78,14 @@ FP32_VECTOR_PEAK_TFLOPS = 157.3
 # Kernels: 2 = exact LDS-staged sweep (the FP32 table), 3 = contracted sweep
 # over the FP32 table, 4 = contracted sweep over the 16-bit store.
 PMC = {
-    (2, 2): {"hbm_bytes": (21_165_243 + 2_653_988) * 1024, "flops_counted": 64 * 727_958_358_714,
-             "valu_insts": 1_212_012_662_697, "valu_issue_frac": 0.657,
-             "lds_bank_conflict_cycles": 213_019_567_520},
-    (2, 3): {"hbm_bytes": (21_119_742 + 2_699_889) * 1024, "flops_counted": 64 * 467_560_510_574,
-             "valu_insts": 491_855_614_361, "valu_issue_frac": 0.627,
+    (2, 2): {"hbm_bytes": (21_235_533 + 2_623_983) * 1024, "flops_counted": 64 * 685_315_275_294,
+             "valu_insts": 1_031_362_332_600, "valu_issue_frac": 0.693,
+             "lds_bank_conflict_cycles": 213_018_474_560},
+    (2, 3): {"hbm_bytes": (21_185_032 + 2_673_361) * 1024, "flops_counted": 64 * 467_560_510_574,
+             "valu_insts": 479_103_616_971, "valu_issue_frac": 0.612,
              "lds_bank_conflict_cycles": 213_036_013_056},
-    (2, 4): {"hbm_bytes": (10_797_493 + 5_723_883) * 1024, "flops_counted": 64 * 478_715_722_289,
-             "valu_insts": 565_784_705_028, "valu_issue_frac": 0.576,
+    (2, 4): {"hbm_bytes": (8_586_874 + 2_592_330) * 1024, "flops_counted": 64 * 478_776_759_961,
+             "valu_insts": 545_896_402_353, "valu_issue_frac": 0.640,
              "lds_bank_conflict_cycles": 213_036_013_056},
 }
 
