@@ -303,6 +303,10 @@ struct DevSrc {
         for (int a = 0; a < 6; ++a) fl += (int64_t)gi[a] * K.stride[a];
         return V[fl];
     }
+    // one corner of a gather (read_acc), and the gather's end (report): the
+    // whole table holds every cell
+    __device__ __forceinline__ float read_acc(const Consts& K, const int gi[6], bool&) const { return read(K, gi); }
+    __device__ __forceinline__ void report(bool) const {}
 };
 
 struct DevDst {
@@ -349,6 +353,20 @@ struct TileSrc {
         atomicOr(miss, 1u);
         return K.cap;
     }
+    // One corner of a gather without the defect check's branch: cap when
+    // outside the window, noted in bad; report raises *miss once for the
+    // gather. (A branch and an atomic per corner, over 64 corners, ran the
+    // tile kernel out of scalar registers.)
+    __device__ __forceinline__ float read_acc(const Consts& K, const int gi[6], bool& bad) const {
+        int32_t fl;
+        const bool in = local(K, gi, fl);
+        bad |= !in;
+        const float v = W[in ? fl : 0];
+        return in ? v : K.cap;
+    }
+    __device__ __forceinline__ void report(bool bad) const {
+        if (bad) atomicOr(miss, 1u);
+    }
 };
 
 // The tile's interior: cells from t0 (x, y, heading) over r cells, every
@@ -382,13 +400,15 @@ __device__ __forceinline__ void fetch_src(const Consts& K, const SRC& src, const
             j1[k] = (i[k] + 1 < K.n[k] - 1) ? i[k] + 1 : K.n[k] - 1;
         }
     }
+    bool bad = false;
 #pragma unroll
     for (int d = 0; d < 64; ++d) {
         int c[6];
 #pragma unroll
         for (int k = 0; k < 6; ++k) c[k] = ((d >> k) & 1) ? j1[k] : j0[k];
-        vals[d] = src.read(K, c);
+        vals[d] = src.read_acc(K, c, bad);
     }
+    src.report(bad);
 }
 
 // The value at the grid node nearest the successor (nearest_cell lookups,
@@ -652,6 +672,15 @@ struct EscDev {
     __device__ __forceinline__ float stage(const Consts& K, const int gi[6], bool walls) const {
         return read(K, gi, walls);
     }
+    // one corner of a gather, without branches: both reads are made and
+    // the wall selects (a branch per corner over 64 corners runs the kernel
+    // out of scalar registers)
+    __device__ __forceinline__ float read_acc(const Consts& K, const int gi[6], bool walls, bool&) const {
+        const bool wall = walls & wall_at(K, occ, gi);
+        const float v = escape_view(K, raw(K, gi));
+        return wall ? K.cap : v;
+    }
+    __device__ __forceinline__ void report(bool) const {}
 };
 
 // ... or (out of core) one tile's window, which holds the tile's interior
@@ -671,6 +700,14 @@ struct EscTile {
         int32_t fl;
         return t.local(K, gi, fl) ? escape_view(K, t.W[fl]) : K.cap;
     }
+    __device__ __forceinline__ float read_acc(const Consts& K, const int gi[6], bool walls, bool& bad) const {
+        const bool wall = walls & wall_at(K, occ, gi);
+        bool out = false;
+        const float v = t.read_acc(K, gi, out);
+        bad |= out && !wall;   // as read: a wall reads as cap without touching the window
+        return wall ? K.cap : escape_view(K, v);
+    }
+    __device__ __forceinline__ void report(bool bad) const { t.report(bad); }
 };
 
 // A source with its walls fixed, as the corner readers take it.
@@ -679,6 +716,10 @@ struct EscView {
     const ESRC& src;
     bool walls;
     __device__ __forceinline__ float read(const Consts& K, const int gi[6]) const { return src.read(K, gi, walls); }
+    __device__ __forceinline__ float read_acc(const Consts& K, const int gi[6], bool& bad) const {
+        return src.read_acc(K, gi, walls, bad);
+    }
+    __device__ __forceinline__ void report(bool bad) const { src.report(bad); }
 };
 
 template <class W, bool DIAG, bool GEN, int MAXS, class ESRC, class DST>
