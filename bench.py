@@ -332,9 +332,14 @@ def run_adjoint(pg, prob, args) -> dict:
         Lm = solve(prob.model.with_theta(tm))[0]
         d = (Lp - Lm) / h_eff
         rel = abs(grad[j] - d) / max(abs(d), 1e-300)
-        fd[j] = {"adjoint": float(grad[j]), "fd": d, "rel": rel}
-        print(f"# theta[{j}] = {th[j]:+.5g}: adjoint {grad[j]:+.6e} fd {d:+.6e} rel {rel:.1e}", file=sys.stderr,
-              flush=True)
+        # one-sided differences: where they disagree, L has a kink at theta
+        # (an argmin that changes with the sign of the step) and the central
+        # difference averages the two sides
+        dp = (Lp - L) / (float(np.float32(tp[j])) - float(np.float32(th[j])))
+        dm = (L - Lm) / (float(np.float32(th[j])) - float(np.float32(tm[j])))
+        fd[j] = {"adjoint": float(grad[j]), "fd": d, "rel": rel, "fd_plus": dp, "fd_minus": dm}
+        print(f"# theta[{j}] = {th[j]:+.5g}: adjoint {grad[j]:+.6e} fd {d:+.6e} rel {rel:.1e} "
+              f"(one-sided {dp:+.6e} / {dm:+.6e})", file=sys.stderr, flush=True)
     return {"metric": "adjoint dL/dtheta vs central differences", "L": L, "grad": [float(v) for v in grad],
             "f64_sweeps": its, "f64_seconds": t_solve, "adjoint_seconds": t_adj, "adjoint_iterations": it_adj,
             "fd_step_rel": args.fd_step, "fd": fd, "max_rel": max((v["rel"] for v in fd.values()), default=None),

@@ -730,11 +730,15 @@ __global__ __launch_bounds__(256, 4) void escape_lds_kernel(
     __shared__ float win[W::SIZE];
     __shared__ BoxShared<MAXS> sm;
     __shared__ uint8_t term[MAXS];
+    __shared__ uint8_t blist[MAXS];   // the blocked states still to back up
+    __shared__ int nblist;
     const int64_t box = box_of_block(G);
     const bool live = box < G.nbox;
     BoxInfo B;
     int o[6];
     bool free_here = false;
+    if (threadIdx.x == 0) nblist = 0;
+    __syncthreads();
     if (live) {
         box_info(G, box, B);
 #pragma unroll
@@ -748,34 +752,52 @@ __global__ __launch_bounds__(256, 4) void escape_lds_kernel(
             term[s] = (old >= 0.0f && old < K.cap) ? 1 : 0;
             sm.best[s] = (unsigned long long)0x7f800000u << 32;
             free_here |= sm.blocked[s] == 0;
+            if (sm.blocked[s] && !term[s]) blist[atomicAdd(&nblist, 1)] = (uint8_t)s;
         }
     }
     // walls in the staged window when any state of the box is free
     const bool walls = __syncthreads_or(free_here ? 1 : 0) != 0;
-    if (live) {
+    auto stage = [&](bool with_walls) {
         for (int j = threadIdx.x; j < W::CELLS; j += kThreads) {
             int l[6], gi[6];
             const bool on = window_cell<W>(K.n, o, j, l, gi);
-            const float v = on ? src.stage(K, gi, walls) : K.cap;
+            const float v = on ? src.stage(K, gi, with_walls) : K.cap;
             int lo = 0;
 #pragma unroll
             for (int a = 0; a < 6; ++a) lo += l[a] * W::S[a];
             win[lo] = v;
         }
-    }
+    };
+    if (live) stage(walls);
     __syncthreads();
-    if (live) {
-        const int ns = B.ns;
-        // the value plan's flags (free states only: the plan skips blocked
-        // ones, whose probes here ignore the obstacles): bit 1, no obstacle
-        // along any probe chord; bit 2, every pair's probes pass
-        const unsigned pf = pfree != nullptr ? pfree[B.gbox] : 0u;
-        const bool pclear = (pf & 1u) != 0, pallpass = (pf & 2u) != 0;
-        const int total = ns * (K.span + 1);
-        int s = threadIdx.x % ns, r = threadIdx.x / ns;
-        const int ds = kThreads % ns, dr = kThreads / ns;
+    // the value plan's flags (free states only: the plan skips blocked
+    // ones, whose probes here ignore the obstacles): bit 1, no obstacle
+    // along any probe chord; bit 2, every pair's probes pass
+    const unsigned pf = live && pfree != nullptr ? pfree[B.gbox] : 0u;
+    const bool pclear = (pf & 1u) != 0, pallpass = (pf & 2u) != 0;
+    // Pass 0: free states (and blocked ones when the window holds no walls).
+    // Pass 1, for a box that holds walls and blocked states to back up: the
+    // window is staged again without the walls and the blocked states' pairs
+    // spread over the threads (a blocked state's probes and corners ignore
+    // the obstacles; read through global memory instead, every corner of
+    // such a pair left the window). One loop, so the backup is compiled once.
+    const int npass = (walls && nblist > 0) ? 2 : 1;
+#pragma unroll 1
+    for (int pass = 0; pass < npass; ++pass) {
+        if (pass == 1) {
+            __syncthreads();
+            if (live) stage(false);
+            __syncthreads();
+        }
+        if (!live) continue;
+        const bool walled = pass == 0 && walls;
+        const int n = pass == 0 ? B.ns : nblist;
+        const int total = n * (K.span + 1);
+        int j = threadIdx.x % n, r = threadIdx.x / n;
+        const int dj = kThreads % n, dr = kThreads / n;
         for (int p = threadIdx.x; p < total; p += kThreads) {
-            if (!term[s]) {
+            const int s = pass == 0 ? j : blist[j];
+            if (!term[s] && !(walled && sm.blocked[s])) {
                 const bool blk = sm.blocked[s] != 0;
                 const OccE og{occ, K.n[1], K.n[2], blk || pclear};
                 const EscView<ESRC> er{src, !blk};
@@ -796,7 +818,7 @@ __global__ __launch_bounds__(256, 4) void escape_lds_kernel(
                         i[a] = (int)b;
                     }
                     int l[6];
-                    bool inwin = !(blk && walls);   // the staged walls apply to free states only
+                    bool inwin = !(blk && walled);
 #pragma unroll
                     for (int a = 0; a < 6; ++a) {
                         int d = i[a] - o[a];
@@ -840,8 +862,8 @@ __global__ __launch_bounds__(256, 4) void escape_lds_kernel(
                 const unsigned long long key = ((unsigned long long)__float_as_uint(val) << 32) | (unsigned)r;
                 atomicMin(&sm.best[s], key);
             }
-            s += ds; r += dr;
-            if (s >= ns) { s -= ns; ++r; }
+            j += dj; r += dr;
+            if (j >= n) { j -= n; ++r; }
         }
     }
     __syncthreads();
