@@ -13,22 +13,30 @@ constexpr int kThreads = 256;
 
 // Padded LDS strides of an E1..E6 window, omega fastest. Each stride above
 // omega's rows is rounded up to an odd number of elements so that
-// neighbouring rows start in different banks.
-template <int E1, int E2, int E3, int E4, int E5, int E6>
+// neighbouring rows start in different banks; P2 and P1 add to the y and x
+// strides. The corners a wave gathers at once belong to successors a cell
+// or two apart on each axis, so two lanes collide in a bank when a small
+// combination of strides is a multiple of the bank count: with x and y
+// strides 1621 and 405 (both 21 modulo 64), a step of +1 in x against -1
+// in y is one. 1631 and 407 avoid it: a model of the gathers of a
+// 564M-state, 125-control grid counts a fifth fewer bank-conflict cycles,
+// and the contracted sweeps run 2-3% faster (the exact sweep, bound by its
+// arithmetic, does not move).
+template <int E1, int E2, int E3, int E4, int E5, int E6, int P2 = 0, int P1 = 0>
 struct Win {
     static constexpr int E[6] = {E1, E2, E3, E4, E5, E6};
     static constexpr int S6 = 1;
     static constexpr int S5 = E6 * S6;
     static constexpr int S4 = (E5 * S5) | 1;
     static constexpr int S3 = (E4 * S4) | 1;
-    static constexpr int S2 = (E3 * S3) | 1;
-    static constexpr int S1 = (E2 * S2) | 1;
+    static constexpr int S2 = ((E3 * S3) | 1) + P2;
+    static constexpr int S1 = ((E2 * S2) | 1) + P1;
     static constexpr int SIZE = E1 * S1;
     static constexpr int CELLS = E1 * E2 * E3 * E4 * E5 * E6;
     static constexpr int S[6] = {S1, S2, S3, S4, S5, S6};
 };
 
-using Win0 = Win<4, 4, 4, 4, 4, 6>;
+using Win0 = Win<4, 4, 4, 4, 4, 6, 2, 2>;
 using Win1 = Win<4, 4, 4, 4, 4, 4>;
 using Win2 = Win<4, 4, 4, 5, 5, 5>;
 

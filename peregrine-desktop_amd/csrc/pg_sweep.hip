@@ -32,6 +32,10 @@ struct VGlobal {
     const float* V;
     __device__ __forceinline__ int64_t off(const Consts& K, int k, int j) const { return (int64_t)j * K.stride[k]; }
     __device__ __forceinline__ float load(const Consts&, int64_t a) const { return V[a]; }
+    // one corner of a 64-corner gather (load_acc), and the gather's end
+    // (report): the whole table holds every offset
+    __device__ __forceinline__ float load_acc(const Consts&, int64_t a, bool&) const { return V[a]; }
+    __device__ __forceinline__ void report(bool) const {}
     // the value of cell i (flat index idx on the whole grid)
     __device__ __forceinline__ float self(const Consts&, int64_t idx, const int32_t*) const { return V[idx]; }
 };
@@ -64,6 +68,19 @@ struct VWindow {
             return K.cap;
         }
         return W[a];
+    }
+    // One corner of a 64-corner gather without the defect check's branch:
+    // cap outside the window, noted in bad; report raises *miss once for the
+    // gather (a branch and an atomic per corner run the kernel out of scalar
+    // registers).
+    __device__ __forceinline__ float load_acc(const Consts& K, int64_t a, bool& bad) const {
+        const bool out = (uint64_t)a >= (uint64_t)size;
+        bad |= out;
+        const float v = W[out ? 0 : a];
+        return out ? K.cap : v;
+    }
+    __device__ __forceinline__ void report(bool bad) const {
+        if (bad) atomicOr(miss, 1u);
     }
     __device__ __forceinline__ float self(const Consts& K, int64_t, const int32_t* i) const {
         int64_t a = 0;
@@ -190,8 +207,10 @@ __device__ __forceinline__ float interp_global(const Consts& K, const VR& V,
         a5[d] = a4[d & 15] + ((d & 16) ? o1[4] : o0[4]);
     }
     float vals[64];
+    bool bad = false;
 #pragma unroll
-    for (int d = 0; d < 64; ++d) vals[d] = V.load(K, a5[d & 31] + ((d & 32) ? o1[5] : o0[5]));
+    for (int d = 0; d < 64; ++d) vals[d] = V.load_acc(K, a5[d & 31] + ((d & 32) ? o1[5] : o0[5]), bad);
+    V.report(bad);
     if (ESC) {
         bool wall[8];
 #pragma unroll
