@@ -78,15 +78,15 @@ FP32_VECTOR_PEAK_TFLOPS = 157.3
 # Kernels: 2 = exact LDS-staged sweep (the FP32 table), 3 = contracted sweep
 # over the FP32 table, 4 = contracted sweep over the 16-bit store.
 PMC = {
-    (2, 2): {"hbm_bytes": (21_235_533 + 2_623_983) * 1024, "flops_counted": 64 * 685_315_275_294,
-             "valu_insts": 1_031_362_332_600, "valu_issue_frac": 0.693,
-             "lds_bank_conflict_cycles": 213_018_474_560},
-    (2, 3): {"hbm_bytes": (21_185_032 + 2_673_361) * 1024, "flops_counted": 64 * 467_560_510_574,
-             "valu_insts": 479_103_616_971, "valu_issue_frac": 0.612,
-             "lds_bank_conflict_cycles": 213_036_013_056},
-    (2, 4): {"hbm_bytes": (8_586_874 + 2_592_330) * 1024, "flops_counted": 64 * 478_776_759_961,
-             "valu_insts": 545_896_402_353, "valu_issue_frac": 0.640,
-             "lds_bank_conflict_cycles": 213_036_013_056},
+    (2, 2): {"hbm_bytes": (21_177_128 + 2_649_233) * 1024, "flops_counted": 64 * 685_315_275_294,
+             "valu_insts": 1_031_362_332_600, "valu_issue_frac": 0.698,
+             "lds_bank_conflict_cycles": 195_850_124_160},
+    (2, 3): {"hbm_bytes": (21_113_265 + 2_715_298) * 1024, "flops_counted": 64 * 467_560_510_574,
+             "valu_insts": 479_103_616_971, "valu_issue_frac": 0.635,
+             "lds_bank_conflict_cycles": 195_867_540_288},
+    (2, 4): {"hbm_bytes": (8_594_514 + 2_592_552) * 1024, "flops_counted": 64 * 478_776_759_961,
+             "valu_insts": 545_896_402_353, "valu_issue_frac": 0.656,
+             "lds_bank_conflict_cycles": 195_867_540_288},
 }
 
 
