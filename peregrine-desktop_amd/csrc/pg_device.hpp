@@ -174,7 +174,9 @@ __device__ __forceinline__ bool euler(const Consts& K, float hs, int nsub, float
 }
 
 // Wrapped heading coordinate f - floor(f / n3) * n3 (PeregrineSolver.jl:407-408).
-__device__ __forceinline__ float wrap_h(const Consts& K, float f) {
+// GC: Consts, or a GridC copy of its grid constants.
+template <class GC>
+__device__ __forceinline__ float wrap_h(const GC& K, float f) {
     return f - __builtin_floorf(qdiv(f, K.rn3)) * K.n3f;
 }
 
@@ -549,29 +551,58 @@ __device__ __forceinline__ bool rk2_state(const Consts& K, float hs, int nsub, f
     return ok;
 }
 
+// The grid constants of a lookup's coordinates, as Consts holds them. A
+// kernel short of scalar registers keeps this copy in LDS and reads it once
+// per lookahead instead of holding 27 more registers across its loop.
+struct GridC {
+    double rstep[6];
+    double rn3;
+    float lo[6];
+    float lim[6];
+    float n3f;
+};
+
+__device__ __forceinline__ void gridc_fill(const Consts& K, GridC& g) {
+#pragma unroll
+    for (int a = 0; a < 6; ++a) {
+        g.rstep[a] = K.rstep[a];
+        g.lo[a] = K.lo[a];
+        g.lim[a] = K.lim[a];
+    }
+    g.rn3 = K.rn3;
+    g.n3f = K.n3f;
+}
+
 // Fractional coordinates of a successor, with the range tests of interp
-// (PeregrineSolver.jl:401-428). Returns false when the lookup rejects.
-template <bool GEN = true>
-__device__ __forceinline__ bool interp_coords(const Consts& K, float x, float y, float h, float vx,
+// (PeregrineSolver.jl:401-428). Returns false when the lookup rejects. The
+// grid constants come from gc: K itself, or a GridC copy of them.
+template <bool GEN = true, class GC>
+__device__ __forceinline__ bool interp_coords(const Consts& K, const GC& gc, float x, float y, float h, float vx,
                                               float vy, float w, float f[6]) {
-    f[0] = qdiv(x - K.lo[0], K.rstep[0]);
-    f[1] = qdiv(y - K.lo[1], K.rstep[1]);
-    f[2] = wrap_h(K, qdiv(h - K.lo[2], K.rstep[2]));
-    f[3] = qdiv(vx - K.lo[3], K.rstep[3]);
-    f[4] = qdiv(vy - K.lo[4], K.rstep[4]);
-    f[5] = qdiv(w - K.lo[5], K.rstep[5]);
+    f[0] = qdiv(x - gc.lo[0], gc.rstep[0]);
+    f[1] = qdiv(y - gc.lo[1], gc.rstep[1]);
+    f[2] = wrap_h(gc, qdiv(h - gc.lo[2], gc.rstep[2]));
+    f[3] = qdiv(vx - gc.lo[3], gc.rstep[3]);
+    f[4] = qdiv(vy - gc.lo[4], gc.rstep[4]);
+    f[5] = qdiv(w - gc.lo[5], gc.rstep[5]);
     bool in = true;
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
         if (k >= 3 && GEN && K.vclamp) {
             // velocity_clamp: a finite velocity off the grid reads the edge
             in = in & finite_f(f[k]);
-            f[k] = f[k] > K.lim[k] ? K.lim[k] : (f[k] < 0.0f ? 0.0f : f[k]);
+            f[k] = f[k] > gc.lim[k] ? gc.lim[k] : (f[k] < 0.0f ? 0.0f : f[k]);
         } else {
-            in = in & (f[k] >= 0.0f) & (f[k] <= K.lim[k]);
+            in = in & (f[k] >= 0.0f) & (f[k] <= gc.lim[k]);
         }
     }
     return in;
+}
+
+template <bool GEN = true>
+__device__ __forceinline__ bool interp_coords(const Consts& K, float x, float y, float h, float vx,
+                                              float vy, float w, float f[6]) {
+    return interp_coords<GEN>(K, K, x, y, h, vx, vy, w, f);
 }
 
 // Level-by-level corner weights of the first five axes; weight d of the

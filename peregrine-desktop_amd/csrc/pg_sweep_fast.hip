@@ -249,11 +249,21 @@ __global__ __launch_bounds__(256, 4) void sweep_fast_kernel(
                 cell_start(K, i[0], i[1], i[2], i[3], i[4], i[5], c);
                 fast_start<DIAG>(K, F, c, S);
             }
-            unsigned pk = 0;
+            // the state's own cell in window coordinates, 4 bits per axis
+            // (15: outside the window, so never a corner of an in-window
+            // successor)
+            unsigned pl = 0;
             float vmir = 0.0f;
             if (SELF) {
 #pragma unroll
-                for (int a = 0; a < 6; ++a) pk |= (unsigned)(i[a] - B.s0[a]) << (3 * a);
+                for (int a = 0; a < 6; ++a) {
+                    int d = i[a] - o[a];
+                    if (a == 2) {
+                        d = d < 0 ? d + K.n[2] : d;
+                        d = d >= K.n[2] ? d - K.n[2] : d;
+                    }
+                    pl |= (unsigned)((unsigned)d < (unsigned)W::E[a] ? d : 15) << (4 * a);
+                }
                 vmir = vin.read(K, Q, i);
             }
             // One backup: the value of control k, or false when a corner of
@@ -284,19 +294,24 @@ __global__ __launch_bounds__(256, 4) void sweep_fast_kernel(
                     l[a] = d;
                     inwin = inwin & ((unsigned)d <= (unsigned)(W::E[a] - 2));
                 }
-                // Weight of the state's own cell in the successor's stencil.
+                // Weight of the state's own cell in the successor's stencil:
+                // the product over the axes of 1 - t where the cell is the
+                // lower corner and t where it is the upper one, 0 when it is
+                // neither on some axis.
                 float wself = 0.0f;
-                if (SELF) {
-                    wself = 1.0f;
-#pragma unroll
-                    for (int a = 0; a < 6; ++a) {
-                        int d = B.s0[a] + (int)((pk >> (3 * a)) & 7u) - si[a];
-                        if (a == 2) d = d < 0 ? d + K.n[2] : d;
-                        wself *= d == 0 ? 1.0f - t[a] : (d == 1 ? t[a] : 0.0f);
-                    }
-                }
                 float acc;
                 if constexpr (OUTSIDE) {
+                    if (SELF) {
+                        int own[6];
+                        state_coords(B, s, own);
+                        wself = 1.0f;
+#pragma unroll
+                        for (int a = 0; a < 6; ++a) {
+                            int d = own[a] - si[a];
+                            if (a == 2) d = d < 0 ? d + K.n[2] : d;
+                            wself *= d == 0 ? 1.0f - t[a] : (d == 1 ? t[a] : 0.0f);
+                        }
+                    }
                     // corners outside the window: global memory, the
                     // reference's clamped / periodic indices
                     acc = tensor_interp_rolled(
@@ -318,6 +333,19 @@ __global__ __launch_bounds__(256, 4) void sweep_fast_kernel(
                         t);
                 } else {
                     if (!inwin) return false;
+                    if (SELF) {
+                        // window coordinates: the cell is a corner on axis a
+                        // when its offset from the base corner is 0 or 1
+                        unsigned bad = 0;
+                        float ws = 1.0f;
+#pragma unroll
+                        for (int a = 0; a < 6; ++a) {
+                            const int d = (int)((pl >> (4 * a)) & 15u) - l[a];
+                            bad |= (unsigned)d;
+                            ws *= d == 0 ? 1.0f - t[a] : t[a];
+                        }
+                        wself = bad > 1u ? 0.0f : ws;
+                    }
                     int base = 0;
 #pragma unroll
                     for (int a = 0; a < 6; ++a) base += l[a] * W::S[a];
@@ -329,13 +357,17 @@ __global__ __launch_bounds__(256, 4) void sweep_fast_kernel(
                         },
                         t);
                 }
-                if (SELF && wself > 0.0f) {
+                if (SELF) {
                     // V = tau + w V + rest solved for V (Jacobi on the
                     // policy system with its diagonal): same fixed point
                     // as tau + min(acc, cap), without the 1/(1-w)-times
-                    // slower approach through the cell's own value.
+                    // slower approach through the cell's own value. With
+                    // w = 0 this is tau + acc clamped to cap, the other
+                    // branch's value (acc and vmir are finite), without a
+                    // branch.
                     const float rest = acc - wself * vmir;
-                    const float cc = (K.tau + rest) * __builtin_amdgcn_rcpf(1.0f - wself);
+                    const float r = wself > 0.0f ? __builtin_amdgcn_rcpf(1.0f - wself) : 1.0f;
+                    const float cc = (K.tau + rest) * r;
                     val = cc < K.cap ? cc : K.cap;
                 } else {
                     const float nv = acc < K.cap ? acc : K.cap;

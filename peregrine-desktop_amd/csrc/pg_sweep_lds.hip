@@ -8,6 +8,8 @@
 // window per box) reads global memory with the reference's clamped indices.
 // The arithmetic is pg_device.hpp's, so the results equal the global-path
 // kernel's, bit for bit.
+#include <type_traits>
+
 #include "pg_box.hpp"
 #include "pg_kernels.h"
 
@@ -75,11 +77,12 @@ __device__ __forceinline__ void setup_box(const Consts& K, const BoxInfo& B,
 // run. REACH: when the probes are reached, reach[] receives the grid x, y
 // and unwrapped heading coordinates of the probe chord's far end (c + d, as
 // the probes form it; every probe lies between the start and that end on
-// each axis) and *probed whether they passed.
-template <bool DIAG, bool GEN, class OCC = OccG, bool REACH = false>
+// each axis) and *probed whether they passed. gc: where the successor's
+// grid coordinates take their constants (K, or an LDS copy).
+template <bool DIAG, bool GEN, class OCC = OccG, bool REACH = false, class GC = Consts>
 __device__ __forceinline__ int successor_t(const Consts& K, const OCC& og, const CellStart& c,
                                            float bux, float buy, float buw, float f[6], bool noprobe = false,
-                                           float* reach = nullptr, int* probed = nullptr) {
+                                           float* reach = nullptr, int* probed = nullptr, const GC* gc = nullptr) {
     float nx, ny, nh, nvx, nvy, nw;
     const bool ok = (GEN && K.rk2) ? rk2_from<DIAG>(K, c, bux, buy, buw, nx, ny, nh, nvx, nvy, nw)
                                    : euler_from<DIAG>(K, c, bux, buy, buw, nx, ny, nh, nvx, nvy, nw);
@@ -101,7 +104,8 @@ __device__ __forceinline__ int successor_t(const Consts& K, const OCC& og, const
     sincos_ref(nh, snh, cnh);
     const float nvfx = cnh * nvx - snh * nvy;
     const float nvfy = snh * nvx + cnh * nvy;
-    return interp_coords<GEN>(K, nx, ny, nh, nvfx, nvfy, nw, f) ? 1 : 0;
+    if constexpr (std::is_same<GC, Consts>::value) return interp_coords<GEN>(K, nx, ny, nh, nvfx, nvfy, nw, f) ? 1 : 0;
+    else return interp_coords<GEN>(K, *gc, nx, ny, nh, nvfx, nvfy, nw, f) ? 1 : 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -476,6 +480,7 @@ __global__ __launch_bounds__(256, 4) void sweep_lds_kernel(
     const uint8_t* __restrict__ pfree, double* __restrict__ part, unsigned* __restrict__ linf_bits, ActiveG A) {
     __shared__ float win[W::SIZE];
     __shared__ BoxShared<MAXS> sm;
+    __shared__ GridC gcs;   // read once per lookahead: 27 fewer scalar registers held across the loop
     const int64_t box = box_of_block(G);
     BoxInfo B;
     int o[6];
@@ -484,6 +489,7 @@ __global__ __launch_bounds__(256, 4) void sweep_lds_kernel(
     const bool live = box < G.nbox && box_active(A, G, K.n, B);
     if (live) {
         if (A.nwork && threadIdx.x == 0) atomicAdd(A.nwork, 1u);
+        if (threadIdx.x == 0) gridc_fill(K, gcs);
 #pragma unroll
         for (int a = 0; a < 6; ++a) o[a] = worig[B.gbox * 6 + a];
         // stage the window; cells off the grid hold cap (read only with zero weight)
@@ -527,8 +533,8 @@ __global__ __launch_bounds__(256, 4) void sweep_lds_kernel(
                 float f[6];
                 float val = K.cap;
                 const int k = slice_control(K, r);
-                if (successor_t<DIAG, GEN>(K, og, c, sm.bu[3 * k], sm.bu[3 * k + 1], sm.bu[3 * k + 2], f,
-                                           noprobe)) {
+                if (successor_t<DIAG, GEN, OccG, false, GridC>(K, og, c, sm.bu[3 * k], sm.bu[3 * k + 1],
+                                                               sm.bu[3 * k + 2], f, noprobe, nullptr, nullptr, &gcs)) {
                     float t[6], u[6];
                     int i[6];
 #pragma unroll
