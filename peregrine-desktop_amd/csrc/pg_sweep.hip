@@ -327,9 +327,19 @@ __global__ __launch_bounds__(256, 3) void sweep_global_kernel(
     VR Vin, float* Vout, TileCells T, const uint8_t* __restrict__ occ,
     const float* __restrict__ bu, const float* __restrict__ uraw, Consts K, uint8_t* __restrict__ pol,
     int8_t* __restrict__ wpol, double* __restrict__ part_sum, unsigned* __restrict__ linf_bits) {
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     float dec = 0.0f;
-    if (idx < T.ncell) {
+    if (tid < T.ncell) {
+        // With a per-cell horizon the substep counts follow the velocity
+        // (cfl_tau), so the threads of a wave take cells of one velocity and
+        // 64 positions / headings: every lane then runs the same number of
+        // substeps. Otherwise consecutive threads take consecutive cells.
+        int64_t idx = tid;
+        if (K.cfl) {
+            const int64_t nvel = (int64_t)K.n[3] * K.n[4] * K.n[5];
+            const int64_t npos = T.ncell / nvel;
+            idx = (tid % npos) * nvel + tid / npos;
+        }
         int64_t r = idx;
         int32_t i[6];
 #pragma unroll
