@@ -322,6 +322,9 @@ __device__ __forceinline__ int8_t pol_quant(float v) {
 // Three waves per SIMD (at most 168 VGPRs): the production value sweep then
 // spills 6 VGPRs (28 B/lane of scratch) and still sweeps 20% faster than at
 // two waves (188 VGPRs, no spill).
+// Neighbouring spins one wave of the per-cell-horizon sweep takes (see below).
+constexpr int64_t kSpinGroup = 2;
+
 template <bool WRITE_V, bool WRITE_POL, bool ESC, bool PROD, class VR>
 __global__ __launch_bounds__(256, 3) void sweep_global_kernel(
     VR Vin, float* Vout, TileCells T, const uint8_t* __restrict__ occ,
@@ -331,14 +334,24 @@ __global__ __launch_bounds__(256, 3) void sweep_global_kernel(
     float dec = 0.0f;
     if (tid < T.ncell) {
         // With a per-cell horizon the substep counts follow the velocity
-        // (cfl_tau), so the threads of a wave take cells of one velocity and
-        // 64 positions / headings: every lane then runs the same number of
-        // substeps. Otherwise consecutive threads take consecutive cells.
+        // (cfl_tau), so the threads of a wave take cells of one (vx, vy),
+        // kSpinGroup neighbouring spins and 64 / kSpinGroup positions and
+        // headings: the lanes then run nearly the same number of substeps,
+        // and the spin neighbours' reads of V still share cache lines.
+        // Otherwise consecutive threads take consecutive cells.
         int64_t idx = tid;
         if (K.cfl) {
-            const int64_t nvel = (int64_t)K.n[3] * K.n[4] * K.n[5];
+            const int64_t n5 = K.n[5];
+            const int64_t nvel = (int64_t)K.n[3] * K.n[4] * n5;
             const int64_t npos = T.ncell / nvel;
-            idx = (tid % npos) * nvel + tid / npos;
+            const int64_t vv = tid / (npos * n5);            // (vx, vy)
+            const int64_t rem = tid - vv * npos * n5;
+            const int64_t kb = rem / (npos * kSpinGroup);      // spin group; the last may be short
+            const int64_t r2 = rem - kb * npos * kSpinGroup;
+            const int64_t kk = n5 - kb * kSpinGroup < kSpinGroup ? n5 - kb * kSpinGroup : kSpinGroup;
+            const int64_t pos = r2 / kk;
+            const int64_t w = kb * kSpinGroup + (r2 - pos * kk);
+            idx = pos * nvel + vv * n5 + w;
         }
         int64_t r = idx;
         int32_t i[6];
