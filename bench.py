@@ -78,14 +78,14 @@ FP32_VECTOR_PEAK_TFLOPS = 157.3
 # Kernels: 2 = exact LDS-staged sweep (the FP32 table), 3 = contracted sweep
 # over the FP32 table, 4 = contracted sweep over the 16-bit store.
 PMC = {
-    (2, 2): {"hbm_bytes": (21_177_128 + 2_649_233) * 1024, "flops_counted": 64 * 685_315_275_294,
-             "valu_insts": 1_031_362_332_600, "valu_issue_frac": 0.698,
+    (2, 2): {"hbm_bytes": (21_171_403 + 2_660_072) * 1024, "flops_counted": 64 * 685_315_275_294,
+             "valu_insts": 1_011_039_627_850, "valu_issue_frac": 0.701,
              "lds_bank_conflict_cycles": 195_850_124_160},
-    (2, 3): {"hbm_bytes": (21_113_265 + 2_715_298) * 1024, "flops_counted": 64 * 467_560_510_574,
-             "valu_insts": 479_103_616_971, "valu_issue_frac": 0.635,
+    (2, 3): {"hbm_bytes": (21_111_973 + 2_717_123) * 1024, "flops_counted": 64 * 467_560_510_574,
+             "valu_insts": 479_103_616_971, "valu_issue_frac": 0.634,
              "lds_bank_conflict_cycles": 195_867_540_288},
-    (2, 4): {"hbm_bytes": (8_594_514 + 2_592_552) * 1024, "flops_counted": 64 * 478_776_759_961,
-             "valu_insts": 545_896_402_353, "valu_issue_frac": 0.656,
+    (2, 4): {"hbm_bytes": (8_573_177 + 2_591_681) * 1024, "flops_counted": 64 * 486_117_086_529,
+             "valu_insts": 541_009_196_454, "valu_issue_frac": 0.661,
              "lds_bank_conflict_cycles": 195_867_540_288},
 }
 
