@@ -301,6 +301,10 @@ __device__ __forceinline__ int8_t pol_quant(float v) {
     return (int8_t)(int32_t)__builtin_rintf(c * 127.0f);
 }
 
+// Neighbouring spins that one wave of the per-cell-horizon sweep takes (see
+// sweep_global_kernel).
+constexpr int64_t kSpinGroup = 2;
+
 // One thread per cell of T (the whole grid, or an out-of-core tile's interior:
 // then Vout, pol and wpol are interior-shaped and V is read through the
 // tile's window). ESC: the escape pass (gpu_escape_kernel!, Gpu.jl:51-67)
@@ -322,9 +326,6 @@ __device__ __forceinline__ int8_t pol_quant(float v) {
 // Three waves per SIMD (at most 168 VGPRs): the production value sweep then
 // spills 6 VGPRs (28 B/lane of scratch) and still sweeps 20% faster than at
 // two waves (188 VGPRs, no spill).
-// Neighbouring spins one wave of the per-cell-horizon sweep takes (see below).
-constexpr int64_t kSpinGroup = 2;
-
 template <bool WRITE_V, bool WRITE_POL, bool ESC, bool PROD, class VR>
 __global__ __launch_bounds__(256, 3) void sweep_global_kernel(
     VR Vin, float* Vout, TileCells T, const uint8_t* __restrict__ occ,
