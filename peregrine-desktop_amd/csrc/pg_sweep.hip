@@ -301,10 +301,6 @@ __device__ __forceinline__ int8_t pol_quant(float v) {
     return (int8_t)(int32_t)__builtin_rintf(c * 127.0f);
 }
 
-// Neighbouring spins that one wave of the per-cell-horizon sweep takes (see
-// sweep_global_kernel).
-constexpr int64_t kSpinGroup = 2;
-
 // One thread per cell of T (the whole grid, or an out-of-core tile's interior:
 // then Vout, pol and wpol are interior-shaped and V is read through the
 // tile's window). ESC: the escape pass (gpu_escape_kernel!, Gpu.jl:51-67)
@@ -335,24 +331,23 @@ __global__ __launch_bounds__(256, 3) void sweep_global_kernel(
     float dec = 0.0f;
     if (tid < T.ncell) {
         // With a per-cell horizon the substep counts follow the velocity
-        // (cfl_tau), so the threads of a wave take cells of one (vx, vy),
-        // kSpinGroup neighbouring spins and 64 / kSpinGroup positions and
-        // headings: the lanes then run nearly the same number of substeps,
-        // and the spin neighbours' reads of V still share cache lines.
-        // Otherwise consecutive threads take consecutive cells.
+        // (cfl_tau), so the threads of a wave take one velocity at 64
+        // consecutive positions (x, y, heading): every lane runs the same
+        // number of substeps. The positions go in blocks of 64 whose every
+        // velocity is taken by consecutive waves, so that the block's reads
+        // of V, which share cache lines across velocities, are served while
+        // those lines are still cached. Otherwise consecutive threads take
+        // consecutive cells.
         int64_t idx = tid;
         if (K.cfl) {
-            const int64_t n5 = K.n[5];
-            const int64_t nvel = (int64_t)K.n[3] * K.n[4] * n5;
+            constexpr int64_t P = 64;
+            const int64_t nvel = (int64_t)K.n[3] * K.n[4] * K.n[5];
             const int64_t npos = T.ncell / nvel;
-            const int64_t vv = tid / (npos * n5);            // (vx, vy)
-            const int64_t rem = tid - vv * npos * n5;
-            const int64_t kb = rem / (npos * kSpinGroup);      // spin group; the last may be short
-            const int64_t r2 = rem - kb * npos * kSpinGroup;
-            const int64_t kk = n5 - kb * kSpinGroup < kSpinGroup ? n5 - kb * kSpinGroup : kSpinGroup;
-            const int64_t pos = r2 / kk;
-            const int64_t w = kb * kSpinGroup + (r2 - pos * kk);
-            idx = pos * nvel + vv * n5 + w;
+            const int64_t pb = tid / (P * nvel);               // the last block may hold fewer positions
+            const int64_t pp = npos - pb * P < P ? npos - pb * P : P;
+            const int64_t rem = tid - pb * P * nvel;
+            const int64_t vel = rem / pp;
+            idx = (pb * P + (rem - vel * pp)) * nvel + vel;
         }
         int64_t r = idx;
         int32_t i[6];
